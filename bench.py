@@ -1,0 +1,164 @@
+"""Benchmark: wall-RPY blob mobility matvec on one MI355X.
+
+One step = one wall-corrected RPY translational product u = B M_UF(z_eff) B f
+over the rigid-multiblob suspension of the rigid-solve configuration
+(256 spheres x 42 blobs = 10752 blobs, blob a = 0.243), FP32 pair terms
+with FP64 per-target sums, inputs resident in HBM.  This is the product
+that every GMRES iteration of that solve applies.
+
+Prints ONE JSON line (rank 0):
+  value        blob-pairs/s = N^2 / (time per step)
+  roofline     pair kernel: algorithmic FLOP per launch (164 FLOP/pair x N^2)
+               / mean pair-kernel duration (HIP events on the launch stream),
+               against the FP32 vector peak; traffic from committed rocprof
+               PMC runs when present
+  cpu_baseline FP64 C oracle (-O3 -ffast-math, OpenMP over all usable host
+               cores) on a bounded sample of target rows of the same workload
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "wall-RPY matvec blob-pairs/s; GMRES time-to-1e-8 residual (learned x0 vs plain)"
+FLOP_PER_PAIR = {"UF": 164, "UT": 95, "WF": 95, "WT": 101, "grand": 386}
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def load_workload(name: str):
+    from rigidmultiblobswall_amd import workloads
+    if name == "c2":
+        w = workloads.c2_rigid_spheres()
+        desc = "C2 rigid-solve matvec: 256 spheres x 42 blobs = 10752 blobs, a=0.243, eta=1, UF block"
+    elif name == "c1":
+        w = workloads.c1_uf_matvec()
+        desc = "C1: N=2048 blobs, a=0.13, eta=1, UF block"
+    elif name == "c3":
+        w = workloads.c3_rollers()
+        desc = "C3 rollers: N=32768 blobs, a=0.656, eta=1e-3, UF block"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return w, desc
+
+
+def cpu_baseline(r, f, eta, a, target_seconds=12.0):
+    """Time the FP64 oracle (fast-math build, all usable cores) on a
+    bounded row sample of the same matvec; returns pairs/s."""
+    from oracle import oracle as O
+    n = r.shape[0]
+    threads = O.num_threads("fast")
+    # calibrate on a small sample, then run ~target_seconds of work
+    rows0 = max(1, min(n, 64))
+    t = time.perf_counter()
+    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows0), kind="fast")
+    dt = max(time.perf_counter() - t, 1e-6)
+    rows = int(min(n, max(rows0, rows0 * target_seconds / dt)))
+    t = time.perf_counter()
+    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows), kind="fast")
+    dt = time.perf_counter() - t
+    return {"value": rows * n / dt, "unit": "blob-pairs/s", "cores": threads, "kind": "port",
+            "sample": f"UF rows 0..{rows} of {n} (x {n} sources), FP64 C oracle -O3 -ffast-math "
+                      f"OpenMP, {dt:.1f} s"}
+
+
+def load_traffic(workload: str):
+    """HBM bytes per pair-kernel launch from the committed rocprof PMC summary."""
+    path = os.path.join(REPO, "profiles", f"pair_kernel_uf_{workload}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("traffic_bytes")
+    return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        return 2
+
+    import torch
+    from rigidmultiblobswall_amd import wall_rpy
+    from rigidmultiblobswall_amd._lib import RMW_UF
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    w, desc = load_workload(args.workload)
+    r_h, eta, a = w["r"], w["eta"], w["a"]
+    n = r_h.shape[0]
+    rng = np.random.default_rng(7)
+    f_h = rng.standard_normal(3 * n)
+    r = torch.from_numpy(r_h).to(dev)
+    f = torch.from_numpy(f_h).to(dev)
+    u = torch.empty((1, n, 3), dtype=torch.float64, device=dev)
+
+    def step():
+        wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with wall_rpy.PairKernelTimer() as pk:
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(args.steps):
+            step()
+        stop.record()
+        torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    pairs = float(n) * n
+    value = pairs / (ms * 1e-3)
+    flop = FLOP_PER_PAIR["UF"] * pairs
+    achieved = flop / (pk.mean_ms * 1e-3) / 1e12
+    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
+    info = wall_rpy.device_info()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "blob-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "blobs": n, "pairs_per_step": pairs,
+                   "precision": args.precision + (" (FP32 pair terms, FP64 per-target sums)"
+                                                  if args.precision == "fp32" else ""),
+                   "device": info},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": load_traffic(args.workload),
+                     "kernel": "pair_kernel_f32<UF,K=1>" if args.precision != "fp64" else "pair_kernel_f64<UF,K=1>",
+                     "kernel_mean_ms": pk.mean_ms, "kernel_launches": pk.launches,
+                     "flop_per_pair": FLOP_PER_PAIR["UF"]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(r_h, f_h, eta, a)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

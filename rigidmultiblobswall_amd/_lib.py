@@ -1,0 +1,115 @@
+"""ctypes binding of librmw_hip.so (the C ABI declared in include/rmw_hip.h).
+
+The library is built in-tree by ``rigidmultiblobswall_amd.build``.  There is
+no fallback: if the shared object is missing or a symbol is absent, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librmw_hip.so")
+
+RMW_OK = 0
+RMW_UF, RMW_UT, RMW_WF, RMW_WT = 1, 2, 4, 8
+RMW_PREC_F32_F64ACC, RMW_PREC_F32_KAHAN, RMW_PREC_F64 = 0, 1, 2
+
+PRECISIONS = {
+    "fp32": RMW_PREC_F32_F64ACC,
+    "fp32_f64acc": RMW_PREC_F32_F64ACC,
+    "fp32_kahan": RMW_PREC_F32_KAHAN,
+    "fp64": RMW_PREC_F64,
+}
+
+
+class RmwError(RuntimeError):
+    """Error reported by librmw_hip (status code + message)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"librmw_hip error {code}: {msg}")
+        self.code = code
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("chunk_len", ctypes.c_int64), ("max_wgs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+_vp = ctypes.c_void_p
+_dp = ctypes.c_void_p  # device pointers are passed as integers
+_i64 = ctypes.c_int64
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_sz = ctypes.c_size_t
+_szp = ctypes.POINTER(ctypes.c_size_t)
+_tp = ctypes.POINTER(Tuning)
+_hdp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every symbol declared in include/rmw_hip.h.
+PROTOTYPES = {
+    "rmw_last_error": (ctypes.c_char_p, []),
+    "rmw_version": (ctypes.c_char_p, []),
+    "rmw_wall_rpy_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, _szp]),
+    "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz, _vp]),
+    "rmw_wall_rpy_ragged_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,
+                                                          ctypes.c_int, _tp, _szp]),
+    "rmw_wall_rpy_ragged_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64p,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                 _hdp, _tp, _vp, _sz, _vp]),
+    "rmw_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "rmw_profile_collect": (ctypes.c_int, [_i64p, _hdp, _hdp]),
+    "rmw_device_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load librmw_hip.so; raise if it is not built or incomplete."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built. Build it with `python -m rigidmultiblobswall_amd.build` "
+                "(there is no CPU fallback).")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(lib, name)  # AttributeError if the symbol is missing
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc: int) -> None:
+    if rc != RMW_OK:
+        msg = load().rmw_last_error()
+        raise RmwError(rc, msg.decode() if msg else "")
+
+
+def precision_code(precision) -> int:
+    if isinstance(precision, int):
+        return precision
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"unknown precision {precision!r}; use one of {sorted(PRECISIONS)}") from None
+
+
+def tuning_ptr(tuning):
+    if tuning is None:
+        return None
+    if isinstance(tuning, Tuning):
+        return ctypes.byref(tuning)
+    t = Tuning(int(tuning.get("chunk_len", 0)), int(tuning.get("max_wgs", 0)), 0)
+    return ctypes.byref(t)

@@ -1,0 +1,332 @@
+// Pair terms of the wall-corrected RPY blob mobility (Rotne-Prager-Yamakawa
+// tensor plus the Swan-Brady single-wall image correction), written once as
+// host/device templates over the real type T (float or double).
+//
+// Reference math: mobility/mobility_numba.py
+//   UF  :200-276  (RPY :209-234, wall :241-276)
+//   UT  :614-679  (wall in the transposed source/target role, :646-679)
+//   WF  :1004-1065
+//   WT  :1256-1321
+//
+// The reference forms the 3x3 block M_ij element by element and multiplies
+// it by the source vector.  Here each block is decomposed into a few scalar
+// coefficients times dyads built from d = r_i - r_j, the image vector
+// R = (d_x, d_y, z_i + z_j) and z-hat, so that the product M_ij v costs
+// ~16-20 operations per right-hand side and the geometry part is shared by
+// k right-hand sides and by the four blocks of the grand mobility:
+//
+//   UF v = (A + f1) v + Bc (d.v) d + [f2 (e.v) + f3 v_z] e + [f4 (e.v) + f5 v_z] z
+//   WF v = c (v x d) - [F1 (v x e) + (F2 v_z + F3 (e.v)) (z x e) + F4 (v x z)]
+//   UT v = c (v x d) - [F1 (e' x v) + (F2 z + F3 e') ((z x e').v) + F4 (z x v)]
+//   WT v = (A + G1) v + Bc (d.v) d + G2 (e.v) e + G3 (e.v) z + G4 ((z x e).v)(z x e)
+//
+// with e = R/|R| and e' = (-R_x, -R_y, R_z)/|R| (UT's image seen from the
+// source).  All lengths are in units of the blob radius a; the 1/(8 pi eta
+// a^p) normalisations are folded into the source vectors by the caller.
+// Self pairs (i == j in the central box) are handled by the caller.
+#pragma once
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define RMW_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define RMW_HD inline
+#endif
+
+namespace rmw {
+
+template <typename T> struct Vec3 { T x, y, z; };
+
+// Reciprocal square root.  FP32 on the device uses the hardware v_rsq_f32
+// (about 1 ulp); FP64 always uses a correctly rounded sqrt and a division.
+RMW_HD float rsq(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / std::sqrt(x);
+#endif
+}
+RMW_HD double rsq(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return 1.0 / __builtin_sqrt(x);
+#else
+  return 1.0 / std::sqrt(x);
+#endif
+}
+
+RMW_HD float rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+RMW_HD double rcp(double x) { return 1.0 / x; }
+
+// ---------------------------------------------------------------------------
+// Shared geometry of one (target, source) pair.
+// ---------------------------------------------------------------------------
+template <typename T> struct Geom {
+  T dx, dy, dz;   // d = r_i - r_j (units of a)
+  T Rz;           // z_i + z_j
+  T r2, invr, r;  // |d|^2, 1/|d|, |d|
+  T invR;         // 1/|R|
+  T ez, ez2;      // R_z/|R| and its square
+};
+
+template <typename T>
+RMW_HD Geom<T> make_geom(T dx, T dy, T dz, T zi, T zj)
+{
+  Geom<T> g;
+  g.dx = dx; g.dy = dy; g.dz = dz;
+  g.Rz = zi + zj;
+  T p2 = dx * dx + dy * dy;
+  g.r2 = p2 + dz * dz;
+  g.invr = rsq(g.r2);
+  g.r = g.r2 * g.invr;
+  T R2 = p2 + g.Rz * g.Rz;
+  g.invR = rsq(R2);
+  g.ez = g.Rz * g.invR;
+  g.ez2 = g.ez * g.ez;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// UF: translation from force.   mobility_numba.py:209-276
+// ---------------------------------------------------------------------------
+template <typename T> struct CoefUF {
+  T s;      // A + f1 (coefficient of v)
+  T s5;     // s + f5 (coefficient of v_z in the z row)
+  T Bc;     // coefficient of (d.v) d
+  T g2;     // f2 / |R|^2  -> coefficient of (R.v) R
+  T g3;     // f3 / |R|    -> coefficient of v_z R
+  T g4;     // f4 / |R|    -> coefficient of (R.v) z
+};
+
+template <typename T>
+RMW_HD CoefUF<T> coef_uf(const Geom<T>& g, T zj)
+{
+  const T one = T(1), two = T(2), three = T(3);
+  CoefUF<T> c;
+  // RPY, far (r > 2) or overlapping (r <= 2) branch as selects.
+  T invr2 = g.invr * g.invr;
+  T A_far = (one + (two / three) * invr2) * g.invr;
+  T B_far = (one - two * invr2) * invr2 * g.invr;
+  T A_near = (T(4) / three) * (one - T(0.28125) * g.r);
+  T B_near = (T(4) / three) * T(0.09375) * g.invr;
+  bool far = g.r > two;
+  T A = far ? A_far : A_near;
+  c.Bc = far ? B_far : B_near;
+
+  // Swan-Brady wall correction with h_hat = z_j / R_z.
+  T hh = zj * rcp(g.Rz);
+  T ez2 = g.ez2;
+  T invR = g.invR;
+  T invR2 = invR * invR;
+  T invR3 = invR2 * invR;
+  T invR5 = invR3 * invR2;
+  T hq = hh * (one - hh);
+  T a1 = one - T(5) * ez2;
+  T f1 = -(three * (one + two * hq * ez2) * invR + two * (one - three * ez2) * invR3
+           - two * a1 * invR5) * (one / three);
+  T f2 = -(three * (one - T(6) * hq * ez2) * invR - T(6) * a1 * invR3
+           + T(10) * (one - T(7) * ez2) * invR5) * (one / three);
+  T f3 = g.ez * (three * hh * (one - T(6) * (one - hh) * ez2) * invR - T(6) * a1 * invR3
+                 + T(10) * (two - T(7) * ez2) * invR5) * (two / three);
+  T f4 = g.ez * (three * hh * invR - T(10) * invR5) * (two / three);
+  T f5 = -(three * hh * hh * ez2 * invR + three * ez2 * invR3 + (two - T(15) * ez2) * invR5)
+         * (T(4) / three);
+  c.s = A + f1;
+  c.s5 = c.s + f5;
+  c.g2 = f2 * invR2;
+  c.g3 = f3 * invR;
+  c.g4 = f4 * invR;
+  return c;
+}
+
+template <typename T>
+RMW_HD Vec3<T> apply_uf(const Geom<T>& g, const CoefUF<T>& c, T vx, T vy, T vz)
+{
+  T p = g.dx * vx + g.dy * vy;
+  T dv = p + g.dz * vz;          // d.v
+  T Rv = p + g.Rz * vz;          // R.v
+  T ge = c.g2 * Rv + c.g3 * vz;  // coefficient of R
+  T q = c.Bc * dv;               // coefficient of d
+  T qe = q + ge;                 // x and y rows share d_x = R_x, d_y = R_y
+  Vec3<T> u;
+  u.x = c.s * vx + qe * g.dx;
+  u.y = c.s * vy + qe * g.dy;
+  u.z = c.s5 * vz + q * g.dz + ge * g.Rz + c.g4 * Rv;
+  return u;
+}
+
+// ---------------------------------------------------------------------------
+// Wall coefficients shared by UT and WF (mobility_numba.py:659-665 and
+// :1046-1052), for a given h_hat.  Returned pre-divided by powers of |R| so
+// that they multiply R rather than e.
+// ---------------------------------------------------------------------------
+template <typename T> struct CoefRot {
+  T c;    // RPY antisymmetric coefficient (1/r^3 far, (1 - 3r/8)/2 near)
+  T F1r;  // F1 / |R|
+  T F2r;  // F2 / |R|
+  T F3r2; // F3 / |R|^2
+  T F4;   // F4
+};
+
+template <typename T>
+RMW_HD CoefRot<T> coef_rot(const Geom<T>& g, T hnum)
+{
+  const T one = T(1), two = T(2);
+  CoefRot<T> c;
+  T invr3 = g.invr * g.invr * g.invr;
+  T c_near = T(0.5) * (one - T(0.375) * g.r);
+  c.c = (g.r >= two) ? invr3 : c_near;
+  T hh = hnum * rcp(g.Rz);
+  T invR = g.invR;
+  T invR2 = invR * invR;
+  T invR4 = invR2 * invR2;
+  T ez2 = g.ez2;
+  T F1 = invR2;
+  T F2 = (T(6) * hh * ez2 * invR2 + (one - T(10) * ez2) * invR4) * two;
+  T F3 = -g.ez * (T(3) * hh * invR2 - T(5) * invR4) * two;
+  T F4 = -g.ez * (hh * invR2 - invR4) * two;
+  c.F1r = F1 * invR;
+  c.F2r = F2 * invR;
+  c.F3r2 = F3 * invR2;
+  c.F4 = F4;
+  return c;
+}
+
+// WF v = c (v x d) - [F1 (v x e) + (F2 v_z + F3 (e.v)) (z x e) + F4 (v x z)]
+template <typename T>
+RMW_HD Vec3<T> apply_wf(const Geom<T>& g, const CoefRot<T>& c, T vx, T vy, T vz)
+{
+  T Rv = g.dx * vx + g.dy * vy + g.Rz * vz;
+  T h = c.F2r * vz + c.F3r2 * Rv;           // (F2 v_z + F3 e.v) / |R|
+  T zc = vx * g.dy - vy * g.dx;              // (v x d)_z = (v x R)_z
+  Vec3<T> w;
+  w.x = c.c * (vy * g.dz - vz * g.dy) - c.F1r * (vy * g.Rz - vz * g.dy) + h * g.dy - c.F4 * vy;
+  w.y = c.c * (vz * g.dx - vx * g.dz) - c.F1r * (vz * g.dx - vx * g.Rz) - h * g.dx + c.F4 * vx;
+  w.z = (c.c - c.F1r) * zc;
+  return w;
+}
+
+// UT v = c (v x d) - [F1 (e' x v) + (F2 z + F3 e') ((z x e').v) + F4 (z x v)],
+// e' = (-d_x, -d_y, R_z)/|R|, coefficients built with h_hat = z_i / R_z.
+template <typename T>
+RMW_HD Vec3<T> apply_ut(const Geom<T>& g, const CoefRot<T>& c, T vx, T vy, T vz)
+{
+  T q = g.dy * vx - g.dx * vy;               // |R| (z x e').v
+  T m3 = c.F3r2 * q;                          // F3 ((z x e').v) / |R|
+  Vec3<T> u;
+  u.x = c.c * (vy * g.dz - vz * g.dy) + c.F1r * (g.dy * vz + g.Rz * vy) + m3 * g.dx + c.F4 * vy;
+  u.y = c.c * (vz * g.dx - vx * g.dz) - c.F1r * (g.Rz * vx + g.dx * vz) + m3 * g.dy - c.F4 * vx;
+  u.z = (c.c - c.F1r - c.F2r) * q - m3 * g.Rz;
+  return u;
+}
+
+// ---------------------------------------------------------------------------
+// WT: rotation from torque.  mobility_numba.py:1256-1321
+// ---------------------------------------------------------------------------
+template <typename T> struct CoefWT {
+  T s;    // A + G1
+  T Bc;   // coefficient of (d.v) d
+  T G2r;  // G2 / |R|^2 -> (R.v) R
+  T G3r;  // G3 / |R|   -> (R.v) z
+  T G4r;  // G4 / |R|^2 -> ((z x R).v)(z x R)
+};
+
+template <typename T>
+RMW_HD CoefWT<T> coef_wt(const Geom<T>& g)
+{
+  const T one = T(1), two = T(2);
+  CoefWT<T> c;
+  T invr2 = g.invr * g.invr;
+  T invr3 = invr2 * g.invr;
+  T r3 = g.r2 * g.r;
+  T A_far = T(-0.5) * invr3;
+  T B_far = T(1.5) * invr2 * invr3;
+  T A_near = one - T(0.84375) * g.r + T(0.078125) * r3;
+  T B_near = T(0.28125) * g.invr - T(0.046875) * g.r;
+  bool far = g.r >= two;
+  T A = far ? A_far : A_near;
+  c.Bc = far ? B_far : B_near;
+  T invR = g.invR;
+  T invR2 = invR * invR;
+  T invR3 = invR2 * invR;
+  T G1 = (one - T(6) * g.ez2) * invR3 * T(0.5);
+  T G2 = T(-1.5) * invR3;
+  T G3 = T(3) * invR3 * g.ez;
+  T G4 = T(3) * invR3;
+  c.s = A + G1;
+  c.G2r = G2 * invR2;
+  c.G3r = G3 * invR;
+  c.G4r = G4 * invR2;
+  return c;
+}
+
+template <typename T>
+RMW_HD Vec3<T> apply_wt(const Geom<T>& g, const CoefWT<T>& c, T vx, T vy, T vz)
+{
+  T p = g.dx * vx + g.dy * vy;
+  T dv = p + g.dz * vz;
+  T Rv = p + g.Rz * vz;
+  T zt = g.dx * vy - g.dy * vx;      // (z x R).v
+  T q = c.Bc * dv;
+  T ge = c.G2r * Rv;
+  T gz = c.G4r * zt;
+  Vec3<T> w;
+  w.x = c.s * vx + (q + ge) * g.dx - gz * g.dy;
+  w.y = c.s * vy + (q + ge) * g.dy + gz * g.dx;
+  w.z = c.s * vz + q * g.dz + ge * g.Rz + c.G3r * Rv;
+  return w;
+}
+
+// ---------------------------------------------------------------------------
+// Self blocks (i == j, central box), mobility_numba.py:202-208 + 245-252,
+// :653-657, :1040-1044, :1291-1297.  h = z_i / a (effective height).
+// ---------------------------------------------------------------------------
+template <typename T>
+RMW_HD Vec3<T> self_uf(T h, T vx, T vy, T vz)
+{
+  T ih = T(1) / h;
+  T ih3 = ih * ih * ih;
+  T ih5 = ih3 * ih * ih;
+  T pxy = T(4) / T(3) - (T(9) * ih - T(2) * ih3 + ih5) / T(12);
+  T pz = T(4) / T(3) - (T(9) * ih - T(4) * ih3 + ih5) / T(6);
+  return Vec3<T>{pxy * vx, pxy * vy, pz * vz};
+}
+
+// UT self: M_xy = +h^-4/8, M_yx = -h^-4/8.
+template <typename T>
+RMW_HD Vec3<T> self_ut(T h, T vx, T vy, T vz)
+{
+  T ih = T(1) / h;
+  T c = T(0.125) * ih * ih * ih * ih;
+  return Vec3<T>{c * vy, -c * vx, T(0)};
+}
+
+// WF self: M_xy = -h^-4/8, M_yx = +h^-4/8.
+template <typename T>
+RMW_HD Vec3<T> self_wf(T h, T vx, T vy, T vz)
+{
+  T ih = T(1) / h;
+  T c = T(0.125) * ih * ih * ih * ih;
+  return Vec3<T>{-c * vy, c * vx, T(0)};
+}
+
+template <typename T>
+RMW_HD Vec3<T> self_wt(T h, T vx, T vy, T vz)
+{
+  T ih = T(1) / h;
+  T ih3 = ih * ih * ih;
+  T pxy = T(1) - T(0.3125) * ih3;
+  T pz = T(1) - T(0.125) * ih3;
+  return Vec3<T>{pxy * vx, pxy * vy, pz * vz};
+}
+
+}  // namespace rmw

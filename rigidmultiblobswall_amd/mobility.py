@@ -1,0 +1,115 @@
+"""Drop-in wall-RPY mobility products with the reference signatures.
+
+Every function has the calling convention of its reference counterpart in
+mobility/mobility.py -- ``f(r_vectors, vector, eta, a, *args, **kwargs)``
+with ``periodic_length=np.array([Lx, Ly, Lz])`` (zeros = unbounded) -- and
+returns a new (3N,) float64 numpy array, so it can be bound wherever the
+reference binds ``single_wall_mobility_*_numba`` / ``*_pycuda``.
+
+Extra keyword arguments understood here:
+  precision: 'fp32' (FP32 pair terms, FP64 per-target sums; default),
+             'fp32_kahan' (compensated FP32 sums) or 'fp64'.
+  tuning:    {'chunk_len': int, 'max_wgs': int} launch shape override.
+
+The computation always runs on the GPU through librmw_hip; inputs given as
+numpy arrays are copied to the device and results copied back.  Inputs
+given as torch device tensors stay on the device (the copy back to the
+host is the only transfer).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import wall_rpy
+from ._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
+
+DEFAULT_PRECISION = "fp32"
+
+
+def shift_heights(r_vectors, blob_radius, *args, **kwargs):
+    """z_eff = max(z, a); mobility/mobility.py:52-64 (host helper)."""
+    r_effective = np.copy(r_vectors)
+    r_effective[r_vectors[:, 2] <= blob_radius, 2] = blob_radius
+    return r_effective
+
+
+def damping_matrix_B(r_vectors, blob_radius, *args, **kwargs):
+    """Diagonal B_ii = z_i/a below the wall-overlap height, else 1, and the
+    overlap flag; mobility/mobility.py:67-84 (host helper, returned as the
+    diagonal vector of length 3N rather than a scipy dia_matrix)."""
+    r = np.asarray(r_vectors).reshape(-1, 3)
+    B = np.ones(r.size)
+    low = r[:, 2] < blob_radius
+    B.reshape(-1, 3)[low] = (r[low, 2] / blob_radius)[:, None]
+    return B, bool(low.any())
+
+
+def _opts(kwargs):
+    return (kwargs.get("periodic_length", np.array([0.0, 0.0, 0.0])),
+            kwargs.get("precision", DEFAULT_PRECISION), kwargs.get("tuning"))
+
+
+def _host(t):
+    return t.reshape(-1).cpu().numpy()
+
+
+def single_wall_mobility_trans_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
+    """u = B M_UF(z_eff) B f; drop-in for mobility.py:1132-1163."""
+    L, prec, tun = _opts(kwargs)
+    u, _ = wall_rpy.apply(RMW_UF, r_vectors, force=force, eta=eta, a=a, periodic_length=L,
+                          precision=prec, tuning=tun)
+    return _host(u)
+
+
+def single_wall_mobility_trans_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
+    """u = B M_UT(z_eff) B t; drop-in for mobility.py:1213-1232."""
+    L, prec, tun = _opts(kwargs)
+    u, _ = wall_rpy.apply(RMW_UT, r_vectors, torque=torque, eta=eta, a=a, periodic_length=L,
+                          precision=prec, tuning=tun)
+    return _host(u)
+
+
+def single_wall_mobility_rot_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
+    """w = B M_WF(z_eff) B f; drop-in for mobility.py:1273-1298."""
+    L, prec, tun = _opts(kwargs)
+    _, w = wall_rpy.apply(RMW_WF, r_vectors, force=force, eta=eta, a=a, periodic_length=L,
+                          precision=prec, tuning=tun)
+    return _host(w)
+
+
+def single_wall_mobility_rot_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
+    """w = B M_WT(z_eff) B t; drop-in for mobility.py:1316-1342."""
+    L, prec, tun = _opts(kwargs)
+    _, w = wall_rpy.apply(RMW_WT, r_vectors, torque=torque, eta=eta, a=a, periodic_length=L,
+                          precision=prec, tuning=tun)
+    return _host(w)
+
+
+def single_wall_mobility_trans_times_force_torque_hip(r_vectors, force, torque, eta, a, *args, **kwargs):
+    """u = M_UF f + M_UT t in one pass; drop-in for mobility.py:384-410."""
+    L, prec, tun = _opts(kwargs)
+    u, _ = wall_rpy.apply(RMW_UF | RMW_UT, r_vectors, force=force, torque=torque, eta=eta, a=a,
+                          periodic_length=L, precision=prec, tuning=tun)
+    return _host(u)
+
+
+def single_wall_grand_mobility_hip(r_vectors, force_torque, eta, a, *args, **kwargs):
+    """[UF UT; WF WT] [f; t] -> [u; w] in one fused pass; drop-in for the
+    rollers' grand_mobility_matrix closure
+    (quaternion_integrator/quaternion_integrator_rollers.py:1114-1122)."""
+    L, prec, tun = _opts(kwargs)
+    ft = np.asarray(force_torque, dtype=np.float64).reshape(-1) if not torch.is_tensor(force_torque) \
+        else force_torque.reshape(-1)
+    half = ft.shape[0] // 2
+    u, w = wall_rpy.apply(RMW_UF | RMW_UT | RMW_WF | RMW_WT, r_vectors, force=ft[:half], torque=ft[half:],
+                          eta=eta, a=a, periodic_length=L, precision=prec, tuning=tun)
+    return np.concatenate([_host(u), _host(w)])
+
+
+__all__ = [
+    "shift_heights", "damping_matrix_B",
+    "single_wall_mobility_trans_times_force_hip", "single_wall_mobility_trans_times_torque_hip",
+    "single_wall_mobility_rot_times_force_hip", "single_wall_mobility_rot_times_torque_hip",
+    "single_wall_mobility_trans_times_force_torque_hip", "single_wall_grand_mobility_hip",
+]

@@ -1,0 +1,164 @@
+"""Device-level wall-RPY products (torch ROCm tensors in, torch tensors out).
+
+Thin host layer over ``rmw_wall_rpy_apply`` / ``rmw_wall_rpy_ragged_apply``:
+it validates shapes, owns a per-device workspace that only grows, and
+passes raw device pointers and the current HIP stream.  PyTorch is used
+only to own device memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
+
+_ws_lock = threading.Lock()
+_workspaces: dict = {}
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("librmw_hip needs a ROCm GPU; none is visible (there is no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def workspace(nbytes: int, device: Optional[torch.device] = None, slot: str = "default") -> torch.Tensor:
+    """A persistent uint8 device buffer of at least nbytes (grown, never shrunk)."""
+    device = device or _device()
+    key = (device.index, slot)
+    with _ws_lock:
+        buf = _workspaces.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+            _workspaces[key] = buf
+        return buf
+
+
+def _as_device_f64(x, n: int, k: Optional[int], device, name: str) -> torch.Tensor:
+    t = torch.as_tensor(x)
+    if t.device != device or t.dtype != torch.float64:
+        t = t.to(device=device, dtype=torch.float64)
+    t = t.contiguous()
+    if k is None:
+        if t.numel() != 3 * n:
+            raise ValueError(f"{name} must have 3*N = {3 * n} entries, got {t.numel()}")
+    else:
+        if t.numel() != 3 * n * k:
+            raise ValueError(f"{name} must have k*3*N = {3 * n * k} entries, got {t.numel()}")
+    return t
+
+
+def _L_array(periodic_length) -> ctypes.Array:
+    L = np.zeros(3) if periodic_length is None else np.asarray(periodic_length, dtype=np.float64).reshape(-1)
+    if L.size != 3:
+        raise ValueError("periodic_length must have 3 entries")
+    return (ctypes.c_double * 3)(*[float(v) for v in L])
+
+
+def _stream_ptr(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: float = 1.0,
+          periodic_length=None, precision="fp32", k: int = 1, tuning=None,
+          out_velocity: Optional[torch.Tensor] = None, out_omega: Optional[torch.Tensor] = None):
+    """Evaluate the blocks of ``mask`` on one suspension.
+
+    r_vectors: (N, 3); force / torque: (k, N, 3) (or (N, 3) for k = 1).
+    Returns (velocity, omega) as float64 device tensors of shape (k, N, 3)
+    (squeezed to (N, 3) when k == 1), None for outputs the mask does not write.
+    """
+    lib = _lib.load()
+    device = _device()
+    r = torch.as_tensor(r_vectors)
+    n = r.numel() // 3
+    r = _as_device_f64(r, n, None, device, "r_vectors")
+    has_u = bool(mask & (RMW_UF | RMW_UT))
+    has_w = bool(mask & (RMW_WF | RMW_WT))
+    f = _as_device_f64(force, n, k, device, "force") if mask & (RMW_UF | RMW_WF) else None
+    t = _as_device_f64(torque, n, k, device, "torque") if mask & (RMW_UT | RMW_WT) else None
+    u = out_velocity if out_velocity is not None else (
+        torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_u else None)
+    w = out_omega if out_omega is not None else (
+        torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None)
+    prec = _lib.precision_code(precision)
+    tp = _lib.tuning_ptr(tuning)
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value, device)
+    _lib.check(lib.rmw_wall_rpy_apply(
+        mask, prec, r.data_ptr(), f.data_ptr() if f is not None else None,
+        t.data_ptr() if t is not None else None, u.data_ptr() if u is not None else None,
+        w.data_ptr() if w is not None else None, n, k, float(eta), float(a), _L_array(periodic_length),
+        tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
+    if k == 1:
+        u = u.view(n, 3) if u is not None else None
+        w = w.view(n, 3) if w is not None else None
+    return u, w
+
+
+def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torque=None, eta: float = 1.0,
+                 a: float = 1.0, periodic_length=None, precision="fp32", k: int = 1, tuning=None):
+    """Independent suspensions stored back to back; system s owns blobs
+    [offsets[s], offsets[s+1]).  One persistent launch for all systems."""
+    lib = _lib.load()
+    device = _device()
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+    nsys = off.size - 1
+    n = int(off[-1])
+    r = _as_device_f64(r_vectors, n, None, device, "r_vectors")
+    has_u = bool(mask & (RMW_UF | RMW_UT))
+    has_w = bool(mask & (RMW_WF | RMW_WT))
+    f = _as_device_f64(force, n, k, device, "force") if mask & (RMW_UF | RMW_WF) else None
+    t = _as_device_f64(torque, n, k, device, "torque") if mask & (RMW_UT | RMW_WT) else None
+    u = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_u else None
+    w = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None
+    prec = _lib.precision_code(precision)
+    tp = _lib.tuning_ptr(tuning)
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, nsys, k, tp, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value, device, slot="ragged")
+    _lib.check(lib.rmw_wall_rpy_ragged_apply(
+        mask, prec, r.data_ptr(), f.data_ptr() if f is not None else None,
+        t.data_ptr() if t is not None else None, u.data_ptr() if u is not None else None,
+        w.data_ptr() if w is not None else None, offp, nsys, k, float(eta), float(a),
+        _L_array(periodic_length), tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
+    if k == 1:
+        u = u.view(n, 3) if u is not None else None
+        w = w.view(n, 3) if w is not None else None
+    return u, w
+
+
+def device_info() -> dict:
+    lib = _lib.load()
+    _device()
+    cu, clk, wf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    name = ctypes.create_string_buffer(64)
+    _lib.check(lib.rmw_device_info(ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(wf), name, 64))
+    return {"cu_count": cu.value, "clock_khz": clk.value, "wavefront": wf.value, "arch": name.value.decode()}
+
+
+class PairKernelTimer:
+    """Context manager timing every pair-kernel launch with HIP events."""
+
+    def __enter__(self):
+        lib = _lib.load()
+        _lib.check(lib.rmw_profile_collect(None, None, None))
+        _lib.check(lib.rmw_profile_enable(1))
+        return self
+
+    def __exit__(self, *exc):
+        lib = _lib.load()
+        _lib.check(lib.rmw_profile_enable(0))
+        n = ctypes.c_int64()
+        tot = ctypes.c_double()
+        mean = ctypes.c_double()
+        _lib.check(lib.rmw_profile_collect(ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mean)))
+        self.launches, self.total_ms, self.mean_ms = n.value, tot.value, mean.value
+        return False

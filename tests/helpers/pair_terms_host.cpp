@@ -1,0 +1,63 @@
+// Test helper: compiles the product's pair-term header for the host so the
+// decomposed block forms can be compared with the oracle on the CPU.
+// Exposes the dense 3x3 block (kernel level, units of a, normalised like
+// the reference: 1/(8 pi eta a^p)) obtained by applying the block to the
+// three unit vectors.
+#include <cmath>
+#include <cstdint>
+#include "../../rigidmultiblobswall_amd/csrc/rmw_pair_terms.h"
+
+using namespace rmw;
+
+template <typename T>
+static Vec3<T> apply_block(int block, const Geom<T>& g, T zi, T zj, T vx, T vy, T vz)
+{
+  switch (block) {
+    case 0: return apply_uf(g, coef_uf(g, zj), vx, vy, vz);
+    case 1: return apply_ut(g, coef_rot(g, zi), vx, vy, vz);
+    case 2: return apply_wf(g, coef_rot(g, zj), vx, vy, vz);
+    default: return apply_wt(g, coef_wt(g), vx, vy, vz);
+  }
+}
+
+template <typename T>
+static Vec3<T> apply_self(int block, T h, T vx, T vy, T vz)
+{
+  switch (block) {
+    case 0: return self_uf(h, vx, vy, vz);
+    case 1: return self_ut(h, vx, vy, vz);
+    case 2: return self_wf(h, vx, vy, vz);
+    default: return self_wt(h, vx, vy, vz);
+  }
+}
+
+template <typename T>
+static int pair_matrix(int block, const double* ri, const double* rj, int self, double eta, double a, double* m9)
+{
+  if (block < 0 || block > 3) return -1;
+  const int p = block == 0 ? 1 : (block == 3 ? 3 : 2);
+  const double norm = 1.0 / (8.0 * M_PI * eta * std::pow(a, p));
+  T dx = T((ri[0] - rj[0]) / a), dy = T((ri[1] - rj[1]) / a), dz = T((ri[2] - rj[2]) / a);
+  T zi = T(ri[2] / a), zj = T(rj[2] / a);
+  Geom<T> g = make_geom(dx, dy, dz, zi, zj);
+  for (int c = 0; c < 3; ++c) {
+    T v[3] = {T(0), T(0), T(0)};
+    v[c] = T(1);
+    Vec3<T> u = self ? apply_self<T>(block, zi, v[0], v[1], v[2])
+                     : apply_block<T>(block, g, zi, zj, v[0], v[1], v[2]);
+    m9[0 * 3 + c] = double(u.x) * norm;
+    m9[1 * 3 + c] = double(u.y) * norm;
+    m9[2 * 3 + c] = double(u.z) * norm;
+  }
+  return 0;
+}
+
+extern "C" int pt_pair_matrix_f64(int block, const double* ri, const double* rj, int self, double eta, double a, double* m9)
+{
+  return pair_matrix<double>(block, ri, rj, self, eta, a, m9);
+}
+
+extern "C" int pt_pair_matrix_f32(int block, const double* ri, const double* rj, int self, double eta, double a, double* m9)
+{
+  return pair_matrix<float>(block, ri, rj, self, eta, a, m9);
+}
