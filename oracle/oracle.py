@@ -278,6 +278,38 @@ def image_singular_stokeslet(r_vectors, eta: float, a: float) -> np.ndarray:
     return M
 
 
+def blake_tensor(x, y, eta: float = 1.0) -> np.ndarray:
+    """Velocity at x due to a unit point force at y above a no-slip wall at
+    z = 0: the Blake (1971) image system, written from the mathematics (not
+    from the reference):
+
+        8 pi eta G = S(r) - S(R) + 2 h P_jk d/dR_k [h R_i/|R|^3 - (d_i3/|R| + R_i R_3/|R|^3)]
+
+    with S(v) = I/|v| + v v^T/|v|^3, r = x - y, R = x - y* (y* the mirror
+    image of y), h = y_3 and P = diag(1, 1, -1).  It is pinned by its
+    defining properties (zero velocity on the wall, reciprocity
+    G(x,y) = G(y,x)^T, incompressibility) in the tests.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    h = y[2]
+    r = x - y
+    R = x - np.array([y[0], y[1], -h])
+    rn, Rn = np.linalg.norm(r), np.linalg.norm(R)
+    I = np.eye(3)
+    G = I / rn + np.outer(r, r) / rn ** 3 - I / Rn - np.outer(R, R) / Rn ** 3
+    P = np.array([1.0, 1.0, -1.0])
+    d3 = np.array([0.0, 0.0, 1.0])
+    RR = np.outer(R, R)
+    # dPhi[i, k] = d/dR_k [h R_i/|R|^3 - d_i3/|R| - R_i R_3/|R|^3]
+    dPhi = (h * (I / Rn ** 3 - 3.0 * RR / Rn ** 5)
+            + np.outer(d3, R) / Rn ** 3
+            - (I * R[2] + np.outer(R, d3)) / Rn ** 3
+            + 3.0 * R[2] * RR / Rn ** 5)
+    G += 2.0 * h * dPhi * P[None, :]
+    return G / (8.0 * np.pi * eta)
+
+
 def self_mobility_with_rotation(location, eta: float, a: float) -> np.ndarray:
     """Closed-form 6x6 wall self mobility, mobility.py:739-772.  NB: its
     coupling block uses the opposite (left-handed) sign convention to the
