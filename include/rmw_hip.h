@@ -40,6 +40,8 @@ extern "C" {
 #define RMW_PREC_F32_F64ACC 0 /* FP32 pair terms, FP64 per-target sums     */
 #define RMW_PREC_F32_KAHAN 1  /* FP32 pair terms, compensated FP32 sums     */
 #define RMW_PREC_F64 2        /* FP64 throughout                           */
+#define RMW_PREC_F32_BLOCKED 3 /* FP32 pair terms, FP32 sub-sums of <= 16
+                                  sources folded into FP64 per-target sums */
 
 /* Launch shape; pass NULL for the built-in heuristic. */
 typedef struct rmw_tuning {

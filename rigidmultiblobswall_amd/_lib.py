@@ -15,12 +15,13 @@ LIB_PATH = os.path.join(HERE, "librmw_hip.so")
 
 RMW_OK = 0
 RMW_UF, RMW_UT, RMW_WF, RMW_WT = 1, 2, 4, 8
-RMW_PREC_F32_F64ACC, RMW_PREC_F32_KAHAN, RMW_PREC_F64 = 0, 1, 2
+RMW_PREC_F32_F64ACC, RMW_PREC_F32_KAHAN, RMW_PREC_F64, RMW_PREC_F32_BLOCKED = 0, 1, 2, 3
 
 PRECISIONS = {
     "fp32": RMW_PREC_F32_F64ACC,
     "fp32_f64acc": RMW_PREC_F32_F64ACC,
     "fp32_kahan": RMW_PREC_F32_KAHAN,
+    "fp32_blocked": RMW_PREC_F32_BLOCKED,
     "fp64": RMW_PREC_F64,
 }
 

@@ -67,6 +67,10 @@ RMW_HD float rcp(float x)
 }
 RMW_HD double rcp(double x) { return 1.0 / x; }
 
+// Fused multiply-add a*b + c (single rounding on the device and the host).
+RMW_HD float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+RMW_HD double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // ---------------------------------------------------------------------------
 // Shared geometry of one (target, source) pair.
 // ---------------------------------------------------------------------------
@@ -108,43 +112,54 @@ template <typename T> struct CoefUF {
 };
 
 template <typename T>
-RMW_HD CoefUF<T> coef_uf(const Geom<T>& g, T zj)
+RMW_HD CoefUF<T> coef_uf(const Geom<T>& g, T zi, T zj)
 {
-  const T one = T(1), two = T(2), three = T(3);
+  const T one = T(1), two = T(2);
   CoefUF<T> c;
   // RPY, far (r > 2) or overlapping (r <= 2) branch as selects.
   T invr2 = g.invr * g.invr;
-  T A_far = (one + (two / three) * invr2) * g.invr;
-  T B_far = (one - two * invr2) * invr2 * g.invr;
-  T A_near = (T(4) / three) * (one - T(0.28125) * g.r);
-  T B_near = (T(4) / three) * T(0.09375) * g.invr;
+  T A_far = (one + T(2.0 / 3.0) * invr2) * g.invr;
+  T B_far = (one - two * invr2) * (invr2 * g.invr);
+  T A_near = T(4.0 / 3.0) - T(4.0 / 3.0 * 0.28125) * g.r;
+  T B_near = T(4.0 / 3.0 * 0.09375) * g.invr;
   bool far = g.r > two;
   T A = far ? A_far : A_near;
   c.Bc = far ? B_far : B_near;
 
-  // Swan-Brady wall correction with h_hat = z_j / R_z.
-  T hh = zj * rcp(g.Rz);
-  T ez2 = g.ez2;
-  T invR = g.invR;
-  T invR2 = invR * invR;
-  T invR3 = invR2 * invR;
-  T invR5 = invR3 * invR2;
-  T hq = hh * (one - hh);
-  T a1 = one - T(5) * ez2;
-  T f1 = -(three * (one + two * hq * ez2) * invR + two * (one - three * ez2) * invR3
-           - two * a1 * invR5) * (one / three);
-  T f2 = -(three * (one - T(6) * hq * ez2) * invR - T(6) * a1 * invR3
-           + T(10) * (one - T(7) * ez2) * invR5) * (one / three);
-  T f3 = g.ez * (three * hh * (one - T(6) * (one - hh) * ez2) * invR - T(6) * a1 * invR3
-                 + T(10) * (two - T(7) * ez2) * invR5) * (two / three);
-  T f4 = g.ez * (three * hh * invR - T(10) * invR5) * (two / three);
-  T f5 = -(three * hh * hh * ez2 * invR + three * ez2 * invR3 + (two - T(15) * ez2) * invR5)
-         * (T(4) / three);
+  // Swan-Brady wall correction (mobility_numba.py:254-264) regrouped as
+  // polynomials in w = 1/|R|^2 with E = e_z^2 = R_z^2 w.  With
+  // h_hat = z_j/R_z the h_hat factors combine into division-free monomials:
+  //   h_hat (1-h_hat) e_z^2 = z_i z_j w,  h_hat e_z / |R| = z_j w,
+  //   (1-h_hat) e_z^2 = z_i R_z w,        h_hat^2 e_z^2 = z_j^2 w,
+  // giving
+  //   f1 = -q [1 + w (2 zz + 2/3 - 2E) + w^2 (10E/3 - 2/3)]
+  //   f2 = -q [1 + w (-6 zz - 2 + 10E) + w^2 (10/3 - 70E/3)]
+  //   f3 = w [2 z_j + R_z w (-12 zz - 4 + 20E + w (40/3 - 140E/3))]
+  //   f4 = w [2 z_j - (20/3) R_z w^2]
+  //   f5 = -q w [4 z_j^2 + 4E + w (8/3 - 20E)]
+  // with q = 1/|R| and zz = z_i z_j.
+  T q = g.invR;
+  T w = q * q;
+  T E = g.ez2;
+  T zz = zi * zj;
+  T t1 = fma_(T(2), zz, fma_(T(-2), E, T(2.0 / 3.0)));
+  T t2 = fma_(T(10.0 / 3.0), E, T(-2.0 / 3.0));
+  T f1 = -q * fma_(w, fma_(w, t2, t1), one);
+  T t3 = fma_(T(-6), zz, fma_(T(10), E, T(-2)));
+  T t4 = fma_(T(-70.0 / 3.0), E, T(10.0 / 3.0));
+  T qw = q * w;
+  c.g2 = -qw * fma_(w, fma_(w, t4, t3), one);              // f2 w
+  T Rzw = g.Rz * w;
+  T t5 = fma_(T(-12), zz, fma_(T(20), E, T(-4)));
+  T t6 = fma_(T(-140.0 / 3.0), E, T(40.0 / 3.0));
+  T two_zj = zj + zj;
+  c.g3 = qw * fma_(Rzw, fma_(w, t6, t5), two_zj);          // f3 q
+  c.g4 = qw * fma_(T(-20.0 / 3.0) * Rzw, w, two_zj);       // f4 q
+  T t7 = fma_(T(-20), E, T(8.0 / 3.0));
+  T t8 = fma_(T(4) * zj, zj, T(4) * E);
+  T f5 = -qw * fma_(w, t7, t8);
   c.s = A + f1;
   c.s5 = c.s + f5;
-  c.g2 = f2 * invR2;
-  c.g3 = f3 * invR;
-  c.g4 = f4 * invR;
   return c;
 }
 
@@ -185,15 +200,16 @@ RMW_HD CoefRot<T> coef_rot(const Geom<T>& g, T hnum)
   T invr3 = g.invr * g.invr * g.invr;
   T c_near = T(0.5) * (one - T(0.375) * g.r);
   c.c = (g.r >= two) ? invr3 : c_near;
-  T hh = hnum * rcp(g.Rz);
+  // h_hat = h/R_z enters only as h_hat e_z^2 = h R_z/|R|^2 and
+  // h_hat e_z = h/|R| (no division needed).
   T invR = g.invR;
   T invR2 = invR * invR;
   T invR4 = invR2 * invR2;
   T ez2 = g.ez2;
   T F1 = invR2;
-  T F2 = (T(6) * hh * ez2 * invR2 + (one - T(10) * ez2) * invR4) * two;
-  T F3 = -g.ez * (T(3) * hh * invR2 - T(5) * invR4) * two;
-  T F4 = -g.ez * (hh * invR2 - invR4) * two;
+  T F2 = (T(6) * hnum * g.Rz + one - T(10) * ez2) * invR4 * two;
+  T F3 = -(T(3) * hnum * invR * invR2 - T(5) * g.ez * invR4) * two;
+  T F4 = -(hnum * invR * invR2 - g.ez * invR4) * two;
   c.F1r = F1 * invR;
   c.F2r = F2 * invR;
   c.F3r2 = F3 * invR2;

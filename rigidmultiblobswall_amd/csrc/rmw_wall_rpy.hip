@@ -122,14 +122,22 @@ __device__ inline SysInfo find_sys_item(const PairParams& p, int64_t item)
 
 // ---------------------------------------------------------------------------
 // 1. extent: max |x|, |y|, |z_eff| over all blobs, in units of a.
+//    Non-negative doubles order like their bit patterns, so the maximum is
+//    an integer atomicMax on the 64-bit pattern; a one-thread fill kernel
+//    clears the word first (a kernel rather than a memset so that the
+//    sequence can be captured in a graph).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void extent_kernel(const double* __restrict__ r, int64_t ntot,
-                                                      double inva, double a, double Lmax,
-                                                      double* __restrict__ extent)
+__global__ void extent_clear_kernel(unsigned long long* word, double Lmax)
 {
-  __shared__ double red[1024];
+  *word = (unsigned long long)__double_as_longlong(Lmax > 0.0 ? Lmax : 0.0);
+}
+
+__global__ __launch_bounds__(256) void extent_kernel(const double* __restrict__ r, int64_t ntot, double inva,
+                                                     double a, unsigned long long* __restrict__ word)
+{
+  __shared__ double red[256];
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < ntot; i += blockDim.x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntot; i += (int64_t)gridDim.x * blockDim.x) {
     double z = r[3 * i + 2];
     double ze = z <= a ? a : z;
     m = fmax(m, fabs(r[3 * i]));
@@ -142,7 +150,7 @@ __global__ __launch_bounds__(1024) void extent_kernel(const double* __restrict__
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) extent[0] = fmax(red[0], Lmax);
+  if (threadIdx.x == 0) atomicMax(word, (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // ---------------------------------------------------------------------------
@@ -222,7 +230,19 @@ template <int ACC> struct Acc;
 template <> struct Acc<RMW_PREC_F32_F64ACC> {
   double v = 0.0;
   __device__ __forceinline__ void add(float x) { v += (double)x; }
+  __device__ __forceinline__ void flush() {}
   __device__ __forceinline__ double get() const { return v; }
+};
+template <> struct Acc<RMW_PREC_F32_BLOCKED> {
+  double v = 0.0;
+  float p = 0.0f;
+  __device__ __forceinline__ void add(float x) { p += x; }
+  __device__ __forceinline__ void flush()
+  {
+    v += (double)p;
+    p = 0.0f;
+  }
+  __device__ __forceinline__ double get() const { return v + (double)p; }
 };
 template <> struct Acc<RMW_PREC_F32_KAHAN> {
   float s = 0.0f, c = 0.0f;
@@ -233,13 +253,29 @@ template <> struct Acc<RMW_PREC_F32_KAHAN> {
     c = (t - s) - y;
     s = t;
   }
+  __device__ __forceinline__ void flush() {}
   __device__ __forceinline__ double get() const { return (double)s - (double)c; }
 };
 template <> struct Acc<RMW_PREC_F64> {
   double v = 0.0;
   __device__ __forceinline__ void add(double x) { v += x; }
+  __device__ __forceinline__ void flush() {}
   __device__ __forceinline__ double get() const { return v; }
 };
+
+constexpr int kSub = 16;  // sources per FP32 sub-sum in the blocked mode
+
+template <int K, int ACC>
+__device__ __forceinline__ void flush_all(Acc<ACC> (&u)[K][3], Acc<ACC> (&w)[K][3])
+{
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[k][c].flush();
+      w[k][c].flush();
+    }
+}
 
 template <int MASK> struct BlockTraits {
   static constexpr bool HU = (MASK & (kUF | kUT)) != 0;  // writes velocity
@@ -266,7 +302,7 @@ __device__ __forceinline__ void pair_eval(T dx, T dy, T dz, T zi, T zj,
   CoefUF<T> cuf;
   CoefRot<T> cut, cwf;
   CoefWT<T> cwt;
-  if constexpr ((MASK & kUF) != 0) cuf = coef_uf(g, zj);
+  if constexpr ((MASK & kUF) != 0) cuf = coef_uf(g, zi, zj);
   if constexpr ((MASK & kUT) != 0) cut = coef_rot(g, zi);
   if constexpr ((MASK & kWF) != 0) cwf = coef_rot(g, zj);
   if constexpr ((MASK & kWT) != 0) cwt = coef_wt(g);
@@ -369,25 +405,30 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
       const bool diag = (s0 < t0 + kBlock) && (t0 < s0 + cnt);
       const int self_j = (int)(il - s0);  // local source index of the target itself
       if (!p.periodic) {
+        auto eval = [&](int j, bool masked) {
+          const float4 sh = s_hi[j], sl = s_lo[j];
+          float dx = (ph.x - sh.x) + (pl.x - sl.x);
+          float dy = (ph.y - sh.y) + (pl.y - sl.y);
+          float dz = (ph.z - sh.z) + (pl.z - sl.z);
+          pair_eval<MASK, K, ACC, float>(dx, dy, dz, ph.w, sh.w, s_f, s_t, j, masked && j == self_j, u, w);
+        };
+        int j = 0;
         if (diag) {
+          for (; j + kSub <= cnt; j += kSub) {
 #pragma unroll 2
-          for (int j = 0; j < cnt; ++j) {
-            const float4 sh = s_hi[j], sl = s_lo[j];
-            float dx = (ph.x - sh.x) + (pl.x - sl.x);
-            float dy = (ph.y - sh.y) + (pl.y - sl.y);
-            float dz = (ph.z - sh.z) + (pl.z - sl.z);
-            pair_eval<MASK, K, ACC, float>(dx, dy, dz, ph.w, sh.w, s_f, s_t, j, j == self_j, u, w);
+            for (int jj = 0; jj < kSub; ++jj) eval(j + jj, true);
+            flush_all<K, ACC>(u, w);
           }
+          for (; j < cnt; ++j) eval(j, true);
         } else {
+          for (; j + kSub <= cnt; j += kSub) {
 #pragma unroll 2
-          for (int j = 0; j < cnt; ++j) {
-            const float4 sh = s_hi[j], sl = s_lo[j];
-            float dx = (ph.x - sh.x) + (pl.x - sl.x);
-            float dy = (ph.y - sh.y) + (pl.y - sl.y);
-            float dz = (ph.z - sh.z) + (pl.z - sl.z);
-            pair_eval<MASK, K, ACC, float>(dx, dy, dz, ph.w, sh.w, s_f, s_t, j, false, u, w);
+            for (int jj = 0; jj < kSub; ++jj) eval(j + jj, false);
+            flush_all<K, ACC>(u, w);
           }
+          for (; j < cnt; ++j) eval(j, false);
         }
+        flush_all<K, ACC>(u, w);
       } else {
         for (int j = 0; j < cnt; ++j) {
           const float4 sh = s_hi[j], sl = s_lo[j];
@@ -408,6 +449,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
                 pair_eval<MASK, K, ACC, float>(dx, dy, dz, ph.w, sh.w, s_f, s_t, j,
                                                diag && central && (j == self_j), u, w);
               }
+          flush_all<K, ACC>(u, w);
         }
       }
     }
@@ -539,26 +581,38 @@ __device__ inline SysInfo find_sys_blob(const ReduceParams& p, int64_t gi)
 
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceParams p)
 {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= p.ntot * p.K) return;
-  const int k = (int)(tid / p.ntot);
-  const int64_t gi = tid - (int64_t)k * p.ntot;
-  const SysInfo S = find_sys_blob(p, gi);
-  const int64_t il = gi - S.b0;
+  // one thread per (rhs k, blob gi, output component c in [0, OC))
   const bool HU = (p.mask & (kUF | kUT)) != 0;
   const bool HW = (p.mask & (kWF | kWT)) != 0;
   const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
-  double su[3] = {0.0, 0.0, 0.0}, sw[3] = {0.0, 0.0, 0.0};
-  for (int64_t c = 0; c < S.n_chunks; ++c) {
-    const double* P = p.partial + S.part0 + ((c * p.K + k) * S.n + il) * OC;
-    int o = 0;
-    if (HU) { su[0] += P[0]; su[1] += P[1]; su[2] += P[2]; o = 3; }
-    if (HW) { sw[0] += P[o]; sw[1] += P[o + 1]; sw[2] += P[o + 2]; }
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_k = p.ntot * OC;
+  if (tid >= per_k * p.K) return;
+  const int k = (int)(tid / per_k);
+  const int64_t rem = tid - (int64_t)k * per_k;
+  const int64_t gi = rem / OC;
+  const int c = (int)(rem - gi * OC);
+  const SysInfo S = find_sys_blob(p, gi);
+  const int64_t il = gi - S.b0;
+  const double* P = p.partial + S.part0 + ((int64_t)k * S.n + il) * OC + c;
+  const int64_t cstride = (int64_t)p.K * S.n * OC;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t ch = 0;
+  for (; ch + 4 <= S.n_chunks; ch += 4) {
+    s0 += P[(ch + 0) * cstride];
+    s1 += P[(ch + 1) * cstride];
+    s2 += P[(ch + 2) * cstride];
+    s3 += P[(ch + 3) * cstride];
   }
+  for (; ch < S.n_chunks; ++ch) s0 += P[ch * cstride];
+  double sum = (s0 + s1) + (s2 + s3);
+
   const double z = p.r[3 * gi + 2];
   const double ze = z <= p.a ? p.a : z;
   const double B = z < p.a ? z * p.inva : 1.0;
   const double h = ze * p.inva;
+  const bool is_w = HW && (!HU || c >= 3);
+  const int comp = c % 3;
   double fv[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
   if (p.f) {
     const double* v = p.f + k * p.rhs_stride + 3 * gi;
@@ -568,18 +622,15 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceParams p)
     const double* v = p.t + k * p.rhs_stride + 3 * gi;
     tv[0] = p.c2 * B * v[0]; tv[1] = p.c2 * B * v[1]; tv[2] = p.c2 * B * v[2];
   }
-  if (p.mask & kUF) { Vec3<double> s = self_uf(h, fv[0], fv[1], fv[2]); su[0] += s.x; su[1] += s.y; su[2] += s.z; }
-  if (p.mask & kUT) { Vec3<double> s = self_ut(h, tv[0], tv[1], tv[2]); su[0] += s.x; su[1] += s.y; su[2] += s.z; }
-  if (p.mask & kWF) { Vec3<double> s = self_wf(h, fv[0], fv[1], fv[2]); sw[0] += s.x; sw[1] += s.y; sw[2] += s.z; }
-  if (p.mask & kWT) { Vec3<double> s = self_wt(h, tv[0], tv[1], tv[2]); sw[0] += s.x; sw[1] += s.y; sw[2] += s.z; }
-  if (HU) {
-    double* o = p.vel + k * p.rhs_stride + 3 * gi;
-    o[0] = B * su[0]; o[1] = B * su[1]; o[2] = B * su[2];
-  }
-  if (HW) {
-    double* o = p.omg + k * p.rhs_stride + 3 * gi;
-    const double s = B * p.inva;
-    o[0] = s * sw[0]; o[1] = s * sw[1]; o[2] = s * sw[2];
+  double self = 0.0;
+  if (!is_w) {
+    if (p.mask & kUF) { Vec3<double> sv = self_uf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (p.mask & kUT) { Vec3<double> sv = self_ut(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    p.vel[k * p.rhs_stride + 3 * gi + comp] = B * (sum + self);
+  } else {
+    if (p.mask & kWF) { Vec3<double> sv = self_wf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (p.mask & kWT) { Vec3<double> sv = self_wt(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    p.omg[k * p.rhs_stride + 3 * gi + comp] = B * p.inva * (sum + self);
   }
 }
 
@@ -604,7 +655,8 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
                       const rmw_tuning* tuning, bool ragged)
 {
   RMW_REQUIRE(valid_mask(mask), "mask must be a non-empty subset of UF|UT|WF|WT");
-  RMW_REQUIRE(prec == RMW_PREC_F32_F64ACC || prec == RMW_PREC_F32_KAHAN || prec == RMW_PREC_F64,
+  RMW_REQUIRE(prec == RMW_PREC_F32_F64ACC || prec == RMW_PREC_F32_KAHAN || prec == RMW_PREC_F64 ||
+                  prec == RMW_PREC_F32_BLOCKED,
               "unknown precision mode");
   RMW_REQUIRE(k >= 1, "k must be >= 1");
   Plan pl;
@@ -679,6 +731,8 @@ static void launch_pair(int prec, const PairParams& pp, int grid, hipStream_t st
 {
   if (prec == RMW_PREC_F32_F64ACC)
     hipLaunchKernelGGL((pair_kernel_f32<MASK, K, RMW_PREC_F32_F64ACC>), dim3(grid), dim3(kBlock), 0, st, pp);
+  else if (prec == RMW_PREC_F32_BLOCKED)
+    hipLaunchKernelGGL((pair_kernel_f32<MASK, K, RMW_PREC_F32_BLOCKED>), dim3(grid), dim3(kBlock), 0, st, pp);
   else if (prec == RMW_PREC_F32_KAHAN)
     hipLaunchKernelGGL((pair_kernel_f32<MASK, K, RMW_PREC_F32_KAHAN>), dim3(grid), dim3(kBlock), 0, st, pp);
   else
@@ -802,8 +856,12 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   }
 
   const bool f32 = pl.prec != RMW_PREC_F64;
-  if (f32 || periodic)
-    hipLaunchKernelGGL(extent_kernel, dim3(1), dim3(1024), 0, st, r, pl.ntot, inva, a, Lmax, extent);
+  if (f32 || periodic) {
+    unsigned long long* word = (unsigned long long*)extent;
+    hipLaunchKernelGGL(extent_clear_kernel, dim3(1), dim3(1), 0, st, word, Lmax);
+    const int eg = (int)std::min<int64_t>((pl.ntot + 255) / 256, 64);
+    hipLaunchKernelGGL(extent_kernel, dim3(eg), dim3(256), 0, st, r, pl.ntot, inva, a, word);
+  }
 
   const int prep_grid = (int)((pl.ntot + 255) / 256);
   int grid = (int)std::min<int64_t>(pl.total_items, (int64_t)INT32_MAX);
@@ -881,7 +939,8 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     rp.ntot = pl.ntot;
     rp.rhs_stride = stride;
     rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
-    const int64_t nthreads = pl.ntot * K;
+    const int OCr = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
+    const int64_t nthreads = pl.ntot * K * OCr;
     hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
     RMW_CHECK_HIP(hipGetLastError());
   }

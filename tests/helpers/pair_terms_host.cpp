@@ -13,7 +13,7 @@ template <typename T>
 static Vec3<T> apply_block(int block, const Geom<T>& g, T zi, T zj, T vx, T vy, T vz)
 {
   switch (block) {
-    case 0: return apply_uf(g, coef_uf(g, zj), vx, vy, vz);
+    case 0: return apply_uf(g, coef_uf(g, zi, zj), vx, vy, vz);
     case 1: return apply_ut(g, coef_rot(g, zi), vx, vy, vz);
     case 2: return apply_wf(g, coef_rot(g, zj), vx, vy, vz);
     default: return apply_wt(g, coef_wt(g), vx, vy, vz);

@@ -15,7 +15,7 @@ from rigidmultiblobswall_amd._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 1e-6, "fp32_kahan": 1e-6, "fp64": 1e-12}
+TOL = {"fp32": 1e-6, "fp32_kahan": 1e-6, "fp32_blocked": 1e-6, "fp64": 1e-12}
 BLOCKS = {"UF": (RMW_UF, O.UF), "UT": (RMW_UT, O.UT), "WF": (RMW_WF, O.WF), "WT": (RMW_WT, O.WT)}
 
 
@@ -31,7 +31,7 @@ def _gpu_block(name, r, v, eta, a, L, prec):
     return fn(r, v, eta, a, periodic_length=L, precision=prec)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp32_kahan", "fp64"])
+@pytest.mark.parametrize("prec", ["fp32", "fp32_kahan", "fp32_blocked", "fp64"])
 @pytest.mark.parametrize("block", ["UF", "UT", "WF", "WT"])
 def test_block_parity_c1(gpu, block, prec):
     w = workloads.c1_uf_matvec()
@@ -44,7 +44,7 @@ def test_block_parity_c1(gpu, block, prec):
     assert err <= TOL[prec]
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("prec", ["fp32", "fp32_blocked", "fp64"])
 @pytest.mark.parametrize("block", ["UF", "UT", "WF", "WT"])
 def test_block_parity_wall_overlap(gpu, block, prec):
     # mobility/test_blobs.py:31-34 stress layout: r = 5 a rand, blobs overlap
@@ -77,8 +77,8 @@ def test_periodic_parity(gpu, prec):
         assert err <= TOL[prec]
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
-@pytest.mark.parametrize("k", [1, 3, 4])
+@pytest.mark.parametrize("prec", ["fp32", "fp32_blocked", "fp64"])
+@pytest.mark.parametrize("k", [1, 3, 4, 7])
 def test_grand_fused_krhs(gpu, prec, k):
     rng = np.random.default_rng(13)
     n, a, eta = 700, 0.656, 1e-3
@@ -181,9 +181,11 @@ def test_c3_size_properties(gpu):
     print("case C3 N=32768 UF fp32/fp64", flush=True)
     u32 = mob.single_wall_mobility_trans_times_force_hip(r, x, eta, a, precision="fp32")
     u64 = mob.single_wall_mobility_trans_times_force_hip(r, x, eta, a, precision="fp64")
+    ub = mob.single_wall_mobility_trans_times_force_hip(r, x, eta, a, precision="fp32_blocked")
     e = rel(u32, u64)
-    print(f"  fp32 vs fp64 rel L2 = {e:.3e}")
-    assert e <= 1e-6
+    eb = rel(ub, u64)
+    print(f"  fp32 vs fp64 rel L2 = {e:.3e}, fp32_blocked vs fp64 = {eb:.3e}")
+    assert e <= 1e-6 and eb <= 1e-6
     rows = (1000, 1256)
     ref = O.uf(r, x, eta, a, rows=rows).reshape(n, 3)[rows[0]:rows[1]]
     assert rel(u64.reshape(n, 3)[rows[0]:rows[1]], ref) <= 1e-12
