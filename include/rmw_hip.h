@@ -109,6 +109,83 @@ int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vectors, const
 int rmw_profile_enable(int on);
 int rmw_profile_collect(int64_t* launches, double* total_ms, double* mean_ms);
 
+/* ------------------------------------------------------------------------
+ * Rigid multiblob bodies (FP64).  body_offsets is a HOST array of nbodies+1
+ * blob offsets (body b owns blobs [off[b], off[b+1])).  rmw_rigid_setup
+ * uploads the offsets into the workspace; every other rigid call reads them
+ * from that workspace (same offsets, stream-ordered after the setup).
+ * Vectors of the rigid system are [lambda (3N); U (6B)] with U_b = (v, w).
+ * ---------------------------------------------------------------------- */
+int rmw_rigid_workspace_size(const int64_t* body_offsets, int nbodies, size_t* bytes,
+                             size_t* minv_doubles, size_t* pc_scratch_doubles);
+int rmw_rigid_setup(const int64_t* body_offsets, int nbodies, void* workspace,
+                    size_t workspace_bytes, void* stream);
+
+/* Blob positions r = R(q_b) ref + c_b and r_rel = r - c_b (quaternions (s,p),
+ * rotation of quaternion_integrator/quaternion.py:42-52).  Replaces
+ * Body.get_r_vectors, body/body.py:64-77, for all bodies at once. */
+int rmw_body_blobs(const double* centres, const double* quats, const double* ref,
+                   const int64_t* body_offsets, int nbodies, double* r_blobs, double* r_rel,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Second half of the rigid operator: given m_lambda = M lambda,
+ *   y_lambda = m_lambda - K U,   y_U = -K^T lambda
+ * (K_b = [J, rot], body/body.py:81-113).  With rmw_wall_rpy_apply for
+ * M lambda this replaces linear_operator_rigid, multi_bodies/multi_bodies.py:
+ * 424-477 (K U and K^T lambda of :328-370); constraints and prescribed
+ * kinematics are not supported. */
+int rmw_rigid_operator_post(const double* r_rel, const int64_t* body_offsets, int nbodies,
+                            const double* x, const double* m_lambda, double* y,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
+/* Dense per-body wall mobility B M(z_eff) B, row-major 3n x 3n blocks packed
+ * back to back (block b at offset sum_{b'<b} (3 n_b')^2).  Replaces
+ * single_wall_fluid_mobility, mobility/mobility.py:1018-1116, as selected by
+ * mobility_blobs_implementation (multi_bodies/multi_bodies.py:207-230). */
+int rmw_dense_wall_mobility(const double* r_blobs, const int64_t* body_offsets, int nbodies,
+                            double eta, double a, double* out, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+/* Block-diagonal body preconditioner build: per body M_b^-1 (via Cholesky,
+ * minv: packed blocks as above, minv_doubles long), M_b^-1 K (minvk:
+ * (3N, 6)) and N_b = pinv(K^T M_b^-1 K) (nbody: (B, 6, 6)); status[b] = 1
+ * when M_b is not positive definite (may be NULL).  Bodies of up to 45 blobs
+ * factorise in LDS; larger ones need `scratch` of pc_scratch_doubles
+ * (else NULL, 0).  Replaces the factorisation loop of
+ * build_block_diagonal_preconditioner, multi_bodies/multi_bodies.py:752-822. */
+int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
+                       int nbodies, double eta, double a, double* minv, double* minvk,
+                       double* nbody, int* status, double* scratch, size_t scratch_doubles,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* y = P^-1 x, the body preconditioner apply of multi_bodies/multi_bodies.py:
+ * 825-849 (x != y). */
+int rmw_rigid_pc_apply(const int64_t* body_offsets, int nbodies, const double* minv,
+                       const double* minvk, const double* nbody, const double* r_rel,
+                       const double* x, double* y, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
+/* ------------------------------------------------------------------------
+ * FP64 Krylov vector kernels (GMRES Arnoldi and Lanczos).  Bases are nvec
+ * rows of stride ldv; coefficient vectors live on the device.  The scipy
+ * gmres / numpy vector algebra they replace is called from
+ * general_application_utils.py:514-635 and
+ * stochastic_forcing/stochastic_forcing.py:112-262.
+ * ---------------------------------------------------------------------- */
+int rmw_krylov_workspace_size(int64_t n, int max_vectors, size_t* bytes);
+int rmw_dot_many(const double* V, int64_t ldv, int nvec, const double* w, int64_t n, double* out,
+                 int sqrt_out, void* workspace, size_t workspace_bytes, void* stream);
+int rmw_sub_combination(const double* V, int64_t ldv, int nvec, const double* coef, double* w,
+                        int64_t n, void* stream);
+int rmw_combination(const double* V, int64_t ldv, int nvec, const double* coef, double* y,
+                    int64_t n, int accumulate, void* stream);
+int rmw_axpby(double alpha, const double* x, double beta, double* y, int64_t n, const double* div,
+              void* stream);
+/* Arnoldi step, classical Gram-Schmidt with one re-orthogonalisation:
+ * h[0..j] = coefficients, h[j+1] = ||w||, V[j+1] = w / ||w||. */
+int rmw_arnoldi_cgs2(double* V, int64_t ldv, int j, double* w, int64_t n, double* h,
+                     double* hscratch, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Device properties used for launch sizing and roofline reporting. */
 int rmw_device_info(int* cu_count, int* clock_khz, int* wavefront, char* arch_name, int arch_len);
 

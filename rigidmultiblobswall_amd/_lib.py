@@ -63,6 +63,23 @@ PROTOTYPES = {
     "rmw_profile_collect": (ctypes.c_int, [_i64p, _hdp, _hdp]),
     "rmw_device_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]),
+    # rigid bodies
+    "rmw_rigid_workspace_size": (ctypes.c_int, [_i64p, ctypes.c_int, _szp, _szp, _szp]),
+    "rmw_rigid_setup": (ctypes.c_int, [_i64p, ctypes.c_int, _vp, _sz, _vp]),
+    "rmw_body_blobs": (ctypes.c_int, [_dp, _dp, _dp, _i64p, ctypes.c_int, _dp, _dp, _vp, _sz, _vp]),
+    "rmw_rigid_operator_post": (ctypes.c_int, [_dp, _i64p, ctypes.c_int, _dp, _dp, _dp, _vp, _sz, _vp]),
+    "rmw_dense_wall_mobility": (ctypes.c_int, [_dp, _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
+                                               _vp, _sz, _vp]),
+    "rmw_rigid_pc_build": (ctypes.c_int, [_dp, _dp, _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
+                                          _dp, _dp, _vp, _dp, _sz, _vp, _sz, _vp]),
+    "rmw_rigid_pc_apply": (ctypes.c_int, [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _sz, _vp]),
+    # Krylov
+    "rmw_krylov_workspace_size": (ctypes.c_int, [_i64, ctypes.c_int, _szp]),
+    "rmw_dot_many": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _i64, _dp, ctypes.c_int, _vp, _sz, _vp]),
+    "rmw_sub_combination": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _dp, _i64, _vp]),
+    "rmw_combination": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _dp, _i64, ctypes.c_int, _vp]),
+    "rmw_axpby": (ctypes.c_int, [ctypes.c_double, _dp, ctypes.c_double, _dp, _i64, _dp, _vp]),
+    "rmw_arnoldi_cgs2": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _i64, _dp, _dp, _vp, _sz, _vp]),
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,611 @@
+// Rigid multiblob bodies on gfx950: blob positions from body poses, the
+// geometric matrix K = [J, rot] and its transpose, the rigid saddle-point
+// operator, the dense per-body wall mobility and the block-diagonal body
+// preconditioner -- all FP64.
+//
+// Reference:
+//   body/body.py:64-77 (blob positions), :81-113 (K), quaternion.py:42-52
+//   multi_bodies/multi_bodies.py:328-370 (K U, K^T lambda), :424-477
+//     (linear_operator_rigid), :752-907 (build_block_diagonal_preconditioner
+//     and its apply), mobility/mobility.py:1018-1116 (dense wall mobility)
+//
+// Preconditioner build, one workgroup per body (n = 3 * blobs of the body):
+//   M_b      dense wall mobility B M(z_eff) B of the body's blobs
+//   L        Cholesky factor, in place                  (M_b = L L^T)
+//   X        L^-1, in place (lower triangular inverse)
+//   M_b^-1 = X^T X             written to global memory (row-major n x n)
+//   XK = X K,  M_b^-1 K = X^T (X K),  K^T M_b^-1 K = (XK)^T (XK)
+//   N_b = pinv(K^T M_b^-1 K)   (6x6, symmetric Jacobi eigen-solve, rcond 1e-15)
+// Bodies with n <= kLdsMaxN keep the whole factorisation in LDS (160 KB per
+// workgroup on gfx950); larger bodies run the same code on a global scratch.
+// Apply, one workgroup per body (multi_bodies.py:836-849):
+//   lt = M_b^-1 s,  Y = N_b (-F - K^T lt),  lambda = lt + (M_b^-1 K) Y
+// (lambda equals M_b^-1 (s + K Y) of the reference up to rounding).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rmw_internal.h"
+#include "rmw_pair_terms.h"
+
+namespace rmw {
+
+constexpr int kLdsMaxN = 136;          // (n^2 + 8 n) doubles <= 160 KB
+constexpr int kRigidBlock = 256;
+
+// Pinned staging of small host arrays (defined in rmw_wall_rpy.hip).
+void stage_upload_public(void* dst, const void* src, size_t bytes, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Blob positions and K
+// ---------------------------------------------------------------------------
+__global__ void body_blobs_kernel(const double* __restrict__ centres, const double* __restrict__ quats,
+                                  const double* __restrict__ ref, const int64_t* __restrict__ offs,
+                                  int nbodies, int64_t nblobs, double* __restrict__ r,
+                                  double* __restrict__ rrel)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nblobs) return;
+  int lo = 0, hi = nbodies - 1;  // body owning blob i
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const double* q = quats + 4 * lo;
+  const double s = q[0], p0 = q[1], p1 = q[2], p2 = q[3];
+  const double dg = s * s - 0.5;
+  // rotation matrix of quaternion.py:42-52
+  const double R00 = 2 * (p0 * p0 + dg), R01 = 2 * (p0 * p1 - s * p2), R02 = 2 * (p0 * p2 + s * p1);
+  const double R10 = 2 * (p1 * p0 + s * p2), R11 = 2 * (p1 * p1 + dg), R12 = 2 * (p1 * p2 - s * p0);
+  const double R20 = 2 * (p2 * p0 - s * p1), R21 = 2 * (p2 * p1 + s * p0), R22 = 2 * (p2 * p2 + dg);
+  const double x = ref[3 * i], y = ref[3 * i + 1], z = ref[3 * i + 2];
+  const double ax = R00 * x + R01 * y + R02 * z;
+  const double ay = R10 * x + R11 * y + R12 * z;
+  const double az = R20 * x + R21 * y + R22 * z;
+  rrel[3 * i] = ax; rrel[3 * i + 1] = ay; rrel[3 * i + 2] = az;
+  r[3 * i] = ax + centres[3 * lo];
+  r[3 * i + 1] = ay + centres[3 * lo + 1];
+  r[3 * i + 2] = az + centres[3 * lo + 2];
+}
+
+// Second half of the rigid operator (multi_bodies.py:424-477) once
+// m_lambda = M lambda is known: y_lambda = m_lambda - K U, y_U = -K^T lambda.
+// One workgroup per body.
+__global__ __launch_bounds__(kRigidBlock) void rigid_post_kernel(const double* __restrict__ rrel,
+                                                                 const int64_t* __restrict__ offs,
+                                                                 const double* __restrict__ x,
+                                                                 const double* __restrict__ mlam,
+                                                                 double* __restrict__ y, int64_t nblobs)
+{
+  const int b = blockIdx.x;
+  const int64_t i0 = offs[b], i1 = offs[b + 1];
+  const double* U = x + 3 * nblobs + 6 * (int64_t)b;
+  const double vx = U[0], vy = U[1], vz = U[2], wx = U[3], wy = U[4], wz = U[5];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double rx = rrel[3 * i], ry = rrel[3 * i + 1], rz = rrel[3 * i + 2];
+    // K U = v + w x r   (body.py:81-113: rot_i x = -(r_i x x))
+    const double kx = vx + (wy * rz - wz * ry);
+    const double ky = vy + (wz * rx - wx * rz);
+    const double kz = vz + (wx * ry - wy * rx);
+    const double lx = x[3 * i], ly = x[3 * i + 1], lz = x[3 * i + 2];
+    y[3 * i] = mlam[3 * i] - kx;
+    y[3 * i + 1] = mlam[3 * i + 1] - ky;
+    y[3 * i + 2] = mlam[3 * i + 2] - kz;
+    // K^T lambda = [sum lambda; sum r x lambda]
+    acc[0] += lx; acc[1] += ly; acc[2] += lz;
+    acc[3] += ry * lz - rz * ly;
+    acc[4] += rz * lx - rx * lz;
+    acc[5] += rx * ly - ry * lx;
+  }
+  __shared__ double red[6][kRigidBlock];
+  for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = kRigidBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) y[3 * nblobs + 6 * (int64_t)b + threadIdx.x] = -red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// Dense wall mobility entries (FP64), B M(z_eff) B, one 3x3 block.
+// ---------------------------------------------------------------------------
+__device__ inline void wall_uf_block(const double* ri, const double* rj, bool self, double a, double inva,
+                                     double norm, double m[9])
+{
+  const double zi_raw = ri[2], zj_raw = rj[2];
+  const double zie = zi_raw <= a ? a : zi_raw, zje = zj_raw <= a ? a : zj_raw;
+  const double Bi = zi_raw < a ? zi_raw * inva : 1.0, Bj = zj_raw < a ? zj_raw * inva : 1.0;
+  const double zi = zie * inva, zj = zje * inva;
+  for (int c = 0; c < 3; ++c) {
+    double v[3] = {0.0, 0.0, 0.0};
+    v[c] = 1.0;
+    Vec3<double> u;
+    if (self) {
+      u = self_uf(zi, v[0], v[1], v[2]);
+    } else {
+      Geom<double> g = make_geom((ri[0] - rj[0]) * inva, (ri[1] - rj[1]) * inva, (zie - zje) * inva, zi, zj);
+      u = apply_uf(g, coef_uf(g, zi, zj), v[0], v[1], v[2]);
+    }
+    const double s = norm * Bi * Bj;
+    m[0 * 3 + c] = s * u.x;
+    m[1 * 3 + c] = s * u.y;
+    m[2 * 3 + c] = s * u.z;
+  }
+}
+
+// Dense 3N x 3N wall mobility of each body (or of one system = one body),
+// row-major at out + moff[b].
+__global__ void dense_mobility_kernel(const double* __restrict__ r, const int64_t* __restrict__ offs,
+                                      const int64_t* __restrict__ moff, double a, double eta,
+                                      double* __restrict__ out)
+{
+  const int b = blockIdx.y;
+  const int64_t i0 = offs[b], nb = offs[b + 1] - i0;
+  const int64_t n = 3 * nb;
+  const double inva = 1.0 / a, norm = 1.0 / (8.0 * M_PI * eta * a);
+  double* M = out + moff[b];
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nb * nb;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / nb, j = idx - (idx / nb) * nb;
+    double m[9];
+    wall_uf_block(r + 3 * (i0 + i), r + 3 * (i0 + j), i == j, a, inva, norm, m);
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) M[(3 * i + p) * n + 3 * j + q] = m[p * 3 + q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric 6x6 pseudo-inverse by cyclic Jacobi (one thread).
+// ---------------------------------------------------------------------------
+__device__ void pinv6(const double* A_in, double* P)
+{
+  double A[36], V[36];
+  for (int i = 0; i < 36; ++i) { A[i] = A_in[i]; V[i] = (i % 7 == 0) ? 1.0 : 0.0; }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < 6; ++p)
+      for (int q = 0; q < 6; ++q) {
+        if (p == q) diag += A[p * 6 + q] * A[p * 6 + q];
+        else off += A[p * 6 + q] * A[p * 6 + q];
+      }
+    if (off <= 1e-32 * diag || off == 0.0) break;
+    for (int p = 0; p < 5; ++p)
+      for (int q = p + 1; q < 6; ++q) {
+        const double apq = A[p * 6 + q];
+        if (apq == 0.0) continue;
+        const double app = A[p * 6 + p], aqq = A[q * 6 + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 6; ++k) {  // A <- A J (columns p, q)
+          const double akp = A[k * 6 + p], akq = A[k * 6 + q];
+          A[k * 6 + p] = c * akp - s * akq;
+          A[k * 6 + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 6; ++k) {  // A <- J^T A (rows p, q)
+          const double apk = A[p * 6 + k], aqk = A[q * 6 + k];
+          A[p * 6 + k] = c * apk - s * aqk;
+          A[q * 6 + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 6; ++k) {
+          const double vkp = V[k * 6 + p], vkq = V[k * 6 + q];
+          V[k * 6 + p] = c * vkp - s * vkq;
+          V[k * 6 + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  double lmax = 0.0;
+  for (int i = 0; i < 6; ++i) lmax = fmax(lmax, fabs(A[i * 7]));
+  const double cut = 1e-15 * lmax;  // numpy.linalg.pinv default rcond
+  for (int i = 0; i < 36; ++i) P[i] = 0.0;
+  for (int e = 0; e < 6; ++e) {
+    const double lam = A[e * 7];
+    if (fabs(lam) <= cut) continue;
+    const double il = 1.0 / lam;
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) P[i * 6 + j] += V[i * 6 + e] * il * V[j * 6 + e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioner build: one workgroup per body.
+// ---------------------------------------------------------------------------
+struct PcBuildParams {
+  const double* r;        // blob positions (N, 3)
+  const double* rrel;     // blob - centre
+  const int64_t* offs;    // body blob offsets (B+1)
+  const int64_t* moff;    // offsets of the per-body n x n blocks (B+1)
+  double* minv;           // out: M_b^-1 blocks
+  double* minvk;          // out: (3N, 6) M_b^-1 K
+  double* nbody;          // out: (B, 6, 6)
+  double* scratch;        // global scratch for bodies above kLdsMaxN (same layout as minv)
+  double* scratch_vec;    // global scratch, 8 n doubles per body at 8 * 3 * offs[b]
+  int* status;            // out: per body 0 ok, 1 not positive definite
+  double a, eta;
+  int use_lds;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(kRigidBlock) void pc_build_kernel(PcBuildParams p)
+{
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  const int64_t i0 = p.offs[b], nb = p.offs[b + 1] - i0;
+  const int n = (int)(3 * nb);
+  if (LDS != (n <= kLdsMaxN)) return;  // the other instantiation owns this body
+  double* A = LDS ? smem : p.scratch + p.moff[b];
+  double* tmp = LDS ? smem + (size_t)n * n : p.scratch_vec + 8 * 3 * i0;
+  double* XK = tmp + n;  // n x 6
+  __shared__ double a6[36];
+  __shared__ int bad;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  const double inva = 1.0 / p.a, norm = 1.0 / (8.0 * M_PI * p.eta * p.a);
+  if (tid == 0) bad = 0;
+
+  // 1. assemble M_b
+  for (int64_t idx = tid; idx < nb * nb; idx += nt) {
+    const int64_t i = idx / nb, j = idx - (idx / nb) * nb;
+    double m[9];
+    wall_uf_block(p.r + 3 * (i0 + i), p.r + 3 * (i0 + j), i == j, p.a, inva, norm, m);
+    for (int q = 0; q < 3; ++q)
+      for (int s = 0; s < 3; ++s) A[(3 * i + q) * n + 3 * j + s] = m[q * 3 + s];
+  }
+  __syncthreads();
+
+  // 2. Cholesky, lower triangle in place (row-major)
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      double d = A[k * n + k];
+      if (!(d > 0.0)) { bad = 1; d = 1.0; }
+      A[k * n + k] = sqrt(d);
+    }
+    __syncthreads();
+    const double dk = A[k * n + k];
+    for (int i = k + 1 + tid; i < n; i += nt) A[i * n + k] /= dk;
+    __syncthreads();
+    for (int i = k + 1 + wv; i < n; i += nw) {
+      const double lik = A[i * n + k];
+      for (int j = k + 1 + lane; j <= i; j += 64) A[i * n + j] -= lik * A[j * n + k];
+    }
+    __syncthreads();
+  }
+
+  // 3. X = L^-1 in place, columns from last to first
+  for (int j = n - 1; j >= 0; --j) {
+    if (tid == 0) A[j * n + j] = 1.0 / A[j * n + j];
+    for (int i = j + 1 + tid; i < n; i += nt) tmp[i] = A[i * n + j];
+    __syncthreads();
+    const double xjj = A[j * n + j];
+    for (int i = j + 1 + tid; i < n; i += nt) {
+      double y = 0.0;
+      for (int k = j + 1; k <= i; ++k) y += A[i * n + k] * tmp[k];
+      A[i * n + j] = -y * xjj;
+    }
+    __syncthreads();
+  }
+
+  // 4. M^-1 = X^T X  (X lower: (X^T X)_ij = sum_{k >= max(i,j)} X_ki X_kj)
+  double* Minv = p.minv + p.moff[b];
+  for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) {
+    const int i = (int)(idx / n), j = (int)(idx - (int64_t)(idx / n) * n);
+    double s = 0.0;
+    for (int k = i > j ? i : j; k < n; ++k) s += A[k * n + i] * A[k * n + j];
+    Minv[idx] = s;
+  }
+
+  // 5. XK = X K  with K rows [e_c | rot_c]; rot_i x = -(r_i x x)
+  for (int idx = tid; idx < n * 6; idx += nt) {
+    const int k = idx / 6, c = idx - (idx / 6) * 6;
+    double s = 0.0;
+    for (int i = 0; i <= k; ++i) {
+      const int blob = i / 3, comp = i - (i / 3) * 3;
+      const double* rr = p.rrel + 3 * (i0 + blob);
+      double kic;
+      if (c < 3) {
+        kic = comp == c ? 1.0 : 0.0;
+      } else {
+        // K[3m+comp][3+d] = rot entries of body.py:92-99
+        const int d = c - 3;
+        const double rx = rr[0], ry = rr[1], rz = rr[2];
+        const double rot[3][3] = {{0.0, rz, -ry}, {-rz, 0.0, rx}, {ry, -rx, 0.0}};
+        kic = rot[comp][d];
+      }
+      s += A[k * n + i] * kic;
+    }
+    XK[idx] = s;
+  }
+  __syncthreads();
+  // M^-1 K = X^T XK
+  double* MK = p.minvk + 3 * i0 * 6;
+  for (int idx = tid; idx < n * 6; idx += nt) {
+    const int i = idx / 6, c = idx - (idx / 6) * 6;
+    double s = 0.0;
+    for (int k = i; k < n; ++k) s += A[k * n + i] * XK[k * 6 + c];
+    MK[idx] = s;
+  }
+  // K^T M^-1 K = XK^T XK
+  if (tid < 36) {
+    const int r6 = tid / 6, c6 = tid - (tid / 6) * 6;
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += XK[k * 6 + r6] * XK[k * 6 + c6];
+    a6[tid] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    pinv6(a6, p.nbody + 36 * (int64_t)b);
+    p.status[b] = bad;
+  }
+}
+
+// Preconditioner apply: y = P^-1 x, one workgroup per body.
+__global__ __launch_bounds__(kRigidBlock) void pc_apply_kernel(const int64_t* __restrict__ offs,
+                                                               const int64_t* __restrict__ moff,
+                                                               const double* __restrict__ minv,
+                                                               const double* __restrict__ minvk,
+                                                               const double* __restrict__ nbody,
+                                                               const double* __restrict__ rrel,
+                                                               const double* __restrict__ x,
+                                                               double* __restrict__ y, int64_t nblobs)
+{
+  extern __shared__ double sm[];  // s (n) | lt (n)
+  const int b = blockIdx.x;
+  const int64_t i0 = offs[b], nb = offs[b + 1] - i0;
+  const int n = (int)(3 * nb);
+  double* s = sm;
+  double* lt = sm + n;
+  __shared__ double t6[6], Y[6];
+  __shared__ double red[6][kRigidBlock / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (int i = tid; i < n; i += blockDim.x) s[i] = x[3 * i0 + i];
+  __syncthreads();
+  // lt = M^-1 s: one wavefront per row, lanes across columns (coalesced)
+  const double* M = minv + moff[b];
+  for (int i = wv; i < n; i += nw) {
+    double acc = 0.0;
+    for (int j = lane; j < n; j += 64) acc += M[(int64_t)i * n + j] * s[j];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) lt[i] = acc;
+  }
+  __syncthreads();
+  // t = -F - K^T lt
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int m = tid; m < nb; m += blockDim.x) {
+    const double* rr = rrel + 3 * (i0 + m);
+    const double lx = lt[3 * m], ly = lt[3 * m + 1], lz = lt[3 * m + 2];
+    acc[0] += lx; acc[1] += ly; acc[2] += lz;
+    acc[3] += rr[1] * lz - rr[2] * ly;
+    acc[4] += rr[2] * lx - rr[0] * lz;
+    acc[5] += rr[0] * ly - rr[1] * lx;
+  }
+  for (int c = 0; c < 6; ++c) {
+    double v = acc[c];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) red[c][wv] = v;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double v = 0.0;
+    for (int w = 0; w < nw; ++w) v += red[tid][w];
+    t6[tid] = -x[3 * nblobs + 6 * (int64_t)b + tid] - v;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    const double* Nb = nbody + 36 * (int64_t)b;
+    double v = 0.0;
+    for (int c = 0; c < 6; ++c) v += Nb[tid * 6 + c] * t6[c];
+    Y[tid] = v;
+    y[3 * nblobs + 6 * (int64_t)b + tid] = v;
+  }
+  __syncthreads();
+  const double* MK = minvk + 3 * i0 * 6;
+  for (int i = tid; i < n; i += blockDim.x) {
+    double v = lt[i];
+    for (int c = 0; c < 6; ++c) v += MK[i * 6 + c] * Y[c];
+    y[3 * i0 + i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct RigidLayout {
+  int nbodies = 0;
+  int64_t nblobs = 0;
+  int max_n = 0;
+  std::vector<int64_t> offs, moff;
+  size_t off_offs = 0, off_moff = 0, off_status = 0, total = 0;
+  size_t minv_doubles = 0;
+  size_t scratch_doubles = 0;  // factorisation scratch for bodies above kLdsMaxN
+};
+
+static RigidLayout rigid_layout(const int64_t* body_offsets, int nbodies)
+{
+  RMW_REQUIRE(body_offsets != nullptr && nbodies >= 1, "body_offsets / nbodies invalid");
+  RMW_REQUIRE(body_offsets[0] == 0, "body_offsets[0] must be 0");
+  RigidLayout L;
+  L.nbodies = nbodies;
+  L.offs.assign(body_offsets, body_offsets + nbodies + 1);
+  L.moff.resize(nbodies + 1);
+  L.moff[0] = 0;
+  bool big = false;
+  for (int b = 0; b < nbodies; ++b) {
+    const int64_t nb = L.offs[b + 1] - L.offs[b];
+    RMW_REQUIRE(nb >= 1, "every body needs at least one blob");
+    const int64_t n = 3 * nb;
+    RMW_REQUIRE(n <= 30000, "body too large for a dense block");
+    L.moff[b + 1] = L.moff[b] + n * n;
+    L.max_n = std::max<int>(L.max_n, (int)n);
+    if (n > kLdsMaxN) big = true;
+  }
+  L.nblobs = L.offs[nbodies];
+  L.minv_doubles = (size_t)L.moff[nbodies];
+  size_t o = 0;
+  L.off_offs = o; o = align_up(o + (nbodies + 1) * sizeof(int64_t));
+  L.off_moff = o; o = align_up(o + (nbodies + 1) * sizeof(int64_t));
+  L.off_status = o; o = align_up(o + nbodies * sizeof(int));
+  L.total = o;
+  if (big) L.scratch_doubles = L.minv_doubles + (size_t)8 * 3 * L.nblobs;
+  return L;
+}
+
+static void upload_offsets(const RigidLayout& L, char* ws, hipStream_t st)
+{
+  stage_upload_public(ws + L.off_offs, L.offs.data(), L.offs.size() * sizeof(int64_t), st);
+  stage_upload_public(ws + L.off_moff, L.moff.data(), L.moff.size() * sizeof(int64_t), st);
+}
+
+// Every rigid entry point below reads the body offsets from the workspace;
+// rmw_rigid_setup must have been called on that workspace (stream-ordered)
+// with the same body_offsets.
+
+}  // namespace rmw
+
+using namespace rmw;
+
+extern "C" int rmw_rigid_workspace_size(const int64_t* body_offsets, int nbodies, size_t* bytes,
+                                        size_t* minv_doubles, size_t* pc_scratch_doubles)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    if (bytes) *bytes = L.total;
+    if (minv_doubles) *minv_doubles = L.minv_doubles;
+    if (pc_scratch_doubles) *pc_scratch_doubles = L.scratch_doubles;
+  });
+}
+
+extern "C" int rmw_rigid_setup(const int64_t* body_offsets, int nbodies, void* workspace,
+                               size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace && workspace_bytes >= L.total, "workspace too small");
+    upload_offsets(L, (char*)workspace, (hipStream_t)stream);
+  });
+}
+
+extern "C" int rmw_body_blobs(const double* centres, const double* quats, const double* ref,
+                              const int64_t* body_offsets, int nbodies, double* r_blobs, double* r_rel,
+                              void* workspace, size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    const int grid = (int)((L.nblobs + 255) / 256);
+    hipLaunchKernelGGL(body_blobs_kernel, dim3(grid), dim3(256), 0, st, centres, quats, ref,
+                       (const int64_t*)(ws + L.off_offs), nbodies, L.nblobs, r_blobs, r_rel);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_rigid_operator_post(const double* r_rel, const int64_t* body_offsets, int nbodies,
+                                       const double* x, const double* m_lambda, double* y, void* workspace,
+                                       size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(x && m_lambda && y && r_rel, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    hipLaunchKernelGGL(rigid_post_kernel, dim3(nbodies), dim3(kRigidBlock), 0, st, r_rel,
+                       (const int64_t*)(ws + L.off_offs), x, m_lambda, y, L.nblobs);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_dense_wall_mobility(const double* r_blobs, const int64_t* body_offsets, int nbodies,
+                                       double eta, double a, double* out, void* workspace,
+                                       size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(eta > 0 && a > 0, "eta and a must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    int64_t maxpairs = 0;
+    for (int b = 0; b < nbodies; ++b) {
+      int64_t nb = L.offs[b + 1] - L.offs[b];
+      maxpairs = std::max(maxpairs, nb * nb);
+    }
+    const int gx = (int)std::min<int64_t>((maxpairs + 255) / 256, 4096);
+    hipLaunchKernelGGL(dense_mobility_kernel, dim3(gx, nbodies), dim3(256), 0, st, r_blobs,
+                       (const int64_t*)(ws + L.off_offs), (const int64_t*)(ws + L.off_moff), a, eta, out);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
+                                  int nbodies, double eta, double a, double* minv, double* minvk,
+                                  double* nbody, int* status, double* scratch, size_t scratch_doubles,
+                                  void* workspace, size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(eta > 0 && a > 0, "eta and a must be positive");
+    RMW_REQUIRE(r_blobs && r_rel && minv && minvk && nbody, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    PcBuildParams p{};
+    p.r = r_blobs; p.rrel = r_rel;
+    p.offs = (const int64_t*)(ws + L.off_offs);
+    p.moff = (const int64_t*)(ws + L.off_moff);
+    p.minv = minv; p.minvk = minvk; p.nbody = nbody;
+    RMW_REQUIRE(scratch_doubles >= L.scratch_doubles && (L.scratch_doubles == 0 || scratch != nullptr),
+                "factorisation scratch too small (bodies above 45 blobs need it)");
+    p.scratch = L.scratch_doubles ? scratch : nullptr;
+    p.scratch_vec = L.scratch_doubles ? scratch + L.minv_doubles : nullptr;
+    p.status = status ? status : (int*)(ws + L.off_status);
+    p.a = a; p.eta = eta;
+    int small = 0, big = 0;
+    int max_small = 0;
+    for (int b = 0; b < nbodies; ++b) {
+      int n = (int)(3 * (L.offs[b + 1] - L.offs[b]));
+      if (n <= kLdsMaxN) { ++small; max_small = std::max(max_small, n); } else { ++big; }
+    }
+    if (small) {
+      const size_t lds = ((size_t)max_small * max_small + 8 * (size_t)max_small) * sizeof(double);
+      RMW_CHECK_HIP(hipFuncSetAttribute((const void*)pc_build_kernel<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(pc_build_kernel<true>, dim3(nbodies), dim3(kRigidBlock), lds, st, p);
+      RMW_CHECK_HIP(hipGetLastError());
+    }
+    if (big) {
+      hipLaunchKernelGGL(pc_build_kernel<false>, dim3(nbodies), dim3(kRigidBlock), 0, st, p);
+      RMW_CHECK_HIP(hipGetLastError());
+    }
+  });
+}
+
+extern "C" int rmw_rigid_pc_apply(const int64_t* body_offsets, int nbodies, const double* minv,
+                                  const double* minvk, const double* nbody, const double* r_rel,
+                                  const double* x, double* y, void* workspace, size_t workspace_bytes,
+                                  void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(x != y, "pc apply is out of place");
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    const size_t lds = 2 * (size_t)L.max_n * sizeof(double);
+    RMW_REQUIRE(lds <= 160 * 1024, "body too large for the LDS preconditioner apply");
+    if (lds > 64 * 1024)
+      RMW_CHECK_HIP(hipFuncSetAttribute((const void*)pc_apply_kernel,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(pc_apply_kernel, dim3(nbodies), dim3(kRigidBlock), lds, st,
+                       (const int64_t*)(ws + L.off_offs), (const int64_t*)(ws + L.off_moff), minv, minvk,
+                       nbody, r_rel, x, y, L.nblobs);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}

@@ -808,6 +808,11 @@ static void stage_upload(void* dst, const void* src, size_t bytes, hipStream_t s
   s.pending = true;
 }
 
+void stage_upload_public(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+  stage_upload(dst, src, bytes, st);
+}
+
 // Profiling record.
 struct ProfRec { hipEvent_t a, b; };
 static std::mutex g_prof_mu;

@@ -113,3 +113,31 @@ __all__ = [
     "single_wall_mobility_rot_times_force_hip", "single_wall_mobility_rot_times_torque_hip",
     "single_wall_mobility_trans_times_force_torque_hip", "single_wall_grand_mobility_hip",
 ]
+
+
+def single_wall_fluid_mobility_hip(r_vectors, eta, a, *args, **kwargs):
+    """Dense 3N x 3N wall mobility B M(z_eff) B (FP64, built on the GPU);
+    drop-in for single_wall_fluid_mobility, mobility/mobility.py:1018-1116,
+    as selected by mobility_blobs_implementation."""
+    import ctypes
+    from . import _lib
+    lib = _lib.load()
+    r = torch.as_tensor(np.asarray(r_vectors, dtype=np.float64).reshape(-1, 3) if not torch.is_tensor(r_vectors)
+                        else r_vectors.reshape(-1, 3)).to(dtype=torch.float64)
+    device = torch.device("cuda", torch.cuda.current_device())
+    r = r.to(device).contiguous()
+    n = r.shape[0]
+    offs = np.array([0, n], dtype=np.int64)
+    offp = offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_rigid_workspace_size(offp, 1, ctypes.byref(nbytes), None, None))
+    ws = wall_rpy.workspace(nbytes.value, device, slot="dense")
+    out = torch.empty((3 * n, 3 * n), dtype=torch.float64, device=device)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    _lib.check(lib.rmw_rigid_setup(offp, 1, ws.data_ptr(), ws.numel(), stream))
+    _lib.check(lib.rmw_dense_wall_mobility(r.data_ptr(), offp, 1, float(eta), float(a), out.data_ptr(),
+                                           ws.data_ptr(), ws.numel(), stream))
+    return out.cpu().numpy()
+
+
+__all__.append("single_wall_fluid_mobility_hip")

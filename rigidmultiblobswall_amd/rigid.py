@@ -1,0 +1,181 @@
+"""Rigid multiblob suspensions on the GPU (FP64 solver, FP32/FP64 matvec).
+
+RigidSuspension holds the bodies of one suspension on the device and
+provides the pieces of the reference's rigid mobility solve:
+
+  operator(x, out)        [M lambda - K U; -K^T lambda]
+                          (multi_bodies/multi_bodies.py:424-477)
+  build_preconditioner()  per-body Cholesky of the dense wall mobility,
+                          N_b = pinv(K^T M_b^-1 K) (multi_bodies.py:752-822)
+  pc_apply(x, out)        block-diagonal preconditioner (multi_bodies.py:825-849)
+  solve_mobility_problem  normalised-RHS right-preconditioned GMRES
+                          (quaternion_integrator_multi_bodies.py:1441-1536)
+
+The blob mobility product M lambda is librmw_hip's wall-RPY kernel; its
+pair arithmetic is chosen by `precision` ('fp64', 'fp32', 'fp32_blocked', ...).
+The operator, K products, preconditioner and Krylov algebra are FP64.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib, krylov, wall_rpy
+from ._lib import RMW_UF
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+class RigidSuspension:
+    """Rigid bodies sharing blob radius `a` above a no-slip wall.
+
+    locations (B, 3); orientations (B, 4) unit quaternions (s, p);
+    reference: one (nb, 3) array shared by all bodies, or a list of per-body
+    (nb_b, 3) arrays (blob coordinates at quaternion (1, 0, 0, 0)).
+    """
+
+    def __init__(self, locations, orientations, reference, eta: float, a: float, periodic_length=None,
+                 precision: str = "fp64", device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("RigidSuspension needs a ROCm GPU (there is no CPU fallback)")
+        self.lib = _lib.load()
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        loc = np.ascontiguousarray(np.asarray(locations, dtype=np.float64).reshape(-1, 3))
+        quat = np.ascontiguousarray(np.asarray(orientations, dtype=np.float64).reshape(-1, 4))
+        self.nbodies = loc.shape[0]
+        if quat.shape[0] != self.nbodies:
+            raise ValueError("one quaternion per body is required")
+        if isinstance(reference, (list, tuple)):
+            refs = [np.asarray(rf, dtype=np.float64).reshape(-1, 3)[:, :3] for rf in reference]
+            if len(refs) != self.nbodies:
+                raise ValueError("one reference configuration per body is required")
+        else:
+            rf = np.asarray(reference, dtype=np.float64).reshape(-1, 3)
+            refs = [rf] * self.nbodies
+        sizes = np.array([rf.shape[0] for rf in refs], dtype=np.int64)
+        self.body_offsets = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64))
+        self.Nblobs = int(self.body_offsets[-1])
+        self.size = 3 * self.Nblobs + 6 * self.nbodies
+        self.eta, self.a = float(eta), float(a)
+        self.L = np.zeros(3) if periodic_length is None else np.asarray(periodic_length, dtype=np.float64)
+        self.precision = precision
+        dev = self.device
+        self.centres = torch.from_numpy(loc).to(dev)
+        self.quats = torch.from_numpy(quat).to(dev)
+        self.ref = torch.from_numpy(np.ascontiguousarray(np.vstack(refs))).to(dev)
+        nbytes, nminv, nscr = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.check(self.lib.rmw_rigid_workspace_size(self._offp(), self.nbodies, ctypes.byref(nbytes),
+                                                     ctypes.byref(nminv), ctypes.byref(nscr)))
+        self.ws = torch.empty(max(nbytes.value, 256), dtype=torch.uint8, device=dev)
+        self.minv_doubles = nminv.value
+        self.pc_scratch_doubles = nscr.value
+        _lib.check(self.lib.rmw_rigid_setup(self._offp(), self.nbodies, self.ws.data_ptr(), self.ws.numel(),
+                                            _stream()))
+        self.r_blobs = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
+        self.r_rel = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
+        _lib.check(self.lib.rmw_body_blobs(self.centres.data_ptr(), self.quats.data_ptr(), self.ref.data_ptr(),
+                                           self._offp(), self.nbodies, self.r_blobs.data_ptr(),
+                                           self.r_rel.data_ptr(), self.ws.data_ptr(), self.ws.numel(), _stream()))
+        self._mlam = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
+        self.minv = None
+
+    def _offp(self):
+        return self.body_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+    # -- blob mobility and operator -------------------------------------------------
+    def mobility_times(self, lam: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
+        """out = B M(z_eff) B lam on all blobs (wall-RPY UF block)."""
+        wall_rpy.apply(RMW_UF, self.r_blobs, force=lam, eta=self.eta, a=self.a, periodic_length=self.L,
+                       precision=precision or self.precision, out_velocity=out.view(1, self.Nblobs, 3))
+
+    def operator(self, x: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
+        """out = [M lambda - K U; -K^T lambda]."""
+        self.mobility_times(x[: 3 * self.Nblobs], self._mlam, precision)
+        _lib.check(self.lib.rmw_rigid_operator_post(self.r_rel.data_ptr(), self._offp(), self.nbodies,
+                                                    x.data_ptr(), self._mlam.data_ptr(), out.data_ptr(),
+                                                    self.ws.data_ptr(), self.ws.numel(), _stream()))
+
+    # -- preconditioner ---------------------------------------------------------------
+    def build_preconditioner(self):
+        dev = self.device
+        self.minv = torch.empty(max(self.minv_doubles, 1), dtype=torch.float64, device=dev)
+        self.minvk = torch.empty((self.Nblobs * 3, 6), dtype=torch.float64, device=dev)
+        self.nbody = torch.empty((self.nbodies, 6, 6), dtype=torch.float64, device=dev)
+        status = torch.zeros(self.nbodies, dtype=torch.int32, device=dev)
+        scratch = (torch.empty(self.pc_scratch_doubles, dtype=torch.float64, device=dev)
+                   if self.pc_scratch_doubles else None)
+        _lib.check(self.lib.rmw_rigid_pc_build(self.r_blobs.data_ptr(), self.r_rel.data_ptr(), self._offp(),
+                                               self.nbodies, self.eta, self.a, self.minv.data_ptr(),
+                                               self.minvk.data_ptr(), self.nbody.data_ptr(), status.data_ptr(),
+                                               scratch.data_ptr() if scratch is not None else None,
+                                               self.pc_scratch_doubles, self.ws.data_ptr(), self.ws.numel(),
+                                               _stream()))
+        del scratch
+        bad = int(status.sum().item())
+        if bad:
+            raise RuntimeError(f"{bad} body mobility blocks are not positive definite")
+
+    def pc_apply(self, x: torch.Tensor, out: torch.Tensor):
+        if self.minv is None:
+            self.build_preconditioner()
+        _lib.check(self.lib.rmw_rigid_pc_apply(self._offp(), self.nbodies, self.minv.data_ptr(),
+                                               self.minvk.data_ptr(), self.nbody.data_ptr(),
+                                               self.r_rel.data_ptr(), x.data_ptr(), out.data_ptr(),
+                                               self.ws.data_ptr(), self.ws.numel(), _stream()))
+
+    # -- solve ------------------------------------------------------------------------
+    def rhs(self, slip, force_torque) -> torch.Tensor:
+        """[slip; -force_torque] as in solve_mobility_problem (no constraints)."""
+        s = torch.as_tensor(slip, dtype=torch.float64).reshape(-1)
+        ft = torch.as_tensor(force_torque, dtype=torch.float64).reshape(-1)
+        if s.numel() != 3 * self.Nblobs or ft.numel() != 6 * self.nbodies:
+            raise ValueError("slip must be (Nblobs, 3) and force_torque (Nbodies, 6)")
+        return torch.cat([s.to(self.device), -ft.to(self.device)])
+
+    def solve_mobility_problem(self, slip=None, force_torque=None, rhs: Optional[torch.Tensor] = None,
+                               tol: float = 1e-8, restart: int = 60, maxiter: int = 1000,
+                               x0: Optional[torch.Tensor] = None, precision: Optional[str] = None,
+                               callback=None) -> krylov.GmresResult:
+        """Solve [M -K; -K^T 0][lambda; U] = [slip; -F] to relative residual tol.
+
+        The RHS is normalised to unit norm and the solution rescaled, as in
+        quaternion_integrator_multi_bodies.py:1515-1536; x0 (unnormalised)
+        is used as a true warm start.  Returns a GmresResult whose x is
+        [lambda; U] (U_b = (v_b, w_b))."""
+        if rhs is None:
+            if slip is None:
+                slip = np.zeros((self.Nblobs, 3))
+            if force_torque is None:
+                force_torque = np.zeros((self.nbodies, 6))
+            rhs = self.rhs(slip, force_torque)
+        ops = krylov.KrylovOps(self.size, 2, self.device)
+        rnorm = ops.norm(rhs)
+        if rnorm == 0.0:
+            return krylov.GmresResult(x=torch.zeros_like(rhs), info=0, iterations=0, matvecs=0, final_residual=0.0)
+        b = torch.empty_like(rhs)
+        ops.axpby(1.0 / rnorm, rhs, 0.0, b)
+        xw = None
+        if x0 is not None:
+            xw = torch.empty_like(rhs)
+            ops.axpby(1.0 / rnorm, x0.to(self.device, torch.float64).reshape(-1), 0.0, xw)
+        if self.minv is None:
+            self.build_preconditioner()
+        prec = precision or self.precision
+        res = krylov.gmres(lambda v, o: self.operator(v, o, prec), b, Pinv=self.pc_apply, x0=xw, tol=tol,
+                           restart=restart, maxiter=maxiter, callback=callback)
+        out = torch.empty_like(rhs)
+        ops.axpby(rnorm, res.x, 0.0, out)
+        res.x = out
+        return res
+
+    def velocities(self, sol: torch.Tensor) -> np.ndarray:
+        """(Nbodies, 6) body velocities [v, w] from a solution vector."""
+        return sol[3 * self.Nblobs:].reshape(self.nbodies, 6).cpu().numpy()
+
+    def constraint_forces(self, sol: torch.Tensor) -> np.ndarray:
+        return sol[: 3 * self.Nblobs].reshape(self.Nblobs, 3).cpu().numpy()

@@ -1,0 +1,109 @@
+"""GPU parity of the rigid-body path (FP64 operator, body preconditioner,
+GMRES) against the oracle and the reference's golden solve outputs."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+import golden_cases as G
+from rigidmultiblobswall_amd import mobility as mob
+from rigidmultiblobswall_amd import workloads
+from rigidmultiblobswall_amd.multi_bodies import set_mobility_blobs, set_mobility_vector_prod
+from rigidmultiblobswall_amd.rigid import RigidSuspension
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _small_spheres(nb=16, seed=21):
+    w = workloads.c2_rigid_spheres(seed=seed, nbodies=nb)
+    return w
+
+
+def _oracle_for(w):
+    bodies = [(c, q, w["ref"]) for c, q in zip(w["centres"], w["quats"])]
+    return O.RigidSystem(bodies, w["eta"], w["a"])
+
+
+def test_dense_wall_mobility_matches_oracle(gpu):
+    rng = np.random.default_rng(3)
+    n, a = 60, 0.13
+    r = np.column_stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(0.05, 0.8, n)])
+    print(f"case dense N={n} (with z < a)", flush=True)
+    M = set_mobility_blobs("hip")(r, 0.9, a)
+    R = O.single_wall_fluid_mobility(r, 0.9, a)
+    assert np.abs(M - R).max() <= 1e-13 * np.abs(R).max()
+
+
+def test_body_blobs_operator_and_pc(gpu):
+    w = _small_spheres()
+    print("case 16 spheres x 42 blobs: blobs, operator, PC", flush=True)
+    S = RigidSuspension(w["centres"], w["quats"], w["ref"], w["eta"], w["a"], precision="fp64")
+    Og = _oracle_for(w)
+    assert np.abs(S.r_blobs.cpu().numpy() - Og.r).max() < 1e-13
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(S.size)
+    xd = torch.from_numpy(x).to(gpu)
+    y = torch.empty_like(xd)
+    S.operator(xd, y)
+    assert rel(y.cpu().numpy(), Og.apply(x)) <= 1e-12
+    S.build_preconditioner()
+    Og.build_pc()
+    S.pc_apply(xd, y)
+    assert rel(y.cpu().numpy(), Og.pc_apply(x)) <= 1e-11
+    # the body blocks are the exact inverse of the isolated-body operator
+    Minv0 = S.minv[: (3 * 42) ** 2].reshape(126, 126).cpu().numpy()
+    assert np.abs(Minv0 - Og.Minv[0]).max() <= 1e-10 * np.abs(Og.Minv[0]).max()
+    assert np.abs(S.nbody[0].cpu().numpy() - Og.N[0]).max() <= 1e-10 * np.abs(Og.N[0]).max()
+
+
+@pytest.mark.parametrize("res,tol", [("low", 1e-8), ("mid", 1e-8), ("high", 5e-8)])
+def test_pair_active_rods_golden_gpu(gpu, res, tol):
+    c = G.load_case(res)
+    print(f"case pair_active_rods {res}: fp64 GMRES tol {c['tol']}", flush=True)
+    S = RigidSuspension(c["locations"], c["quats"], c["ref"], c["eta"], c["a"], precision="fp64")
+    out = S.solve_mobility_problem(c["slip"], c["force_torque"], tol=c["tol"])
+    assert out.info == 0
+    vel = S.velocities(out.x)
+    err = rel(vel, c["expected"])
+    print(f"  iterations {out.iterations}, rel err vs reference file {err:.2e}")
+    assert err < tol
+    # FP32 pair terms: same solve, velocities within the FP32 budget
+    out32 = S.solve_mobility_problem(c["slip"], c["force_torque"], tol=1e-7, precision="fp32")
+    assert rel(S.velocities(out32.x), c["expected"]) < 1e-6
+
+
+def test_c2_solve_residual_and_warm_start(gpu):
+    w = workloads.c2_rigid_spheres()
+    print("case C2 256 spheres x 42 blobs: fp64 GMRES to 1e-8", flush=True)
+    S = RigidSuspension(w["centres"], w["quats"], w["ref"], w["eta"], w["a"], precision="fp64")
+    slip = np.zeros((S.Nblobs, 3))
+    out = S.solve_mobility_problem(slip, w["force_torque"], tol=1e-8)
+    assert out.info == 0
+    rhs = S.rhs(slip, w["force_torque"])
+    r = torch.empty_like(rhs)
+    S.operator(out.x, r)
+    true_res = float(torch.linalg.norm(r - rhs) / torch.linalg.norm(rhs))
+    print(f"  iterations {out.iterations}, true residual {true_res:.2e}")
+    assert true_res <= 1.5e-8
+    # warm start from the solution: converged at once
+    again = S.solve_mobility_problem(slip, w["force_torque"], tol=1e-8, x0=out.x)
+    assert again.info == 0 and again.iterations <= 1
+    # warm start from a perturbed solution: fewer iterations than cold
+    rng = np.random.default_rng(5)
+    pert = out.x * (1 + 1e-4 * torch.from_numpy(rng.standard_normal(S.size)).to(gpu))
+    warm = S.solve_mobility_problem(slip, w["force_torque"], tol=1e-8, x0=pert)
+    assert warm.info == 0 and warm.iterations < out.iterations
+
+
+def test_dispatch_names(gpu):
+    f = set_mobility_vector_prod("hip_fp64")
+    rng = np.random.default_rng(6)
+    r = np.column_stack([rng.uniform(0, 5, 50), rng.uniform(0, 5, 50), rng.uniform(0.3, 2, 50)])
+    v = rng.standard_normal(150)
+    assert rel(f(r, v, 1.0, 0.2), O.uf(r, v, 1.0, 0.2)) <= 1e-12
+    with pytest.raises(ValueError):
+        set_mobility_vector_prod("numba")
