@@ -65,8 +65,9 @@ int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
  *     omega    = WF * force + WT * torque
  * with the wall-overlap regularisation B M(z_eff) B of the reference
  * wrappers (z_eff = max(z, a), B_ii = z_i/a below z = a).  r_vectors (n,3);
- * force/torque (k,n,3) or NULL when no block of the mask reads them;
- * velocity/omega (k,n,3) or NULL when no block writes them.  L[3] are the
+ * force/torque: k right-hand sides of (n,3), `ld` doubles apart (0 = 3n),
+ * or NULL when no block of the mask reads them; velocity/omega likewise
+ * (same ld), or NULL when no block writes them.  L[3] are the
  * pseudo-periodic lengths (host array; NULL or entries <= 0 = unbounded).
  *
  * Replaces, for mask = RMW_UF / RMW_UT / RMW_WF / RMW_WT:
@@ -80,7 +81,7 @@ int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
  */
 int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double* force,
                        const double* torque, double* velocity, double* omega, int64_t n, int k,
-                       double eta, double a, const double* L, const rmw_tuning* tuning,
+                       int64_t ld, double eta, double a, const double* L, const rmw_tuning* tuning,
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /*
@@ -96,8 +97,8 @@ int rmw_wall_rpy_ragged_workspace_size(int mask, int prec, const int64_t* offset
                                        int k, const rmw_tuning* tuning, size_t* bytes);
 int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vectors, const double* force,
                               const double* torque, double* velocity, double* omega,
-                              const int64_t* offsets, int nsys, int k, double eta, double a,
-                              const double* L, const rmw_tuning* tuning, void* workspace,
+                              const int64_t* offsets, int nsys, int k, int64_t ld, double eta,
+                              double a, const double* L, const rmw_tuning* tuning, void* workspace,
                               size_t workspace_bytes, void* stream);
 
 /*
@@ -185,6 +186,21 @@ int rmw_axpby(double alpha, const double* x, double beta, double* y, int64_t n, 
  * h[0..j] = coefficients, h[j+1] = ||w||, V[j+1] = w / ||w||. */
 int rmw_arnoldi_cgs2(double* V, int64_t ldv, int j, double* w, int64_t n, double* h,
                      double* hscratch, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Lanczos square root (FP64).  For k tridiagonals T_r of order m whose
+ * entries the Lanczos steps left in H (alpha_i = H_r[i*ldh + i], beta_i =
+ * H_r[i*ldh + i + 1], H_r = H + r*h_rhs_stride), writes
+ *   c_out[r*c_stride + j] = scale[r] * (sqrt+(T_r) e_1)_j,   j < m,
+ * sqrt+ taking the square root of the non-negative eigenvalues (negative
+ * ones -> 0) of an implicit-QL eigen-decomposition; status[r] = 1 when QL
+ * did not converge.  This is the eigh step of stochastic_forcing_lanczos,
+ * stochastic_forcing/stochastic_forcing.py:208-227 (noise = V c).
+ * ---------------------------------------------------------------------- */
+int rmw_lanczos_scratch_doubles(int m, int k, size_t* doubles);
+int rmw_lanczos_sqrt_coeffs(const double* H, int64_t h_rhs_stride, int64_t ldh, int m, int k,
+                            const double* scale, double* c_out, int64_t c_stride, double* scratch,
+                            size_t scratch_doubles, int* status, void* stream);
 
 /* Device properties used for launch sizing and roofline reporting. */
 int rmw_device_info(int* cu_count, int* clock_khz, int* wavefront, char* arch_name, int arch_len);

@@ -53,11 +53,11 @@ PROTOTYPES = {
     "rmw_version": (ctypes.c_char_p, []),
     "rmw_wall_rpy_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, _szp]),
     "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,
-                                          ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz, _vp]),
+                                          _i64, ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz, _vp]),
     "rmw_wall_rpy_ragged_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,
                                                           ctypes.c_int, _tp, _szp]),
     "rmw_wall_rpy_ragged_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64p,
-                                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_int, ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double,
                                                  _hdp, _tp, _vp, _sz, _vp]),
     "rmw_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "rmw_profile_collect": (ctypes.c_int, [_i64p, _hdp, _hdp]),
@@ -80,6 +80,10 @@ PROTOTYPES = {
     "rmw_combination": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _dp, _i64, ctypes.c_int, _vp]),
     "rmw_axpby": (ctypes.c_int, [ctypes.c_double, _dp, ctypes.c_double, _dp, _i64, _dp, _vp]),
     "rmw_arnoldi_cgs2": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _i64, _dp, _dp, _vp, _sz, _vp]),
+    # Lanczos
+    "rmw_lanczos_scratch_doubles": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _szp]),
+    "rmw_lanczos_sqrt_coeffs": (ctypes.c_int, [_dp, _i64, _i64, ctypes.c_int, ctypes.c_int, _dp, _dp, _i64, _dp,
+                                               _sz, _vp, _vp]),
 }
 
 _lock = threading.Lock()

@@ -820,9 +820,10 @@ static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
 
 static void run_plan(const Plan& pl, const double* r, const double* f, const double* t, double* vel,
-                     double* omg, int k, double eta, double a, const double* L,
+                     double* omg, int k, int64_t ld, double eta, double a, const double* L,
                      const rmw_tuning* tuning, void* ws, size_t ws_bytes, hipStream_t st, bool ragged)
 {
+  RMW_REQUIRE(ld == 0 || ld >= 3 * pl.ntot, "ld must be 0 or >= 3 n");
   RMW_REQUIRE(ws_bytes >= pl.total, "workspace too small: need " + std::to_string(pl.total) + " bytes");
   if (pl.ntot == 0) return;
   RMW_REQUIRE(r != nullptr && ws != nullptr, "r_vectors / workspace is null");
@@ -878,7 +879,7 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
 
   for (int k0 = 0, K = group_k(k); k0 < k; k0 += K) {
     K = group_k(k - k0);
-    const int64_t stride = 3 * pl.ntot;
+    const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
     PrepParams pp{};
     pp.r = r;
     pp.f = NF ? f + (size_t)k0 * stride : nullptr;
@@ -975,13 +976,13 @@ extern "C" int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
 
 extern "C" int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double* force,
                                   const double* torque, double* velocity, double* omega, int64_t n, int k,
-                                  double eta, double a, const double* L, const rmw_tuning* tuning,
+                                  int64_t ld, double eta, double a, const double* L, const rmw_tuning* tuning,
                                   void* workspace, size_t workspace_bytes, void* stream)
 {
   return guarded([&] {
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
-    run_plan(pl, r_vectors, force, torque, velocity, omega, k, eta, a, L, tuning, workspace,
+    run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
   });
 }
@@ -999,15 +1000,15 @@ extern "C" int rmw_wall_rpy_ragged_workspace_size(int mask, int prec, const int6
 
 extern "C" int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vectors, const double* force,
                                          const double* torque, double* velocity, double* omega,
-                                         const int64_t* offsets, int nsys, int k, double eta, double a,
-                                         const double* L, const rmw_tuning* tuning, void* workspace,
+                                         const int64_t* offsets, int nsys, int k, int64_t ld, double eta,
+                                         double a, const double* L, const rmw_tuning* tuning, void* workspace,
                                          size_t workspace_bytes, void* stream)
 {
   return guarded([&] {
     RMW_REQUIRE(offsets != nullptr && nsys >= 1, "bad offsets");
     std::vector<int64_t> off(offsets, offsets + nsys + 1);
     Plan pl = make_plan(mask, prec, off, k, tuning, true);
-    run_plan(pl, r_vectors, force, torque, velocity, omega, k, eta, a, L, tuning, workspace,
+    run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, true);
   });
 }

@@ -1,0 +1,47 @@
+"""Single-blob rollers: the grand (translation + rotation) mobility of the
+rollers integrator and its Brownian slip by block Lanczos.
+
+Reference: quaternion_integrator/quaternion_integrator_rollers.py, the
+grand_mobility_matrix closure (:1114-1122) -- four separate block products
+UF, UT, WF, WT per application -- and compute_stochastic_velocity (:1082-
+1201), which draws factor * N^1/2 z by stochastic_forcing_lanczos.  Here one
+fused launch evaluates all four blocks for k right-hand sides at once.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import lanczos, wall_rpy
+from ._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
+
+GRAND = RMW_UF | RMW_UT | RMW_WF | RMW_WT
+
+
+class Rollers:
+    def __init__(self, r_vectors, eta: float, a: float, periodic_length=None, precision: str = "fp32_blocked",
+                 device=None):
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.r = torch.as_tensor(np.asarray(r_vectors, dtype=np.float64).reshape(-1, 3)
+                                 if not torch.is_tensor(r_vectors) else r_vectors).to(self.device,
+                                                                                      torch.float64).contiguous()
+        self.n = self.r.shape[0]
+        self.eta, self.a = float(eta), float(a)
+        self.L = periodic_length
+        self.precision = precision
+
+    def grand_matvec(self, X: torch.Tensor, Y: torch.Tensor, precision=None):
+        """Y[r] = [UF UT; WF WT] X[r] for X, Y (k, 6N) with X[r] = [f; t]."""
+        k = X.shape[0]
+        n3 = 3 * self.n
+        ld = X.stride(0)
+        assert Y.stride(0) == ld and X.is_contiguous() and Y.is_contiguous()
+        wall_rpy.apply_strided(GRAND, self.r, self.n, k, ld, X.data_ptr(), X.data_ptr() + 8 * n3, Y.data_ptr(),
+                               Y.data_ptr() + 8 * n3, self.eta, self.a, self.L, precision or self.precision)
+
+    def brownian_noise(self, Z: torch.Tensor, factor: float = 1.0, tolerance: float = 1e-3, max_iter: int = 1000,
+                       precision=None):
+        """factor * N^1/2 Z[r] for the k rows of Z (k, 6N), one fused grand
+        matvec per Lanczos step for all rows."""
+        return lanczos.lanczos_block(lambda X, Y: self.grand_matvec(X, Y, precision), Z, factor=factor,
+                                     tolerance=tolerance, max_iter=max_iter)

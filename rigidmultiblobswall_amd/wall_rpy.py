@@ -94,12 +94,28 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     _lib.check(lib.rmw_wall_rpy_apply(
         mask, prec, r.data_ptr(), f.data_ptr() if f is not None else None,
         t.data_ptr() if t is not None else None, u.data_ptr() if u is not None else None,
-        w.data_ptr() if w is not None else None, n, k, float(eta), float(a), _L_array(periodic_length),
+        w.data_ptr() if w is not None else None, n, k, 0, float(eta), float(a), _L_array(periodic_length),
         tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
     if k == 1:
         u = u.view(n, 3) if u is not None else None
         w = w.view(n, 3) if w is not None else None
     return u, w
+
+
+def apply_strided(mask: int, r: torch.Tensor, n: int, k: int, ld: int, force_ptr, torque_ptr, vel_ptr, omg_ptr,
+                  eta: float, a: float, periodic_length=None, precision="fp32", tuning=None):
+    """Low-level product on raw device pointers with k right-hand sides `ld`
+    doubles apart (e.g. force and torque halves of stacked 6N vectors)."""
+    lib = _lib.load()
+    device = _device()
+    prec = _lib.precision_code(precision)
+    tp = _lib.tuning_ptr(tuning)
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value, device)
+    _lib.check(lib.rmw_wall_rpy_apply(mask, prec, r.data_ptr(), force_ptr, torque_ptr, vel_ptr, omg_ptr, n, k, ld,
+                                      float(eta), float(a), _L_array(periodic_length), tp, ws.data_ptr(),
+                                      ws.numel(), _stream_ptr(device)))
 
 
 def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torque=None, eta: float = 1.0,
@@ -127,7 +143,7 @@ def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torqu
     _lib.check(lib.rmw_wall_rpy_ragged_apply(
         mask, prec, r.data_ptr(), f.data_ptr() if f is not None else None,
         t.data_ptr() if t is not None else None, u.data_ptr() if u is not None else None,
-        w.data_ptr() if w is not None else None, offp, nsys, k, float(eta), float(a),
+        w.data_ptr() if w is not None else None, offp, nsys, k, 0, float(eta), float(a),
         _L_array(periodic_length), tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
     if k == 1:
         u = u.view(n, 3) if u is not None else None
