@@ -202,6 +202,45 @@ int rmw_lanczos_sqrt_coeffs(const double* H, int64_t h_rhs_stride, int64_t ldh, 
                             const double* scale, double* c_out, int64_t c_stride, double* scratch,
                             size_t scratch_doubles, int* status, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned GMRES warm start (no reference counterpart: the network only
+ * supplies the initial guess x0 of the rigid-body solve, GMRES to the same
+ * tolerance stays the judge).  BF16 operands are uint16 arrays.
+ * ---------------------------------------------------------------------- */
+/* C = op(A) op(B) + bias (+ beta C), H = act(op(A) op(B) + bias) in BF16;
+ * op(A) is M x K (A stored K x M when transA), op(B) is K x N (B stored N x K
+ * when transB); FP32 accumulation on MFMA (v_mfma_f32_16x16x16bf16_1k).
+ * C or H may be NULL; act 0 = identity, 1 = SiLU. */
+int rmw_gemm_bf16(int transA, int transB, int M, int N, int K, const uint16_t* A, int64_t lda,
+                  const uint16_t* B, int64_t ldb, float* C, int64_t ldc, float beta, uint16_t* H,
+                  int64_t ldh, const float* bias, int act, void* stream);
+int rmw_f32_to_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
+/* dZ = dH * SiLU'(Z), BF16 out */
+int rmw_silu_backward(const float* dH, const float* Z, uint16_t* dZ, int64_t n, void* stream);
+/* out[j] = sum_i X[i, j] (+ beta out[j]) */
+int rmw_colsum_bf16(const uint16_t* X, int64_t ldx, int rows, int cols, float* out, float beta,
+                    void* stream);
+/* node input rows [Xn | sum of the deg edge messages | 0] and its adjoint */
+int rmw_gnn_gather(const float* Xn, int nin, const float* Msg, int nmsg, int deg, int nb, uint16_t* Xc,
+                   int ldc, void* stream);
+int rmw_gnn_scatter(const float* dXc, int ldc, int nin, int nmsg, int deg, int nb, uint16_t* dE,
+                    void* stream);
+/* weighted mean-square error: per-block partial losses and dY = 2 w (Y - T) / rows */
+int rmw_mse(const float* Y, const float* T, const float* w, int rows, int cols, uint16_t* dY,
+            float* loss_part, int nparts, void* stream);
+/* Adam step (bias-corrected, step >= 1) on FP32 weights; wb (BF16 copy) may be NULL */
+int rmw_adam(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n, float lr, float b1,
+             float b2, float eps, int step, float weight_decay, void* stream);
+/* deg nearest neighbours of every body centre (brute force) */
+int rmw_gnn_knn(const double* centres, int nb, int deg, int* nbr, void* stream);
+/* node (10) and edge (16 per neighbour) features in each body's frame */
+int rmw_gnn_features(const double* centres, const double* quats, const double* force_torque,
+                     const int* nbr, int nb, int deg, double fscale, float* Xn, float* Xe, void* stream);
+/* mode 0: T = standardised body-frame (sol - pc) / fscale; mode 1: sol = pc + R (T sd + mean) fscale */
+int rmw_gnn_body_frame(const double* quats, const int64_t* body_offsets_dev, int nb, int64_t nblobs,
+                       int blobs_per_body, double* sol, const double* pc, float* T, const float* mean,
+                       const float* sd, double fscale, int mode, void* stream);
+
 /* Device properties used for launch sizing and roofline reporting. */
 int rmw_device_info(int* cu_count, int* clock_khz, int* wavefront, char* arch_name, int arch_len);
 

@@ -80,6 +80,24 @@ PROTOTYPES = {
     "rmw_combination": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _dp, _i64, ctypes.c_int, _vp]),
     "rmw_axpby": (ctypes.c_int, [ctypes.c_double, _dp, ctypes.c_double, _dp, _i64, _dp, _vp]),
     "rmw_arnoldi_cgs2": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _i64, _dp, _dp, _vp, _sz, _vp]),
+    # learned warm start
+    "rmw_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _i64,
+                                     _dp, _i64, _dp, _i64, ctypes.c_float, _dp, _i64, _dp, ctypes.c_int, _vp]),
+    "rmw_f32_to_bf16": (ctypes.c_int, [_dp, _dp, _i64, _vp]),
+    "rmw_silu_backward": (ctypes.c_int, [_dp, _dp, _dp, _i64, _vp]),
+    "rmw_colsum_bf16": (ctypes.c_int, [_dp, _i64, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_float, _vp]),
+    "rmw_gnn_gather": (ctypes.c_int, [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                      ctypes.c_int, _vp]),
+    "rmw_gnn_scatter": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _dp, _vp]),
+    "rmw_mse": (ctypes.c_int, [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _vp]),
+    "rmw_adam": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_int, ctypes.c_float, _vp]),
+    "rmw_gnn_knn": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _vp]),
+    "rmw_gnn_features": (ctypes.c_int, [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                        _vp]),
+    "rmw_gnn_body_frame": (ctypes.c_int, [_dp, _dp, ctypes.c_int, _i64, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                          ctypes.c_double, ctypes.c_int, _vp]),
     # Lanczos
     "rmw_lanczos_scratch_doubles": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _szp]),
     "rmw_lanczos_sqrt_coeffs": (ctypes.c_int, [_dp, _i64, _i64, ctypes.c_int, ctypes.c_int, _dp, _dp, _i64, _dp,

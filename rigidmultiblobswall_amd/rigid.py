@@ -173,6 +173,54 @@ class RigidSuspension:
         res.x = out
         return res
 
+    def solve_mixed(self, slip=None, force_torque=None, rhs: Optional[torch.Tensor] = None, tol: float = 1e-8,
+                    restart: int = 60, maxiter: int = 1000, x0: Optional[torch.Tensor] = None,
+                    inner_precision: str = "fp32_blocked", inner_floor: float = 2e-7, max_outer: int = 8):
+        """Mixed-precision solve to the FP64 tolerance: FP64 residuals
+        r = b - A_64 x (outer), GMRES on A_32 d = r with FP32-pair products
+        and FP64 Krylov algebra (inner), x += d.  The inner tolerance asks for
+        the reduction still needed, floored at `inner_floor` (the FP32-pair
+        operator's accuracy).  Returns a GmresResult whose iterations and
+        matvecs sum over the inner solves; final_residual is the FP64 one."""
+        if rhs is None:
+            rhs = self.rhs(np.zeros((self.Nblobs, 3)) if slip is None else slip,
+                           np.zeros((self.nbodies, 6)) if force_torque is None else force_torque)
+        if self.minv is None:
+            self.build_preconditioner()
+        ops = krylov.KrylovOps(self.size, 2, self.device)
+        bnorm = ops.norm(rhs)
+        x = torch.zeros_like(rhs) if x0 is None else x0.to(self.device, torch.float64).reshape(-1).clone()
+        r = torch.empty_like(rhs)
+        out = krylov.GmresResult(x=x, info=0, iterations=0, matvecs=0)
+        if bnorm == 0.0:
+            out.final_residual = 0.0
+            return out
+        for outer in range(max_outer + 1):
+            if x0 is None and outer == 0:
+                ops.axpby(1.0, rhs, 0.0, r)
+                rel = 1.0
+            else:
+                self.operator(x, r, precision="fp64")
+                out.matvecs += 1
+                ops.axpby(1.0, rhs, -1.0, r)
+                rel = ops.norm(r) / bnorm
+            out.final_residual = rel
+            out.residuals.append(rel)
+            if rel <= tol or outer == max_outer:
+                out.info = 0 if rel <= tol else out.iterations
+                break
+            inner_tol = max(0.5 * tol / rel, inner_floor)
+            rn = rel * bnorm
+            rs = torch.empty_like(r)
+            ops.axpby(1.0 / rn, r, 0.0, rs)
+            res = krylov.gmres(lambda v, o: self.operator(v, o, inner_precision), rs, Pinv=self.pc_apply, tol=inner_tol,
+                               restart=restart, maxiter=maxiter)
+            out.iterations += res.iterations
+            out.matvecs += res.matvecs
+            ops.axpby(rn, res.x, 1.0, x)
+        out.x = x
+        return out
+
     def velocities(self, sol: torch.Tensor) -> np.ndarray:
         """(Nbodies, 6) body velocities [v, w] from a solution vector."""
         return sol[3 * self.Nblobs:].reshape(self.nbodies, 6).cpu().numpy()
