@@ -188,6 +188,46 @@ int rmw_arnoldi_cgs2(double* V, int64_t ldv, int j, double* w, int64_t n, double
                      double* hscratch, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Segmented (ragged-batch) FP64 Krylov kernels: the vector algebra of many
+ * independent suspensions stacked in one vector, one launch sequence per
+ * step for the whole batch.  System s owns blobs [offsets[s], offsets[s+1])
+ * (offsets: DEVICE int64 array of nsys + 1 entries, nblobs = offsets[nsys]);
+ * a vector holds nparts parts of stride part_stride with comp values per
+ * blob.  The reference runs one suspension per process, so these replace
+ * calling the per-system algebra of stochastic_forcing_lanczos
+ * (stochastic_forcing/stochastic_forcing.py:112-262) once per replica.
+ * Results are bit-reproducible (fixed-order two-pass reductions).
+ * ---------------------------------------------------------------------- */
+typedef struct rmw_segments {
+  const int64_t* offsets; /* device, nsys + 1 */
+  int32_t nsys;
+  int32_t comp;           /* values per blob in each part (3) */
+  int32_t nparts;         /* 2 for [force; torque] grand vectors */
+  int32_t reserved;
+  int64_t nblobs;         /* offsets[nsys], known to the host */
+  int64_t part_stride;    /* element offset between parts (>= comp * nblobs) */
+} rmw_segments;
+
+int rmw_seg_workspace_size(const rmw_segments* seg, int max_vectors, size_t* bytes);
+/* out[s*ldo + j] = (V_j . w) restricted to system s (sqrt when sqrt_out) */
+int rmw_seg_dot_many(const double* V, int64_t ldv, int nvec, const double* w, const rmw_segments* seg,
+                     double* out, int64_t ldo, int sqrt_out, void* workspace, size_t workspace_bytes,
+                     void* stream);
+/* y|s = sum_j coef[s*ldc + j] V_j|s (+ y|s when accumulate) */
+int rmw_seg_combination(const double* V, int64_t ldv, int nvec, const double* coef, int64_t ldc, double* y,
+                        const rmw_segments* seg, int accumulate, void* stream);
+/* y|s = x|s / div[s*ld_div] (a non-positive divisor writes e_0 of system s) */
+int rmw_seg_scale(const double* x, double* y, const double* div, int64_t ld_div, const rmw_segments* seg,
+                  void* stream);
+/* Arnoldi/Lanczos step with CGS2 for every system at once: with
+ * h_s = H + s*h_sys_stride, h_s[0..j] = coefficients, h_s[j+1] = ||w|s||,
+ * V[j+1]|s = w|s / ||w|s|| (e_0 of the system when zero); hscratch holds
+ * nsys*(j+1) doubles. */
+int rmw_seg_arnoldi_cgs2(double* V, int64_t ldv, int j, double* w, const rmw_segments* seg, double* H,
+                         int64_t h_sys_stride, double* hscratch, void* workspace, size_t workspace_bytes,
+                         void* stream);
+
+/* ------------------------------------------------------------------------
  * Lanczos square root (FP64).  For k tridiagonals T_r of order m whose
  * entries the Lanczos steps left in H (alpha_i = H_r[i*ldh + i], beta_i =
  * H_r[i*ldh + i + 1], H_r = H + r*h_rhs_stride), writes

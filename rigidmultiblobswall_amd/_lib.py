@@ -38,6 +38,12 @@ class Tuning(ctypes.Structure):
     _fields_ = [("chunk_len", ctypes.c_int64), ("max_wgs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class Segments(ctypes.Structure):
+    _fields_ = [("offsets", ctypes.c_void_p), ("nsys", ctypes.c_int32), ("comp", ctypes.c_int32),
+                ("nparts", ctypes.c_int32), ("reserved", ctypes.c_int32), ("nblobs", ctypes.c_int64),
+                ("part_stride", ctypes.c_int64)]
+
+
 _vp = ctypes.c_void_p
 _dp = ctypes.c_void_p  # device pointers are passed as integers
 _i64 = ctypes.c_int64
@@ -46,6 +52,7 @@ _sz = ctypes.c_size_t
 _szp = ctypes.POINTER(ctypes.c_size_t)
 _tp = ctypes.POINTER(Tuning)
 _hdp = ctypes.POINTER(ctypes.c_double)
+_sgp = ctypes.POINTER(Segments)
 
 # name -> (restype, argtypes); every symbol declared in include/rmw_hip.h.
 PROTOTYPES = {
@@ -80,6 +87,12 @@ PROTOTYPES = {
     "rmw_combination": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _dp, _i64, ctypes.c_int, _vp]),
     "rmw_axpby": (ctypes.c_int, [ctypes.c_double, _dp, ctypes.c_double, _dp, _i64, _dp, _vp]),
     "rmw_arnoldi_cgs2": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _i64, _dp, _dp, _vp, _sz, _vp]),
+    # segmented (ragged-batch) Krylov kernels
+    "rmw_seg_workspace_size": (ctypes.c_int, [_sgp, ctypes.c_int, _szp]),
+    "rmw_seg_dot_many": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _sgp, _dp, _i64, ctypes.c_int, _vp, _sz, _vp]),
+    "rmw_seg_combination": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _i64, _dp, _sgp, ctypes.c_int, _vp]),
+    "rmw_seg_scale": (ctypes.c_int, [_dp, _dp, _dp, _i64, _sgp, _vp]),
+    "rmw_seg_arnoldi_cgs2": (ctypes.c_int, [_dp, _i64, ctypes.c_int, _dp, _sgp, _dp, _i64, _dp, _vp, _sz, _vp]),
     # learned warm start
     "rmw_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _i64,
                                      _dp, _i64, _dp, _i64, ctypes.c_float, _dp, _i64, _dp, ctypes.c_int, _vp]),

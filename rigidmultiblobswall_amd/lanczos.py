@@ -113,6 +113,153 @@ def lanczos_block(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch
     return noise, int(it_conv.max())
 
 
+class Segments:
+    """A ragged batch of independent suspensions for the segmented kernels:
+    system s owns blobs [offsets[s], offsets[s+1]); a batch vector holds
+    `nparts` parts of comp * nblobs values (for the grand mobility the
+    forces of all blobs, then their torques)."""
+
+    def __init__(self, offsets, comp: int = 3, nparts: int = 2, device=None):
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        if off.ndim != 1 or off.size < 2 or off[0] != 0 or np.any(np.diff(off) < 0):
+            raise ValueError("offsets must start at 0 and be non-decreasing")
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.offsets = off
+        self.nsys = off.size - 1
+        self.nblobs = int(off[-1])
+        self.comp, self.nparts = comp, nparts
+        self.part_stride = comp * self.nblobs
+        self.dim = nparts * self.part_stride
+        self.off_dev = torch.from_numpy(off).to(self.device)
+        self.c = _lib.Segments(self.off_dev.data_ptr(), self.nsys, comp, nparts, 0, self.nblobs, self.part_stride)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.c)
+
+    def workspace(self, max_vectors: int) -> torch.Tensor:
+        nb = ctypes.c_size_t()
+        _lib.check(_lib.load().rmw_seg_workspace_size(self.ref, max_vectors, ctypes.byref(nb)))
+        return torch.empty(max(nb.value, 8), dtype=torch.uint8, device=self.device)
+
+    def dot(self, x: torch.Tensor, y: torch.Tensor, sqrt: bool = False) -> torch.Tensor:
+        """Per-system x|s . y|s (nsys,) on the device."""
+        ws = self.workspace(1)
+        out = torch.empty(self.nsys, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().rmw_seg_dot_many(x.data_ptr(), 0, 1, y.data_ptr(), self.ref, out.data_ptr(), 1,
+                                                int(sqrt), ws.data_ptr(), ws.numel(), _stream()))
+        return out
+
+    def element_system(self) -> np.ndarray:
+        """System index of every element of a batch vector (host, for checks)."""
+        sys_of_blob = np.repeat(np.arange(self.nsys), np.diff(self.offsets))
+        return np.tile(np.repeat(sys_of_blob, self.comp), self.nparts)
+
+
+def lanczos_ragged(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch.Tensor, seg: Segments,
+                   factor: float = 1.0, tolerance: float = 1e-6, max_iter: int = 1000):
+    """factor * M_s^1/2 z_s for every system s of a ragged batch at once.
+
+    Z (k, seg.dim): k batch vectors; matvec(X, Y) writes Y[r] = M X[r] for
+    all systems (one ragged launch).  Every (row, system) pair is its own
+    Lanczos recurrence with its own convergence test, as the reference runs
+    one suspension at a time (stochastic_forcing.py:112-262); the Gram-
+    Schmidt, norms and scalings of all of them are the segmented kernels,
+    one launch sequence per step for the whole batch.  A system's
+    coefficients are frozen at the step its test passes, so its noise is
+    the one a separate run would return.  Returns (noise (k, dim),
+    iterations (k, nsys))."""
+    lib = _lib.load()
+    if Z.dim() == 1:
+        Z = Z.reshape(1, -1)
+    k, dim = Z.shape
+    if dim != seg.dim:
+        raise ValueError(f"Z has {dim} columns, the batch needs {seg.dim}")
+    dev = Z.device
+    nsys = seg.nsys
+    R = k * nsys
+    sizes = np.diff(seg.offsets)
+    if factor == 0.0:
+        return torch.zeros((k, dim), dtype=torch.float64, device=dev), np.zeros((k, nsys), dtype=int)
+    mmax = max_iter + 2
+    if mmax > 1024:
+        raise ValueError("max_iter above 1022 is not supported")
+    mcap = min(mmax, 32)
+    V = torch.empty((mcap, k, dim), dtype=torch.float64, device=dev)
+    H = torch.zeros((k, nsys, mcap, mcap + 1), dtype=torch.float64, device=dev)
+    W = torch.empty((k, dim), dtype=torch.float64, device=dev)
+    ws = seg.workspace(mmax)
+    hs = torch.empty(nsys * (mmax + 1), dtype=torch.float64, device=dev)
+    zsrc = Z.to(torch.float64).contiguous()
+    znorm = torch.empty((k, nsys), dtype=torch.float64, device=dev)
+    for r in range(k):
+        _lib.check(lib.rmw_seg_dot_many(zsrc[r].data_ptr(), 0, 1, zsrc[r].data_ptr(), seg.ref, znorm[r].data_ptr(), 1,
+                                        1, ws.data_ptr(), ws.numel(), _stream()))
+        _lib.check(lib.rmw_seg_scale(zsrc[r].data_ptr(), V[0, r].data_ptr(), znorm[r].data_ptr(), 1, seg.ref,
+                                     _stream()))
+    zn = znorm.cpu().numpy()
+    live = np.broadcast_to(sizes > 0, (k, nsys))
+    if np.any((zn == 0.0) & live):
+        raise ValueError("a zero right-hand side has no Lanczos basis")
+    scale = (znorm * factor).reshape(R).contiguous()
+    status = torch.zeros(R, dtype=torch.int32, device=dev)
+    nscr = ctypes.c_size_t()
+    converged = ~live.copy()
+    it_conv = np.where(live, max_iter, 0)
+    c_final = np.zeros((k, nsys, mmax))
+    m_used = np.ones((k, nsys), dtype=int)
+    c_prev = None
+    i_last = 0
+    for i in range(max_iter + 1):
+        if i + 2 > mcap:                      # grow the basis and H geometrically
+            m2 = min(mmax, 2 * mcap)
+            V2 = torch.empty((m2, k, dim), dtype=torch.float64, device=dev)
+            V2[:mcap] = V
+            H2 = torch.zeros((k, nsys, m2, m2 + 1), dtype=torch.float64, device=dev)
+            H2[:, :, :mcap, :mcap + 1] = H
+            V, H, mcap = V2, H2, m2
+        matvec(V[i], W)
+        for r in range(k):
+            _lib.check(lib.rmw_seg_arnoldi_cgs2(V[0, r].data_ptr(), k * dim, i, W[r].data_ptr(), seg.ref,
+                                                H[r, 0, i].data_ptr(), H.stride(1), hs.data_ptr(), ws.data_ptr(),
+                                                ws.numel(), _stream()))
+        m = i + 1
+        _lib.check(lib.rmw_lanczos_scratch_doubles(m, R, ctypes.byref(nscr)))
+        zscr = torch.empty(max(nscr.value, 1), dtype=torch.float64, device=dev)
+        C = torch.empty((R, m), dtype=torch.float64, device=dev)
+        _lib.check(lib.rmw_lanczos_sqrt_coeffs(H.data_ptr(), H.stride(1), H.stride(2), m, R, scale.data_ptr(),
+                                               C.data_ptr(), C.stride(0), zscr.data_ptr(), nscr.value,
+                                               status.data_ptr(), _stream()))
+        c_now = C.cpu().numpy().reshape(k, nsys, m)
+        if int(status.sum().item()):
+            raise RuntimeError("tridiagonal eigen-solve did not converge")
+        i_last = i
+        if i > 0:
+            prev = np.concatenate([c_prev, np.zeros((k, nsys, 1))], axis=2)
+            diff = np.linalg.norm(c_now - prev, axis=2)
+            den = np.maximum(np.linalg.norm(c_prev, axis=2), np.finfo(float).eps)
+            newly = (~converged) & (diff / den < tolerance)
+            for r, s in zip(*np.nonzero(newly)):
+                c_final[r, s, :m] = c_now[r, s]
+                m_used[r, s] = m
+                it_conv[r, s] = i
+            converged |= newly
+        c_prev = c_now
+        if converged.all():
+            break
+    # systems that never passed keep their last coefficients (as the reference)
+    for r, s in zip(*np.nonzero(~converged)):
+        c_final[r, s, :i_last + 1] = c_prev[r, s]
+        m_used[r, s] = i_last + 1
+    mfin = int(m_used.max())
+    coef = torch.from_numpy(np.ascontiguousarray(c_final[:, :, :mfin])).to(dev)
+    noise = torch.empty((k, dim), dtype=torch.float64, device=dev)
+    for r in range(k):
+        _lib.check(lib.rmw_seg_combination(V[0, r].data_ptr(), k * dim, mfin, coef[r].data_ptr(), mfin,
+                                           noise[r].data_ptr(), seg.ref, 0, _stream()))
+    return noise, it_conv
+
+
 def stochastic_forcing_lanczos(factor=1.0, tolerance=1e-06, max_iter=1000, dim=None, mobility=None,
                                mobility_mult=None, L_mult=None, z=None, print_residual=False):
     """Drop-in for stochastic_forcing.stochastic_forcing_lanczos

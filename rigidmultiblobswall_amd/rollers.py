@@ -45,3 +45,51 @@ class Rollers:
         matvec per Lanczos step for all rows."""
         return lanczos.lanczos_block(lambda X, Y: self.grand_matvec(X, Y, precision), Z, factor=factor,
                                      tolerance=tolerance, max_iter=max_iter)
+
+
+class RaggedRollers:
+    """A batch of independent roller suspensions stored back to back (system
+    s owns rollers [offsets[s], offsets[s+1])).  The grand product of every
+    system is one persistent ragged launch, and the Brownian slip of all
+    systems advances together through the segmented Lanczos kernels.
+
+    Batch vectors are (k, 6 N_total): the forces of all rollers, then their
+    torques (system s's part of each is contiguous)."""
+
+    def __init__(self, r_vectors, offsets, eta: float, a: float, periodic_length=None,
+                 precision: str = "fp32_blocked", device=None, tuning=None):
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.offsets = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        self.r = torch.as_tensor(np.asarray(r_vectors, dtype=np.float64).reshape(-1, 3)
+                                 if not torch.is_tensor(r_vectors) else r_vectors).to(self.device,
+                                                                                      torch.float64).contiguous()
+        self.n = self.r.shape[0]
+        if self.n != int(self.offsets[-1]):
+            raise ValueError("offsets[-1] must equal the number of rollers")
+        self.eta, self.a = float(eta), float(a)
+        self.L = periodic_length
+        self.precision = precision
+        self.tuning = tuning
+        self.seg = lanczos.Segments(self.offsets, comp=3, nparts=2, device=self.device)
+
+    def grand_matvec(self, X: torch.Tensor, Y: torch.Tensor, precision=None):
+        """Y[r] = blockdiag_s([UF UT; WF WT]_s) X[r] for X, Y (k, 6N)."""
+        k = X.shape[0]
+        n3 = 3 * self.n
+        ld = X.stride(0)
+        assert Y.stride(0) == ld and X.is_contiguous() and Y.is_contiguous()
+        wall_rpy.apply_ragged_strided(GRAND, self.r, self.offsets, k, ld, X.data_ptr(), X.data_ptr() + 8 * n3,
+                                      Y.data_ptr(), Y.data_ptr() + 8 * n3, self.eta, self.a, self.L,
+                                      precision or self.precision, tuning=self.tuning)
+
+    def brownian_noise(self, Z: torch.Tensor, factor: float = 1.0, tolerance: float = 1e-3, max_iter: int = 1000,
+                       precision=None):
+        """factor * N_s^1/2 z_s for every system; returns (noise (k, 6N),
+        iterations (k, nsys))."""
+        return lanczos.lanczos_ragged(lambda X, Y: self.grand_matvec(X, Y, precision), Z, self.seg, factor=factor,
+                                      tolerance=tolerance, max_iter=max_iter)
+
+    def system_slices(self, s: int):
+        """Index arrays of system s's forces and torques in a batch vector."""
+        b0, b1 = int(self.offsets[s]), int(self.offsets[s + 1])
+        return np.r_[3 * b0:3 * b1, 3 * self.n + 3 * b0:3 * self.n + 3 * b1]

@@ -151,6 +151,25 @@ def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torqu
     return u, w
 
 
+def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int, ld: int, force_ptr, torque_ptr,
+                         vel_ptr, omg_ptr, eta: float, a: float, periodic_length=None, precision="fp32_blocked",
+                         tuning=None):
+    """Ragged product on raw device pointers, k right-hand sides `ld` doubles
+    apart (force and torque parts of stacked grand vectors of a batch)."""
+    lib = _lib.load()
+    device = _device()
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+    prec = _lib.precision_code(precision)
+    tp = _lib.tuning_ptr(tuning)
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, off.size - 1, k, tp, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value, device, slot="ragged")
+    _lib.check(lib.rmw_wall_rpy_ragged_apply(mask, prec, r.data_ptr(), force_ptr, torque_ptr, vel_ptr, omg_ptr, offp,
+                                             off.size - 1, k, ld, float(eta), float(a), _L_array(periodic_length),
+                                             tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
+
+
 def device_info() -> dict:
     lib = _lib.load()
     _device()
