@@ -3,7 +3,8 @@
 One step = one wall-corrected RPY translational product u = B M_UF(z_eff) B f
 over the rigid-multiblob suspension of the rigid-solve configuration
 (256 spheres x 42 blobs = 10752 blobs, blob a = 0.243), FP32 pair terms
-with FP64 per-target sums, inputs resident in HBM.  This is the product
+with FP32 sub-sums folded into FP64 per-target sums (--precision), inputs
+resident in HBM.  This is the product
 that every GMRES iteration of that solve applies.
 
 Prints ONE JSON line (rank 0):
@@ -14,6 +15,9 @@ Prints ONE JSON line (rank 0):
                PMC runs when present
   cpu_baseline FP64 C oracle (-O3 -ffast-math, OpenMP over all usable host
                cores) on a bounded sample of target rows of the same workload
+  gmres        rigid solve to 1e-8 on the 1024-sphere monolayer: plain FP64
+               GMRES from x0 = 0 against the learned warm start (network
+               trained on the GPU from a short trajectory), per-solve times
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
 """
@@ -33,6 +37,10 @@ sys.path.insert(0, REPO)
 METRIC = "wall-RPY matvec blob-pairs/s; GMRES time-to-1e-8 residual (learned x0 vs plain)"
 FLOP_PER_PAIR = {"UF": 164, "UT": 95, "WF": 95, "WT": 101, "grand": 386}
 PEAK_FP32_TFLOPS = 157.3
+PRECISION_NOTE = {"fp32": " (FP32 pair terms, FP64 per-target sums)",
+                  "fp32_blocked": " (FP32 pair terms, FP32 sub-sums of <= 16 sources folded into FP64 per-target sums)",
+                  "fp32_kahan": " (FP32 pair terms, compensated FP32 per-target sums)",
+                  "fp64": " (FP64 throughout)"}
 PEAK_FP64_TFLOPS = 78.6
 
 
@@ -52,7 +60,7 @@ def load_workload(name: str):
     return w, desc
 
 
-def cpu_baseline(r, f, eta, a, target_seconds=12.0):
+def cpu_baseline(r, f, eta, a, target_seconds=16.0):
     """Time the FP64 oracle (fast-math build, all usable cores) on a
     bounded sample of the same matvec (whole products repeated, or a row
     block when one product is longer than the budget); returns pairs/s."""
@@ -75,9 +83,46 @@ def cpu_baseline(r, f, eta, a, target_seconds=12.0):
                       f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads; {dt:.1f} s"}
 
 
-def load_traffic(workload: str):
-    """HBM bytes per pair-kernel launch from the committed rocprof PMC summary."""
-    path = os.path.join(REPO, "profiles", f"pair_kernel_uf_{workload}_traffic.json")
+def gmres_leg(steps: int, epochs: int, evals: int, log):
+    """Rigid-body solve to 1e-8 on the learned-warm-start monolayer (1024
+    spheres x 42 blobs): a short Brownian trajectory generates the training
+    solves, the graph network is trained on the GPU, and held-out steps are
+    solved from x0 = 0 with plain FP64 GMRES and from the network's x0 with
+    FP64 GMRES and with mixed precision (FP32-pair inner GMRES, FP64
+    residual refinement).  Every solve's returned x is checked against the
+    FP64 operator; times include the network's inference."""
+    from rigidmultiblobswall_amd import warmstart, workloads
+    from rigidmultiblobswall_amd.trajectory import run_c4_trajectory
+    w = workloads.c4_monolayer()
+    t0 = time.perf_counter()
+    traj = run_c4_trajectory(w["centres"], w["quats"], w["ref"], w["a"], w["eta"], nsteps=steps)
+    t_traj = time.perf_counter() - t0
+    log(f"gmres leg: trajectory {steps} steps in {t_traj:.1f} s")
+    ntr = int(0.75 * steps)
+    ws = warmstart.WarmStart(w["ref"], w["a"], w["eta"])
+    fit = ws.fit(traj.snapshots[:ntr], epochs=epochs)
+    log(f"gmres leg: trained {fit['steps']} Adam steps in {fit['train_seconds']:.1f} s, loss {fit['loss_history'][-1]:.3e}")
+    held = traj.snapshots[ntr:]
+    picks = np.linspace(0, len(held) - 1, min(evals, len(held))).astype(int)
+    rows = warmstart.evaluate(ws, [held[i] for i in picks], [traj.snapshots[ntr + i - 1] for i in picks])
+    sm = warmstart.summarise(rows)
+    pick = lambda k: {"ms": sm[k]["total_ms"], "iterations": sm[k]["iterations"],  # noqa: E731
+                      "x0_rel_err": sm[k]["x0_rel_err"], "converged": sm[k]["all_converged"]}
+    out = {"workload": f"C4 monolayer: 1024 spheres x 42 blobs = 43008 blobs, tol 1e-8, {steps}-step trajectory, "
+                       f"trained on {ntr} steps, {len(picks)} held-out solves",
+           "plain_fp64": pick("zero/fp64"), "learned_fp64": pick("gnn/fp64"), "learned_mixed": pick("gnn/mixed"),
+           "zero_mixed": pick("zero/mixed"), "previous_step_mixed": pick("previous/mixed"),
+           "gnn_inference_ms": float(np.mean([r["gnn_inference_ms"] for r in rows])),
+           "train_seconds": fit["train_seconds"], "parameters": fit["parameters"]}
+    out["speedup_learned_vs_plain"] = out["plain_fp64"]["ms"] / out["learned_mixed"]["ms"]
+    out["speedup_learned_fp64_vs_plain"] = out["plain_fp64"]["ms"] / out["learned_fp64"]["ms"]
+    return out
+
+
+def load_traffic(workload: str, precision: str):
+    """HBM bytes per pair-kernel launch from the committed rocprof PMC summary
+    of the same workload and precision (None when there is none)."""
+    path = os.path.join(REPO, "profiles", f"pair_kernel_uf_{workload}_{precision}_traffic.json")
     if os.path.exists(path):
         with open(path) as fh:
             return json.load(fh).get("traffic_bytes")
@@ -90,8 +135,14 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="fp32_blocked")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-gmres", action="store_true", help="skip the learned-warm-start GMRES leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a streaming copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--gmres-steps", type=int, default=64)
+    ap.add_argument("--gmres-epochs", type=int, default=40)
+    ap.add_argument("--gmres-evals", type=int, default=3)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -103,6 +154,21 @@ def main(argv=None):
     from rigidmultiblobswall_amd._lib import RMW_UF
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        # plain streaming copy of known size (y = 1 * x, beta = 0) for scaling
+        # the FETCH_SIZE / WRITE_SIZE counters of the traffic runs
+        from rigidmultiblobswall_amd import _lib
+        nbytes = int(args.calibrate_copy) << 20
+        x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        lib = _lib.load()
+        for _ in range(5):
+            _lib.check(lib.rmw_axpby(1.0, x.data_ptr(), 0.0, y.data_ptr(), x.numel(), None,
+                                     torch.cuda.current_stream().cuda_stream))
+        torch.cuda.synchronize()
+        assert bool(torch.equal(x, y))
+        print(json.dumps({"calibrate_copy_bytes": nbytes}), flush=True)
+        return 0
     dev = torch.device("cuda", 0)
     w, desc = load_workload(args.workload)
     r_h, eta, a = w["r"], w["eta"], w["a"]
@@ -148,15 +214,19 @@ def main(argv=None):
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": desc, "blobs": n, "pairs_per_step": pairs,
-                   "precision": args.precision + (" (FP32 pair terms, FP64 per-target sums)"
-                                                  if args.precision == "fp32" else ""),
+                   "precision": args.precision + PRECISION_NOTE.get(args.precision, ""),
                    "device": info},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": load_traffic(args.workload),
-                     "kernel": "pair_kernel_f32<UF,K=1>" if args.precision != "fp64" else "pair_kernel_f64<UF,K=1>",
+                     "frac": achieved / peak, "traffic": load_traffic(args.workload, args.precision),
+                     "kernel": (f"pair_kernel_f32<UF,K=1,{args.precision}>" if args.precision != "fp64"
+                                else "pair_kernel_f64<UF,K=1>"),
                      "kernel_mean_ms": pk.mean_ms, "kernel_launches": pk.launches,
                      "flop_per_pair": FLOP_PER_PAIR["UF"]},
     }
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    log(f"matvec: {ms:.4f} ms/step, {value:.4g} pairs/s, pair kernel {pk.mean_ms:.4f} ms")
+    if not args.no_gmres:
+        out["gmres"] = gmres_leg(args.gmres_steps, args.gmres_epochs, args.gmres_evals, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(r_h, f_h, eta, a)
     print(json.dumps(out), flush=True)
