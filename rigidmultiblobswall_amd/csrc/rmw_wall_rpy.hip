@@ -2,32 +2,30 @@
 //
 // One call evaluates any subset of the grand mobility blocks
 // [[UF, UT], [WF, WT]] for k right-hand sides on one suspension, or on a
-// ragged batch of independent suspensions, as
+// ragged batch of independent suspensions, in two launches:
 //
-//   1. extent   : max |coordinate| / a over all blobs (one workgroup)
-//   2. prepare  : z_eff = max(z, a), damping B = min(z/a, 1) applied to the
-//                 sources, 1/(8 pi eta a^p) folded into the sources,
-//                 coordinates split on a power-of-two grid (FP32 modes)
-//   3. pair     : all-pairs sums; each workgroup owns 256 targets (one per
-//                 lane, in registers) and one chunk of sources that it
-//                 streams through LDS in tiles of 256; the RPY and wall
-//                 image terms are fused in one pass and the geometry is
-//                 shared by all blocks and right-hand sides.  Partial sums
-//                 (FP64) go to the workspace, one slot per (chunk, target).
-//   4. reduce   : fixed-order FP64 sum over chunks, FP64 self term, B and
-//                 the remaining 1/a factor, written as (k, n, 3) float64.
+//   1. pair   : all-pairs sums; each workgroup owns 256 targets (one per
+//               lane, in registers) and one chunk of sources that it
+//               streams through LDS in tiles of 256, reading positions and
+//               sources straight from the caller's float64 arrays: z_eff =
+//               max(z, a), damping B = min(z/a, 1) and 1/(8 pi eta a^p) are
+//               applied as a tile is staged.  The RPY and wall image terms
+//               are fused in one pass and the geometry is shared by all
+//               blocks and right-hand sides.  Partial sums (FP64) go to the
+//               workspace, one slot per (chunk, target).
+//   2. reduce : fixed-order FP64 sum over chunks, FP64 self term, B and
+//               the remaining 1/a factor, written as (k, n, 3) float64.
 //
 // Reference: mobility/mobility_numba.py:123-287 (UF), :547-686 (UT),
 // :937-1073 (WF), :1188-1328 (WT); wrappers mobility/mobility.py:52-84,
 // 1132-1342; pseudo-periodic images mobility_numba.py:149-198.
 //
-// FP32 modes.  Subtracting two FP32 coordinates of size X loses
-// ~6e-8 X of absolute accuracy, which for near pairs (|d| ~ a, X up to
-// ~500 a) would be a 3e-5 relative error.  Each coordinate is therefore
-// stored as hi + lo with hi on the grid 2^-q (q chosen so that every hi,
-// every difference of two hi and every periodic shift fits 24 bits):
-// hi_i - hi_j is then exact and d = (hi_i - hi_j) + (lo_i - lo_j) carries
-// FP32 relative accuracy in d itself.
+// FP32 modes.  Subtracting two FP32 coordinates of size X loses ~6e-8 X of
+// absolute accuracy, which for near pairs (|d| ~ a, X up to ~500 a) would be
+// a 3e-5 relative error.  The difference d = r_i - r_j (and its periodic
+// shifts) is therefore formed in FP64 and rounded once to FP32, so d
+// carries FP32 relative accuracy in d itself; everything after that is
+// FP32.  (On gfx950 an FP64 add issues at the rate of an unpacked FP32 add.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,13 +81,12 @@ struct SysInfo {
 };
 
 struct PairParams {
-  const float4* pos_hi;   // FP32 modes: (x_hi, y_hi, z_hi, z) / a
-  const float4* pos_lo;   // FP32 modes: (x_lo, y_lo, z_lo, 0) / a
-  const double4* pos64;   // FP64 mode : (x, y, z, 0) / a
-  const void* frc;        // [K][ntot] float4 / double4, scaled sources
-  const void* trq;
+  const double* r;        // (ntot, 3) positions
+  const double* f;        // K right-hand sides rhs_stride apart, or null
+  const double* t;
+  int64_t rhs_stride;
+  double a, inva, c1, c2; // c1 = 1/(8 pi eta a), c2 = c1 / a
   double* partial;
-  const double* extent;   // device word written by extent_kernel
   const SysInfo* table;   // nullptr -> single
   SysInfo single;
   int nsys;
@@ -98,16 +95,6 @@ struct PairParams {
   int periodic;           // bit d set when L_d > 0
   double L[3];            // periodic lengths / a
 };
-
-// Grid exponent q for the hi/lo split of coordinates of size <= xmax.
-__host__ __device__ inline int split_exponent(double xmax)
-{
-  // every |value| * 2^q must stay below 2^23 for differences and shifts
-  double v = 4.0 * xmax + 1e-300;
-  int e = ilogb(v) + 1;  // v < 2^e
-  int q = 23 - e;
-  return q > 60 ? 60 : (q < -60 ? -60 : q);
-}
 
 __device__ inline SysInfo find_sys_item(const PairParams& p, int64_t item)
 {
@@ -121,106 +108,27 @@ __device__ inline SysInfo find_sys_item(const PairParams& p, int64_t item)
 }
 
 // ---------------------------------------------------------------------------
-// 1. extent: max |x|, |y|, |z_eff| over all blobs, in units of a.
-//    Non-negative doubles order like their bit patterns, so the maximum is
-//    an integer atomicMax on the 64-bit pattern; a one-thread fill kernel
-//    clears the word first (a kernel rather than a memset so that the
-//    sequence can be captured in a graph).
+// Blob staging: scaled position (x/a, y/a, z_eff/a) with z_eff = max(z, a)
+// (shift_heights, mobility.py:52-64) and damping B = min(z/a, 1)
+// (damping_matrix_B, mobility.py:67-84).
 // ---------------------------------------------------------------------------
-__global__ void extent_clear_kernel(unsigned long long* word, double Lmax)
+__device__ __forceinline__ void load_blob(const double* __restrict__ r, int64_t g, double a, double inva,
+                                          double& X, double& Y, double& Z, double& B)
 {
-  *word = (unsigned long long)__double_as_longlong(Lmax > 0.0 ? Lmax : 0.0);
+  const double x = r[3 * g], y = r[3 * g + 1], z = r[3 * g + 2];
+  const double ze = z <= a ? a : z;
+  X = x * inva;
+  Y = y * inva;
+  Z = ze * inva;
+  B = z < a ? z * inva : 1.0;
 }
 
-__global__ __launch_bounds__(256) void extent_kernel(const double* __restrict__ r, int64_t ntot, double inva,
-                                                     double a, unsigned long long* __restrict__ word)
-{
-  __shared__ double red[256];
-  double m = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ntot; i += (int64_t)gridDim.x * blockDim.x) {
-    double z = r[3 * i + 2];
-    double ze = z <= a ? a : z;
-    m = fmax(m, fabs(r[3 * i]));
-    m = fmax(m, fabs(r[3 * i + 1]));
-    m = fmax(m, fabs(ze));
-  }
-  red[threadIdx.x] = m * inva;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicMax(word, (unsigned long long)__double_as_longlong(red[0]));
-}
-
-// ---------------------------------------------------------------------------
-// 2. prepare
-// ---------------------------------------------------------------------------
-struct PrepParams {
-  const double* r;
-  const double* f;        // (K, ntot, 3) or null
-  const double* t;
-  float4* pos_hi; float4* pos_lo; double4* pos64;
-  void* frc; void* trq;
-  const double* extent;
-  int64_t ntot;
-  int K;                  // right-hand sides in this group
-  int64_t rhs_stride;     // doubles between consecutive RHS of f/t (= 3 ntot)
-  double a, inva, c1, c2;
-  int prec;
-  int do_pos;
+// Source record of the FP32 kernels: FP64 position for the difference, FP32
+// height for the wall terms (two 16-byte LDS reads).
+struct __align__(16) SrcF32 {
+  double x, y, z;
+  float zf, pad;
 };
-
-__device__ inline void split_coord(double X, double scale, double inv_scale, float& hi, float& lo)
-{
-  double h = rint(X * scale) * inv_scale;
-  hi = (float)h;
-  lo = (float)(X - h);
-}
-
-__global__ __launch_bounds__(256) void prepare_kernel(PrepParams p)
-{
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.ntot) return;
-  double x = p.r[3 * i], y = p.r[3 * i + 1], z = p.r[3 * i + 2];
-  double ze = z <= p.a ? p.a : z;          // shift_heights, mobility.py:52-64
-  double B = z < p.a ? z * p.inva : 1.0;   // damping_matrix_B, mobility.py:67-84
-  double X = x * p.inva, Y = y * p.inva, Z = ze * p.inva;
-  if (p.do_pos) {
-    if (p.prec == RMW_PREC_F64) {
-      p.pos64[i] = make_double4(X, Y, Z, 0.0);
-    } else {
-      int q = split_exponent(p.extent[0]);
-      double scale = ldexp(1.0, q), inv_scale = ldexp(1.0, -q);
-      float xh, xl, yh, yl, zh, zl;
-      split_coord(X, scale, inv_scale, xh, xl);
-      split_coord(Y, scale, inv_scale, yh, yl);
-      split_coord(Z, scale, inv_scale, zh, zl);
-      p.pos_hi[i] = make_float4(xh, yh, zh, (float)Z);
-      p.pos_lo[i] = make_float4(xl, yl, zl, 0.0f);
-    }
-  }
-  for (int k = 0; k < p.K; ++k) {
-    if (p.f) {
-      const double* v = p.f + k * p.rhs_stride + 3 * i;
-      double s = p.c1 * B;
-      if (p.prec == RMW_PREC_F64)
-        ((double4*)p.frc)[k * p.ntot + i] = make_double4(s * v[0], s * v[1], s * v[2], 0.0);
-      else
-        ((float4*)p.frc)[k * p.ntot + i] =
-            make_float4((float)(s * v[0]), (float)(s * v[1]), (float)(s * v[2]), 0.0f);
-    }
-    if (p.t) {
-      const double* v = p.t + k * p.rhs_stride + 3 * i;
-      double s = p.c2 * B;
-      if (p.prec == RMW_PREC_F64)
-        ((double4*)p.trq)[k * p.ntot + i] = make_double4(s * v[0], s * v[1], s * v[2], 0.0);
-      else
-        ((float4*)p.trq)[k * p.ntot + i] =
-            make_float4((float)(s * v[0]), (float)(s * v[1]), (float)(s * v[2]), 0.0f);
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // 3. pair kernels
@@ -348,28 +256,66 @@ __device__ __forceinline__ double image_count(double d, double L)
   return trunc(d / L + 0.5 * s);
 }
 
+// Partial sums of one work item: slot (chunk, rhs, target) of the system.
+template <int MASK, int K, typename A>
+__device__ __forceinline__ void store_partials(const PairParams& p, const SysInfo& S, int64_t ch, int64_t il,
+                                               A (&u)[K][3], A (&w)[K][3])
+{
+  using BT = BlockTraits<MASK>;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double* P = p.partial + S.part0 + ((ch * K + k) * S.n + il) * BT::OC;
+    int o = 0;
+    if constexpr (BT::HU) {
+      P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
+      o = 3;
+    }
+    if constexpr (BT::HW) {
+      P[o] = w[k][0].get(); P[o + 1] = w[k][1].get(); P[o + 2] = w[k][2].get();
+    }
+  }
+}
+
+template <typename V> __device__ V make_v4(double x, double y, double z);
+template <> __device__ __forceinline__ float4 make_v4<float4>(double x, double y, double z)
+{
+  return make_float4((float)x, (float)y, (float)z, 0.0f);
+}
+template <> __device__ __forceinline__ double4 make_v4<double4>(double x, double y, double z)
+{
+  return make_double4(x, y, z, 0.0);
+}
+
+// Stage the sources of blob gj for K right-hand sides into LDS slot `slot`:
+// B f / (8 pi eta a) and B t / (8 pi eta a^2) as float4 or double4.
+template <int MASK, int K, typename V>
+__device__ __forceinline__ void stage_sources(const PairParams& p, int64_t gj, double B, int slot, V* s_f, V* s_t)
+{
+  using BT = BlockTraits<MASK>;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if constexpr (BT::NF) {
+      const double* v = p.f + k * p.rhs_stride + 3 * gj;
+      const double c = p.c1 * B;
+      s_f[k * kBlock + slot] = make_v4<V>(c * v[0], c * v[1], c * v[2]);
+    }
+    if constexpr (BT::NT) {
+      const double* v = p.t + k * p.rhs_stride + 3 * gj;
+      const double c = p.c2 * B;
+      s_t[k * kBlock + slot] = make_v4<V>(c * v[0], c * v[1], c * v[2]);
+    }
+  }
+}
+
 template <int MASK, int K, int ACC>
 __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
 {
   using BT = BlockTraits<MASK>;
-  __shared__ float4 s_hi[kBlock];
-  __shared__ float4 s_lo[kBlock];
+  __shared__ SrcF32 s_pos[kBlock];
   __shared__ float4 s_f[BT::NF ? K * kBlock : 1];
   __shared__ float4 s_t[BT::NT ? K * kBlock : 1];
-
-  // Periodic lengths split on the same grid as the coordinates.
-  float Lh[3] = {0.f, 0.f, 0.f}, Ll[3] = {0.f, 0.f, 0.f};
-  int pbox[3] = {0, 0, 0};
-  if (p.periodic) {
-    int q = split_exponent(p.extent[0]);
-    double scale = ldexp(1.0, q), inv_scale = ldexp(1.0, -q);
-    for (int d = 0; d < 3; ++d) {
-      if (p.periodic & (1 << d)) {
-        split_coord(p.L[d], scale, inv_scale, Lh[d], Ll[d]);
-        pbox[d] = 1;
-      }
-    }
-  }
+  int pbox[3];
+  for (int d = 0; d < 3; ++d) pbox[d] = (p.periodic >> d) & 1;
 
   for (int64_t item = blockIdx.x; item < p.total_items; item += gridDim.x) {
     const SysInfo S = find_sys_item(p, item);
@@ -379,9 +325,9 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
     const int64_t t0 = tt * kBlock;
     const int64_t il = t0 + threadIdx.x;
     const bool valid = il < S.n;
-    const int64_t gi = S.b0 + (valid ? il : 0);
-    const float4 ph = p.pos_hi[gi];
-    const float4 pl = p.pos_lo[gi];
+    double Xi, Yi, Zi, Bi;
+    load_blob(p.r, S.b0 + (valid ? il : 0), p.a, p.inva, Xi, Yi, Zi, Bi);
+    const float zi = (float)Zi;
 
     Acc<ACC> u[K][3], w[K][3];
 
@@ -392,13 +338,10 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
       __syncthreads();  // the previous tile has been consumed
       if ((int)threadIdx.x < cnt) {
         const int64_t gj = S.b0 + s0 + threadIdx.x;
-        s_hi[threadIdx.x] = p.pos_hi[gj];
-        s_lo[threadIdx.x] = p.pos_lo[gj];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if constexpr (BT::NF) s_f[k * kBlock + threadIdx.x] = ((const float4*)p.frc)[k * p.ntot + gj];
-          if constexpr (BT::NT) s_t[k * kBlock + threadIdx.x] = ((const float4*)p.trq)[k * p.ntot + gj];
-        }
+        double X, Y, Z, B;
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B);
+        s_pos[threadIdx.x] = SrcF32{X, Y, Z, (float)Z, 0.0f};
+        stage_sources<MASK, K, float4>(p, gj, B, threadIdx.x, s_f, s_t);
       }
       __syncthreads();
       // Tiles that contain this workgroup's own targets need the self mask.
@@ -406,11 +349,11 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
       const int self_j = (int)(il - s0);  // local source index of the target itself
       if (!p.periodic) {
         auto eval = [&](int j, bool masked) {
-          const float4 sh = s_hi[j], sl = s_lo[j];
-          float dx = (ph.x - sh.x) + (pl.x - sl.x);
-          float dy = (ph.y - sh.y) + (pl.y - sl.y);
-          float dz = (ph.z - sh.z) + (pl.z - sl.z);
-          pair_eval<MASK, K, ACC, float>(dx, dy, dz, ph.w, sh.w, s_f, s_t, j, masked && j == self_j, u, w);
+          const SrcF32 sp = s_pos[j];
+          const float dx = (float)(Xi - sp.x);
+          const float dy = (float)(Yi - sp.y);
+          const float dz = (float)(Zi - sp.z);
+          pair_eval<MASK, K, ACC, float>(dx, dy, dz, zi, sp.zf, s_f, s_t, j, masked && j == self_j, u, w);
         };
         int j = 0;
         if (diag) {
@@ -431,42 +374,27 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
         flush_all<K, ACC>(u, w);
       } else {
         for (int j = 0; j < cnt; ++j) {
-          const float4 sh = s_hi[j], sl = s_lo[j];
-          const float hx = ph.x - sh.x, hy = ph.y - sh.y, hz = ph.z - sh.z;
-          const float lx = pl.x - sl.x, ly = pl.y - sl.y, lz = pl.z - sl.z;
-          // Minimum image decided in FP64 on the accurate difference.
-          const float nx = pbox[0] ? (float)image_count((double)hx + (double)lx, p.L[0]) : 0.f;
-          const float ny = pbox[1] ? (float)image_count((double)hy + (double)ly, p.L[1]) : 0.f;
-          const float nz = pbox[2] ? (float)image_count((double)hz + (double)lz, p.L[2]) : 0.f;
+          const SrcF32 sp = s_pos[j];
+          // minimum image of the FP64 difference, then every box shift in FP64
+          double ddx = Xi - sp.x, ddy = Yi - sp.y, ddz = Zi - sp.z;
+          if (pbox[0]) ddx -= image_count(ddx, p.L[0]) * p.L[0];
+          if (pbox[1]) ddy -= image_count(ddy, p.L[1]) * p.L[1];
+          if (pbox[2]) ddz -= image_count(ddz, p.L[2]) * p.L[2];
           for (int bx = -pbox[0]; bx <= pbox[0]; ++bx)
             for (int by = -pbox[1]; by <= pbox[1]; ++by)
               for (int bz = -pbox[2]; bz <= pbox[2]; ++bz) {
-                const float sx = (float)bx - nx, sy = (float)by - ny, sz = (float)bz - nz;
-                float dx = (hx + sx * Lh[0]) + (lx + sx * Ll[0]);
-                float dy = (hy + sy * Lh[1]) + (ly + sy * Ll[1]);
-                float dz = (hz + sz * Lh[2]) + (lz + sz * Ll[2]);
+                const float dx = (float)(ddx + bx * p.L[0]);
+                const float dy = (float)(ddy + by * p.L[1]);
+                const float dz = (float)(ddz + bz * p.L[2]);
                 const bool central = (bx == 0) && (by == 0) && (bz == 0);
-                pair_eval<MASK, K, ACC, float>(dx, dy, dz, ph.w, sh.w, s_f, s_t, j,
+                pair_eval<MASK, K, ACC, float>(dx, dy, dz, zi, sp.zf, s_f, s_t, j,
                                                diag && central && (j == self_j), u, w);
               }
           flush_all<K, ACC>(u, w);
         }
       }
     }
-    if (valid) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        double* P = p.partial + S.part0 + ((ch * K + k) * S.n + il) * BT::OC;
-        int o = 0;
-        if constexpr (BT::HU) {
-          P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
-          o = 3;
-        }
-        if constexpr (BT::HW) {
-          P[o] = w[k][0].get(); P[o + 1] = w[k][1].get(); P[o + 2] = w[k][2].get();
-        }
-      }
-    }
+    if (valid) store_partials<MASK, K>(p, S, ch, il, u, w);
   }
 }
 
@@ -477,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
   __shared__ double4 s_pos[kBlock];
   __shared__ double4 s_f[BT::NF ? K * kBlock : 1];
   __shared__ double4 s_t[BT::NT ? K * kBlock : 1];
-  int pbox[3] = {0, 0, 0};
+  int pbox[3];
   for (int d = 0; d < 3; ++d) pbox[d] = (p.periodic >> d) & 1;
 
   for (int64_t item = blockIdx.x; item < p.total_items; item += gridDim.x) {
@@ -488,8 +416,8 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
     const int64_t t0 = tt * kBlock;
     const int64_t il = t0 + threadIdx.x;
     const bool valid = il < S.n;
-    const int64_t gi = S.b0 + (valid ? il : 0);
-    const double4 pi = p.pos64[gi];
+    double Xi, Yi, Zi, Bi;
+    load_blob(p.r, S.b0 + (valid ? il : 0), p.a, p.inva, Xi, Yi, Zi, Bi);
 
     Acc<RMW_PREC_F64> u[K][3], w[K][3];
     const int64_t s_begin = ch * S.chunk_len;
@@ -499,52 +427,34 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
       __syncthreads();
       if ((int)threadIdx.x < cnt) {
         const int64_t gj = S.b0 + s0 + threadIdx.x;
-        s_pos[threadIdx.x] = p.pos64[gj];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if constexpr (BT::NF) s_f[k * kBlock + threadIdx.x] = ((const double4*)p.frc)[k * p.ntot + gj];
-          if constexpr (BT::NT) s_t[k * kBlock + threadIdx.x] = ((const double4*)p.trq)[k * p.ntot + gj];
-        }
+        double X, Y, Z, B;
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B);
+        s_pos[threadIdx.x] = make_double4(X, Y, Z, 0.0);
+        stage_sources<MASK, K, double4>(p, gj, B, threadIdx.x, s_f, s_t);
       }
       __syncthreads();
       const bool diag = (s0 < t0 + kBlock) && (t0 < s0 + cnt);
       const int self_j = (int)(il - s0);
       for (int j = 0; j < cnt; ++j) {
         const double4 pj = s_pos[j];
-        double rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        double rx = Xi - pj.x, ry = Yi - pj.y, rz = Zi - pj.z;
         if (!p.periodic) {
-          pair_eval<MASK, K, RMW_PREC_F64, double>(rx, ry, rz, pi.z, pj.z, s_f, s_t, j,
-                                                   diag && j == self_j, u, w);
+          pair_eval<MASK, K, RMW_PREC_F64, double>(rx, ry, rz, Zi, pj.z, s_f, s_t, j, diag && j == self_j, u, w);
         } else {
-          const double nx = pbox[0] ? image_count(rx, p.L[0]) : 0.0;
-          const double ny = pbox[1] ? image_count(ry, p.L[1]) : 0.0;
-          const double nz = pbox[2] ? image_count(rz, p.L[2]) : 0.0;
-          const double bx0 = rx - nx * p.L[0], by0 = ry - ny * p.L[1], bz0 = rz - nz * p.L[2];
+          if (pbox[0]) rx -= image_count(rx, p.L[0]) * p.L[0];
+          if (pbox[1]) ry -= image_count(ry, p.L[1]) * p.L[1];
+          if (pbox[2]) rz -= image_count(rz, p.L[2]) * p.L[2];
           for (int bx = -pbox[0]; bx <= pbox[0]; ++bx)
             for (int by = -pbox[1]; by <= pbox[1]; ++by)
               for (int bz = -pbox[2]; bz <= pbox[2]; ++bz) {
                 const bool central = (bx == 0) && (by == 0) && (bz == 0);
-                pair_eval<MASK, K, RMW_PREC_F64, double>(
-                    bx0 + bx * p.L[0], by0 + by * p.L[1], bz0 + bz * p.L[2], pi.z, pj.z, s_f, s_t, j,
-                    diag && central && (j == self_j), u, w);
+                pair_eval<MASK, K, RMW_PREC_F64, double>(rx + bx * p.L[0], ry + by * p.L[1], rz + bz * p.L[2], Zi,
+                                                         pj.z, s_f, s_t, j, diag && central && (j == self_j), u, w);
               }
         }
       }
     }
-    if (valid) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        double* P = p.partial + S.part0 + ((ch * K + k) * S.n + il) * BT::OC;
-        int o = 0;
-        if constexpr (BT::HU) {
-          P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
-          o = 3;
-        }
-        if constexpr (BT::HW) {
-          P[o] = w[k][0].get(); P[o + 1] = w[k][1].get(); P[o + 2] = w[k][2].get();
-        }
-      }
-    }
+    if (valid) store_partials<MASK, K>(p, S, ch, il, u, w);
   }
 }
 
@@ -643,8 +553,7 @@ struct Plan {
   std::vector<SysInfo> sys;  // non-empty systems only
   int64_t total_items = 0;
   size_t partial_doubles = 0;
-  size_t off_extent = 0, off_hi = 0, off_lo = 0, off_pos64 = 0, off_frc = 0, off_trq = 0,
-         off_partial = 0, off_table = 0, total = 0;
+  size_t off_partial = 0, off_table = 0, total = 0;
 };
 
 static bool valid_mask(int mask) { return mask > 0 && mask < 16; }
@@ -706,19 +615,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   pl.total_items = item;
   pl.partial_doubles = (size_t)part;
 
-  const bool f32 = prec != RMW_PREC_F64;
-  const bool NF = (mask & (kUF | kWF)) != 0, NT = (mask & (kUT | kWT)) != 0;
   size_t o = 0;
-  pl.off_extent = o; o = align_up(o + 64);
-  if (f32) {
-    pl.off_hi = o; o = align_up(o + (size_t)pl.ntot * sizeof(float4));
-    pl.off_lo = o; o = align_up(o + (size_t)pl.ntot * sizeof(float4));
-  } else {
-    pl.off_pos64 = o; o = align_up(o + (size_t)pl.ntot * sizeof(double4));
-  }
-  const size_t vsz = f32 ? sizeof(float4) : sizeof(double4);
-  if (NF) { pl.off_frc = o; o = align_up(o + (size_t)pl.kmax * pl.ntot * vsz); }
-  if (NT) { pl.off_trq = o; o = align_up(o + (size_t)pl.kmax * pl.ntot * vsz); }
   pl.off_partial = o; o = align_up(o + pl.partial_doubles * sizeof(double));
   if (ragged) { pl.off_table = o; o = align_up(o + pl.sys.size() * sizeof(SysInfo)); }
   pl.total = o;
@@ -839,16 +736,13 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   RMW_REQUIRE(!(HW && omg == nullptr), "omega output is required by the mask");
 
   char* base = (char*)ws;
-  double* extent = (double*)(base + pl.off_extent);
   double Lp[3] = {0, 0, 0};
   int periodic = 0;
-  double Lmax = 0.0;
   for (int d = 0; d < 3; ++d) {
     double Ld = L ? L[d] : 0.0;
     if (Ld > 0.0) {
       periodic |= 1 << d;
       Lp[d] = Ld / a;
-      Lmax = std::max(Lmax, Lp[d]);
     }
   }
   const double inva = 1.0 / a;
@@ -861,15 +755,6 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     stage_upload(dtable, pl.sys.data(), pl.sys.size() * sizeof(SysInfo), st);
   }
 
-  const bool f32 = pl.prec != RMW_PREC_F64;
-  if (f32 || periodic) {
-    unsigned long long* word = (unsigned long long*)extent;
-    hipLaunchKernelGGL(extent_clear_kernel, dim3(1), dim3(1), 0, st, word, Lmax);
-    const int eg = (int)std::min<int64_t>((pl.ntot + 255) / 256, 64);
-    hipLaunchKernelGGL(extent_kernel, dim3(eg), dim3(256), 0, st, r, pl.ntot, inva, a, word);
-  }
-
-  const int prep_grid = (int)((pl.ntot + 255) / 256);
   int grid = (int)std::min<int64_t>(pl.total_items, (int64_t)INT32_MAX);
   if (ragged || (tuning && tuning->max_wgs > 0)) {
     const DeviceProps& dp = device_props();
@@ -877,32 +762,16 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     grid = (int)std::min<int64_t>(grid, cap);
   }
 
+  const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
   for (int k0 = 0, K = group_k(k); k0 < k; k0 += K) {
     K = group_k(k - k0);
-    const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
-    PrepParams pp{};
-    pp.r = r;
-    pp.f = NF ? f + (size_t)k0 * stride : nullptr;
-    pp.t = NT ? t + (size_t)k0 * stride : nullptr;
-    pp.pos_hi = (float4*)(base + pl.off_hi);
-    pp.pos_lo = (float4*)(base + pl.off_lo);
-    pp.pos64 = (double4*)(base + pl.off_pos64);
-    pp.frc = base + pl.off_frc;
-    pp.trq = base + pl.off_trq;
-    pp.extent = extent;
-    pp.ntot = pl.ntot;
-    pp.K = K;
-    pp.rhs_stride = stride;
-    pp.a = a; pp.inva = inva; pp.c1 = c1; pp.c2 = c2;
-    pp.prec = pl.prec;
-    pp.do_pos = (k0 == 0);
-    hipLaunchKernelGGL(prepare_kernel, dim3(prep_grid), dim3(256), 0, st, pp);
-
     PairParams q{};
-    q.pos_hi = pp.pos_hi; q.pos_lo = pp.pos_lo; q.pos64 = pp.pos64;
-    q.frc = pp.frc; q.trq = pp.trq;
+    q.r = r;
+    q.f = NF ? f + (size_t)k0 * stride : nullptr;
+    q.t = NT ? t + (size_t)k0 * stride : nullptr;
+    q.rhs_stride = stride;
+    q.a = a; q.inva = inva; q.c1 = c1; q.c2 = c2;
     q.partial = (double*)(base + pl.off_partial);
-    q.extent = extent;
     q.table = dtable;
     q.single = pl.sys.empty() ? SysInfo{} : pl.sys[0];
     q.nsys = (int)pl.sys.size();
@@ -933,7 +802,7 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
 
     ReduceParams rp{};
     rp.r = r;
-    rp.f = pp.f; rp.t = pp.t;
+    rp.f = q.f; rp.t = q.t;
     rp.vel = HU ? vel + (size_t)k0 * stride : nullptr;
     rp.omg = HW ? omg + (size_t)k0 * stride : nullptr;
     rp.partial = q.partial;
