@@ -200,6 +200,10 @@ def main(argv=None):
     achieved = flop / (pk.mean_ms * 1e-3) / 1e12
     peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
     info = wall_rpy.device_info()
+    from rigidmultiblobswall_amd import autotune
+    shape = autotune.lookup(RMW_UF, args.precision, 1, float(n) * n, False)
+    launch = ({"chunk_len": shape.chunk_len, "max_wgs": shape.max_wgs, "rhs_group": shape.rhs_group,
+               "source": "autotuner cache"} if shape is not None else {"source": "built-in heuristic"})
     out = {
         "metric": METRIC,
         "value": value,
@@ -215,7 +219,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": desc, "blobs": n, "pairs_per_step": pairs,
                    "precision": args.precision + PRECISION_NOTE.get(args.precision, ""),
-                   "device": info},
+                   "device": info, "launch_shape": launch},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": load_traffic(args.workload, args.precision),
                      "kernel": (f"pair_kernel_f32<UF,K=1,{args.precision}>" if args.precision != "fp64"

@@ -43,11 +43,12 @@ extern "C" {
 #define RMW_PREC_F32_BLOCKED 3 /* FP32 pair terms, FP32 sub-sums of <= 16
                                   sources folded into FP64 per-target sums */
 
-/* Launch shape; pass NULL for the built-in heuristic. */
+/* Launch shape; pass NULL for the built-in heuristic (the Python layer
+ * passes the autotuner's choice when its cache has one). */
 typedef struct rmw_tuning {
   int64_t chunk_len;  /* sources per work item (multiple of 256; 0 = auto) */
   int32_t max_wgs;    /* cap on resident workgroups of persistent launches (0 = auto) */
-  int32_t reserved;
+  int32_t rhs_group;  /* right-hand sides per pair-kernel pass: 1, 2 or 4 (0 = auto) */
 } rmw_tuning;
 
 const char* rmw_last_error(void);

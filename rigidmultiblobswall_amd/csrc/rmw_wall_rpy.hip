@@ -558,7 +558,14 @@ struct Plan {
 
 static bool valid_mask(int mask) { return mask > 0 && mask < 16; }
 
-static int group_k(int k) { return k >= 4 ? 4 : (k >= 2 ? 2 : 1); }
+// Right-hand sides per pair-kernel pass: 4, 2 or 1, at most the tuned cap.
+static int group_k(int k, int cap)
+{
+  const int g = k >= 4 ? 4 : (k >= 2 ? 2 : 1);
+  return cap > 0 && cap < g ? (cap >= 2 ? 2 : 1) : g;
+}
+
+static int rhs_cap(const rmw_tuning* tuning) { return tuning ? tuning->rhs_group : 0; }
 
 static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, int k,
                       const rmw_tuning* tuning, bool ragged)
@@ -571,7 +578,10 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   Plan pl;
   pl.mask = mask;
   pl.prec = prec;
-  pl.kmax = group_k(k);
+  RMW_REQUIRE(!tuning || tuning->rhs_group == 0 || tuning->rhs_group == 1 || tuning->rhs_group == 2 ||
+                  tuning->rhs_group == 4,
+              "rhs_group must be 0, 1, 2 or 4");
+  pl.kmax = group_k(k, rhs_cap(tuning));
   const int nsys = (int)offsets.size() - 1;
   RMW_REQUIRE(nsys >= 1 && offsets[0] == 0, "offsets must start at 0");
   for (int s = 0; s < nsys; ++s) RMW_REQUIRE(offsets[s + 1] >= offsets[s], "offsets must be non-decreasing");
@@ -763,8 +773,8 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   }
 
   const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
-  for (int k0 = 0, K = group_k(k); k0 < k; k0 += K) {
-    K = group_k(k - k0);
+  for (int k0 = 0, K = group_k(k, rhs_cap(tuning)); k0 < k; k0 += K) {
+    K = group_k(k - k0, rhs_cap(tuning));
     PairParams q{};
     q.r = r;
     q.f = NF ? f + (size_t)k0 * stride : nullptr;

@@ -64,6 +64,19 @@ def _stream_ptr(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+def _tuning_for(tuning, mask: int, precision: str, k: int, pairs: float, ragged: bool):
+    """Launch shape for a call: the caller's, the autotuner's cached one
+    (tuning=None), or the library's built-in heuristic (tuning="heuristic")."""
+    if isinstance(tuning, str):
+        if tuning != "heuristic":
+            raise ValueError("tuning must be None, 'heuristic', a dict or an rmw_tuning")
+        return None
+    if tuning is not None:
+        return tuning
+    from . import autotune
+    return autotune.lookup(mask, precision if isinstance(precision, str) else str(precision), k, pairs, ragged)
+
+
 def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: float = 1.0,
           periodic_length=None, precision="fp32", k: int = 1, tuning=None,
           out_velocity: Optional[torch.Tensor] = None, out_omega: Optional[torch.Tensor] = None):
@@ -87,7 +100,7 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     w = out_omega if out_omega is not None else (
         torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None)
     prec = _lib.precision_code(precision)
-    tp = _lib.tuning_ptr(tuning)
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device)
@@ -109,7 +122,7 @@ def apply_strided(mask: int, r: torch.Tensor, n: int, k: int, ld: int, force_ptr
     lib = _lib.load()
     device = _device()
     prec = _lib.precision_code(precision)
-    tp = _lib.tuning_ptr(tuning)
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device)
@@ -135,7 +148,8 @@ def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torqu
     u = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_u else None
     w = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None
     prec = _lib.precision_code(precision)
-    tp = _lib.tuning_ptr(tuning)
+    pairs = float(np.sum(np.diff(off).astype(np.float64) ** 2))
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True))
     offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, nsys, k, tp, ctypes.byref(nbytes)))
@@ -160,7 +174,8 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int
     device = _device()
     off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
     prec = _lib.precision_code(precision)
-    tp = _lib.tuning_ptr(tuning)
+    pairs = float(np.sum(np.diff(off).astype(np.float64) ** 2))
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True))
     offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, off.size - 1, k, tp, ctypes.byref(nbytes)))
@@ -170,13 +185,20 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int
                                              tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
 
 
+_info_cache: dict = {}
+
+
 def device_info() -> dict:
+    dev = _device()
+    if dev.index in _info_cache:
+        return dict(_info_cache[dev.index])
     lib = _lib.load()
-    _device()
     cu, clk, wf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     name = ctypes.create_string_buffer(64)
     _lib.check(lib.rmw_device_info(ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(wf), name, 64))
-    return {"cu_count": cu.value, "clock_khz": clk.value, "wavefront": wf.value, "arch": name.value.decode()}
+    info = {"cu_count": cu.value, "clock_khz": clk.value, "wavefront": wf.value, "arch": name.value.decode()}
+    _info_cache[dev.index] = info
+    return dict(info)
 
 
 class PairKernelTimer:
