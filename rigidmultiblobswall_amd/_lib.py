@@ -35,7 +35,8 @@ class RmwError(RuntimeError):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("chunk_len", ctypes.c_int64), ("max_wgs", ctypes.c_int32), ("rhs_group", ctypes.c_int32)]
+    _fields_ = [("chunk_len", ctypes.c_int64), ("max_wgs", ctypes.c_int32), ("rhs_group", ctypes.c_int32),
+                ("symmetric", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class Segments(ctypes.Structure):
@@ -164,5 +165,6 @@ def tuning_ptr(tuning):
         return None
     if isinstance(tuning, Tuning):
         return ctypes.byref(tuning)
-    t = Tuning(int(tuning.get("chunk_len", 0)), int(tuning.get("max_wgs", 0)), int(tuning.get("rhs_group", 0)))
+    t = Tuning(int(tuning.get("chunk_len", 0)), int(tuning.get("max_wgs", 0)), int(tuning.get("rhs_group", 0)),
+               int(tuning.get("symmetric", 0)), 0)
     return ctypes.byref(t)

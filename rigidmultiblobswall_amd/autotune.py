@@ -3,18 +3,21 @@
 A pair-kernel work item is a tile of 256 targets against one chunk of
 sources.  Three launch parameters are tuned (``rmw_tuning``):
 
-  chunk_len  sources per work item (multiples of 256): short chunks give
+  chunk_len  sources per work item (multiples of 64): short chunks give
              more workgroups and more FP64 partial sums to reduce, long
              chunks fewer of both;
   max_wgs    cap on the resident grid of the persistent loop (0 = one
              workgroup per work item, ragged batches default to 8 per CU);
   rhs_group  right-hand sides per pass (1, 2 or 4): more share the
-             geometry of a pair, but need more registers.
+             geometry of a pair, but need more registers;
+  symmetric  for the UF block on one suspension: each unordered pair once
+             for both blobs (on) or the ordered kernel (off).
 
-`tune` times every candidate with HIP events on the launch stream (whole
-product: pair kernel + reduce), keeps only candidates whose result agrees
-with the built-in heuristic's to the precision's tolerance, and returns the
-fastest.  Results live in a JSON cache keyed by device architecture, block
+`tune` times candidates with HIP events on the launch stream (whole
+product: pair kernel + reduce) in two stages -- every chunk length first,
+then grid caps and RHS groups for the three best -- keeps only candidates
+whose result agrees with the built-in heuristic's to the precision's
+tolerance, and returns the fastest.  Results live in a JSON cache keyed by device architecture, block
 mask, precision, right-hand sides, single/ragged and the problem size
 (half-octave bucket of the pair count); `wall_rpy` consults the cache
 whenever the caller passes no tuning.
@@ -122,27 +125,35 @@ def lookup(mask: int, precision: str, k: int, pairs: float, ragged: bool) -> Opt
     e = _shared_cache(arch).lookup(cache_key(arch, mask, precision, k, pairs, ragged))
     if e is None:
         return None
-    return _lib.Tuning(int(e["chunk_len"]), int(e["max_wgs"]), int(e.get("rhs_group", 0)))
+    return _lib.Tuning(int(e["chunk_len"]), int(e["max_wgs"]), int(e.get("rhs_group", 0)),
+                       int(e.get("symmetric", 0)), 0)
 
 
-def candidates(nmax: int, k: int, cu: int, ragged: bool) -> List[dict]:
-    """Launch shapes to try: chunk lengths 256 * 2^i up to the largest
-    system, grid caps of 2..16 workgroups per CU (and none), RHS groups."""
-    top = max(BLOCK, (nmax + BLOCK - 1) // BLOCK * BLOCK)
-    chunks = [0]
-    c = BLOCK
+def chunk_candidates(nmax: int, max_chunks: int = 48) -> List[int]:
+    """Chunk lengths to try (explicit, so a cached entry does not depend on
+    the library's built-in rule): powers of two from 256 and every distinct
+    length ceil(n / c) rounded to 64 for c = 1..max_chunks chunks."""
+    q = 64
+    top = max(q, (nmax + q - 1) // q * q)
+    out = set()
+    c = 256
     while c < top:
-        chunks.append(c)
+        out.add(c)
         c *= 2
-    chunks.append(top)
-    wgs = [0, 2 * cu, 4 * cu, 8 * cu, 16 * cu]
-    groups = [0] + [g for g in (1, 2) if g < min(k, 4)] if k > 1 else [0]
-    out = []
-    for ch in sorted(set(chunks)):
-        for w in wgs:
-            for g in groups:
-                out.append({"chunk_len": ch, "max_wgs": w, "rhs_group": g})
-    return out
+    for nch in range(1, max_chunks + 1):
+        ln = ((nmax + nch - 1) // nch + q - 1) // q * q
+        out.add(max(q, min(ln, top)))
+    return sorted(out)
+
+
+def candidates(nmax: int, k: int, cu: int, ragged: bool, symmetric_possible: bool = False) -> List[dict]:
+    """Second-stage shapes for one chunk length: grid caps (none, 4 and 8
+    per CU, plus 16 per CU for ragged batches) x RHS groups x (where the
+    symmetric UF kernel applies) symmetric on / off."""
+    wgs = [0, 4 * cu, 8 * cu] + ([16 * cu] if ragged else [])
+    groups = ([0] + [g for g in (1, 2) if g < min(k, 4)]) if k > 1 else [0]
+    syms = [0, -1] if symmetric_possible else [0]
+    return [{"max_wgs": w, "rhs_group": g, "symmetric": sy} for w in wgs for g in groups for sy in syms]
 
 
 def _tolerance(precision: str) -> float:
@@ -194,25 +205,41 @@ def tune(mask: int, precision: str, r: np.ndarray, offsets: Optional[np.ndarray]
     refn = float(torch.linalg.vector_norm(ref).item())
     tol = _tolerance(precision)
     cu = wall_rpy.device_info()["cu_count"]
-    cands = candidates(int(sizes.max()) if sizes.size else 0, k, cu, ragged)
-    if max_candidates:
-        cands = cands[:max_candidates]
+    nmax = int(sizes.max()) if sizes.size else 0
     table = []
-    best = None
-    for c in cands:
-        tun = _lib.Tuning(c["chunk_len"], c["max_wgs"], c["rhs_group"])
+
+    def trial(c):
+        tun = _lib.Tuning(c["chunk_len"], c["max_wgs"], c["rhs_group"], c.get("symmetric", 0), 0)
         ms, out = timed(tun)
         got = torch.cat([x.reshape(-1) for x in out if x is not None])
         err = float(torch.linalg.vector_norm(got - ref).item()) / max(refn, 1e-300)
-        ok = err <= tol
-        row = dict(c, ms=ms, rel_diff=err, ok=ok)
+        row = dict(c, ms=ms, rel_diff=err, ok=err <= tol)
         table.append(row)
-        if ok and (best is None or ms < best["ms"]):
-            best = row
+        return row
+
+    # stage 1: chunk length at the default grid and RHS grouping
+    chunks = chunk_candidates(nmax)
+    if max_candidates:
+        chunks = chunks[:max_candidates]
+    sym_possible = mask == RMW_UF and not ragged and periodic_length is None
+    stage1 = [trial({"chunk_len": c, "max_wgs": 0, "rhs_group": 0, "symmetric": 0}) for c in chunks]
+    if sym_possible:
+        stage1 += [trial({"chunk_len": c, "max_wgs": 0, "rhs_group": 0, "symmetric": -1}) for c in chunks]
+    good = sorted((r for r in stage1 if r["ok"]), key=lambda r: r["ms"])[:3]
+    # stage 2: grid cap and RHS group for the three best (chunk length, kernel)
+    for r0 in good:
+        for c in candidates(nmax, k, cu, ragged):
+            if c["max_wgs"] == 0 and c["rhs_group"] == 0:
+                continue
+            trial(dict(c, chunk_len=r0["chunk_len"], symmetric=r0["symmetric"]))
+    cands = table
+    best = min((r for r in table if r["ok"]), key=lambda r: r["ms"])
     if log:
         log(f"  {len(cands)} shapes: heuristic {base_ms:.4f} ms, best {best['ms']:.4f} ms "
-            f"(chunk {best['chunk_len']}, max_wgs {best['max_wgs']}, rhs_group {best['rhs_group']})")
+            f"(chunk {best['chunk_len']}, max_wgs {best['max_wgs']}, rhs_group {best['rhs_group']}, "
+            f"symmetric {best.get('symmetric', 0)})")
     entry = {"chunk_len": best["chunk_len"], "max_wgs": best["max_wgs"], "rhs_group": best["rhs_group"],
+             "symmetric": best.get("symmetric", 0),
              "ms": best["ms"], "heuristic_ms": base_ms, "pairs": pairs, "n_max": int(sizes.max()),
              "systems": int(sizes.size)}
     return {"best": entry, "default_ms": base_ms, "table": table, "pairs": pairs, "ragged": ragged}

@@ -179,6 +179,25 @@ RMW_HD Vec3<T> apply_uf(const Geom<T>& g, const CoefUF<T>& c, T vx, T vy, T vz)
   return u;
 }
 
+// Transposed block: UF_ij^T v, which is UF_ji v (the mobility is symmetric),
+// so one evaluation of a pair serves both of its blobs.  The coefficients of
+// v_z R and (R.v) z trade places.
+template <typename T>
+RMW_HD Vec3<T> apply_uf_t(const Geom<T>& g, const CoefUF<T>& c, T vx, T vy, T vz)
+{
+  T p = g.dx * vx + g.dy * vy;
+  T dv = p + g.dz * vz;
+  T Rv = p + g.Rz * vz;
+  T ge = c.g2 * Rv + c.g4 * vz;
+  T q = c.Bc * dv;
+  T qe = q + ge;
+  Vec3<T> u;
+  u.x = c.s * vx + qe * g.dx;
+  u.y = c.s * vy + qe * g.dy;
+  u.z = c.s5 * vz + q * g.dz + ge * g.Rz + c.g3 * Rv;
+  return u;
+}
+
 // ---------------------------------------------------------------------------
 // Wall coefficients shared by UT and WF (mobility_numba.py:659-665 and
 // :1046-1052), for a given h_hat.  Returned pre-divided by powers of |R| so

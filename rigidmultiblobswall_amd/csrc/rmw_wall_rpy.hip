@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "rmw_internal.h"
@@ -307,8 +308,19 @@ __device__ __forceinline__ void stage_sources(const PairParams& p, int64_t gj, d
   }
 }
 
+// Minimum resident waves per SIMD asked of the register allocator for the
+// FP32 pair kernels (1 = no constraint).
+#ifndef RMW_PAIR_WAVES
+#define RMW_PAIR_WAVES 1
+#endif
+template <int MASK, int K>
+constexpr int pair_min_waves()
+{
+  return (K == 1 && (MASK == kUF || MASK == kUT || MASK == kWF || MASK == kWT)) ? RMW_PAIR_WAVES : 1;
+}
+
 template <int MASK, int K, int ACC>
-__global__ __launch_bounds__(kBlock) void pair_kernel_f32(PairParams p)
+__global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kernel_f32(PairParams p)
 {
   using BT = BlockTraits<MASK>;
   __shared__ SrcF32 s_pos[kBlock];
@@ -459,6 +471,281 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
 }
 
 // ---------------------------------------------------------------------------
+// 3b. symmetric ("mutual") UF kernels: UF_ji = UF_ij^T, so each unordered pair
+//     is evaluated once and serves both blobs.  Work items are (target tile
+//     T, run of source tiles S >= T).  Each wave owns 64 targets (registers)
+//     and sweeps the source tile in blocks of 64: at step s lane l takes
+//     source (l + s) mod 64 of the block, so the 64 lanes touch 64 different
+//     sources (structure-of-arrays LDS, conflict-free).  The source-side sum
+//     UF_ij^T v_i of a source travels with it: every step the accumulator
+//     register moves one lane down (DPP wave_rol:1, fused into the add), so
+//     after the 64 steps each source's sum over the wave's targets sits in
+//     one lane.  The four waves' sums are then added in a fixed order.  On
+//     the diagonal tile (S == T) only sources above the target are taken.
+//     Partial sums: one slot per item (target side) and one per tile pair
+//     (S, T) (source side), reduced in a fixed order.
+// ---------------------------------------------------------------------------
+struct MutParams {
+  const double* r;
+  const double* f;          // K right-hand sides rhs_stride apart
+  int64_t rhs_stride;
+  double a, inva, c1;
+  double* part_i;           // [item][K][kBlock][3]
+  double* part_j;           // [tri(S, T)][K][kBlock][3]
+  int64_t n, n_t, n_items, chunk_tiles;
+};
+
+__host__ __device__ __forceinline__ int64_t tri_index(int64_t S, int64_t T) { return S * (S + 1) / 2 + T; }
+
+// sum_{m=1}^{M} ceil(m / C)
+__host__ __device__ __forceinline__ int64_t ceil_sum(int64_t M, int64_t C)
+{
+  const int64_t q = M / C, r = M - q * C;
+  return C * q * (q + 1) / 2 + r * (q + 1);
+}
+
+// Work items of target tiles 0..T-1 (target tile t has ceil((n_t - t) / C)
+// runs of C source tiles): closed form, no table.
+__host__ __device__ __forceinline__ int64_t items_before(int64_t T, int64_t n_t, int64_t C)
+{
+  return ceil_sum(n_t, C) - ceil_sum(n_t - T, C);
+}
+
+// Lane l receives lane (l + 1) mod 64 (DPP wave_rol:1).
+__device__ __forceinline__ float rol1(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double rol1(double v)
+{
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ int64_t mut_target_tile(const MutParams& p, int64_t item)
+{
+  int64_t lo = 0, hi = p.n_t - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (items_before(mid, p.n_t, p.chunk_tiles) <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Source-side sums of one tile: the four waves' values added in order.
+template <int K>
+__device__ __forceinline__ void mut_store_source_side(const MutParams& p, int64_t S, int64_t T, int cnt,
+                                                      const double (*s_jd)[K][3][kBlock])
+{
+  const int t = threadIdx.x;
+  if (t < cnt) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double* P = p.part_j + ((tri_index(S, T) * K + k) * kBlock + t) * 3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) P[c] = ((s_jd[0][k][c][t] + s_jd[1][k][c][t]) + s_jd[2][k][c][t]) + s_jd[3][k][c][t];
+    }
+  }
+}
+
+// FP32 pair terms.  Target side: the mode's accumulator.  Source side:
+// blocked mode -- FP32 sums over 16 steps folded into the FP64 LDS sums;
+// the other modes -- the travelling accumulator itself is FP64.
+template <int K, int ACC>
+__global__ __launch_bounds__(kBlock) void mutual_uf_f32(MutParams p)
+{
+  constexpr bool kFoldJ = ACC == RMW_PREC_F32_BLOCKED;
+  using JT = typename std::conditional<kFoldJ, float, double>::type;
+  __shared__ double s_x[kBlock], s_y[kBlock], s_z[kBlock];
+  __shared__ float s_zf[kBlock];
+  __shared__ float s_f[K][3][kBlock];
+  __shared__ double s_jd[kBlock / 64][K][3][kBlock];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int il = threadIdx.x;
+
+  for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
+    const int64_t T = mut_target_tile(p, item);
+    const int64_t S0 = T + (item - items_before(T, p.n_t, p.chunk_tiles)) * p.chunk_tiles;
+    const int64_t S1 = S0 + p.chunk_tiles < p.n_t ? S0 + p.chunk_tiles : p.n_t;
+    const int64_t i = T * kBlock + il;
+    const bool valid = i < p.n;
+    double Xi, Yi, Zi, Bi;
+    load_blob(p.r, valid ? i : 0, p.a, p.inva, Xi, Yi, Zi, Bi);
+    const float zi = (float)Zi;
+    float fi[K][3];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double* v = p.f + k * p.rhs_stride + 3 * (valid ? i : 0);
+      const double c = valid ? p.c1 * Bi : 0.0;
+      fi[k][0] = (float)(c * v[0]); fi[k][1] = (float)(c * v[1]); fi[k][2] = (float)(c * v[2]);
+    }
+    Acc<ACC> u[K][3], unused[K][3];
+
+    for (int64_t S = S0; S < S1; ++S) {
+      const int64_t s0 = S * kBlock;
+      const int cnt = (int)(p.n - s0 < kBlock ? p.n - s0 : kBlock);
+      __syncthreads();  // the previous tile and its source-side sums have been consumed
+      if (il < cnt) {
+        double X, Y, Z, B;
+        load_blob(p.r, s0 + il, p.a, p.inva, X, Y, Z, B);
+        s_x[il] = X; s_y[il] = Y; s_z[il] = Z; s_zf[il] = (float)Z;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double* v = p.f + k * p.rhs_stride + 3 * (s0 + il);
+          const double c = p.c1 * B;
+          s_f[k][0][il] = (float)(c * v[0]); s_f[k][1][il] = (float)(c * v[1]); s_f[k][2][il] = (float)(c * v[2]);
+        }
+      }
+      __syncthreads();
+      const bool diag = S == T;
+      for (int b = 0; b < kBlock / 64 && 64 * b < cnt; ++b) {
+        JT jacc[K][3];
+#pragma unroll
+        for (int k = 0; k < K; ++k) jacc[k][0] = jacc[k][1] = jacc[k][2] = JT(0);
+        for (int st = 0; st < 64; ++st) {
+          const int jl = 64 * b + ((lane + st) & 63);
+          const bool on = valid && jl < cnt && (!diag || jl > il);
+          const float dx = (float)(Xi - s_x[jl]);
+          const float dy = (float)(Yi - s_y[jl]);
+          const float dz = (float)(Zi - s_z[jl]);
+          const float zj = s_zf[jl];
+          const Geom<float> g = make_geom(dx, dy, dz, zi, zj);
+          const CoefUF<float> cu = coef_uf(g, zi, zj);
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const Vec3<float> a = apply_uf(g, cu, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
+            u[k][0].add(on ? a.x : 0.0f);
+            u[k][1].add(on ? a.y : 0.0f);
+            u[k][2].add(on ? a.z : 0.0f);
+            const Vec3<float> bt = apply_uf_t(g, cu, fi[k][0], fi[k][1], fi[k][2]);
+            jacc[k][0] = rol1(jacc[k][0]) + (on ? (JT)bt.x : JT(0));
+            jacc[k][1] = rol1(jacc[k][1]) + (on ? (JT)bt.y : JT(0));
+            jacc[k][2] = rol1(jacc[k][2]) + (on ? (JT)bt.z : JT(0));
+          }
+          if ((st & (kSub - 1)) == kSub - 1) {
+            flush_all<K, ACC>(u, unused);
+            if constexpr (kFoldJ) {
+              // lane l now holds the sum of source (l + st) mod 64 of the block
+              const int jw = 64 * b + ((lane + st) & 63);
+#pragma unroll
+              for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  const double prev = st < kSub ? 0.0 : s_jd[wv][k][c][jw];
+                  s_jd[wv][k][c][jw] = prev + (double)jacc[k][c];
+                  jacc[k][c] = JT(0);
+                }
+            }
+          }
+        }
+        if constexpr (!kFoldJ) {
+          const int jw = 64 * b + ((lane + 63) & 63);
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s_jd[wv][k][c][jw] = (double)jacc[k][c];
+        }
+      }
+      flush_all<K, ACC>(u, unused);
+      __syncthreads();
+      mut_store_source_side<K>(p, S, T, cnt, s_jd);
+    }
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double* P = p.part_i + ((item * K + k) * kBlock + il) * 3;
+        P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
+      }
+    }
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void mutual_uf_f64(MutParams p)
+{
+  __shared__ double s_x[kBlock], s_y[kBlock], s_z[kBlock];
+  __shared__ double s_f[K][3][kBlock];
+  __shared__ double s_jd[kBlock / 64][K][3][kBlock];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int il = threadIdx.x;
+
+  for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
+    const int64_t T = mut_target_tile(p, item);
+    const int64_t S0 = T + (item - items_before(T, p.n_t, p.chunk_tiles)) * p.chunk_tiles;
+    const int64_t S1 = S0 + p.chunk_tiles < p.n_t ? S0 + p.chunk_tiles : p.n_t;
+    const int64_t i = T * kBlock + il;
+    const bool valid = i < p.n;
+    double Xi, Yi, Zi, Bi;
+    load_blob(p.r, valid ? i : 0, p.a, p.inva, Xi, Yi, Zi, Bi);
+    double fi[K][3];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double* v = p.f + k * p.rhs_stride + 3 * (valid ? i : 0);
+      const double c = valid ? p.c1 * Bi : 0.0;
+      fi[k][0] = c * v[0]; fi[k][1] = c * v[1]; fi[k][2] = c * v[2];
+    }
+    Acc<RMW_PREC_F64> u[K][3];
+
+    for (int64_t S = S0; S < S1; ++S) {
+      const int64_t s0 = S * kBlock;
+      const int cnt = (int)(p.n - s0 < kBlock ? p.n - s0 : kBlock);
+      __syncthreads();
+      if (il < cnt) {
+        double X, Y, Z, B;
+        load_blob(p.r, s0 + il, p.a, p.inva, X, Y, Z, B);
+        s_x[il] = X; s_y[il] = Y; s_z[il] = Z;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double* v = p.f + k * p.rhs_stride + 3 * (s0 + il);
+          const double c = p.c1 * B;
+          s_f[k][0][il] = c * v[0]; s_f[k][1][il] = c * v[1]; s_f[k][2][il] = c * v[2];
+        }
+      }
+      __syncthreads();
+      const bool diag = S == T;
+      for (int b = 0; b < kBlock / 64 && 64 * b < cnt; ++b) {
+        double jacc[K][3];
+#pragma unroll
+        for (int k = 0; k < K; ++k) jacc[k][0] = jacc[k][1] = jacc[k][2] = 0.0;
+        for (int st = 0; st < 64; ++st) {
+          const int jl = 64 * b + ((lane + st) & 63);
+          const bool on = valid && jl < cnt && (!diag || jl > il);
+          const double zj = s_z[jl];
+          const Geom<double> g = make_geom(Xi - s_x[jl], Yi - s_y[jl], Zi - zj, Zi, zj);
+          const CoefUF<double> cu = coef_uf(g, Zi, zj);
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const Vec3<double> a = apply_uf(g, cu, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
+            u[k][0].add(on ? a.x : 0.0);
+            u[k][1].add(on ? a.y : 0.0);
+            u[k][2].add(on ? a.z : 0.0);
+            const Vec3<double> bt = apply_uf_t(g, cu, fi[k][0], fi[k][1], fi[k][2]);
+            jacc[k][0] = rol1(jacc[k][0]) + (on ? bt.x : 0.0);
+            jacc[k][1] = rol1(jacc[k][1]) + (on ? bt.y : 0.0);
+            jacc[k][2] = rol1(jacc[k][2]) + (on ? bt.z : 0.0);
+          }
+        }
+        const int jw = 64 * b + ((lane + 63) & 63);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) s_jd[wv][k][c][jw] = jacc[k][c];
+      }
+      __syncthreads();
+      mut_store_source_side<K>(p, S, T, cnt, s_jd);
+    }
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double* P = p.part_i + ((item * K + k) * kBlock + il) * 3;
+        P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 4. reduce: fixed-order chunk sum + FP64 self term + damping.
 // ---------------------------------------------------------------------------
 struct ReduceParams {
@@ -544,16 +831,77 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceParams p)
   }
 }
 
+// Reduce of the symmetric UF path: for blob gi of tile X, the target-side
+// slots of X's items, then the source-side slots (X, T) for T = 0..X, in
+// that order; FP64 self term and damping as in reduce_kernel.
+struct MutReduceParams {
+  const double* r;
+  const double* f;
+  double* vel;
+  const double* part_i;
+  const double* part_j;
+  int K;
+  int64_t n, n_t, chunk_tiles;
+  int64_t rhs_stride;
+  double a, inva, c1;
+};
+
+__global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
+{
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_k = p.n * 3;
+  if (tid >= per_k * p.K) return;
+  const int k = (int)(tid / per_k);
+  const int64_t rem = tid - (int64_t)k * per_k;
+  const int64_t gi = rem / 3;
+  const int c = (int)(rem - gi * 3);
+  const int64_t X = gi / kBlock, l = gi - X * kBlock;
+  // target side: X's items; source side: slots (X, 0..X), two running sums each
+  const int64_t it0 = items_before(X, p.n_t, p.chunk_tiles), it1 = items_before(X + 1, p.n_t, p.chunk_tiles);
+  const int64_t istep = (int64_t)p.K * kBlock * 3;
+  const double* Pi = p.part_i + ((it0 * p.K + k) * kBlock + l) * 3 + c;
+  const double* Pj = p.part_j + ((tri_index(X, 0) * p.K + k) * kBlock + l) * 3 + c;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t m = 0;
+  const int64_t ni = it1 - it0;
+  for (; m + 2 <= ni; m += 2) {
+    s0 += Pi[m * istep];
+    s1 += Pi[(m + 1) * istep];
+  }
+  if (m < ni) s0 += Pi[m * istep];
+  for (m = 0; m + 2 <= X + 1; m += 2) {
+    s2 += Pj[m * istep];
+    s3 += Pj[(m + 1) * istep];
+  }
+  if (m < X + 1) s2 += Pj[m * istep];
+  s0 += s1;
+  s1 = s2 + s3;
+  const double z = p.r[3 * gi + 2];
+  const double ze = z <= p.a ? p.a : z;
+  const double B = z < p.a ? z * p.inva : 1.0;
+  const double* v = p.f + k * p.rhs_stride + 3 * gi;
+  const double s = p.c1 * B;
+  const Vec3<double> sv = self_uf(ze * p.inva, s * v[0], s * v[1], s * v[2]);
+  const double self = c == 0 ? sv.x : (c == 1 ? sv.y : sv.z);
+  p.vel[k * p.rhs_stride + 3 * gi + c] = B * ((s0 + s1) + self);
+}
+
 // ---------------------------------------------------------------------------
 // Host side: plan, workspace layout, dispatch.
 // ---------------------------------------------------------------------------
 struct Plan {
   int mask = 0, prec = 0, kmax = 1;
+  int slots = 0;             // resident pair-kernel workgroups on the device
   int64_t ntot = 0;
   std::vector<SysInfo> sys;  // non-empty systems only
   int64_t total_items = 0;
   size_t partial_doubles = 0;
   size_t off_partial = 0, off_table = 0, total = 0;
+  // symmetric UF path (single suspension, no periodic images)
+  bool mut_ok = false;
+  int kmax_mut = 1;
+  int64_t n_t = 0, mut_items = 0, chunk_tiles = 1;
+  size_t off_pi = 0, off_pj = 0;
 };
 
 static bool valid_mask(int mask) { return mask > 0 && mask < 16; }
@@ -566,6 +914,71 @@ static int group_k(int k, int cap)
 }
 
 static int rhs_cap(const rmw_tuning* tuning) { return tuning ? tuning->rhs_group : 0; }
+
+constexpr int kChunkQuantum = 64;  // chunk lengths are multiples of this
+
+template <int MASK, int K>
+static const void* pair_ptr(int prec)
+{
+  if (prec == RMW_PREC_F32_F64ACC) return reinterpret_cast<const void*>(&pair_kernel_f32<MASK, K, RMW_PREC_F32_F64ACC>);
+  if (prec == RMW_PREC_F32_BLOCKED) return reinterpret_cast<const void*>(&pair_kernel_f32<MASK, K, RMW_PREC_F32_BLOCKED>);
+  if (prec == RMW_PREC_F32_KAHAN) return reinterpret_cast<const void*>(&pair_kernel_f32<MASK, K, RMW_PREC_F32_KAHAN>);
+  return reinterpret_cast<const void*>(&pair_kernel_f64<MASK, K>);
+}
+
+template <int MASK>
+static const void* pair_ptr_k(int prec, int K)
+{
+  return K == 4 ? pair_ptr<MASK, 4>(prec) : (K == 2 ? pair_ptr<MASK, 2>(prec) : pair_ptr<MASK, 1>(prec));
+}
+
+static const void* pair_kernel_ptr(int mask, int prec, int K)
+{
+  switch (mask) {
+    case 1: return pair_ptr_k<1>(prec, K);
+    case 2: return pair_ptr_k<2>(prec, K);
+    case 3: return pair_ptr_k<3>(prec, K);
+    case 4: return pair_ptr_k<4>(prec, K);
+    case 5: return pair_ptr_k<5>(prec, K);
+    case 8: return pair_ptr_k<8>(prec, K);
+    case 10: return pair_ptr_k<10>(prec, K);
+    case 12: return pair_ptr_k<12>(prec, K);
+    default: return pair_ptr_k<15>(prec, K);
+  }
+}
+
+// Workgroups of the (mask, prec, K) pair kernel resident on the whole
+// device at once (register and LDS limited), cached per device.
+static int resident_slots(int mask, int prec, int K)
+{
+  static std::mutex mu;
+  static std::vector<std::pair<int64_t, int>> cache;
+  const DeviceProps& dp = device_props();
+  const int64_t key = ((((int64_t)dp.device * 16 + mask) * 8 + prec) * 8) + K;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& e : cache)
+      if (e.first == key) return e.second;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel_ptr(mask, prec, K), kBlock, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int slots = per_cu * std::max(dp.cu_count, 1);
+  std::lock_guard<std::mutex> lk(mu);
+  cache.emplace_back(key, slots);
+  return slots;
+}
+
+// Built-in chunk length: ~16 work items per CU over the whole batch (small
+// items keep the workgroups of a CU out of step and balance the persistent
+// loop of ragged batches), at least one LDS tile.  The autotuner refines it.
+static int64_t default_chunk(double pairs, int cu_count)
+{
+  const double c = pairs / (16.0 * std::max(cu_count, 1) * kBlock);
+  return std::max<int64_t>(kBlock, (int64_t)std::ceil(c / kChunkQuantum) * kChunkQuantum);
+}
 
 static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, int k,
                       const rmw_tuning* tuning, bool ragged)
@@ -587,7 +1000,6 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   for (int s = 0; s < nsys; ++s) RMW_REQUIRE(offsets[s + 1] >= offsets[s], "offsets must be non-decreasing");
   pl.ntot = offsets[nsys];
 
-  // chunk length: aim for ~8 workgroups per CU over the whole batch.
   double pairs = 0.0;
   int64_t nmax = 0;
   for (int s = 0; s < nsys; ++s) {
@@ -595,15 +1007,11 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
     pairs += n * n;
     nmax = std::max<int64_t>(nmax, offsets[s + 1] - offsets[s]);
   }
+  pl.slots = resident_slots(mask, prec, pl.kmax);
   int64_t chunk = tuning && tuning->chunk_len > 0 ? tuning->chunk_len : 0;
-  if (chunk <= 0) {
-    const DeviceProps& dp = device_props();
-    double target_items = 8.0 * std::max(dp.cu_count, 1);
-    double c = pairs / (target_items * kBlock);
-    chunk = (int64_t)std::ceil(c / kBlock) * kBlock;
-  }
-  chunk = std::max<int64_t>(kBlock, (chunk + kBlock - 1) / kBlock * kBlock);
-  if (nmax > 0) chunk = std::min<int64_t>(chunk, (nmax + kBlock - 1) / kBlock * kBlock);
+  if (chunk <= 0) chunk = default_chunk(pairs, device_props().cu_count);
+  chunk = std::max<int64_t>(kChunkQuantum, (chunk + kChunkQuantum - 1) / kChunkQuantum * kChunkQuantum);
+  if (nmax > 0) chunk = std::min<int64_t>(chunk, (nmax + kChunkQuantum - 1) / kChunkQuantum * kChunkQuantum);
 
   const bool HU = (mask & (kUF | kUT)) != 0, HW = (mask & (kWF | kWT)) != 0;
   const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
@@ -625,10 +1033,40 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   pl.total_items = item;
   pl.partial_doubles = (size_t)part;
 
+  // Symmetric layout: UF only, one suspension, sym not switched off.
+  const int sym = tuning ? tuning->symmetric : 0;
+  pl.mut_ok = mask == kUF && nsys == 1 && pl.ntot > 0 && sym >= 0;
+  RMW_REQUIRE(!(sym > 0 && !pl.mut_ok), "symmetric evaluation needs the UF block alone on one suspension");
+  if (pl.mut_ok && sym == 0) {
+    // automatic choice: only with enough tile pairs to fill the device
+    const int64_t nt = (pl.ntot + kBlock - 1) / kBlock;
+    pl.mut_ok = nt * (nt + 1) / 2 >= device_props().cu_count;
+  }
+  size_t mut_total = 0;
+  if (pl.mut_ok) {
+    // FP64 source-side sums need 24 KB of LDS per right-hand side: at most 2.
+    const int cap_mut = prec == RMW_PREC_F32_BLOCKED ? 4 : 2;
+    const int rc = rhs_cap(tuning);
+    pl.kmax_mut = group_k(k, rc > 0 ? std::min(rc, cap_mut) : cap_mut);
+    pl.n_t = (pl.ntot + kBlock - 1) / kBlock;
+    const int64_t pairs_t = pl.n_t * (pl.n_t + 1) / 2;
+    if (tuning && tuning->chunk_len > 0) {
+      pl.chunk_tiles = std::max<int64_t>(1, (tuning->chunk_len + kBlock - 1) / kBlock);
+    } else {
+      const DeviceProps& dp = device_props();
+      pl.chunk_tiles = std::max<int64_t>(1, pairs_t / (16 * (int64_t)std::max(dp.cu_count, 1)));
+    }
+    pl.mut_items = items_before(pl.n_t, pl.n_t, pl.chunk_tiles);
+    size_t o = 0;
+    pl.off_pi = o; o = align_up(o + (size_t)pl.mut_items * pl.kmax_mut * kBlock * 3 * sizeof(double));
+    pl.off_pj = o; o = align_up(o + (size_t)pairs_t * pl.kmax_mut * kBlock * 3 * sizeof(double));
+    mut_total = o;
+  }
+
   size_t o = 0;
   pl.off_partial = o; o = align_up(o + pl.partial_doubles * sizeof(double));
   if (ragged) { pl.off_table = o; o = align_up(o + pl.sys.size() * sizeof(SysInfo)); }
-  pl.total = o;
+  pl.total = std::max(o, mut_total);
   return pl;
 }
 
@@ -726,6 +1164,91 @@ static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
 
+static void dispatch_mutual(int prec, int K, const MutParams& q, int grid, hipStream_t st)
+{
+  const dim3 g(grid), b(kBlock);
+  if (prec == RMW_PREC_F32_BLOCKED) {
+    if (K == 4) hipLaunchKernelGGL((mutual_uf_f32<4, RMW_PREC_F32_BLOCKED>), g, b, 0, st, q);
+    else if (K == 2) hipLaunchKernelGGL((mutual_uf_f32<2, RMW_PREC_F32_BLOCKED>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_uf_f32<1, RMW_PREC_F32_BLOCKED>), g, b, 0, st, q);
+  } else if (prec == RMW_PREC_F32_F64ACC) {
+    if (K == 2) hipLaunchKernelGGL((mutual_uf_f32<2, RMW_PREC_F32_F64ACC>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_uf_f32<1, RMW_PREC_F32_F64ACC>), g, b, 0, st, q);
+  } else if (prec == RMW_PREC_F32_KAHAN) {
+    if (K == 2) hipLaunchKernelGGL((mutual_uf_f32<2, RMW_PREC_F32_KAHAN>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_uf_f32<1, RMW_PREC_F32_KAHAN>), g, b, 0, st, q);
+  } else {
+    if (K == 2) hipLaunchKernelGGL((mutual_uf_f64<2>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_uf_f64<1>), g, b, 0, st, q);
+  }
+}
+
+// Pair-kernel timing records (rmw_profile_*).
+static void prof_begin(ProfRec& rec, bool& on, hipStream_t st)
+{
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    on = g_prof_on;
+  }
+  if (on) {
+    RMW_CHECK_HIP(hipEventCreate(&rec.a));
+    RMW_CHECK_HIP(hipEventCreate(&rec.b));
+    RMW_CHECK_HIP(hipEventRecord(rec.a, st));
+  }
+}
+
+static void prof_end(const ProfRec& rec, bool on, hipStream_t st)
+{
+  if (on) {
+    RMW_CHECK_HIP(hipEventRecord(rec.b, st));
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.push_back(rec);
+  }
+}
+
+static void run_mutual(const Plan& pl, const double* r, const double* f, double* vel, int k, int64_t stride,
+                       double a, double inva, double c1, const rmw_tuning* tuning, char* base, hipStream_t st)
+{
+  int grid = (int)std::min<int64_t>(pl.mut_items, (int64_t)INT32_MAX);
+  if (tuning && tuning->max_wgs > 0) grid = std::min(grid, tuning->max_wgs);
+  for (int k0 = 0, K = group_k(k, pl.kmax_mut); k0 < k; k0 += K) {
+    K = group_k(k - k0, pl.kmax_mut);
+    MutParams q{};
+    q.r = r;
+    q.f = f + (size_t)k0 * stride;
+    q.rhs_stride = stride;
+    q.a = a; q.inva = inva; q.c1 = c1;
+    q.part_i = (double*)(base + pl.off_pi);
+    q.part_j = (double*)(base + pl.off_pj);
+    q.n = pl.ntot;
+    q.n_t = pl.n_t;
+    q.n_items = pl.mut_items;
+    q.chunk_tiles = pl.chunk_tiles;
+    ProfRec rec{};
+    bool prof = false;
+    prof_begin(rec, prof, st);
+    dispatch_mutual(pl.prec, K, q, grid, st);
+    RMW_CHECK_HIP(hipGetLastError());
+    prof_end(rec, prof, st);
+
+    MutReduceParams rp{};
+    rp.r = r;
+    rp.f = q.f;
+    rp.vel = vel + (size_t)k0 * stride;
+    rp.part_i = q.part_i;
+    rp.part_j = q.part_j;
+    rp.K = K;
+    rp.n = pl.ntot;
+    rp.n_t = pl.n_t;
+    rp.chunk_tiles = pl.chunk_tiles;
+    rp.rhs_stride = stride;
+    rp.a = a; rp.inva = inva; rp.c1 = c1;
+    const int64_t nthreads = pl.ntot * K * 3;
+    hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
+    RMW_CHECK_HIP(hipGetLastError());
+  }
+}
+
 static void run_plan(const Plan& pl, const double* r, const double* f, const double* t, double* vel,
                      double* omg, int k, int64_t ld, double eta, double a, const double* L,
                      const rmw_tuning* tuning, void* ws, size_t ws_bytes, hipStream_t st, bool ragged)
@@ -759,6 +1282,12 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   const double c1 = 1.0 / (8.0 * M_PI * eta * a);
   const double c2 = c1 / a;
 
+  const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
+  if (pl.mut_ok && periodic == 0) {
+    run_mutual(pl, r, f, vel, k, stride, a, inva, c1, tuning, base, st);
+    return;
+  }
+
   SysInfo* dtable = nullptr;
   if (ragged) {
     dtable = (SysInfo*)(base + pl.off_table);
@@ -767,12 +1296,10 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
 
   int grid = (int)std::min<int64_t>(pl.total_items, (int64_t)INT32_MAX);
   if (ragged || (tuning && tuning->max_wgs > 0)) {
-    const DeviceProps& dp = device_props();
-    int cap = tuning && tuning->max_wgs > 0 ? tuning->max_wgs : dp.cu_count * 8;
+    const int cap = tuning && tuning->max_wgs > 0 ? tuning->max_wgs : std::max(pl.slots, 1);
     grid = (int)std::min<int64_t>(grid, cap);
   }
 
-  const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
   for (int k0 = 0, K = group_k(k, rhs_cap(tuning)); k0 < k; k0 += K) {
     K = group_k(k - k0, rhs_cap(tuning));
     PairParams q{};
@@ -793,22 +1320,10 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     // reuses it with its own K (items only touch their own slots).
     ProfRec rec{};
     bool prof = false;
-    {
-      std::lock_guard<std::mutex> lk(g_prof_mu);
-      prof = g_prof_on;
-    }
-    if (prof) {
-      RMW_CHECK_HIP(hipEventCreate(&rec.a));
-      RMW_CHECK_HIP(hipEventCreate(&rec.b));
-      RMW_CHECK_HIP(hipEventRecord(rec.a, st));
-    }
+    prof_begin(rec, prof, st);
     dispatch_pair(kmask, pl.prec, K, q, grid, st);
     RMW_CHECK_HIP(hipGetLastError());
-    if (prof) {
-      RMW_CHECK_HIP(hipEventRecord(rec.b, st));
-      std::lock_guard<std::mutex> lk(g_prof_mu);
-      g_prof.push_back(rec);
-    }
+    prof_end(rec, prof, st);
 
     ReduceParams rp{};
     rp.r = r;

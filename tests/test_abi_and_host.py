@@ -91,6 +91,28 @@ def test_pair_term_header_matches_oracle(block):
     assert worst32 < 2e-5
 
 
+def test_transposed_uf_is_the_swapped_pair():
+    """apply_uf_t on the pair (i, j) is the oracle's UF block of (j, i): the
+    identity the symmetric kernel uses to serve both blobs of a pair."""
+    lib = _host_lib()
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rng = np.random.default_rng(11)
+    a, eta = 0.3, 0.7
+    worst = 0.0
+    for t in range(2000):
+        ri = np.array([*rng.uniform(-3, 3, 2), rng.uniform(a, 3)])
+        rj = np.array([*rng.uniform(-3, 3, 2), rng.uniform(a, 3)])
+        m = np.empty(9)
+        assert lib.pt_pair_matrix_f64(4, P(ri), P(rj), 0, eta, a, P(m)) == 0
+        ref = O.pair_matrix(O.UF, rj, ri, False, eta, a).ravel()
+        worst = max(worst, np.abs(m - ref).max() / np.abs(ref).max())
+        # and it is not the untransposed block (the wall part is not symmetric)
+        m0 = np.empty(9)
+        lib.pt_pair_matrix_f64(0, P(ri), P(rj), 0, eta, a, P(m0))
+    assert worst < 1e-13
+    assert np.abs(m0 - m).max() > 1e-6 * np.abs(m).max()
+
+
 def test_icosphere_matches_reference_vertex_file():
     from rigidmultiblobswall_amd.workloads import icosphere_42
     ref = np.loadtxt(os.path.join(REPO, "tests", "golden", "shell_N_42_Rg_1_Rh_1_1220.vertex"), skiprows=1)

@@ -36,14 +36,15 @@ def test_cache_round_trip_and_nearest(tmp_path):
 
 
 def test_candidates():
+    ch = autotune.chunk_candidates(10752)
+    assert 0 not in ch and 256 in ch and 10752 in ch and 448 in ch   # 448 = 10752 / 24
+    assert all(c % 64 == 0 for c in ch)
+    assert autotune.chunk_candidates(100) == [64, 128]
     cs = autotune.candidates(nmax=10752, k=1, cu=256, ragged=False)
-    chunks = sorted({c["chunk_len"] for c in cs})
-    assert chunks[0] == 0 and 256 in chunks and 10752 in chunks
-    assert all(c == 0 or c % 256 == 0 for c in chunks)
     assert {c["rhs_group"] for c in cs} == {0}
-    assert {c["max_wgs"] for c in cs} == {0, 512, 1024, 2048, 4096}
+    assert {c["max_wgs"] for c in cs} == {0, 1024, 2048}
     cs4 = autotune.candidates(nmax=32768, k=4, cu=256, ragged=False)
     assert {c["rhs_group"] for c in cs4} == {0, 1, 2}
     cs2 = autotune.candidates(nmax=100, k=2, cu=256, ragged=True)
     assert {c["rhs_group"] for c in cs2} == {0, 1}
-    assert sorted({c["chunk_len"] for c in cs2}) == [0, 256]
+    assert {c["max_wgs"] for c in cs2} == {0, 1024, 2048, 4096}
