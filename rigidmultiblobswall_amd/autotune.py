@@ -10,8 +10,8 @@ sources.  Three launch parameters are tuned (``rmw_tuning``):
              workgroup per work item, ragged batches default to 8 per CU);
   rhs_group  right-hand sides per pass (1, 2 or 4): more share the
              geometry of a pair, but need more registers;
-  symmetric  for the UF block on one suspension: each unordered pair once
-             for both blobs (on) or the ordered kernel (off).
+  symmetric  for the UF block alone or all four blocks: each unordered
+             pair once for both blobs (on) or the ordered kernel (off).
 
 `tune` times candidates with HIP events on the launch stream (whole
 product: pair kernel + reduce) in two stages -- every chunk length first,
@@ -221,7 +221,7 @@ def tune(mask: int, precision: str, r: np.ndarray, offsets: Optional[np.ndarray]
     chunks = chunk_candidates(nmax)
     if max_candidates:
         chunks = chunks[:max_candidates]
-    sym_possible = mask == RMW_UF and not ragged and periodic_length is None
+    sym_possible = mask in (RMW_UF, RMW_UF | RMW_UT | RMW_WF | RMW_WT) and periodic_length is None
     stage1 = [trial({"chunk_len": c, "max_wgs": 0, "rhs_group": 0, "symmetric": 0}) for c in chunks]
     if sym_possible:
         stage1 += [trial({"chunk_len": c, "max_wgs": 0, "rhs_group": 0, "symmetric": -1}) for c in chunks]

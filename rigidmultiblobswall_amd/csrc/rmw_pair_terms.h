@@ -264,6 +264,47 @@ RMW_HD Vec3<T> apply_ut(const Geom<T>& g, const CoefRot<T>& c, T vx, T vy, T vz)
   return u;
 }
 
+// Transposes of the rotational couplings, read off the component forms
+// above.  Reciprocity gives UT_ji = WF_ij^T and WF_ji = UT_ij^T, so with
+// the coefficients of the pair (i, j) these are the blocks of (j, i).
+// WF_ij^T v, with p = d_x v_x + d_y v_y, q = d_y v_x - d_x v_y:
+//   x = F3 d_x q / |R|^2 + (F1 R_z / |R| - c d_z + F4) v_y + (c - F1/|R|) d_y v_z
+//   y = F3 d_y q / |R|^2 - (F1 R_z / |R| - c d_z + F4) v_x - (c - F1/|R|) d_x v_z
+//   z = (F1/|R| + F2/|R| + F3 R_z/|R|^2 - c) q
+template <typename T>
+RMW_HD Vec3<T> apply_wf_t(const Geom<T>& g, const CoefRot<T>& c, T vx, T vy, T vz)
+{
+  T q = g.dy * vx - g.dx * vy;
+  T m3 = c.F3r2 * q;
+  T G = c.F1r * g.Rz - c.c * g.dz + c.F4;
+  T cf = c.c - c.F1r;
+  Vec3<T> w;
+  w.x = m3 * g.dx + G * vy + cf * g.dy * vz;
+  w.y = m3 * g.dy - G * vx - cf * g.dx * vz;
+  w.z = (c.F1r + c.F2r + c.F3r2 * g.Rz - c.c) * q;
+  return w;
+}
+
+// UT_ij^T v:
+//   x =  F3 d_y p / |R|^2 - G v_y + K d_y v_z
+//   y = -F3 d_x p / |R|^2 + G v_x - K d_x v_z
+//   z = (F1/|R| - c) q
+// with G = c d_z + F1 R_z / |R| + F4 and K = c - F1/|R| - F2/|R| - F3 R_z/|R|^2.
+template <typename T>
+RMW_HD Vec3<T> apply_ut_t(const Geom<T>& g, const CoefRot<T>& c, T vx, T vy, T vz)
+{
+  T p = g.dx * vx + g.dy * vy;
+  T q = g.dy * vx - g.dx * vy;
+  T m3 = c.F3r2 * p;
+  T G = c.c * g.dz + c.F1r * g.Rz + c.F4;
+  T K = c.c - c.F1r - c.F2r - c.F3r2 * g.Rz;
+  Vec3<T> u;
+  u.x = m3 * g.dy - G * vy + K * g.dy * vz;
+  u.y = -m3 * g.dx + G * vx - K * g.dx * vz;
+  u.z = (c.F1r - c.c) * q;
+  return u;
+}
+
 // ---------------------------------------------------------------------------
 // WT: rotation from torque.  mobility_numba.py:1256-1321
 // ---------------------------------------------------------------------------
@@ -318,6 +359,24 @@ RMW_HD Vec3<T> apply_wt(const Geom<T>& g, const CoefWT<T>& c, T vx, T vy, T vz)
   w.x = c.s * vx + (q + ge) * g.dx - gz * g.dy;
   w.y = c.s * vy + (q + ge) * g.dy + gz * g.dx;
   w.z = c.s * vz + q * g.dz + ge * g.Rz + c.G3r * Rv;
+  return w;
+}
+
+// WT_ij^T v = WT_ji v: the (R.v) z term becomes v_z R.
+template <typename T>
+RMW_HD Vec3<T> apply_wt_t(const Geom<T>& g, const CoefWT<T>& c, T vx, T vy, T vz)
+{
+  T p = g.dx * vx + g.dy * vy;
+  T dv = p + g.dz * vz;
+  T Rv = p + g.Rz * vz;
+  T zt = g.dx * vy - g.dy * vx;
+  T q = c.Bc * dv;
+  T ge = c.G2r * Rv + c.G3r * vz;
+  T gz = c.G4r * zt;
+  Vec3<T> w;
+  w.x = c.s * vx + (q + ge) * g.dx - gz * g.dy;
+  w.y = c.s * vy + (q + ge) * g.dy + gz * g.dx;
+  w.z = c.s * vz + q * g.dz + ge * g.Rz;
   return w;
 }
 

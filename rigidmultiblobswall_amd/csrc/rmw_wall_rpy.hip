@@ -471,28 +471,40 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
 }
 
 // ---------------------------------------------------------------------------
-// 3b. symmetric ("mutual") UF kernels: UF_ji = UF_ij^T, so each unordered pair
-//     is evaluated once and serves both blobs.  Work items are (target tile
-//     T, run of source tiles S >= T).  Each wave owns 64 targets (registers)
-//     and sweeps the source tile in blocks of 64: at step s lane l takes
-//     source (l + s) mod 64 of the block, so the 64 lanes touch 64 different
-//     sources (structure-of-arrays LDS, conflict-free).  The source-side sum
-//     UF_ij^T v_i of a source travels with it: every step the accumulator
-//     register moves one lane down (DPP wave_rol:1, fused into the add), so
-//     after the 64 steps each source's sum over the wave's targets sits in
-//     one lane.  The four waves' sums are then added in a fixed order.  On
-//     the diagonal tile (S == T) only sources above the target are taken.
-//     Partial sums: one slot per item (target side) and one per tile pair
-//     (S, T) (source side), reduced in a fixed order.
+// 3b. symmetric ("mutual") kernels.  The wall mobility is symmetric:
+//     UF_ji = UF_ij^T, WT_ji = WT_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, so
+//     each unordered pair is evaluated once and serves both blobs.  Work
+//     items are (target tile T, run of source tiles S >= T) of one system.
+//     Each wave owns 64 targets (registers) and sweeps a source tile in
+//     blocks of 64: at step s lane l takes source (l + s) mod 64 of the
+//     block, so the 64 lanes touch 64 different sources (structure-of-arrays
+//     LDS, conflict-free).  The source-side sum of a source travels with it:
+//     every step the accumulator register moves one lane down (DPP
+//     wave_rol:1, fused into the add), so after the 64 steps each source's
+//     sum over the wave's targets sits in one lane.  After each block the
+//     four waves' sums are added in a fixed order.  On the diagonal tile
+//     (S == T) only sources above the target are taken.  Partial sums: one
+//     slot per item (target side) and one per tile pair (S, T) (source side),
+//     reduced in a fixed order.
 // ---------------------------------------------------------------------------
+struct MutSys {
+  int64_t b0, n, n_t;   // first blob, blobs, tiles of kBlock
+  int64_t item0;        // first work item
+  int64_t pi0, pj0;     // first target-side / source-side slot
+};
+
 struct MutParams {
   const double* r;
-  const double* f;          // K right-hand sides rhs_stride apart
+  const double* f;          // K right-hand sides rhs_stride apart, or null
+  const double* t;
   int64_t rhs_stride;
-  double a, inva, c1;
-  double* part_i;           // [item][K][kBlock][3]
-  double* part_j;           // [tri(S, T)][K][kBlock][3]
-  int64_t n, n_t, n_items, chunk_tiles;
+  double a, inva, c1, c2;
+  double* part_i;           // [slot][K][kBlock][OC]
+  double* part_j;
+  const MutSys* table;      // nullptr -> single
+  MutSys single;
+  int nsys;
+  int64_t n_items, chunk_tiles;
 };
 
 __host__ __device__ __forceinline__ int64_t tri_index(int64_t S, int64_t T) { return S * (S + 1) / 2 + T; }
@@ -504,8 +516,8 @@ __host__ __device__ __forceinline__ int64_t ceil_sum(int64_t M, int64_t C)
   return C * q * (q + 1) / 2 + r * (q + 1);
 }
 
-// Work items of target tiles 0..T-1 (target tile t has ceil((n_t - t) / C)
-// runs of C source tiles): closed form, no table.
+// Work items of target tiles 0..T-1 of a system (target tile t has
+// ceil((n_t - t) / C) runs of C source tiles): closed form, no table.
 __host__ __device__ __forceinline__ int64_t items_before(int64_t T, int64_t n_t, int64_t C)
 {
   return ceil_sum(n_t, C) - ceil_sum(n_t - T, C);
@@ -523,223 +535,221 @@ __device__ __forceinline__ double rol1(double v)
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ int64_t mut_target_tile(const MutParams& p, int64_t item)
+__device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t item)
 {
-  int64_t lo = 0, hi = p.n_t - 1;
+  if (!p.table) return p.single;
+  int lo = 0, hi = p.nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.table[mid].item0 <= item) lo = mid; else hi = mid - 1;
+  }
+  return p.table[lo];
+}
+
+__device__ __forceinline__ int64_t mut_target_tile(int64_t local, int64_t n_t, int64_t C)
+{
+  int64_t lo = 0, hi = n_t - 1;
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
-    if (items_before(mid, p.n_t, p.chunk_tiles) <= item) lo = mid; else hi = mid - 1;
+    if (items_before(mid, n_t, C) <= local) lo = mid; else hi = mid - 1;
   }
   return lo;
 }
 
-// Source-side sums of one tile: the four waves' values added in order.
-template <int K>
-__device__ __forceinline__ void mut_store_source_side(const MutParams& p, int64_t S, int64_t T, int cnt,
-                                                      const double (*s_jd)[K][3][kBlock])
+// Symmetric pair kernel.  R = float: FP32 pair terms (ACC: the target-side
+// accumulator; source side: FP32 sums over 16 steps folded into FP64 in the
+// blocked mode, an FP64 travelling accumulator otherwise); R = double: FP64.
+template <int MASK, int K, int ACC, typename R>
+__global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
 {
-  const int t = threadIdx.x;
-  if (t < cnt) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double* P = p.part_j + ((tri_index(S, T) * K + k) * kBlock + t) * 3;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) P[c] = ((s_jd[0][k][c][t] + s_jd[1][k][c][t]) + s_jd[2][k][c][t]) + s_jd[3][k][c][t];
-    }
-  }
-}
-
-// FP32 pair terms.  Target side: the mode's accumulator.  Source side:
-// blocked mode -- FP32 sums over 16 steps folded into the FP64 LDS sums;
-// the other modes -- the travelling accumulator itself is FP64.
-template <int K, int ACC>
-__global__ __launch_bounds__(kBlock) void mutual_uf_f32(MutParams p)
-{
+  using BT = BlockTraits<MASK>;
+  constexpr bool UF = (MASK & kUF) != 0, UT = (MASK & kUT) != 0, WF = (MASK & kWF) != 0, WT = (MASK & kWT) != 0;
+  constexpr int OC = BT::OC;
   constexpr bool kFoldJ = ACC == RMW_PREC_F32_BLOCKED;
   using JT = typename std::conditional<kFoldJ, float, double>::type;
   __shared__ double s_x[kBlock], s_y[kBlock], s_z[kBlock];
-  __shared__ float s_zf[kBlock];
-  __shared__ float s_f[K][3][kBlock];
-  __shared__ double s_jd[kBlock / 64][K][3][kBlock];
+  __shared__ R s_zr[kBlock];
+  __shared__ R s_f[BT::NF ? K : 1][3][kBlock];
+  __shared__ R s_t[BT::NT ? K : 1][3][kBlock];
+  __shared__ double s_jd[kBlock / 64][K][OC][64];  // per-wave source-side sums of one block
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int il = threadIdx.x;
 
   for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
-    const int64_t T = mut_target_tile(p, item);
-    const int64_t S0 = T + (item - items_before(T, p.n_t, p.chunk_tiles)) * p.chunk_tiles;
-    const int64_t S1 = S0 + p.chunk_tiles < p.n_t ? S0 + p.chunk_tiles : p.n_t;
-    const int64_t i = T * kBlock + il;
-    const bool valid = i < p.n;
+    const MutSys Sy = mut_sys_of_item(p, item);
+    const int64_t local = item - Sy.item0;
+    const int64_t T = mut_target_tile(local, Sy.n_t, p.chunk_tiles);
+    const int64_t S0 = T + (local - items_before(T, Sy.n_t, p.chunk_tiles)) * p.chunk_tiles;
+    const int64_t S1 = S0 + p.chunk_tiles < Sy.n_t ? S0 + p.chunk_tiles : Sy.n_t;
+    const int64_t i = T * kBlock + il;  // within the system
+    const bool valid = i < Sy.n;
+    const int64_t gi = Sy.b0 + (valid ? i : 0);
     double Xi, Yi, Zi, Bi;
-    load_blob(p.r, valid ? i : 0, p.a, p.inva, Xi, Yi, Zi, Bi);
-    const float zi = (float)Zi;
-    float fi[K][3];
+    load_blob(p.r, gi, p.a, p.inva, Xi, Yi, Zi, Bi);
+    const R zi = (R)Zi;
+    R fi[BT::NF ? K : 1][3], ti[BT::NT ? K : 1][3];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const double* v = p.f + k * p.rhs_stride + 3 * (valid ? i : 0);
-      const double c = valid ? p.c1 * Bi : 0.0;
-      fi[k][0] = (float)(c * v[0]); fi[k][1] = (float)(c * v[1]); fi[k][2] = (float)(c * v[2]);
+      if constexpr (BT::NF) {
+        const double* v = p.f + k * p.rhs_stride + 3 * gi;
+        const double c = valid ? p.c1 * Bi : 0.0;
+        fi[k][0] = (R)(c * v[0]); fi[k][1] = (R)(c * v[1]); fi[k][2] = (R)(c * v[2]);
+      }
+      if constexpr (BT::NT) {
+        const double* v = p.t + k * p.rhs_stride + 3 * gi;
+        const double c = valid ? p.c2 * Bi : 0.0;
+        ti[k][0] = (R)(c * v[0]); ti[k][1] = (R)(c * v[1]); ti[k][2] = (R)(c * v[2]);
+      }
     }
-    Acc<ACC> u[K][3], unused[K][3];
+    Acc<ACC> u[K][3], w[K][3];
 
     for (int64_t S = S0; S < S1; ++S) {
       const int64_t s0 = S * kBlock;
-      const int cnt = (int)(p.n - s0 < kBlock ? p.n - s0 : kBlock);
-      __syncthreads();  // the previous tile and its source-side sums have been consumed
+      const int cnt = (int)(Sy.n - s0 < kBlock ? Sy.n - s0 : kBlock);
+      __syncthreads();  // the previous tile has been consumed
       if (il < cnt) {
+        const int64_t gj = Sy.b0 + s0 + il;
         double X, Y, Z, B;
-        load_blob(p.r, s0 + il, p.a, p.inva, X, Y, Z, B);
-        s_x[il] = X; s_y[il] = Y; s_z[il] = Z; s_zf[il] = (float)Z;
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B);
+        s_x[il] = X; s_y[il] = Y; s_z[il] = Z; s_zr[il] = (R)Z;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const double* v = p.f + k * p.rhs_stride + 3 * (s0 + il);
-          const double c = p.c1 * B;
-          s_f[k][0][il] = (float)(c * v[0]); s_f[k][1][il] = (float)(c * v[1]); s_f[k][2][il] = (float)(c * v[2]);
+          if constexpr (BT::NF) {
+            const double* v = p.f + k * p.rhs_stride + 3 * gj;
+            const double c = p.c1 * B;
+            s_f[k][0][il] = (R)(c * v[0]); s_f[k][1][il] = (R)(c * v[1]); s_f[k][2][il] = (R)(c * v[2]);
+          }
+          if constexpr (BT::NT) {
+            const double* v = p.t + k * p.rhs_stride + 3 * gj;
+            const double c = p.c2 * B;
+            s_t[k][0][il] = (R)(c * v[0]); s_t[k][1][il] = (R)(c * v[1]); s_t[k][2][il] = (R)(c * v[2]);
+          }
         }
       }
       __syncthreads();
       const bool diag = S == T;
       for (int b = 0; b < kBlock / 64 && 64 * b < cnt; ++b) {
-        JT jacc[K][3];
+        JT ja[K][OC];
 #pragma unroll
-        for (int k = 0; k < K; ++k) jacc[k][0] = jacc[k][1] = jacc[k][2] = JT(0);
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int c = 0; c < OC; ++c) ja[k][c] = JT(0);
         for (int st = 0; st < 64; ++st) {
           const int jl = 64 * b + ((lane + st) & 63);
           const bool on = valid && jl < cnt && (!diag || jl > il);
-          const float dx = (float)(Xi - s_x[jl]);
-          const float dy = (float)(Yi - s_y[jl]);
-          const float dz = (float)(Zi - s_z[jl]);
-          const float zj = s_zf[jl];
-          const Geom<float> g = make_geom(dx, dy, dz, zi, zj);
-          const CoefUF<float> cu = coef_uf(g, zi, zj);
+          R dx, dy, dz;
+          if constexpr (sizeof(R) == 4) {
+            dx = (R)(Xi - s_x[jl]); dy = (R)(Yi - s_y[jl]); dz = (R)(Zi - s_z[jl]);
+          } else {
+            dx = Xi - s_x[jl]; dy = Yi - s_y[jl]; dz = Zi - s_z[jl];
+          }
+          const R zj = s_zr[jl];
+          const Geom<R> g = make_geom(dx, dy, dz, zi, zj);
+          CoefUF<R> cuf;
+          CoefRot<R> cut, cwf;
+          CoefWT<R> cwt;
+          if constexpr (UF) cuf = coef_uf(g, zi, zj);
+          if constexpr (UT || WF) { cut = coef_rot(g, zi); cwf = coef_rot(g, zj); }
+          if constexpr (WT) cwt = coef_wt(g);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const Vec3<float> a = apply_uf(g, cu, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
-            u[k][0].add(on ? a.x : 0.0f);
-            u[k][1].add(on ? a.y : 0.0f);
-            u[k][2].add(on ? a.z : 0.0f);
-            const Vec3<float> bt = apply_uf_t(g, cu, fi[k][0], fi[k][1], fi[k][2]);
-            jacc[k][0] = rol1(jacc[k][0]) + (on ? (JT)bt.x : JT(0));
-            jacc[k][1] = rol1(jacc[k][1]) + (on ? (JT)bt.y : JT(0));
-            jacc[k][2] = rol1(jacc[k][2]) + (on ? (JT)bt.z : JT(0));
+            // target side: u_i += UF f_j + UT t_j, w_i += WF f_j + WT t_j
+            Vec3<R> ui{R(0), R(0), R(0)}, wi{R(0), R(0), R(0)};
+            // source side: u_j += UF^T f_i + WF^T t_i, w_j += UT^T f_i + WT^T t_i
+            Vec3<R> uj{R(0), R(0), R(0)}, wj{R(0), R(0), R(0)};
+            if constexpr (UF) {
+              const Vec3<R> a = apply_uf(g, cuf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
+              const Vec3<R> bt = apply_uf_t(g, cuf, fi[k][0], fi[k][1], fi[k][2]);
+              ui.x += a.x; ui.y += a.y; ui.z += a.z;
+              uj.x += bt.x; uj.y += bt.y; uj.z += bt.z;
+            }
+            if constexpr (UT) {
+              const Vec3<R> a = apply_ut(g, cut, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]);
+              const Vec3<R> bt = apply_wf_t(g, cwf, ti[k][0], ti[k][1], ti[k][2]);
+              ui.x += a.x; ui.y += a.y; ui.z += a.z;
+              uj.x += bt.x; uj.y += bt.y; uj.z += bt.z;
+            }
+            if constexpr (WF) {
+              const Vec3<R> a = apply_wf(g, cwf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
+              const Vec3<R> bt = apply_ut_t(g, cut, fi[k][0], fi[k][1], fi[k][2]);
+              wi.x += a.x; wi.y += a.y; wi.z += a.z;
+              wj.x += bt.x; wj.y += bt.y; wj.z += bt.z;
+            }
+            if constexpr (WT) {
+              const Vec3<R> a = apply_wt(g, cwt, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]);
+              const Vec3<R> bt = apply_wt_t(g, cwt, ti[k][0], ti[k][1], ti[k][2]);
+              wi.x += a.x; wi.y += a.y; wi.z += a.z;
+              wj.x += bt.x; wj.y += bt.y; wj.z += bt.z;
+            }
+            if constexpr (BT::HU) {
+              u[k][0].add(on ? ui.x : R(0));
+              u[k][1].add(on ? ui.y : R(0));
+              u[k][2].add(on ? ui.z : R(0));
+              ja[k][0] = rol1(ja[k][0]) + (on ? (JT)uj.x : JT(0));
+              ja[k][1] = rol1(ja[k][1]) + (on ? (JT)uj.y : JT(0));
+              ja[k][2] = rol1(ja[k][2]) + (on ? (JT)uj.z : JT(0));
+            }
+            if constexpr (BT::HW) {
+              constexpr int o = BT::HU ? 3 : 0;
+              w[k][0].add(on ? wi.x : R(0));
+              w[k][1].add(on ? wi.y : R(0));
+              w[k][2].add(on ? wi.z : R(0));
+              ja[k][o] = rol1(ja[k][o]) + (on ? (JT)wj.x : JT(0));
+              ja[k][o + 1] = rol1(ja[k][o + 1]) + (on ? (JT)wj.y : JT(0));
+              ja[k][o + 2] = rol1(ja[k][o + 2]) + (on ? (JT)wj.z : JT(0));
+            }
           }
           if ((st & (kSub - 1)) == kSub - 1) {
-            flush_all<K, ACC>(u, unused);
+            flush_all<K, ACC>(u, w);
             if constexpr (kFoldJ) {
-              // lane l now holds the sum of source (l + st) mod 64 of the block
-              const int jw = 64 * b + ((lane + st) & 63);
+              // lane l holds the sum of source (l + st) mod 64 of the block
+              const int jw = (lane + st) & 63;
 #pragma unroll
               for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
+                for (int c = 0; c < OC; ++c) {
                   const double prev = st < kSub ? 0.0 : s_jd[wv][k][c][jw];
-                  s_jd[wv][k][c][jw] = prev + (double)jacc[k][c];
-                  jacc[k][c] = JT(0);
+                  s_jd[wv][k][c][jw] = prev + (double)ja[k][c];
+                  ja[k][c] = JT(0);
                 }
             }
           }
         }
         if constexpr (!kFoldJ) {
-          const int jw = 64 * b + ((lane + 63) & 63);
+          const int jw = (lane + 63) & 63;
 #pragma unroll
           for (int k = 0; k < K; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) s_jd[wv][k][c][jw] = (double)jacc[k][c];
+            for (int c = 0; c < OC; ++c) s_jd[wv][k][c][jw] = (double)ja[k][c];
         }
-      }
-      flush_all<K, ACC>(u, unused);
-      __syncthreads();
-      mut_store_source_side<K>(p, S, T, cnt, s_jd);
-    }
-    if (valid) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        double* P = p.part_i + ((item * K + k) * kBlock + il) * 3;
-        P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
-      }
-    }
-  }
-}
-
-template <int K>
-__global__ __launch_bounds__(kBlock) void mutual_uf_f64(MutParams p)
-{
-  __shared__ double s_x[kBlock], s_y[kBlock], s_z[kBlock];
-  __shared__ double s_f[K][3][kBlock];
-  __shared__ double s_jd[kBlock / 64][K][3][kBlock];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int il = threadIdx.x;
-
-  for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
-    const int64_t T = mut_target_tile(p, item);
-    const int64_t S0 = T + (item - items_before(T, p.n_t, p.chunk_tiles)) * p.chunk_tiles;
-    const int64_t S1 = S0 + p.chunk_tiles < p.n_t ? S0 + p.chunk_tiles : p.n_t;
-    const int64_t i = T * kBlock + il;
-    const bool valid = i < p.n;
-    double Xi, Yi, Zi, Bi;
-    load_blob(p.r, valid ? i : 0, p.a, p.inva, Xi, Yi, Zi, Bi);
-    double fi[K][3];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double* v = p.f + k * p.rhs_stride + 3 * (valid ? i : 0);
-      const double c = valid ? p.c1 * Bi : 0.0;
-      fi[k][0] = c * v[0]; fi[k][1] = c * v[1]; fi[k][2] = c * v[2];
-    }
-    Acc<RMW_PREC_F64> u[K][3];
-
-    for (int64_t S = S0; S < S1; ++S) {
-      const int64_t s0 = S * kBlock;
-      const int cnt = (int)(p.n - s0 < kBlock ? p.n - s0 : kBlock);
-      __syncthreads();
-      if (il < cnt) {
-        double X, Y, Z, B;
-        load_blob(p.r, s0 + il, p.a, p.inva, X, Y, Z, B);
-        s_x[il] = X; s_y[il] = Y; s_z[il] = Z;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const double* v = p.f + k * p.rhs_stride + 3 * (s0 + il);
-          const double c = p.c1 * B;
-          s_f[k][0][il] = c * v[0]; s_f[k][1][il] = c * v[1]; s_f[k][2][il] = c * v[2];
-        }
-      }
-      __syncthreads();
-      const bool diag = S == T;
-      for (int b = 0; b < kBlock / 64 && 64 * b < cnt; ++b) {
-        double jacc[K][3];
-#pragma unroll
-        for (int k = 0; k < K; ++k) jacc[k][0] = jacc[k][1] = jacc[k][2] = 0.0;
-        for (int st = 0; st < 64; ++st) {
-          const int jl = 64 * b + ((lane + st) & 63);
-          const bool on = valid && jl < cnt && (!diag || jl > il);
-          const double zj = s_z[jl];
-          const Geom<double> g = make_geom(Xi - s_x[jl], Yi - s_y[jl], Zi - zj, Zi, zj);
-          const CoefUF<double> cu = coef_uf(g, Zi, zj);
+        __syncthreads();  // every wave's sums of this block are in LDS
+        if (il < 64 && 64 * b + il < cnt) {
+          const int64_t slot = Sy.pj0 + tri_index(S, T);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            const Vec3<double> a = apply_uf(g, cu, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
-            u[k][0].add(on ? a.x : 0.0);
-            u[k][1].add(on ? a.y : 0.0);
-            u[k][2].add(on ? a.z : 0.0);
-            const Vec3<double> bt = apply_uf_t(g, cu, fi[k][0], fi[k][1], fi[k][2]);
-            jacc[k][0] = rol1(jacc[k][0]) + (on ? bt.x : 0.0);
-            jacc[k][1] = rol1(jacc[k][1]) + (on ? bt.y : 0.0);
-            jacc[k][2] = rol1(jacc[k][2]) + (on ? bt.z : 0.0);
+            double* P = p.part_j + ((slot * K + k) * kBlock + 64 * b + il) * OC;
+#pragma unroll
+            for (int c = 0; c < OC; ++c)
+              P[c] = ((s_jd[0][k][c][il] + s_jd[1][k][c][il]) + s_jd[2][k][c][il]) + s_jd[3][k][c][il];
           }
         }
-        const int jw = 64 * b + ((lane + 63) & 63);
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) s_jd[wv][k][c][jw] = jacc[k][c];
+        __syncthreads();  // s_jd is free for the next block
       }
-      __syncthreads();
-      mut_store_source_side<K>(p, S, T, cnt, s_jd);
+      flush_all<K, ACC>(u, w);
     }
     if (valid) {
+      const int64_t slot = Sy.pi0 + local;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        double* P = p.part_i + ((item * K + k) * kBlock + il) * 3;
-        P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
+        double* P = p.part_i + ((slot * K + k) * kBlock + il) * OC;
+        int o = 0;
+        if constexpr (BT::HU) {
+          P[0] = u[k][0].get(); P[1] = u[k][1].get(); P[2] = u[k][2].get();
+          o = 3;
+        }
+        if constexpr (BT::HW) {
+          P[o] = w[k][0].get(); P[o + 1] = w[k][1].get(); P[o + 2] = w[k][2].get();
+        }
       }
     }
   }
@@ -776,6 +786,44 @@ __device__ inline SysInfo find_sys_blob(const ReduceParams& p, int64_t gi)
   return p.table[lo];
 }
 
+// Output component c (0..OC-1) of right-hand side k for blob gi: the FP64
+// self term, damping B and the remaining 1/a of the rotational rows added
+// to the pair sum.
+__device__ __forceinline__ void finish_output(const double* __restrict__ r, const double* __restrict__ f,
+                                              const double* __restrict__ t, double* __restrict__ vel,
+                                              double* __restrict__ omg, int mask, int k, int64_t gi, int c,
+                                              double sum, int64_t rhs_stride, double a, double inva, double c1,
+                                              double c2)
+{
+  const bool HU = (mask & (kUF | kUT)) != 0;
+  const bool HW = (mask & (kWF | kWT)) != 0;
+  const double z = r[3 * gi + 2];
+  const double ze = z <= a ? a : z;
+  const double B = z < a ? z * inva : 1.0;
+  const double h = ze * inva;
+  const bool is_w = HW && (!HU || c >= 3);
+  const int comp = c % 3;
+  double fv[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
+  if (f) {
+    const double* v = f + k * rhs_stride + 3 * gi;
+    fv[0] = c1 * B * v[0]; fv[1] = c1 * B * v[1]; fv[2] = c1 * B * v[2];
+  }
+  if (t) {
+    const double* v = t + k * rhs_stride + 3 * gi;
+    tv[0] = c2 * B * v[0]; tv[1] = c2 * B * v[1]; tv[2] = c2 * B * v[2];
+  }
+  double self = 0.0;
+  if (!is_w) {
+    if (mask & kUF) { Vec3<double> sv = self_uf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (mask & kUT) { Vec3<double> sv = self_ut(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    vel[k * rhs_stride + 3 * gi + comp] = B * (sum + self);
+  } else {
+    if (mask & kWF) { Vec3<double> sv = self_wf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (mask & kWT) { Vec3<double> sv = self_wt(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    omg[k * rhs_stride + 3 * gi + comp] = B * inva * (sum + self);
+  }
+}
+
 __global__ __launch_bounds__(256) void reduce_kernel(ReduceParams p)
 {
   // one thread per (rhs k, blob gi, output component c in [0, OC))
@@ -804,86 +852,72 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceParams p)
   for (; ch < S.n_chunks; ++ch) s0 += P[ch * cstride];
   double sum = (s0 + s1) + (s2 + s3);
 
-  const double z = p.r[3 * gi + 2];
-  const double ze = z <= p.a ? p.a : z;
-  const double B = z < p.a ? z * p.inva : 1.0;
-  const double h = ze * p.inva;
-  const bool is_w = HW && (!HU || c >= 3);
-  const int comp = c % 3;
-  double fv[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
-  if (p.f) {
-    const double* v = p.f + k * p.rhs_stride + 3 * gi;
-    fv[0] = p.c1 * B * v[0]; fv[1] = p.c1 * B * v[1]; fv[2] = p.c1 * B * v[2];
-  }
-  if (p.t) {
-    const double* v = p.t + k * p.rhs_stride + 3 * gi;
-    tv[0] = p.c2 * B * v[0]; tv[1] = p.c2 * B * v[1]; tv[2] = p.c2 * B * v[2];
-  }
-  double self = 0.0;
-  if (!is_w) {
-    if (p.mask & kUF) { Vec3<double> sv = self_uf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    if (p.mask & kUT) { Vec3<double> sv = self_ut(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    p.vel[k * p.rhs_stride + 3 * gi + comp] = B * (sum + self);
-  } else {
-    if (p.mask & kWF) { Vec3<double> sv = self_wf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    if (p.mask & kWT) { Vec3<double> sv = self_wt(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    p.omg[k * p.rhs_stride + 3 * gi + comp] = B * p.inva * (sum + self);
-  }
+  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2);
 }
 
-// Reduce of the symmetric UF path: for blob gi of tile X, the target-side
-// slots of X's items, then the source-side slots (X, T) for T = 0..X, in
-// that order; FP64 self term and damping as in reduce_kernel.
+// Reduce of the symmetric path: for blob gi of tile X of its system, the
+// target-side slots of X's items, then the source-side slots (X, T) for
+// T = 0..X, each with two running sums, in that order.
 struct MutReduceParams {
   const double* r;
   const double* f;
+  const double* t;
   double* vel;
+  double* omg;
   const double* part_i;
   const double* part_j;
+  const MutSys* table;
+  MutSys single;
+  int nsys;
+  int mask;
   int K;
-  int64_t n, n_t, chunk_tiles;
+  int64_t ntot, chunk_tiles;
   int64_t rhs_stride;
-  double a, inva, c1;
+  double a, inva, c1, c2;
 };
 
 __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
 {
+  const bool HU = (p.mask & (kUF | kUT)) != 0;
+  const bool HW = (p.mask & (kWF | kWT)) != 0;
+  const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per_k = p.n * 3;
+  const int64_t per_k = p.ntot * OC;
   if (tid >= per_k * p.K) return;
   const int k = (int)(tid / per_k);
   const int64_t rem = tid - (int64_t)k * per_k;
-  const int64_t gi = rem / 3;
-  const int c = (int)(rem - gi * 3);
-  const int64_t X = gi / kBlock, l = gi - X * kBlock;
-  // target side: X's items; source side: slots (X, 0..X), two running sums each
-  const int64_t it0 = items_before(X, p.n_t, p.chunk_tiles), it1 = items_before(X + 1, p.n_t, p.chunk_tiles);
-  const int64_t istep = (int64_t)p.K * kBlock * 3;
-  const double* Pi = p.part_i + ((it0 * p.K + k) * kBlock + l) * 3 + c;
-  const double* Pj = p.part_j + ((tri_index(X, 0) * p.K + k) * kBlock + l) * 3 + c;
+  const int64_t gi = rem / OC;
+  const int c = (int)(rem - gi * OC);
+  MutSys Sy = p.single;
+  if (p.table) {
+    int lo = 0, hi = p.nsys - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (p.table[mid].b0 <= gi) lo = mid; else hi = mid - 1;
+    }
+    Sy = p.table[lo];
+  }
+  const int64_t il = gi - Sy.b0;
+  const int64_t X = il / kBlock, l = il - X * kBlock;
+  const int64_t it0 = items_before(X, Sy.n_t, p.chunk_tiles), it1 = items_before(X + 1, Sy.n_t, p.chunk_tiles);
+  const int64_t step = (int64_t)p.K * kBlock * OC;
+  const double* Pi = p.part_i + (((Sy.pi0 + it0) * p.K + k) * kBlock + l) * OC + c;
+  const double* Pj = p.part_j + (((Sy.pj0 + tri_index(X, 0)) * p.K + k) * kBlock + l) * OC + c;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int64_t m = 0;
   const int64_t ni = it1 - it0;
+  int64_t m = 0;
   for (; m + 2 <= ni; m += 2) {
-    s0 += Pi[m * istep];
-    s1 += Pi[(m + 1) * istep];
+    s0 += Pi[m * step];
+    s1 += Pi[(m + 1) * step];
   }
-  if (m < ni) s0 += Pi[m * istep];
+  if (m < ni) s0 += Pi[m * step];
   for (m = 0; m + 2 <= X + 1; m += 2) {
-    s2 += Pj[m * istep];
-    s3 += Pj[(m + 1) * istep];
+    s2 += Pj[m * step];
+    s3 += Pj[(m + 1) * step];
   }
-  if (m < X + 1) s2 += Pj[m * istep];
-  s0 += s1;
-  s1 = s2 + s3;
-  const double z = p.r[3 * gi + 2];
-  const double ze = z <= p.a ? p.a : z;
-  const double B = z < p.a ? z * p.inva : 1.0;
-  const double* v = p.f + k * p.rhs_stride + 3 * gi;
-  const double s = p.c1 * B;
-  const Vec3<double> sv = self_uf(ze * p.inva, s * v[0], s * v[1], s * v[2]);
-  const double self = c == 0 ? sv.x : (c == 1 ? sv.y : sv.z);
-  p.vel[k * p.rhs_stride + 3 * gi + c] = B * ((s0 + s1) + self);
+  if (m < X + 1) s2 += Pj[m * step];
+  const double sum = (s0 + s1) + (s2 + s3);
+  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2);
 }
 
 // ---------------------------------------------------------------------------
@@ -897,11 +931,12 @@ struct Plan {
   int64_t total_items = 0;
   size_t partial_doubles = 0;
   size_t off_partial = 0, off_table = 0, total = 0;
-  // symmetric UF path (single suspension, no periodic images)
+  // symmetric path (UF or all four blocks, no periodic images)
   bool mut_ok = false;
   int kmax_mut = 1;
-  int64_t n_t = 0, mut_items = 0, chunk_tiles = 1;
-  size_t off_pi = 0, off_pj = 0;
+  int64_t mut_items = 0, chunk_tiles = 1;
+  std::vector<MutSys> msys;  // non-empty systems only
+  size_t off_pi = 0, off_pj = 0, off_mtable = 0;
 };
 
 static bool valid_mask(int mask) { return mask > 0 && mask < 16; }
@@ -1033,33 +1068,50 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   pl.total_items = item;
   pl.partial_doubles = (size_t)part;
 
-  // Symmetric layout: UF only, one suspension, sym not switched off.
+  // Symmetric layout: the UF block alone or all four blocks, unless switched off.
   const int sym = tuning ? tuning->symmetric : 0;
-  pl.mut_ok = mask == kUF && nsys == 1 && pl.ntot > 0 && sym >= 0;
-  RMW_REQUIRE(!(sym > 0 && !pl.mut_ok), "symmetric evaluation needs the UF block alone on one suspension");
-  if (pl.mut_ok && sym == 0) {
-    // automatic choice: only with enough tile pairs to fill the device
-    const int64_t nt = (pl.ntot + kBlock - 1) / kBlock;
-    pl.mut_ok = nt * (nt + 1) / 2 >= device_props().cu_count;
+  pl.mut_ok = (mask == kUF || mask == (kUF | kUT | kWF | kWT)) && pl.ntot > 0 && sym >= 0;
+  RMW_REQUIRE(!(sym > 0 && !pl.mut_ok), "symmetric evaluation needs the UF block alone or all four blocks");
+  int64_t tile_pairs = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int64_t nt = (offsets[s + 1] - offsets[s] + kBlock - 1) / kBlock;
+    tile_pairs += nt * (nt + 1) / 2;
   }
+  // automatic choice: only with enough tile pairs to fill the device
+  if (pl.mut_ok && sym == 0) pl.mut_ok = tile_pairs >= device_props().cu_count;
   size_t mut_total = 0;
   if (pl.mut_ok) {
-    // FP64 source-side sums need 24 KB of LDS per right-hand side: at most 2.
-    const int cap_mut = prec == RMW_PREC_F32_BLOCKED ? 4 : 2;
+    // at most 2 right-hand sides per pass (LDS of the source-side sums, registers)
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = group_k(k, rc > 0 ? std::min(rc, cap_mut) : cap_mut);
-    pl.n_t = (pl.ntot + kBlock - 1) / kBlock;
-    const int64_t pairs_t = pl.n_t * (pl.n_t + 1) / 2;
+    pl.kmax_mut = group_k(k, rc > 0 ? std::min(rc, 2) : 2);
     if (tuning && tuning->chunk_len > 0) {
       pl.chunk_tiles = std::max<int64_t>(1, (tuning->chunk_len + kBlock - 1) / kBlock);
     } else {
       const DeviceProps& dp = device_props();
-      pl.chunk_tiles = std::max<int64_t>(1, pairs_t / (16 * (int64_t)std::max(dp.cu_count, 1)));
+      pl.chunk_tiles = std::max<int64_t>(1, tile_pairs / (16 * (int64_t)std::max(dp.cu_count, 1)));
     }
-    pl.mut_items = items_before(pl.n_t, pl.n_t, pl.chunk_tiles);
+    int64_t item = 0, pi = 0, pj = 0;
+    for (int s = 0; s < nsys; ++s) {
+      MutSys M;
+      M.b0 = offsets[s];
+      M.n = offsets[s + 1] - offsets[s];
+      if (M.n == 0) continue;
+      M.n_t = (M.n + kBlock - 1) / kBlock;
+      M.item0 = item;
+      M.pi0 = pi;
+      M.pj0 = pj;
+      const int64_t its = items_before(M.n_t, M.n_t, pl.chunk_tiles);
+      item += its;
+      pi += its;
+      pj += M.n_t * (M.n_t + 1) / 2;
+      pl.msys.push_back(M);
+    }
+    pl.mut_items = item;
+    const int OCm = mask == kUF ? 3 : 6;
     size_t o = 0;
-    pl.off_pi = o; o = align_up(o + (size_t)pl.mut_items * pl.kmax_mut * kBlock * 3 * sizeof(double));
-    pl.off_pj = o; o = align_up(o + (size_t)pairs_t * pl.kmax_mut * kBlock * 3 * sizeof(double));
+    pl.off_pi = o; o = align_up(o + (size_t)pi * pl.kmax_mut * kBlock * OCm * sizeof(double));
+    pl.off_pj = o; o = align_up(o + (size_t)pj * pl.kmax_mut * kBlock * OCm * sizeof(double));
+    if (pl.msys.size() > 1) { pl.off_mtable = o; o = align_up(o + pl.msys.size() * sizeof(MutSys)); }
     mut_total = o;
   }
 
@@ -1164,23 +1216,29 @@ static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
 
-static void dispatch_mutual(int prec, int K, const MutParams& q, int grid, hipStream_t st)
+template <int MASK>
+static void dispatch_mutual_m(int prec, int K, const MutParams& q, int grid, hipStream_t st)
 {
   const dim3 g(grid), b(kBlock);
   if (prec == RMW_PREC_F32_BLOCKED) {
-    if (K == 4) hipLaunchKernelGGL((mutual_uf_f32<4, RMW_PREC_F32_BLOCKED>), g, b, 0, st, q);
-    else if (K == 2) hipLaunchKernelGGL((mutual_uf_f32<2, RMW_PREC_F32_BLOCKED>), g, b, 0, st, q);
-    else hipLaunchKernelGGL((mutual_uf_f32<1, RMW_PREC_F32_BLOCKED>), g, b, 0, st, q);
+    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
   } else if (prec == RMW_PREC_F32_F64ACC) {
-    if (K == 2) hipLaunchKernelGGL((mutual_uf_f32<2, RMW_PREC_F32_F64ACC>), g, b, 0, st, q);
-    else hipLaunchKernelGGL((mutual_uf_f32<1, RMW_PREC_F32_F64ACC>), g, b, 0, st, q);
+    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
   } else if (prec == RMW_PREC_F32_KAHAN) {
-    if (K == 2) hipLaunchKernelGGL((mutual_uf_f32<2, RMW_PREC_F32_KAHAN>), g, b, 0, st, q);
-    else hipLaunchKernelGGL((mutual_uf_f32<1, RMW_PREC_F32_KAHAN>), g, b, 0, st, q);
+    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
   } else {
-    if (K == 2) hipLaunchKernelGGL((mutual_uf_f64<2>), g, b, 0, st, q);
-    else hipLaunchKernelGGL((mutual_uf_f64<1>), g, b, 0, st, q);
+    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F64, double>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F64, double>), g, b, 0, st, q);
   }
+}
+
+static void dispatch_mutual(int mask, int prec, int K, const MutParams& q, int grid, hipStream_t st)
+{
+  if (mask == kUF) dispatch_mutual_m<kUF>(prec, K, q, grid, st);
+  else dispatch_mutual_m<kUF | kUT | kWF | kWT>(prec, K, q, grid, st);
 }
 
 // Pair-kernel timing records (rmw_profile_*).
@@ -1206,44 +1264,59 @@ static void prof_end(const ProfRec& rec, bool on, hipStream_t st)
   }
 }
 
-static void run_mutual(const Plan& pl, const double* r, const double* f, double* vel, int k, int64_t stride,
-                       double a, double inva, double c1, const rmw_tuning* tuning, char* base, hipStream_t st)
+static void run_mutual(const Plan& pl, const double* r, const double* f, const double* t, double* vel,
+                       double* omg, int k, int64_t stride, double a, double inva, double c1, double c2,
+                       const rmw_tuning* tuning, char* base, hipStream_t st)
 {
+  const bool HU = (pl.mask & (kUF | kUT)) != 0, HW = (pl.mask & (kWF | kWT)) != 0;
+  const bool NF = (pl.mask & (kUF | kWF)) != 0, NT = (pl.mask & (kUT | kWT)) != 0;
+  MutSys* dtable = nullptr;
+  if (pl.msys.size() > 1) {
+    dtable = (MutSys*)(base + pl.off_mtable);
+    stage_upload(dtable, pl.msys.data(), pl.msys.size() * sizeof(MutSys), st);
+  }
   int grid = (int)std::min<int64_t>(pl.mut_items, (int64_t)INT32_MAX);
   if (tuning && tuning->max_wgs > 0) grid = std::min(grid, tuning->max_wgs);
   for (int k0 = 0, K = group_k(k, pl.kmax_mut); k0 < k; k0 += K) {
     K = group_k(k - k0, pl.kmax_mut);
     MutParams q{};
     q.r = r;
-    q.f = f + (size_t)k0 * stride;
+    q.f = NF ? f + (size_t)k0 * stride : nullptr;
+    q.t = NT ? t + (size_t)k0 * stride : nullptr;
     q.rhs_stride = stride;
-    q.a = a; q.inva = inva; q.c1 = c1;
+    q.a = a; q.inva = inva; q.c1 = c1; q.c2 = c2;
     q.part_i = (double*)(base + pl.off_pi);
     q.part_j = (double*)(base + pl.off_pj);
-    q.n = pl.ntot;
-    q.n_t = pl.n_t;
+    q.table = dtable;
+    q.single = pl.msys[0];
+    q.nsys = (int)pl.msys.size();
     q.n_items = pl.mut_items;
     q.chunk_tiles = pl.chunk_tiles;
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st);
-    dispatch_mutual(pl.prec, K, q, grid, st);
+    dispatch_mutual(pl.mask, pl.prec, K, q, grid, st);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
 
     MutReduceParams rp{};
     rp.r = r;
-    rp.f = q.f;
-    rp.vel = vel + (size_t)k0 * stride;
+    rp.f = q.f; rp.t = q.t;
+    rp.vel = HU ? vel + (size_t)k0 * stride : nullptr;
+    rp.omg = HW ? omg + (size_t)k0 * stride : nullptr;
     rp.part_i = q.part_i;
     rp.part_j = q.part_j;
+    rp.table = dtable;
+    rp.single = q.single;
+    rp.nsys = q.nsys;
+    rp.mask = pl.mask;
     rp.K = K;
-    rp.n = pl.ntot;
-    rp.n_t = pl.n_t;
+    rp.ntot = pl.ntot;
     rp.chunk_tiles = pl.chunk_tiles;
     rp.rhs_stride = stride;
-    rp.a = a; rp.inva = inva; rp.c1 = c1;
-    const int64_t nthreads = pl.ntot * K * 3;
+    rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
+    const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
+    const int64_t nthreads = pl.ntot * K * OC;
     hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
     RMW_CHECK_HIP(hipGetLastError());
   }
@@ -1284,7 +1357,7 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
 
   const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
   if (pl.mut_ok && periodic == 0) {
-    run_mutual(pl, r, f, vel, k, stride, a, inva, c1, tuning, base, st);
+    run_mutual(pl, r, f, t, vel, omg, k, stride, a, inva, c1, c2, tuning, base, st);
     return;
   }
 

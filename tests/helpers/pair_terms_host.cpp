@@ -17,6 +17,9 @@ static Vec3<T> apply_block(int block, const Geom<T>& g, T zi, T zj, T vx, T vy, 
     case 1: return apply_ut(g, coef_rot(g, zi), vx, vy, vz);
     case 2: return apply_wf(g, coef_rot(g, zj), vx, vy, vz);
     case 4: return apply_uf_t(g, coef_uf(g, zi, zj), vx, vy, vz);  // UF_ij^T
+    case 5: return apply_ut_t(g, coef_rot(g, zi), vx, vy, vz);     // UT_ij^T
+    case 6: return apply_wf_t(g, coef_rot(g, zj), vx, vy, vz);     // WF_ij^T
+    case 7: return apply_wt_t(g, coef_wt(g), vx, vy, vz);          // WT_ij^T
     default: return apply_wt(g, coef_wt(g), vx, vy, vz);
   }
 }
@@ -35,8 +38,8 @@ static Vec3<T> apply_self(int block, T h, T vx, T vy, T vz)
 template <typename T>
 static int pair_matrix(int block, const double* ri, const double* rj, int self, double eta, double a, double* m9)
 {
-  if (block < 0 || block > 4) return -1;
-  const int p = (block == 0 || block == 4) ? 1 : (block == 3 ? 3 : 2);
+  if (block < 0 || block > 7) return -1;
+  const int p = (block == 0 || block == 4) ? 1 : ((block == 3 || block == 7) ? 3 : 2);
   const double norm = 1.0 / (8.0 * M_PI * eta * std::pow(a, p));
   T dx = T((ri[0] - rj[0]) / a), dy = T((ri[1] - rj[1]) / a), dz = T((ri[2] - rj[2]) / a);
   T zi = T(ri[2] / a), zj = T(rj[2] / a);
@@ -44,7 +47,7 @@ static int pair_matrix(int block, const double* ri, const double* rj, int self, 
   for (int c = 0; c < 3; ++c) {
     T v[3] = {T(0), T(0), T(0)};
     v[c] = T(1);
-    Vec3<T> u = self ? apply_self<T>(block & 3, zi, v[0], v[1], v[2])
+    Vec3<T> u = self ? apply_self<T>(block < 4 ? block : 0, zi, v[0], v[1], v[2])
                      : apply_block<T>(block, g, zi, zj, v[0], v[1], v[2]);
     m9[0 * 3 + c] = double(u.x) * norm;
     m9[1 * 3 + c] = double(u.y) * norm;

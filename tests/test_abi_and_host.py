@@ -91,26 +91,33 @@ def test_pair_term_header_matches_oracle(block):
     assert worst32 < 2e-5
 
 
-def test_transposed_uf_is_the_swapped_pair():
-    """apply_uf_t on the pair (i, j) is the oracle's UF block of (j, i): the
-    identity the symmetric kernel uses to serve both blobs of a pair."""
+# transposed block of the pair (i, j) -> (oracle block of the pair (j, i),
+# untransposed block of (i, j) that it must differ from)
+TRANSPOSES = {4: (O.UF, 0), 5: (O.WF, 1), 6: (O.UT, 2), 7: (O.WT, 3)}
+
+
+@pytest.mark.parametrize("tblock", sorted(TRANSPOSES))
+def test_transposed_blocks_are_the_swapped_pair(tblock):
+    """apply_*_t on the pair (i, j) is the oracle's block of (j, i) (UF_ji =
+    UF_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, WT_ji = WT_ij^T): the
+    identities the symmetric kernels use to serve both blobs of a pair."""
     lib = _host_lib()
     P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    rng = np.random.default_rng(11)
+    rng = np.random.default_rng(11 + tblock)
     a, eta = 0.3, 0.7
-    worst = 0.0
+    oblock, plain = TRANSPOSES[tblock]
+    worst, differs = 0.0, 0.0
     for t in range(2000):
         ri = np.array([*rng.uniform(-3, 3, 2), rng.uniform(a, 3)])
         rj = np.array([*rng.uniform(-3, 3, 2), rng.uniform(a, 3)])
-        m = np.empty(9)
-        assert lib.pt_pair_matrix_f64(4, P(ri), P(rj), 0, eta, a, P(m)) == 0
-        ref = O.pair_matrix(O.UF, rj, ri, False, eta, a).ravel()
+        m, m0 = np.empty(9), np.empty(9)
+        assert lib.pt_pair_matrix_f64(tblock, P(ri), P(rj), 0, eta, a, P(m)) == 0
+        ref = O.pair_matrix(oblock, rj, ri, False, eta, a).ravel()
         worst = max(worst, np.abs(m - ref).max() / np.abs(ref).max())
-        # and it is not the untransposed block (the wall part is not symmetric)
-        m0 = np.empty(9)
-        lib.pt_pair_matrix_f64(0, P(ri), P(rj), 0, eta, a, P(m0))
+        lib.pt_pair_matrix_f64(plain, P(ri), P(rj), 0, eta, a, P(m0))
+        differs = max(differs, np.abs(m0 - m).max() / np.abs(m).max())
     assert worst < 1e-13
-    assert np.abs(m0 - m).max() > 1e-6 * np.abs(m).max()
+    assert differs > 1e-3          # the transpose is not the block itself
 
 
 def test_icosphere_matches_reference_vertex_file():

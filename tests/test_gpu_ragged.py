@@ -198,21 +198,18 @@ def test_c5_ragged_grand_matvec_per_replica(gpu, c5):
     errs = np.array(errs)
     print(f"  per-replica rel L2 vs FP64 oracle (first {rows} rows): median {np.median(errs):.2e} max {errs.max():.2e}")
     assert errs.max() <= 1e-6
-    # With FP64 pairs and the same work split a replica of the batch gives the
-    # bits of the replica alone (FP32 modes split coordinates on a grid set
-    # by the whole batch's extent, so there the match is to rounding).
-    tun = _lib.Tuning(1024, 0, 0)
-    for prec, tol in (("fp64", 0.0), ("fp32_blocked", 1e-6)):
-        ub, wb = wall_rpy.apply_ragged(GRAND, r, off, force=f, torque=t, eta=eta, a=a, precision=prec, tuning=tun)
-        for s in (0, 17, 63):
-            b0, b1 = int(off[s]), int(off[s + 1])
-            us, wsv = wall_rpy.apply(GRAND, r[b0:b1], force=f[b0:b1], torque=t[b0:b1], eta=eta, a=a,
-                                     precision=prec, tuning=tun)
-            if tol == 0.0:
-                assert torch.equal(ub[b0:b1], us) and torch.equal(wb[b0:b1], wsv), s
-            else:
-                assert rel(ub[b0:b1].cpu().numpy(), us.cpu().numpy()) < tol
-                assert rel(wb[b0:b1].cpu().numpy(), wsv.cpu().numpy()) < tol
+    # With the same kernel (ordered or symmetric) and the same work split, a
+    # replica of the batch gives the bits of the replica alone.
+    for sym in (-1, 1):
+        tun = _lib.Tuning(1024, 0, 0, sym, 0)
+        for prec in ("fp64", "fp32_blocked"):
+            ub, wb = wall_rpy.apply_ragged(GRAND, r, off, force=f, torque=t, eta=eta, a=a, precision=prec,
+                                           tuning=tun)
+            for s in (0, 17, 63):
+                b0, b1 = int(off[s]), int(off[s + 1])
+                us, wsv = wall_rpy.apply(GRAND, r[b0:b1], force=f[b0:b1], torque=t[b0:b1], eta=eta, a=a,
+                                         precision=prec, tuning=tun)
+                assert torch.equal(ub[b0:b1], us) and torch.equal(wb[b0:b1], wsv), (sym, prec, s)
 
 
 def test_c5_ragged_lanczos_per_replica(gpu, c5):

@@ -1,6 +1,8 @@
-"""Symmetric UF evaluation (each unordered pair once, UF_ji = UF_ij^T)
-against the ordered kernels and the FP64 oracle: ragged tile edges, every
-precision mode, k right-hand sides, multi-tile work items, persistent grids."""
+"""Symmetric evaluation (each unordered pair once: UF_ji = UF_ij^T, WT_ji =
+WT_ij^T, UT_ji = WF_ij^T) of the UF block and of the grand mobility, on one
+suspension and on ragged batches, against the ordered kernels and the FP64
+oracle: tile edges, every precision mode, k right-hand sides, multi-tile
+work items, persistent grids."""
 import numpy as np
 import pytest
 import torch
@@ -97,10 +99,102 @@ def test_symmetric_c3_size_matches_ordered(gpu):
 def test_symmetric_rejected_where_it_does_not_apply(gpu):
     r = _geom(50, 1)
     f = np.ones((50, 3))
-    with pytest.raises(_lib.RmwError):
+    with pytest.raises(_lib.RmwError):      # only UF alone or all four blocks
         wall_rpy.apply(RMW_UT, r, torque=f, eta=1.0, a=0.2, tuning=_lib.Tuning(0, 0, 0, 1, 0))
     # periodic images: the ordered kernel is used (auto), and agrees with the oracle
     L = np.array([6.0, 5.0, 0.0])
     u, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=1.0, a=0.2, periodic_length=L, precision="fp64")
     ref = O.block_apply(O.UF, r, f, 1.0, 0.2, periodic_length=L).reshape(-1, 3)
     assert rel(u.cpu().numpy(), ref) <= 1e-12
+
+
+GRAND = 15
+
+
+def _grand(r, f, t, prec, k, sym, offsets=None, chunk=0, wgs=0):
+    tun = _lib.Tuning(chunk, wgs, 0, sym, 0)
+    if offsets is None:
+        u, w = wall_rpy.apply(GRAND, r, force=f, torque=t, eta=0.9, a=0.3, precision=prec, k=k, tuning=tun)
+    else:
+        u, w = wall_rpy.apply_ragged(GRAND, r, offsets, force=f, torque=t, eta=0.9, a=0.3, precision=prec, k=k,
+                                     tuning=tun)
+    return np.concatenate([u.cpu().numpy().reshape(k, -1), w.cpu().numpy().reshape(k, -1)], axis=1)
+
+
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
+@pytest.mark.parametrize("n,k", [(300, 1), (1000, 2), (513, 3)])
+def test_symmetric_grand_matches_ordered(gpu, prec, n, k):
+    r = _geom(n, 20 + n)
+    rng = np.random.default_rng(n)
+    f = rng.standard_normal((k, n, 3))
+    t = rng.standard_normal((k, n, 3))
+    print(f"case symmetric grand n={n} k={k} {prec}", flush=True)
+    sym = _grand(r, f, t, prec, k, 1)
+    ordered = _grand(r, f, t, prec, k, -1)
+    for j in range(k):
+        assert rel(sym[j], ordered[j]) <= TOL[prec], j
+
+
+def test_symmetric_grand_vs_oracle(gpu):
+    n = 400
+    r = _geom(n, 31)
+    rng = np.random.default_rng(32)
+    f, t = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    print("case symmetric grand n=400 vs oracle", flush=True)
+    ou = O.block_apply(O.UF, r, f, 0.9, 0.3) + O.block_apply(O.UT, r, t, 0.9, 0.3)
+    ow = O.block_apply(O.WF, r, f, 0.9, 0.3) + O.block_apply(O.WT, r, t, 0.9, 0.3)
+    ref = np.concatenate([ou, ow])
+    for prec, tol in (("fp64", 1e-12), ("fp32_blocked", 1e-6), ("fp32", 1e-6)):
+        got = _grand(r, f[None], t[None], prec, 1, 1)[0]
+        assert rel(got, ref) <= tol, prec
+
+
+SIZES_R = [0, 300, 7, 0, 600, 1, 256, 257]
+
+
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp64"])
+@pytest.mark.parametrize("mask", [RMW_UF, GRAND])
+def test_symmetric_ragged_matches_ordered_and_oracle(gpu, prec, mask):
+    off = np.concatenate([[0], np.cumsum(SIZES_R)]).astype(np.int64)
+    n = int(off[-1])
+    r = np.vstack([_geom(m, 40 + m) for m in SIZES_R if m > 0])
+    rng = np.random.default_rng(41)
+    f, t = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    print(f"case symmetric ragged mask={mask} {prec}", flush=True)
+
+    def run(sym, chunk=0, wgs=0):
+        u, w = wall_rpy.apply_ragged(mask, r, off, force=f, torque=t if mask == GRAND else None, eta=0.9, a=0.3,
+                                     precision=prec, tuning=_lib.Tuning(chunk, wgs, 0, sym, 0))
+        parts = [u.cpu().numpy()] + ([w.cpu().numpy()] if w is not None else [])
+        return parts
+    sym = run(1)
+    ordered = run(-1)
+    for a_, b_ in zip(sym, ordered):
+        assert rel(a_, b_) <= TOL[prec]
+    split = run(1, 512, 5)                      # multi-tile items, small persistent grid
+    for a_, b_ in zip(split, ordered):
+        assert rel(a_, b_) <= TOL[prec]
+    # per system against the oracle
+    for s in range(len(SIZES_R)):
+        b0, b1 = int(off[s]), int(off[s + 1])
+        if b1 == b0:
+            continue
+        rr = r[b0:b1]
+        ou = O.block_apply(O.UF, rr, f[b0:b1], 0.9, 0.3)
+        if mask == GRAND:
+            ou = ou + O.block_apply(O.UT, rr, t[b0:b1], 0.9, 0.3)
+            ow = O.block_apply(O.WF, rr, f[b0:b1], 0.9, 0.3) + O.block_apply(O.WT, rr, t[b0:b1], 0.9, 0.3)
+            assert rel(sym[1][b0:b1].ravel(), ow) <= (1e-12 if prec == "fp64" else 2e-6), s
+        assert rel(sym[0][b0:b1].ravel(), ou) <= (1e-12 if prec == "fp64" else 2e-6), s
+
+
+def test_symmetric_c5_matches_ordered(gpu):
+    w = workloads.c5_ragged_rollers()
+    off, r = w["offsets"], w["r"]
+    n = int(off[-1])
+    rng = np.random.default_rng(51)
+    f, t = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    print("case symmetric C5 grand ragged fp32_blocked", flush=True)
+    sym = _grand(r, f[None], t[None], "fp32_blocked", 1, 1, offsets=off)
+    ordered = _grand(r, f[None], t[None], "fp32_blocked", 1, -1, offsets=off)
+    assert rel(sym, ordered) <= 1e-6
