@@ -202,8 +202,13 @@ def main(argv=None):
     info = wall_rpy.device_info()
     from rigidmultiblobswall_amd import autotune
     shape = autotune.lookup(RMW_UF, args.precision, 1, float(n) * n, False)
-    launch = ({"chunk_len": shape.chunk_len, "max_wgs": shape.max_wgs, "rhs_group": shape.rhs_group,
-               "source": "autotuner cache"} if shape is not None else {"source": "built-in heuristic"})
+    launch = wall_rpy.describe(RMW_UF, n, 1, args.precision)
+    launch["source"] = "autotuner cache" if shape is not None else "built-in heuristic"
+    if shape is not None:
+        launch["max_wgs"] = shape.max_wgs
+    kname = {("symmetric", True): "mutual_kernel<UF,K=1,{p}>", ("symmetric", False): "mutual_kernel<UF,K=1,fp64>",
+             ("ordered", True): "pair_kernel_f32<UF,K=1,{p}>", ("ordered", False): "pair_kernel_f64<UF,K=1>"}[
+        (launch["kernel"], args.precision != "fp64")].format(p=args.precision)
     out = {
         "metric": METRIC,
         "value": value,
@@ -222,8 +227,7 @@ def main(argv=None):
                    "device": info, "launch_shape": launch},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": load_traffic(args.workload, args.precision),
-                     "kernel": (f"pair_kernel_f32<UF,K=1,{args.precision}>" if args.precision != "fp64"
-                                else "pair_kernel_f64<UF,K=1>"),
+                     "kernel": kname,
                      "kernel_mean_ms": pk.mean_ms, "kernel_launches": pk.launches,
                      "flop_per_pair": FLOP_PER_PAIR["UF"]},
     }

@@ -90,6 +90,22 @@ int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * The launch plan rmw_wall_rpy_apply would use for these sizes (periodic:
+ * whether L has a positive entry): which kernel, the right-hand sides per
+ * pass, the source chunk and the number of work items.  No reference
+ * counterpart; for benchmarks and tuning reports.
+ */
+typedef struct rmw_plan_info {
+  int32_t symmetric;       /* 1: each unordered pair once for both blobs */
+  int32_t rhs_group;       /* right-hand sides per pair-kernel pass */
+  int64_t chunk_len;       /* sources per work item */
+  int64_t work_items;      /* workgroup work items of one pass */
+  int64_t workspace_bytes;
+} rmw_plan_info;
+int rmw_wall_rpy_describe(int mask, int prec, int64_t n, int k, const rmw_tuning* tuning, int periodic,
+                          rmw_plan_info* info);
+
+/*
  * Ragged batch: nsys independent suspensions stored back to back, system s
  * owning blobs [offsets[s], offsets[s+1]) (offsets is a HOST array of nsys+1
  * entries, offsets[0] = 0).  Same arrays and semantics as rmw_wall_rpy_apply

@@ -45,6 +45,11 @@ class Segments(ctypes.Structure):
                 ("part_stride", ctypes.c_int64)]
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("symmetric", ctypes.c_int32), ("rhs_group", ctypes.c_int32), ("chunk_len", ctypes.c_int64),
+                ("work_items", ctypes.c_int64), ("workspace_bytes", ctypes.c_int64)]
+
+
 _vp = ctypes.c_void_p
 _dp = ctypes.c_void_p  # device pointers are passed as integers
 _i64 = ctypes.c_int64
@@ -60,6 +65,8 @@ PROTOTYPES = {
     "rmw_last_error": (ctypes.c_char_p, []),
     "rmw_version": (ctypes.c_char_p, []),
     "rmw_wall_rpy_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, _szp]),
+    "rmw_wall_rpy_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, ctypes.c_int,
+                                             ctypes.POINTER(PlanInfo)]),
     "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,
                                           _i64, ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz, _vp]),
     "rmw_wall_rpy_ragged_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,

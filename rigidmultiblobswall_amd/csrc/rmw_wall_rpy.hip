@@ -1441,6 +1441,22 @@ extern "C" int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
   });
 }
 
+extern "C" int rmw_wall_rpy_describe(int mask, int prec, int64_t n, int k, const rmw_tuning* tuning,
+                                     int periodic, rmw_plan_info* info)
+{
+  return guarded([&] {
+    RMW_REQUIRE(info != nullptr, "info is null");
+    RMW_REQUIRE(n >= 0, "n must be >= 0");
+    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
+    const bool mut = pl.mut_ok && !periodic;
+    info->symmetric = mut ? 1 : 0;
+    info->rhs_group = mut ? pl.kmax_mut : pl.kmax;
+    info->chunk_len = mut ? pl.chunk_tiles * kBlock : (pl.sys.empty() ? 0 : pl.sys[0].chunk_len);
+    info->work_items = mut ? pl.mut_items : pl.total_items;
+    info->workspace_bytes = (int64_t)pl.total;
+  });
+}
+
 extern "C" int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double* force,
                                   const double* torque, double* velocity, double* omega, int64_t n, int k,
                                   int64_t ld, double eta, double a, const double* L, const rmw_tuning* tuning,

@@ -185,6 +185,20 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int
                                              tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
 
 
+def describe(mask: int, n: int, k: int = 1, precision="fp32", periodic_length=None, tuning=None) -> dict:
+    """The launch plan ``apply`` would use (kernel, RHS per pass, chunk,
+    work items), with the same tuning resolution as ``apply``."""
+    lib = _lib.load()
+    _device()
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
+    L = np.zeros(3) if periodic_length is None else np.asarray(periodic_length, dtype=np.float64)
+    info = _lib.PlanInfo()
+    _lib.check(lib.rmw_wall_rpy_describe(mask, _lib.precision_code(precision), n, k, tp, int(bool(np.any(L > 0))),
+                                         ctypes.byref(info)))
+    return {"kernel": "symmetric" if info.symmetric else "ordered", "rhs_group": info.rhs_group,
+            "chunk_len": info.chunk_len, "work_items": info.work_items, "workspace_bytes": info.workspace_bytes}
+
+
 _info_cache: dict = {}
 
 

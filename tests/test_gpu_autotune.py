@@ -24,7 +24,10 @@ def test_tune_small_uf(gpu):
     assert len(res["table"]) == 2 * 8 + 3 * 2
     assert all(row["rel_diff"] <= 2e-6 for row in res["table"])
     best = res["best"]
-    assert best["ms"] <= min(row["ms"] for row in res["table"] if row["ok"]) + 1e-12
+    shapes = {(r["chunk_len"], r["max_wgs"], r["rhs_group"], r["symmetric"]) for r in res["table"] if r["ok"]}
+    assert (best["chunk_len"], best["max_wgs"], best["rhs_group"], best["symmetric"]) in shapes
+    confirmed = [r for r in res["table"] if "ms_confirmed" in r]
+    assert len(confirmed) == 3 and best["ms"] == min(r["ms_confirmed"] for r in confirmed)
     assert best["chunk_len"] % 64 == 0
 
 

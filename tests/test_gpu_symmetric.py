@@ -198,3 +198,15 @@ def test_symmetric_c5_matches_ordered(gpu):
     sym = _grand(r, f[None], t[None], "fp32_blocked", 1, 1, offsets=off)
     ordered = _grand(r, f[None], t[None], "fp32_blocked", 1, -1, offsets=off)
     assert rel(sym, ordered) <= 1e-6
+
+
+def test_describe_reports_the_plan(gpu):
+    d = wall_rpy.describe(RMW_UF, 32768, 1, "fp32_blocked", tuning=_lib.Tuning(0, 0, 0, 0, 0))
+    assert d["kernel"] == "symmetric" and d["work_items"] > 0 and d["workspace_bytes"] > 0
+    d = wall_rpy.describe(RMW_UF, 32768, 1, "fp32_blocked", periodic_length=[10.0, 0, 0],
+                          tuning=_lib.Tuning(0, 0, 0, 0, 0))
+    assert d["kernel"] == "ordered"
+    d = wall_rpy.describe(RMW_UF, 32768, 1, "fp32_blocked", tuning=_lib.Tuning(1024, 0, 0, -1, 0))
+    assert d["kernel"] == "ordered" and d["chunk_len"] == 1024 and d["work_items"] == 128 * 32
+    d = wall_rpy.describe(RMW_UF, 2048, 1, "fp64", tuning="heuristic")      # too few tile pairs
+    assert d["kernel"] == "ordered"
