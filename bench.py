@@ -60,14 +60,28 @@ def load_workload(name: str):
     return w, desc
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(r, f, eta, a, target_seconds=16.0):
     """Time the FP64 oracle (fast-math build, all usable cores) on a
     bounded sample of the same matvec (whole products repeated, or a row
-    block when one product is longer than the budget); returns pairs/s."""
+    block when one product is longer than the budget); returns pairs/s,
+    with the CPU model, core count and a single-thread rate on a small
+    row block."""
     from oracle import oracle as O
     n = r.shape[0]
     threads = O.num_threads("fast")
     rows0 = max(1, min(n, 256))
+    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows0), kind="fast")          # warm-up
     t = time.perf_counter()
     O.block_apply(O.UF, r, f, eta, a, rows=(0, rows0), kind="fast")
     dt = max(time.perf_counter() - t, 1e-6)
@@ -78,9 +92,37 @@ def cpu_baseline(r, f, eta, a, target_seconds=16.0):
     for _ in range(reps):
         O.block_apply(O.UF, r, f, eta, a, rows=(0, rows), kind="fast")
     dt = time.perf_counter() - t
+    # one thread on a small block, for the per-core rate
+    O.set_num_threads(1, "fast")
+    rows1 = max(1, min(n, 32))
+    t = time.perf_counter()
+    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows1), kind="fast")
+    single = rows1 * n / (time.perf_counter() - t)
+    O.set_num_threads(threads, "fast")
     return {"value": reps * rows * n / dt, "unit": "blob-pairs/s", "cores": threads, "kind": "port",
             "sample": f"{reps} x UF rows 0..{rows} of {n} (x {n} sources) of the same workload; "
-                      f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads; {dt:.1f} s"}
+                      f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads; {dt:.1f} s",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "single_thread_pairs_per_s": single}
+
+
+def end_to_end(r_h, f_h, eta, a, precision, reps=20):
+    """Median time of the numpy drop-in signature (host arrays in, host
+    array out: H2D copies, the product, D2H copy), the path a caller of the
+    reference's mobility functions sees."""
+    import torch
+    from rigidmultiblobswall_amd import mobility
+    for _ in range(3):
+        mobility.single_wall_mobility_trans_times_force_hip(r_h, f_h, eta, a, precision=precision)
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        mobility.single_wall_mobility_trans_times_force_hip(r_h, f_h, eta, a, precision=precision)
+        ts.append(time.perf_counter() - t)
+    ms = 1e3 * float(np.median(ts))
+    n = r_h.shape[0]
+    return {"ms": ms, "pairs_per_s": float(n) * n / (ms * 1e-3),
+            "path": "mobility.single_wall_mobility_trans_times_force_hip(numpy r, numpy f) -> numpy u"}
 
 
 def gmres_leg(steps: int, epochs: int, evals: int, log):
@@ -233,6 +275,8 @@ def main(argv=None):
     }
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     log(f"matvec: {ms:.4f} ms/step, {value:.4g} pairs/s, pair kernel {pk.mean_ms:.4f} ms")
+    out["end_to_end"] = end_to_end(r_h, f_h.reshape(-1, 3), eta, a, args.precision)
+    log(f"end to end through the numpy signature: {out['end_to_end']['ms']:.3f} ms")
     if not args.no_gmres:
         out["gmres"] = gmres_leg(args.gmres_steps, args.gmres_epochs, args.gmres_evals, log)
     if not args.no_cpu:

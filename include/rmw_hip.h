@@ -123,6 +123,26 @@ int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vectors, const
                               size_t workspace_bytes, void* stream);
 
 /*
+ * Blobs of different radii, source -> target: velocity (nt, 3) =
+ * B_t M(x, y) B_s force, force (ns, 3), radii per blob, lengths in absolute
+ * units (not scaled by a radius), normalisation 1/(8 pi eta).  wall != 0:
+ * single-wall Rotne-Prager-Blake with heights shifted to max(z, radius)
+ * and damping B = z/radius below the radius; wall == 0: unbounded RPY of
+ * different radii.  L: pseudo-periodic lengths as rmw_wall_rpy_apply.
+ * Sources and targets may be the same arrays (then the self pair is the
+ * RPY self mobility 1/(6 pi eta a) plus its wall image).  Replaces
+ *   mobility_trans_times_force_source_target_numba  mobility/mobility_numba.py:1479-1658
+ *   single_wall_mobility_trans_times_force_source_target_numba  mobility/mobility.py:551-591
+ *   no_wall_mobility_trans_times_force_source_target_numba      mobility/mobility.py:593-615
+ * as selected by the 'radii*' options of multi_bodies/multi_bodies.py:266-287.
+ */
+int rmw_radii_workspace_size(int prec, int64_t ns, int64_t nt, const rmw_tuning* tuning, size_t* bytes);
+int rmw_radii_apply(int prec, int wall, const double* source, const double* target, const double* force,
+                    const double* radius_source, const double* radius_target, double* velocity, int64_t ns,
+                    int64_t nt, double eta, const double* L, const rmw_tuning* tuning, void* workspace,
+                    size_t workspace_bytes, void* stream);
+
+/*
  * Optional timing of the pair kernels with HIP events recorded on the
  * launch stream.  rmw_profile_collect synchronises on the recorded events,
  * returns the number of pair-kernel launches since the last collect and

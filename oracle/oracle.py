@@ -61,6 +61,12 @@ def _lib(kind: str = "strict") -> ctypes.CDLL:
                                           ctypes.c_double, dp]
     lib.orc_block_pair_matrix.restype = ctypes.c_int
     lib.orc_num_threads.restype = ctypes.c_int
+    lib.orc_radii_apply.argtypes = [dp, dp, dp, dp, dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_int64, ctypes.c_double, dp, ctypes.c_int, dp]
+    lib.orc_radii_apply.restype = ctypes.c_int
+    lib.orc_radii_pair_matrix.argtypes = [dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          dp]
+    lib.orc_radii_pair_matrix.restype = ctypes.c_int
     if kind == "mutant":
         lib.orc_set_mutation.argtypes = [ctypes.c_int]
     _LIBS[kind] = lib
@@ -78,6 +84,10 @@ def set_mutation(m: int) -> None:
 
 def num_threads(kind: str = "strict") -> int:
     return int(_lib(kind).orc_num_threads())
+
+
+def set_num_threads(n: int, kind: str = "strict") -> None:
+    _lib(kind).orc_set_num_threads(int(n))
 
 
 def block_apply(block: int, r_vectors, vec, eta: float, a: float, periodic_length=None,
@@ -136,6 +146,39 @@ def pair_matrix(block: int, ri, rj, self_term: bool, eta: float, a: float, kind=
     rc = _lib(kind).orc_block_pair_matrix(block, _ptr(ri), _ptr(rj), int(self_term), eta, a, _ptr(m))
     if rc:
         raise ValueError("bad block")
+    return m.reshape(3, 3)
+
+
+def radii_apply(source, target, force, radius_source, radius_target, eta: float, periodic_length=None,
+                wall: bool = True, rows: Optional[tuple] = None, kind: str = "strict") -> np.ndarray:
+    """u = B_t M B_s f for blobs of different radii, source -> target
+    (mobility_numba.py:1479-1658 with the wrapper mobility.py:551-591;
+    wall=False: the unbounded kernel of mobility.py:593-615).  (3 N_t,)."""
+    src = np.ascontiguousarray(np.asarray(source, dtype=np.float64).reshape(-1, 3))
+    trg = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(-1, 3))
+    f = np.ascontiguousarray(np.asarray(force, dtype=np.float64).reshape(-1, 3))
+    rs = np.ascontiguousarray(np.asarray(radius_source, dtype=np.float64).reshape(-1))
+    rt = np.ascontiguousarray(np.asarray(radius_target, dtype=np.float64).reshape(-1))
+    ns, nt = src.shape[0], trg.shape[0]
+    if f.shape[0] != ns or rs.size != ns or rt.size != nt:
+        raise ValueError("size mismatch")
+    L = np.zeros(3) if periodic_length is None else np.ascontiguousarray(
+        np.asarray(periodic_length, dtype=np.float64).reshape(3))
+    out = np.zeros((nt, 3))
+    t0, t1 = (0, nt) if rows is None else rows
+    rc = _lib(kind).orc_radii_apply(_ptr(src), _ptr(trg), _ptr(f), _ptr(rs), _ptr(rt), ns, nt, t0, t1, float(eta),
+                                    _ptr(L), int(bool(wall)), _ptr(out))
+    if rc:
+        raise ValueError("bad row range")
+    return out.reshape(-1)
+
+
+def radii_pair_matrix(ri, rj, a: float, b: float, eta: float, wall: bool = True, kind="strict") -> np.ndarray:
+    """3x3 block of one (target radius a, source radius b) pair, no wrapper."""
+    m = np.empty(9)
+    ri = np.ascontiguousarray(ri, dtype=np.float64)
+    rj = np.ascontiguousarray(rj, dtype=np.float64)
+    _lib(kind).orc_radii_pair_matrix(_ptr(ri), _ptr(rj), float(a), float(b), float(eta), int(bool(wall)), _ptr(m))
     return m.reshape(3, 3)
 
 

@@ -538,3 +538,173 @@ int orc_num_threads(void)
   return 1;
 #endif
 }
+
+/* Threads of the following calls (timing of the CPU baseline). */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+/* ---------------------------------------------------------------------- */
+/* Blobs of different radii, source -> target:                             */
+/* mobility_numba.py:1479-1658 (mobility_trans_times_force_source_target_  */
+/* numba) in its expression order; lengths are NOT scaled by a radius, the */
+/* normalisation is 1/(8 pi eta).  a = target radius, b = source radius;   */
+/* x3, y3 = target / source heights.  wall = 0: the RPY part only.         */
+/* ---------------------------------------------------------------------- */
+static inline mat3 radii_pair(double rx, double ry, double rz, double x3, double y3, double a, double b, int wall)
+{
+  const double fourOverThree = 4.0 / 3.0;
+  mat3 M;
+  double r2 = rx * rx + ry * ry + rz * rz;
+  double r = sqrt(r2);
+  double C1, C2;
+  if (r > (a + b)) {
+    double a2 = a * a, b2 = b * b;
+    C1 = (1 + (b2 + a2) / (3 * r2)) / r;
+    C2 = ((1 - (b2 + a2) / r2) / r2) / r;
+  } else if (r > fabs(b - a)) {
+    double r3 = r2 * r;
+    C1 = ((16 * (b + a) * r3 - pow(pow(b - a, 2) + 3 * r2, 2)) / (32 * r3)) * fourOverThree / (b * a);
+    C2 = ((3 * pow(pow(b - a, 2) - r2, 2) / (32 * r3)) / r2) * fourOverThree / (b * a);
+  } else {
+    double largest_radius = a > b ? a : b;
+    C1 = fourOverThree / largest_radius;
+    C2 = 0;
+  }
+  M.xx = C1 + C2 * rx * rx;
+  M.xy = C2 * rx * ry;
+  M.xz = C2 * rx * rz;
+  M.yy = C1 + C2 * ry * ry;
+  M.yz = C2 * ry * rz;
+  M.zz = C1 + C2 * rz * rz;
+  M.yx = M.xy;
+  M.zx = M.xz;
+  M.zy = M.yz;
+  if (wall) {
+    rz = x3 + y3;
+    r2 = rx * rx + ry * ry + rz * rz;
+    r = sqrt(r2);
+    double a2 = a * a, b2 = b * b;
+    double r3 = r2 * r, r5 = r3 * r2, r7 = r5 * r2, r9 = r7 * r2;
+
+    M.xx -= ((1 + (b2 + a2) / (3.0 * r2)) + (1 - (b2 + a2) / r2) * rx * rx / r2) / r;
+    M.xy -= ((1 - (b2 + a2) / r2) * rx * ry / r2) / r;
+    M.xz += ((1 - (b2 + a2) / r2) * rx * rz / r2) / r;
+    M.yx -= ((1 - (b2 + a2) / r2) * ry * rx / r2) / r;
+    M.yy -= ((1 + (b2 + a2) / (3.0 * r2)) + (1 - (b2 + a2) / r2) * ry * ry / r2) / r;
+    M.yz += ((1 - (b2 + a2) / r2) * ry * rz / r2) / r;
+    M.zx -= ((1 - (b2 + a2) / r2) * rz * rx / r2) / r;
+    M.zy -= ((1 - (b2 + a2) / r2) * rz * ry / r2) / r;
+    M.zz += ((1 + (b2 + a2) / (3.0 * r2)) + (1 - (b2 + a2) / r2) * rz * rz / r2) / r;
+
+    M.xx -= 2 * (x3 * y3 * (1.0 / r3 - 3 * rx * rx / r5));
+    M.xy -= 2 * (x3 * y3 * (-3 * rx * ry / r5));
+    M.xz += 2 * (-rx * x3 / r3 + x3 * y3 * (-3 * rx * rz / r5));
+    M.yx -= 2 * (x3 * y3 * (-3 * ry * rx / r5));
+    M.yy -= 2 * (x3 * y3 * (1.0 / r3 - 3 * ry * ry / r5));
+    M.yz += 2 * (-ry * x3 / r3 + x3 * y3 * (-3 * ry * rz / r5));
+    M.zx -= 2 * (-y3 * rx / r3 + x3 * y3 * (-3 * rz * rx / r5));
+    M.zy -= 2 * (-y3 * ry / r3 + x3 * y3 * (-3 * rz * ry / r5));
+    M.zz += 2 * (-1.0 / r - rz * x3 / r3 - y3 * rz / r3 + x3 * y3 * (1.0 / r3 - 3 * rz * rz / r5));
+
+    double ca = 2 * a2 / 3.0;
+    M.xx -= ca * (-y3 * (3 * rz / r5 - 15 * rz * rx * rx / r7));
+    M.xy -= ca * (-y3 * (-15 * rz * rx * ry / r7));
+    M.xz += ca * (3 * rx * rz / r5 - y3 * (3 * rx / r5 - 15 * rz * rx * rz / r7));
+    M.yx -= ca * (-y3 * (-15 * rz * ry * rx / r7));
+    M.yy -= ca * (-y3 * (3 * rz / r5 - 15 * rz * ry * ry / r7));
+    M.yz += ca * (3 * ry * rz / r5 - y3 * (3 * ry / r5 - 15 * rz * ry * rz / r7));
+    M.zx -= ca * (-y3 * (3 * rx / r5 - 15 * rz * rz * rx / r7));
+    M.zy -= ca * (-y3 * (3 * ry / r5 - 15 * rz * rz * ry / r7));
+    M.zz += ca * (-1.0 / r3 + 3 * rz * rz / r5 - y3 * (3 * rz / r5 + 3 * rz / r5 + 3 * rz / r5 - 15 * rz * rz * rz / r7));
+
+    double cb = 2 * b2 / 3.0;
+    M.xx -= cb * (-x3 * (3 * rz / r5 - 15 * rz * rx * rx / r7));
+    M.xy -= cb * (-x3 * (-15 * rz * rx * ry / r7));
+    M.xz += cb * (-x3 * (3 * rx / r5 - 15 * rz * rx * rz / r7));
+    M.yx -= cb * (-x3 * (-15 * rz * ry * rx / r7));
+    M.yy -= cb * (-x3 * (3 * rz / r5 - 15 * rz * ry * ry / r7));
+    M.yz += cb * (-x3 * (3 * ry / r5 - 15 * rz * ry * rz / r7));
+    M.zx -= cb * (3 * rz * rx / r5 - x3 * (3 * rx / r5 - 15 * rz * rz * rx / r7));
+    M.zy -= cb * (3 * rz * ry / r5 - x3 * (3 * ry / r5 - 15 * rz * rz * ry / r7));
+    M.zz += cb * (-1.0 / r3 + 3 * rz * rz / r5 - x3 * (3 * rz / r5 + 3 * rz / r5 + 3 * rz / r5 - 15 * rz * rz * rz / r7));
+
+    double cab = 2 * b2 * a2 / 3.0;
+    M.xx -= cab * (-1.0 / r5 + 5 * rz * rz / r7 + 5 * rx * rx / r7 - 35 * rz * rz * rx * rx / r9);
+    M.xy -= cab * (5 * rx * ry / r7 - 35 * rz * rz * rx * ry / r9);
+    M.xz += cab * (5 * rx * rz / r7 + 5 * rx * rz / r7 + 5 * rx * rz / r7 - 35 * rz * rz * rx * rz / r9);
+    M.yx -= cab * (5 * ry * rx / r7 - 35 * rz * rz * ry * rx / r9);
+    M.yy -= cab * (-1.0 / r5 + 5 * rz * rz / r7 + 5 * ry * ry / r7 - 35 * rz * rz * ry * ry / r9);
+    M.yz += cab * (5 * ry * rz / r7 + 5 * ry * rz / r7 + 5 * ry * rz / r7 - 35 * rz * rz * rz * ry / r9);
+    M.zx -= cab * (5 * rz * rx / r7 + 5 * rz * rx / r7 + 5 * rz * rx / r7 - 35 * rz * rz * rz * rx / r9);
+    M.zy -= cab * (5 * rz * ry / r7 + 5 * rz * ry / r7 + 5 * rz * ry / r7 - 35 * rz * rz * rz * ry / r9);
+    M.zz += cab * (-1.0 / r5 + 5 * rz * rz / r7 - 1.0 / r5 + 5 * rz * rz / r7 - 1.0 / r5 + 5 * rz * rz / r7 +
+                   5 * rz * rz / r7 + 5 * rz * rz / r7 + 5 * rz * rz / r7 - 35 * rz * rz * rz * rz / r9);
+  }
+  return M;
+}
+
+/* u = B_t M(x, y) B_s f for targets [t0, t1) (wall != 0: heights shifted to
+ * max(z, radius) and damping B = z / radius below the radius, as the
+ * wrapper mobility.py:551-591 with shift_heights_different_radius :87-99
+ * and damping_matrix_B_different_radius :102-119; wall == 0: the bare
+ * kernel as no_wall_mobility_trans_times_force_source_target_numba
+ * :593-615).  out has nt*3 entries; rows outside [t0, t1) are untouched. */
+int orc_radii_apply(const double *src, const double *trg, const double *force, const double *rs,
+                    const double *rt, int64_t ns, int64_t nt, int64_t t0, int64_t t1, double eta,
+                    const double *L, int wall, double *out)
+{
+  if (t0 < 0 || t1 > nt || t0 > t1) return -1;
+  const double norm_fact_f = 1.0 / (8.0 * M_PI * eta);
+  const double Lx = L ? L[0] : 0.0, Ly = L ? L[1] : 0.0, Lz = L ? L[2] : 0.0;
+  const int px = Lx > 0, py = Ly > 0, pz = Lz > 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = t0; i < t1; ++i) {
+    double a = rt[i];
+    double xi = trg[3 * i], yi = trg[3 * i + 1], zi = trg[3 * i + 2];
+    double Bt = 1.0;
+    if (wall) {
+      if (!(zi > a)) { Bt = zi < a ? zi / a : 1.0; zi = a; }
+    }
+    double ux = 0, uy = 0, uz = 0;
+    for (int bx = -px; bx <= px; ++bx)
+      for (int by = -py; by <= py; ++by)
+        for (int bz = -pz; bz <= pz; ++bz)
+          for (int64_t j = 0; j < ns; ++j) {
+            double b = rs[j];
+            double zj = src[3 * j + 2], Bs = 1.0;
+            if (wall) {
+              if (!(zj > b)) { Bs = zj < b ? zj / b : 1.0; zj = b; }
+            }
+            double rx = project_image(xi - src[3 * j], Lx, bx);
+            double ry = project_image(yi - src[3 * j + 1], Ly, by);
+            double rz = project_image(zi - zj, Lz, bz);
+            mat3 M = radii_pair(rx, ry, rz, zi, zj, a, b, wall);
+            double fx = Bs * force[3 * j], fy = Bs * force[3 * j + 1], fz = Bs * force[3 * j + 2];
+            ux += (M.xx * fx + M.xy * fy + M.xz * fz) * norm_fact_f;
+            uy += (M.yx * fx + M.yy * fy + M.yz * fz) * norm_fact_f;
+            uz += (M.zx * fx + M.zy * fy + M.zz * fz) * norm_fact_f;
+          }
+    out[3 * i] = Bt * ux;
+    out[3 * i + 1] = Bt * uy;
+    out[3 * i + 2] = Bt * uz;
+  }
+  return 0;
+}
+
+/* The 3x3 block of one (target, source) pair, normalised, no wrapper. */
+int orc_radii_pair_matrix(const double *ri, const double *rj, double a, double b, double eta, int wall, double *m9)
+{
+  mat3 M = radii_pair(ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2], ri[2], rj[2], a, b, wall);
+  const double n = 1.0 / (8.0 * M_PI * eta);
+  m9[0] = M.xx * n; m9[1] = M.xy * n; m9[2] = M.xz * n;
+  m9[3] = M.yx * n; m9[4] = M.yy * n; m9[5] = M.yz * n;
+  m9[6] = M.zx * n; m9[7] = M.zy * n; m9[8] = M.zz * n;
+  return 0;
+}

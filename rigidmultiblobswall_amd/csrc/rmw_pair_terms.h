@@ -381,6 +381,91 @@ RMW_HD Vec3<T> apply_wt_t(const Geom<T>& g, const CoefWT<T>& c, T vx, T vy, T vz
 }
 
 // ---------------------------------------------------------------------------
+// Blobs of different radii (target radius a, source radius b), lengths in
+// absolute units, normalisation 1/(8 pi eta) folded in by the caller.
+// mobility_numba.py:1479-1658.  The self pair (d = 0) needs no special
+// case: it falls in the inner RPY regime.
+//
+//   RPY:  C1 I + C2 d d^T with, for r = |d| and s = a^2 + b^2,
+//         r > a + b:     C1 = (1 + s/(3r^2))/r,  C2 = (1 - s/r^2)/r^3
+//         r > |b - a|:   C1 = 4/(3ab) [(a + b)/2 - ((b-a)^2 + 3r^2)^2/(32 r^3)]
+//                        C2 = 4/(3ab) (3/32) ((b-a)^2 - r^2)^2 / r^5
+//         otherwise:     C1 = 4/(3 max(a, b)),   C2 = 0
+//   wall: -S D v with D = diag(1, 1, -1), R = (d_x, d_y, x3 + y3) (x3, y3 the
+//         target and source heights), rho = |R| and
+//         S = alpha I + beta R R^T + g1 R z^T + g2 z R^T + delta z z^T,
+//   the image Stokeslet, its doublets weighted by ca = 2a^2/3, cb = 2b^2/3
+//   and the quadrupole weighted by cab = 2a^2 b^2/3 collected per dyad.
+// ---------------------------------------------------------------------------
+template <typename T> struct CoefRadii {
+  T C1, C2;                 // RPY
+  T al, be, g1, g2, de;     // wall dyads
+};
+
+template <typename T>
+RMW_HD CoefRadii<T> coef_radii(T dx, T dy, T dz, T Rz, T x3, T y3, T a, T b, T inva, T invb, bool wall)
+{
+  CoefRadii<T> c;
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  const T invr = rsq(r2);
+  const T r = r2 > T(0) ? r2 * invr : T(0);
+  const T w = invr * invr;
+  const T s = a * a + b * b;
+  const T C1_far = fma_(s * w, T(1.0 / 3.0), T(1)) * invr;
+  const T C2_far = fma_(-s, w, T(1)) * w * invr;
+  const T dab = b - a;
+  const T dab2 = dab * dab;
+  const T k1 = T(4.0 / 3.0) * inva * invb;
+  const T t = fma_(T(3), r2, dab2);
+  const T invr3 = w * invr;
+  const T C1_mid = k1 * (T(0.5) * (a + b) - T(1.0 / 32.0) * t * t * invr3);
+  const T u = dab2 - r2;
+  const T C2_mid = k1 * T(3.0 / 32.0) * u * u * invr3 * w;
+  const T C1_in = T(4.0 / 3.0) * (inva < invb ? inva : invb);
+  const bool far = r > a + b;
+  const bool mid = r > (dab < T(0) ? -dab : dab);
+  c.C1 = far ? C1_far : (mid ? C1_mid : C1_in);
+  c.C2 = far ? C2_far : (mid ? C2_mid : T(0));
+  if (wall) {
+    const T rho2 = dx * dx + dy * dy + Rz * Rz;
+    const T q = rsq(rho2);
+    const T qw = q * q;
+    const T q3 = q * qw, q5 = q3 * qw, q7 = q5 * qw, q9 = q7 * qw;
+    const T ca = T(2.0 / 3.0) * a * a, cb = T(2.0 / 3.0) * b * b, cab = ca * b * b;
+    const T Rz2 = Rz * Rz;
+    const T xy = x3 * y3;
+    const T m = ca * y3 + cb * x3;
+    const T P = fma_(s * qw, T(1.0 / 3.0), T(1)) * q;
+    const T Q = fma_(-s, qw, T(1)) * q3;
+    c.al = P + T(2) * xy * q3 - T(3) * m * Rz * q5 + cab * (T(5) * Rz2 * q7 - q5);
+    c.be = Q - T(6) * xy * q5 + T(15) * m * Rz * q7 + cab * (T(5) * q7 - T(35) * Rz2 * q9);
+    const T e = T(3) * (ca - cb) * q5;
+    const T z10 = T(10) * cab * Rz * q7;
+    c.g1 = x3 * (e - T(2) * q3) + z10;
+    c.g2 = y3 * (-e - T(2) * q3) + z10;
+    c.de = -T(2) * q - (ca + cb) * q3 - T(2) * cab * q5;
+  } else {
+    c.al = c.be = c.g1 = c.g2 = c.de = T(0);
+  }
+  return c;
+}
+
+template <typename T>
+RMW_HD Vec3<T> apply_radii(T dx, T dy, T dz, T Rz, const CoefRadii<T>& c, T vx, T vy, T vz)
+{
+  const T p = dx * vx + dy * vy;
+  const T dv = p + dz * vz;
+  const T Rv = p - Rz * vz;               // R . D v
+  const T h = c.be * Rv - c.g1 * vz;      // coefficient of R
+  const T q = c.C2 * dv;
+  Vec3<T> u;
+  u.x = (c.C1 - c.al) * vx + (q - h) * dx;
+  u.y = (c.C1 - c.al) * vy + (q - h) * dy;
+  u.z = (c.C1 + c.al) * vz + q * dz - h * Rz - (c.g2 * Rv - c.de * vz);
+  return u;
+}
+
+// ---------------------------------------------------------------------------
 // Self blocks (i == j, central box), mobility_numba.py:202-208 + 245-252,
 // :653-657, :1040-1044, :1291-1297.  h = z_i / a (effective height).
 // ---------------------------------------------------------------------------

@@ -37,6 +37,7 @@
 
 #include "rmw_internal.h"
 #include "rmw_pair_terms.h"
+#include "rmw_accum.h"
 
 namespace rmw {
 
@@ -134,45 +135,7 @@ struct __align__(16) SrcF32 {
 // ---------------------------------------------------------------------------
 // 3. pair kernels
 // ---------------------------------------------------------------------------
-// Per-target accumulator policies.
-template <int ACC> struct Acc;
-template <> struct Acc<RMW_PREC_F32_F64ACC> {
-  double v = 0.0;
-  __device__ __forceinline__ void add(float x) { v += (double)x; }
-  __device__ __forceinline__ void flush() {}
-  __device__ __forceinline__ double get() const { return v; }
-};
-template <> struct Acc<RMW_PREC_F32_BLOCKED> {
-  double v = 0.0;
-  float p = 0.0f;
-  __device__ __forceinline__ void add(float x) { p += x; }
-  __device__ __forceinline__ void flush()
-  {
-    v += (double)p;
-    p = 0.0f;
-  }
-  __device__ __forceinline__ double get() const { return v + (double)p; }
-};
-template <> struct Acc<RMW_PREC_F32_KAHAN> {
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float x)
-  {
-    float y = x - c;
-    float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ void flush() {}
-  __device__ __forceinline__ double get() const { return (double)s - (double)c; }
-};
-template <> struct Acc<RMW_PREC_F64> {
-  double v = 0.0;
-  __device__ __forceinline__ void add(double x) { v += x; }
-  __device__ __forceinline__ void flush() {}
-  __device__ __forceinline__ double get() const { return v; }
-};
-
-constexpr int kSub = 16;  // sources per FP32 sub-sum in the blocked mode
+// Per-target accumulator policies: rmw_accum.h.
 
 template <int K, int ACC>
 __device__ __forceinline__ void flush_all(Acc<ACC> (&u)[K][3], Acc<ACC> (&w)[K][3])
@@ -250,12 +213,6 @@ __device__ __forceinline__ void pair_eval(T dx, T dy, T dz, T zi, T zj,
   }
 }
 
-// Minimum-image shift count of mobility_numba.py:184-193: int(d/L + 0.5 sign(d)).
-__device__ __forceinline__ double image_count(double d, double L)
-{
-  double s = (double)((d > 0.0) - (d < 0.0));
-  return trunc(d / L + 0.5 * s);
-}
 
 // Partial sums of one work item: slot (chunk, rhs, target) of the system.
 template <int MASK, int K, typename A>

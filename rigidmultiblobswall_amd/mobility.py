@@ -141,3 +141,32 @@ def single_wall_fluid_mobility_hip(r_vectors, eta, a, *args, **kwargs):
 
 
 __all__.append("single_wall_fluid_mobility_hip")
+
+
+# ---------------------------------------------------------------------------
+# Blobs of different radii (source -> target), the 'radii*' implementations
+# ---------------------------------------------------------------------------
+def single_wall_mobility_trans_times_force_source_target_hip(source, target, force, radius_source, radius_target,
+                                                             eta, *args, **kwargs):
+    """u = B_t M(z_eff) B_s f for blobs of different radii above the wall;
+    drop-in for mobility.py:551-591 (kernel mobility_numba.py:1479-1658)."""
+    L, prec, tun = _opts(kwargs)
+    u = wall_rpy.apply_radii(source, target, force, radius_source, radius_target, eta, periodic_length=L, wall=True,
+                             precision=prec, tuning=tun)
+    return _host(u)
+
+
+def no_wall_mobility_trans_times_force_source_target_hip(source, target, force, radius_source, radius_target,
+                                                         eta, *args, **kwargs):
+    """Unbounded RPY of different radii; drop-in for mobility.py:593-615."""
+    L, prec, tun = _opts(kwargs)
+    u = wall_rpy.apply_radii(source, target, force, radius_source, radius_target, eta, periodic_length=L,
+                             wall=False, precision=prec, tuning=tun)
+    return _host(u)
+
+
+def mobility_radii_trans_times_force_hip(r_vectors, force, eta, a, radius_blobs, function=None, *args, **kwargs):
+    """M f with per-blob radii, sources = targets; drop-in for
+    mobility.py:1369-1374 (``function`` defaults to the wall kernel)."""
+    fn = function or single_wall_mobility_trans_times_force_source_target_hip
+    return fn(r_vectors, r_vectors, force, radius_blobs, radius_blobs, eta, *args, **kwargs)

@@ -7,7 +7,8 @@
                                              (quaternion_integrator_rollers.py:59-92)
 
 New implementation values served by librmw_hip: 'hip' (FP32 pair terms, FP64
-per-target sums), 'hip_fp32_blocked', 'hip_fp32_kahan' and 'hip_fp64'.  None
+per-target sums), 'hip_fp32_blocked', 'hip_fp32_kahan' and 'hip_fp64', and
+for blobs of different radii 'radii_hip[_no_wall][_<precision>]'.  None
 contains 'numba' or 'pycuda', so the reference's substring dispatch never
 mistakes them for its own backends.
 """
@@ -35,7 +36,25 @@ def _precision(implementation: str) -> str:
 
 
 def set_mobility_vector_prod(implementation, *args, **kwargs):
-    """M*F at the blob level, signature f(r_vectors, force, eta, a, *args, **kwargs)."""
+    """M*F at the blob level, signature f(r_vectors, force, eta, a, *args, **kwargs).
+
+    'radii_hip' / 'radii_hip_no_wall' (optionally with a precision suffix,
+    e.g. 'radii_hip_fp64') select the blobs-of-different-radii product as the
+    reference's 'radii*' options do (multi_bodies.py:266-287); the radii come
+    from kwargs['radius_blobs'] (the reference reads them from the bodies)."""
+    if implementation.startswith("radii_hip"):
+        rest = implementation[len("radii_hip"):]
+        no_wall = rest.startswith("_no_wall")
+        if no_wall:
+            rest = rest[len("_no_wall"):]
+        prec = _precision("hip" + rest)
+        radius_blobs = kwargs.get("radius_blobs")
+        if radius_blobs is None:
+            raise ValueError("radii implementations need radius_blobs=")
+        fn = mb.no_wall_mobility_trans_times_force_source_target_hip if no_wall else \
+            mb.single_wall_mobility_trans_times_force_source_target_hip
+        return partial(mb.mobility_radii_trans_times_force_hip, radius_blobs=radius_blobs,
+                       function=partial(fn, precision=prec))
     return partial(mb.single_wall_mobility_trans_times_force_hip, precision=_precision(implementation))
 
 

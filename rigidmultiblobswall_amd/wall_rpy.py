@@ -185,6 +185,35 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int
                                              tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
 
 
+def apply_radii(source, target, force, radius_source, radius_target, eta: float, periodic_length=None,
+                wall: bool = True, precision="fp32_blocked", tuning=None, out: Optional[torch.Tensor] = None):
+    """Different radii, source -> target: u (N_t, 3) = B_t M B_s f (wall) or
+    the unbounded RPY of different radii (wall=False)."""
+    lib = _lib.load()
+    device = _device()
+    src = torch.as_tensor(source)
+    ns = src.numel() // 3
+    trg = torch.as_tensor(target)
+    nt = trg.numel() // 3
+    src = _as_device_f64(src, ns, None, device, "source")
+    trg = _as_device_f64(trg, nt, None, device, "target")
+    f = _as_device_f64(force, ns, None, device, "force")
+    rs = torch.as_tensor(radius_source).to(device=device, dtype=torch.float64).contiguous().reshape(-1)
+    rt = torch.as_tensor(radius_target).to(device=device, dtype=torch.float64).contiguous().reshape(-1)
+    if rs.numel() != ns or rt.numel() != nt:
+        raise ValueError("one radius per source and per target blob is required")
+    u = out if out is not None else torch.empty((nt, 3), dtype=torch.float64, device=device)
+    prec = _lib.precision_code(precision)
+    tp = _lib.tuning_ptr(None if isinstance(tuning, str) else tuning)
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_radii_workspace_size(prec, ns, nt, tp, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value, device, slot="radii")
+    _lib.check(lib.rmw_radii_apply(prec, int(bool(wall)), src.data_ptr(), trg.data_ptr(), f.data_ptr(),
+                                   rs.data_ptr(), rt.data_ptr(), u.data_ptr(), ns, nt, float(eta),
+                                   _L_array(periodic_length), tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
+    return u
+
+
 def describe(mask: int, n: int, k: int = 1, precision="fp32", periodic_length=None, tuning=None) -> dict:
     """The launch plan ``apply`` would use (kernel, RHS per pass, chunk,
     work items), with the same tuning resolution as ``apply``."""

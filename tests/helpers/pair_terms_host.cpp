@@ -65,3 +65,35 @@ extern "C" int pt_pair_matrix_f32(int block, const double* ri, const double* rj,
 {
   return pair_matrix<float>(block, ri, rj, self, eta, a, m9);
 }
+
+// Different radii: the (target radius a, source radius b) block, normalised
+// by 1/(8 pi eta), positions and radii in absolute units.
+template <typename T>
+static int radii_matrix(const double* ri, const double* rj, double a, double b, double eta, int wall, double* m9)
+{
+  const double norm = 1.0 / (8.0 * M_PI * eta);
+  T dx = T(ri[0] - rj[0]), dy = T(ri[1] - rj[1]), dz = T(ri[2] - rj[2]);
+  T Rz = T(ri[2] + rj[2]);
+  CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, T(ri[2]), T(rj[2]), T(a), T(b), T(1.0 / a), T(1.0 / b), wall != 0);
+  for (int k = 0; k < 3; ++k) {
+    T v[3] = {T(0), T(0), T(0)};
+    v[k] = T(1);
+    Vec3<T> u = apply_radii<T>(dx, dy, dz, Rz, c, v[0], v[1], v[2]);
+    m9[0 * 3 + k] = double(u.x) * norm;
+    m9[1 * 3 + k] = double(u.y) * norm;
+    m9[2 * 3 + k] = double(u.z) * norm;
+  }
+  return 0;
+}
+
+extern "C" int pt_radii_matrix_f64(const double* ri, const double* rj, double a, double b, double eta, int wall,
+                                   double* m9)
+{
+  return radii_matrix<double>(ri, rj, a, b, eta, wall, m9);
+}
+
+extern "C" int pt_radii_matrix_f32(const double* ri, const double* rj, double a, double b, double eta, int wall,
+                                   double* m9)
+{
+  return radii_matrix<float>(ri, rj, a, b, eta, wall, m9);
+}
