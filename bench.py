@@ -19,7 +19,9 @@ Prints ONE JSON line (rank 0):
                GMRES from x0 = 0 against the learned warm start (network
                trained on the GPU from a short trajectory), per-solve times
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f]
+       (c3r: the different-radii product on the C3 geometry; c3f: the
+       blob-blob soft-repulsion forces on it)
 """
 from __future__ import annotations
 
@@ -35,7 +37,10 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 METRIC = "wall-RPY matvec blob-pairs/s; GMRES time-to-1e-8 residual (learned x0 vs plain)"
-FLOP_PER_PAIR = {"UF": 164, "UT": 95, "WF": 95, "WT": 101, "grand": 386}
+FLOP_PER_PAIR = {"UF": 164, "UT": 95, "WF": 95, "WT": 101, "grand": 386,
+                 "radii": 154,   # radii: counted on the dyadic form of the kernel (FMA = 2)
+                 "forces": 22}   # difference 3, r^2 5, sqrt + reciprocal 2, exponent 2, exp 1,
+                                 # prefactor 2, radius sum 1, accumulation 6
 PEAK_FP32_TFLOPS = 157.3
 PRECISION_NOTE = {"fp32": " (FP32 pair terms, FP64 per-target sums)",
                   "fp32_blocked": " (FP32 pair terms, FP32 sub-sums of <= 16 sources folded into FP64 per-target sums)",
@@ -55,6 +60,16 @@ def load_workload(name: str):
     elif name == "c3":
         w = workloads.c3_rollers()
         desc = "C3 rollers: N=32768 blobs, a=0.656, eta=1e-3, UF block"
+    elif name == "c3f":
+        w = workloads.c3_rollers()
+        w = dict(w, forces=dict(repulsion_strength=0.0165677856, debye_length=0.0656, blob_radius=0.656))
+        desc = ("C3 geometry, blob-blob soft repulsion with the rollers example's parameters "
+                "(eps=0.0165677856, b=0.0656, a=0.656): N=32768, every pair, open box")
+    elif name == "c3r":
+        w = workloads.c3_rollers()
+        w = dict(w, radii=np.random.default_rng(33).uniform(0.5, 0.8, w["r"].shape[0]))
+        desc = ("C3 geometry with blob radii U[0.5, 0.8] (seed 33): N=32768, eta=1e-3, different-radii wall "
+                "RPY (radii kernel, sources = targets)")
     else:
         raise SystemExit(f"unknown workload {name}")
     return w, desc
@@ -71,7 +86,7 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(r, f, eta, a, target_seconds=16.0):
+def cpu_baseline(r, f, eta, a, target_seconds=16.0, radii=None, force_params=None):
     """Time the FP64 oracle (fast-math build, all usable cores) on a
     bounded sample of the same matvec (whole products repeated, or a row
     block when one product is longer than the budget); returns pairs/s,
@@ -80,27 +95,40 @@ def cpu_baseline(r, f, eta, a, target_seconds=16.0):
     from oracle import oracle as O
     n = r.shape[0]
     threads = O.num_threads("fast")
+    if force_params is not None:
+        fp = force_params
+        def run(rows):
+            O.blob_forces(r, fp["repulsion_strength"], fp["debye_length"], fp["blob_radius"], rows=rows, kind="fast")
+    elif radii is not None:
+        # different radii: the oracle's radii product on a block of rows
+        def run(rows):
+            O.radii_apply(r, r, f, radii, radii, eta, rows=rows, kind="fast")
+    else:
+        def run(rows):
+            O.block_apply(O.UF, r, f, eta, a, rows=rows, kind="fast")
     rows0 = max(1, min(n, 256))
-    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows0), kind="fast")          # warm-up
+    run((0, rows0))          # warm-up
     t = time.perf_counter()
-    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows0), kind="fast")
+    run((0, rows0))
     dt = max(time.perf_counter() - t, 1e-6)
     per_pair = dt / (rows0 * n)
     rows = int(min(n, max(rows0, target_seconds / (per_pair * n))))
     reps = max(1, int(target_seconds / (per_pair * rows * n)))
     t = time.perf_counter()
     for _ in range(reps):
-        O.block_apply(O.UF, r, f, eta, a, rows=(0, rows), kind="fast")
+        run((0, rows))
     dt = time.perf_counter() - t
     # one thread on a small block, for the per-core rate
     O.set_num_threads(1, "fast")
     rows1 = max(1, min(n, 32))
     t = time.perf_counter()
-    O.block_apply(O.UF, r, f, eta, a, rows=(0, rows1), kind="fast")
+    run((0, rows1))
     single = rows1 * n / (time.perf_counter() - t)
     O.set_num_threads(threads, "fast")
     return {"value": reps * rows * n / dt, "unit": "blob-pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} x UF rows 0..{rows} of {n} (x {n} sources) of the same workload; "
+            "sample": f"{reps} x {'forces' if force_params else 'radii' if radii is not None else 'UF'} rows 0..{rows} "
+                      f"of {n} (x {n} sources) "
+                      f"of the same workload; "
                       f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads; {dt:.1f} s",
             "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "single_thread_pairs_per_s": single}
 
@@ -221,8 +249,23 @@ def main(argv=None):
     f = torch.from_numpy(f_h).to(dev)
     u = torch.empty((1, n, 3), dtype=torch.float64, device=dev)
 
+    radii = "radii" in w
+    fparams = w.get("forces")
+    if radii or fparams:
+        u2 = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    if radii:
+        rad = torch.from_numpy(w["radii"]).to(dev)
+    if fparams:
+        from rigidmultiblobswall_amd import forces
+        fprec = "fp64" if args.precision == "fp64" else "fp32"
+
     def step():
-        wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u)
+        if fparams:
+            forces.blob_forces(r, precision=fprec, out=u2, **fparams)
+        elif radii:
+            wall_rpy.apply_radii(r, r, f, rad, rad, eta, precision=args.precision, out=u2)
+        else:
+            wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u)
 
     for _ in range(args.warmup):
         step()
@@ -238,9 +281,12 @@ def main(argv=None):
     ms = start.elapsed_time(stop) / args.steps
     pairs = float(n) * n
     value = pairs / (ms * 1e-3)
-    flop = FLOP_PER_PAIR["UF"] * pairs
-    achieved = flop / (pk.mean_ms * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
+    op = "forces" if fparams else "radii" if radii else "UF"
+    kernel_ms = pk.mean_ms
+    flop = FLOP_PER_PAIR[op] * pairs
+    achieved = flop / (kernel_ms * 1e-3) / 1e12
+    # the force kernel computes in FP64 in both modes (FP32 only for exp)
+    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" or fparams else PEAK_FP32_TFLOPS
     info = wall_rpy.device_info()
     from rigidmultiblobswall_amd import autotune
     shape = autotune.lookup(RMW_UF, args.precision, 1, float(n) * n, False)
@@ -251,6 +297,12 @@ def main(argv=None):
     kname = {("symmetric", True): "mutual_kernel<UF,K=1,{p}>", ("symmetric", False): "mutual_kernel<UF,K=1,fp64>",
              ("ordered", True): "pair_kernel_f32<UF,K=1,{p}>", ("ordered", False): "pair_kernel_f64<UF,K=1>"}[
         (launch["kernel"], args.precision != "fp64")].format(p=args.precision)
+    if radii:
+        launch = {"kernel": "radii (ordered)"}
+        kname = f"radii_kernel<{args.precision}>"
+    if fparams:
+        launch = {"kernel": "forces (ordered)", "mode": fprec}
+        kname = f"force_kernel<{'fp64' if fprec == 'fp64' else 'fp32 exp'}>"
     out = {
         "metric": METRIC,
         "value": value,
@@ -270,17 +322,23 @@ def main(argv=None):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": load_traffic(args.workload, args.precision),
                      "kernel": kname,
-                     "kernel_mean_ms": pk.mean_ms, "kernel_launches": pk.launches,
-                     "flop_per_pair": FLOP_PER_PAIR["UF"]},
+                     "kernel_mean_ms": kernel_ms, "kernel_launches": pk.launches,
+                     "flop_per_pair": FLOP_PER_PAIR[op]},
     }
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    log(f"matvec: {ms:.4f} ms/step, {value:.4g} pairs/s, pair kernel {pk.mean_ms:.4f} ms")
-    out["end_to_end"] = end_to_end(r_h, f_h.reshape(-1, 3), eta, a, args.precision)
-    log(f"end to end through the numpy signature: {out['end_to_end']['ms']:.3f} ms")
+    log(f"matvec: {ms:.4f} ms/step, {value:.4g} pairs/s, pair kernel {kernel_ms:.4f} ms")
+    if fparams:
+        out["metric"] = "blob-blob soft-repulsion forces, blob-pairs/s"
+        out["dtype"] = "f64"
+        out["config"]["precision"] = {"fp64": "FP64 sqrt, quotient and exp; FP64 sums",
+                                      "fp32": "FP64 geometry and sums, FP32 exponential"}[fprec]
+    if not radii and not fparams:
+        out["end_to_end"] = end_to_end(r_h, f_h.reshape(-1, 3), eta, a, args.precision)
+        log(f"end to end through the numpy signature: {out['end_to_end']['ms']:.3f} ms")
     if not args.no_gmres:
         out["gmres"] = gmres_leg(args.gmres_steps, args.gmres_epochs, args.gmres_evals, log)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(r_h, f_h, eta, a)
+        out["cpu_baseline"] = cpu_baseline(r_h, f_h, eta, a, radii=w.get("radii"), force_params=fparams)
     print(json.dumps(out), flush=True)
     return 0
 

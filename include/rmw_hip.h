@@ -142,9 +142,34 @@ int rmw_radii_apply(int prec, int wall, const double* source, const double* targ
                     int64_t nt, double eta, const double* L, const rmw_tuning* tuning, void* workspace,
                     size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Blob-blob soft repulsion (FP64 positions, radii and forces, n x 3 row
+ * major).  force_i = sum_{j != i} f0(r) (r_j - r_i), dr taken to its
+ * minimum image where L[d] > 0 (L may be null), with
+ *   f0 = -(eps/b) exp(-(r - 2a)/b) / r  for r > 2a,  -(eps/b) / r  otherwise,
+ * a = blob_radius, or (a_i + a_j)/2 when radius_blobs is given.  cutoff > 0
+ * drops the pairs with r > cutoff (the neighbour-list form uses
+ * 2a + 30b); 0 keeps every pair.  prec RMW_PREC_F64: FP64 sqrt, quotient
+ * and exp; any other mode: FP64 geometry and sums with a hardware
+ * reciprocal square root (one Newton step) and an FP32 exponential.
+ * Replaces
+ *   blob_blob_force_numba / calc_blob_blob_forces_numba
+ *                                  multi_bodies/forces_numba.py:12-72
+ *   blob_blob_force_radii_numba    multi_bodies/forces_numba.py:75-138
+ *   blob_blob_force_tree_numba     multi_bodies/forces_numba.py:141-241
+ *   blob_blob_force (C++)          multi_bodies/forces.cpp:11-42
+ *   calc_blob_blob_forces_pycuda   multi_bodies/forces_pycuda.py:148-180
+ * as selected by set_blob_blob_forces, multi_bodies_functions.py:249-279.
+ */
+int rmw_blob_forces_workspace_size(int prec, int64_t n, const rmw_tuning* tuning, size_t* bytes);
+int rmw_blob_forces(int prec, const double* r_vectors, const double* radius_blobs, double* force, int64_t n,
+                    double repulsion_strength, double debye_length, double blob_radius, double cutoff,
+                    const double* L, const rmw_tuning* tuning, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
 /*
- * Optional timing of the pair kernels with HIP events recorded on the
- * launch stream.  rmw_profile_collect synchronises on the recorded events,
+ * Optional timing of the pair kernels (wall RPY, different radii and
+ * blob-blob forces) with HIP events recorded on the launch stream.  rmw_profile_collect synchronises on the recorded events,
  * returns the number of pair-kernel launches since the last collect and
  * their total and mean duration in ms, and clears the record.
  */

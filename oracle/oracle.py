@@ -67,6 +67,9 @@ def _lib(kind: str = "strict") -> ctypes.CDLL:
     lib.orc_radii_pair_matrix.argtypes = [dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                           dp]
     lib.orc_radii_pair_matrix.restype = ctypes.c_int
+    lib.orc_blob_forces.argtypes = [dp, dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
+    lib.orc_blob_forces.restype = ctypes.c_int
     if kind == "mutant":
         lib.orc_set_mutation.argtypes = [ctypes.c_int]
     _LIBS[kind] = lib
@@ -171,6 +174,25 @@ def radii_apply(source, target, force, radius_source, radius_target, eta: float,
     if rc:
         raise ValueError("bad row range")
     return out.reshape(-1)
+
+
+def blob_forces(r_vectors, eps: float, debye: float, a: float, periodic_length=None, radii=None,
+                cutoff: float = 0.0, rows: Optional[tuple] = None, kind: str = "strict") -> np.ndarray:
+    """Blob-blob soft repulsion forces (N, 3): multi_bodies/forces_numba.py
+    :12-56 (radii given: :75-122, a = (a_i + a_j)/2; cutoff > 0: only pairs
+    with r <= cutoff, the neighbour-list version :141-201)."""
+    r = np.ascontiguousarray(np.asarray(r_vectors, dtype=np.float64).reshape(-1, 3))
+    n = r.shape[0]
+    L = np.zeros(3) if periodic_length is None else np.ascontiguousarray(
+        np.asarray(periodic_length, dtype=np.float64).reshape(3))
+    rad = None if radii is None else np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(n))
+    out = np.zeros((n, 3))
+    t0, t1 = (0, n) if rows is None else rows
+    rc = _lib(kind).orc_blob_forces(_ptr(r), _ptr(rad) if rad is not None else None, n, t0, t1, float(eps),
+                                    float(debye), float(a), float(cutoff), _ptr(L), _ptr(out))
+    if rc:
+        raise ValueError("bad row range")
+    return out
 
 
 def radii_pair_matrix(ri, rj, a: float, b: float, eta: float, wall: bool = True, kind="strict") -> np.ndarray:

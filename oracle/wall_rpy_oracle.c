@@ -708,3 +708,49 @@ int orc_radii_pair_matrix(const double *ri, const double *rj, double a, double b
   m9[6] = M.zx * n; m9[7] = M.zy * n; m9[8] = M.zz * n;
   return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Blob-blob soft repulsion (multi_bodies/forces_numba.py:12-56,           */
+/* blob_blob_force_numba; radii variant :75-122 with a = (a_i + a_j)/2):   */
+/* F_i = sum_{j != i} f0 dr, dr = r_j - r_i (minimum image for L_d > 0),   */
+/* f0 = -(eps/b) exp(-(r - 2a)/b) / r for r > 2a, else -(eps/b)/max(r,     */
+/* 1e-25).  radii may be NULL (one radius a).  cutoff > 0 keeps only pairs */
+/* with r <= cutoff (the neighbour-list version :141-201 with cutoff       */
+/* 2a + 30b).  Rows [t0, t1).                                              */
+/* ---------------------------------------------------------------------- */
+int orc_blob_forces(const double *r, const double *radii, int64_t n, int64_t t0, int64_t t1, double eps,
+                    double b, double a, double cutoff, const double *L, double *out)
+{
+  if (t0 < 0 || t1 > n || t0 > t1) return -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = t0; i < t1; ++i) {
+    double fx = 0, fy = 0, fz = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (i == j) continue;
+      double dr[3];
+      for (int k = 0; k < 3; ++k) {
+        dr[k] = r[3 * j + k] - r[3 * i + k];
+        double Lk = L ? L[k] : 0.0;
+        if (Lk > 0) {
+          double s = (double)((dr[k] > 0) - (dr[k] < 0));
+          dr[k] -= (double)(long)(dr[k] / Lk + MUT(13, 0.5, 0.0) * s) * Lk;
+        }
+      }
+      double aa = radii ? (radii[i] + radii[MUT(14, j, i)]) * 0.5 : a;
+      double r_norm = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+      double f0;
+      if (cutoff > 0 && r_norm > cutoff) continue;
+      if (r_norm > 2 * aa)
+        f0 = -((eps / b) * exp(-(r_norm - MUT(11, 2.0, 1.0) * aa) / b) / r_norm);
+      else
+        f0 = -((eps / b) / MUT(12, (r_norm > 1e-25 ? r_norm : 1e-25), 1.0));
+      fx += f0 * dr[0];
+      fy += f0 * dr[1];
+      fz += f0 * dr[2];
+    }
+    out[3 * i] = fx;
+    out[3 * i + 1] = fy;
+    out[3 * i + 2] = fz;
+  }
+  return 0;
+}

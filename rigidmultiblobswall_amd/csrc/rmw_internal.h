@@ -65,4 +65,11 @@ struct DeviceProps {
 // Cached per current device.
 const DeviceProps& device_props();
 
+// Pair-kernel timing (rmw_profile_enable / rmw_profile_collect): when
+// enabled, prof_begin/prof_end bracket one pair-kernel launch with HIP
+// events recorded on its stream.
+struct ProfRec { hipEvent_t a, b; };
+void prof_begin(ProfRec& rec, bool& on, hipStream_t st);
+void prof_end(const ProfRec& rec, bool on, hipStream_t st);
+
 }  // namespace rmw

@@ -243,11 +243,15 @@ extern "C" int rmw_radii_apply(int prec, int wall, const double* source, const d
     int grid = (int)std::min<int64_t>(pl.total_items, (int64_t)INT32_MAX);
     if (tuning && tuning->max_wgs > 0) grid = std::min(grid, tuning->max_wgs);
     const dim3 g(grid), b(kRBlock);
+    ProfRec rec{};
+    bool prof = false;
+    prof_begin(rec, prof, st);
     if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
     else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
     else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
     else hipLaunchKernelGGL((radii_kernel<RMW_PREC_F64, double>), g, b, 0, st, q);
     RMW_CHECK_HIP(hipGetLastError());
+    prof_end(rec, prof, st);
     hipLaunchKernelGGL(radii_reduce_kernel, dim3((unsigned)((nt * 3 + 255) / 256)), dim3(256), 0, st, q.partial,
                        pl.n_chunks, nt, target, radius_target, q.wall, velocity);
     RMW_CHECK_HIP(hipGetLastError());

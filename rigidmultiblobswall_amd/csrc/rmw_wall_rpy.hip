@@ -1167,8 +1167,6 @@ void stage_upload_public(void* dst, const void* src, size_t bytes, hipStream_t s
   stage_upload(dst, src, bytes, st);
 }
 
-// Profiling record.
-struct ProfRec { hipEvent_t a, b; };
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
@@ -1199,7 +1197,7 @@ static void dispatch_mutual(int mask, int prec, int K, const MutParams& q, int g
 }
 
 // Pair-kernel timing records (rmw_profile_*).
-static void prof_begin(ProfRec& rec, bool& on, hipStream_t st)
+void prof_begin(ProfRec& rec, bool& on, hipStream_t st)
 {
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -1212,7 +1210,7 @@ static void prof_begin(ProfRec& rec, bool& on, hipStream_t st)
   }
 }
 
-static void prof_end(const ProfRec& rec, bool on, hipStream_t st)
+void prof_end(const ProfRec& rec, bool on, hipStream_t st)
 {
   if (on) {
     RMW_CHECK_HIP(hipEventRecord(rec.b, st));
