@@ -162,6 +162,11 @@ int rmw_radii_apply(int prec, int wall, const double* source, const double* targ
  * as selected by set_blob_blob_forces, multi_bodies_functions.py:249-279.
  */
 int rmw_blob_forces_workspace_size(int prec, int64_t n, const rmw_tuning* tuning, size_t* bytes);
+/* Launch plan of rmw_blob_forces: symmetric (each unordered pair once, the
+ * default when there are at least as many tile pairs as CUs and the
+ * workspace stays within 2 GiB; tuning->symmetric 1 requires it, -1 turns
+ * it off) or ordered, chunk length, work items, workspace bytes. */
+int rmw_blob_forces_describe(int prec, int64_t n, const rmw_tuning* tuning, rmw_plan_info* info);
 int rmw_blob_forces(int prec, const double* r_vectors, const double* radius_blobs, double* force, int64_t n,
                     double repulsion_strength, double debye_length, double blob_radius, double cutoff,
                     const double* L, const rmw_tuning* tuning, void* workspace, size_t workspace_bytes,

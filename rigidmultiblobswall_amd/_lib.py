@@ -69,6 +69,7 @@ PROTOTYPES = {
     "rmw_radii_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _i64, _i64,
                                        ctypes.c_double, _vp, _tp, _vp, _sz, _vp]),
     "rmw_blob_forces_workspace_size": (ctypes.c_int, [ctypes.c_int, _i64, _tp, _szp]),
+    "rmw_blob_forces_describe": (ctypes.c_int, [ctypes.c_int, _i64, _tp, ctypes.POINTER(PlanInfo)]),
     "rmw_blob_forces": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, _vp, _tp, _vp, _sz, _vp]),
     "rmw_wall_rpy_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, ctypes.c_int,

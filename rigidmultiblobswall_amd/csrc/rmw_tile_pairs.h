@@ -1,0 +1,51 @@
+// Tile-pair schedule of the symmetric pair kernels: each unordered pair of
+// blob tiles (S >= T) is visited once; a work item is a target tile T and
+// a run of C consecutive source tiles starting at S >= T.  Also the DPP
+// lane rotation that carries source-side sums across a wave.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rmw {
+
+__host__ __device__ __forceinline__ int64_t tri_index(int64_t S, int64_t T) { return S * (S + 1) / 2 + T; }
+
+// sum_{m=1}^{M} ceil(m / C)
+__host__ __device__ __forceinline__ int64_t ceil_sum(int64_t M, int64_t C)
+{
+  const int64_t q = M / C, r = M - q * C;
+  return C * q * (q + 1) / 2 + r * (q + 1);
+}
+
+// Work items of target tiles 0..T-1 of a system (target tile t has
+// ceil((n_t - t) / C) runs of C source tiles): closed form, no table.
+__host__ __device__ __forceinline__ int64_t items_before(int64_t T, int64_t n_t, int64_t C)
+{
+  return ceil_sum(n_t, C) - ceil_sum(n_t - T, C);
+}
+
+// Lane l receives lane (l + 1) mod 64 (DPP wave_rol:1).
+__device__ __forceinline__ float rol1(float v)
+{
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double rol1(double v)
+{
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ int64_t pair_target_tile(int64_t local, int64_t n_t, int64_t C)
+{
+  int64_t lo = 0, hi = n_t - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (items_before(mid, n_t, C) <= local) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace rmw

@@ -37,6 +37,7 @@
 
 #include "rmw_internal.h"
 #include "rmw_pair_terms.h"
+#include "rmw_tile_pairs.h"
 #include "rmw_accum.h"
 
 namespace rmw {
@@ -444,6 +445,9 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
 //     slot per item (target side) and one per tile pair (S, T) (source side),
 //     reduced in a fixed order.
 // ---------------------------------------------------------------------------
+// Largest workspace the automatic choice gives the symmetric kernels.
+constexpr size_t kMutAutoBytes = size_t(2) << 30;
+
 struct MutSys {
   int64_t b0, n, n_t;   // first blob, blobs, tiles of kBlock
   int64_t item0;        // first work item
@@ -464,34 +468,6 @@ struct MutParams {
   int64_t n_items, chunk_tiles;
 };
 
-__host__ __device__ __forceinline__ int64_t tri_index(int64_t S, int64_t T) { return S * (S + 1) / 2 + T; }
-
-// sum_{m=1}^{M} ceil(m / C)
-__host__ __device__ __forceinline__ int64_t ceil_sum(int64_t M, int64_t C)
-{
-  const int64_t q = M / C, r = M - q * C;
-  return C * q * (q + 1) / 2 + r * (q + 1);
-}
-
-// Work items of target tiles 0..T-1 of a system (target tile t has
-// ceil((n_t - t) / C) runs of C source tiles): closed form, no table.
-__host__ __device__ __forceinline__ int64_t items_before(int64_t T, int64_t n_t, int64_t C)
-{
-  return ceil_sum(n_t, C) - ceil_sum(n_t - T, C);
-}
-
-// Lane l receives lane (l + 1) mod 64 (DPP wave_rol:1).
-__device__ __forceinline__ float rol1(float v)
-{
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double rol1(double v)
-{
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
 __device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t item)
 {
   if (!p.table) return p.single;
@@ -501,16 +477,6 @@ __device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t it
     if (p.table[mid].item0 <= item) lo = mid; else hi = mid - 1;
   }
   return p.table[lo];
-}
-
-__device__ __forceinline__ int64_t mut_target_tile(int64_t local, int64_t n_t, int64_t C)
-{
-  int64_t lo = 0, hi = n_t - 1;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi + 1) >> 1;
-    if (items_before(mid, n_t, C) <= local) lo = mid; else hi = mid - 1;
-  }
-  return lo;
 }
 
 // Symmetric pair kernel.  R = float: FP32 pair terms (ACC: the target-side
@@ -535,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
   for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
     const MutSys Sy = mut_sys_of_item(p, item);
     const int64_t local = item - Sy.item0;
-    const int64_t T = mut_target_tile(local, Sy.n_t, p.chunk_tiles);
+    const int64_t T = pair_target_tile(local, Sy.n_t, p.chunk_tiles);
     const int64_t S0 = T + (local - items_before(T, Sy.n_t, p.chunk_tiles)) * p.chunk_tiles;
     const int64_t S1 = S0 + p.chunk_tiles < Sy.n_t ? S0 + p.chunk_tiles : Sy.n_t;
     const int64_t i = T * kBlock + il;  // within the system
@@ -1070,6 +1036,13 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
     pl.off_pj = o; o = align_up(o + (size_t)pj * pl.kmax_mut * kBlock * OCm * sizeof(double));
     if (pl.msys.size() > 1) { pl.off_mtable = o; o = align_up(o + pl.msys.size() * sizeof(MutSys)); }
     mut_total = o;
+    // the source-side slots grow as N^2 / kBlock: the automatic choice
+    // keeps them within kMutAutoBytes and uses the ordered kernel beyond
+    if (sym == 0 && mut_total > kMutAutoBytes) {
+      pl.mut_ok = false;
+      pl.msys.clear();
+      mut_total = 0;
+    }
   }
 
   size_t o = 0;

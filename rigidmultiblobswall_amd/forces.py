@@ -63,6 +63,17 @@ def blob_forces(r_vectors, repulsion_strength: float, debye_length: float, blob_
     return F
 
 
+def describe(n: int, precision: str = "fp32", tuning=None) -> dict:
+    """The launch plan ``blob_forces`` would use for n blobs."""
+    lib = _lib.load()
+    _device()
+    info = _lib.PlanInfo()
+    _lib.check(lib.rmw_blob_forces_describe(_lib.precision_code(precision), int(n), _lib.tuning_ptr(tuning),
+                                            ctypes.byref(info)))
+    return {"kernel": "symmetric" if info.symmetric else "ordered", "chunk_len": info.chunk_len,
+            "work_items": info.work_items, "workspace_bytes": info.workspace_bytes}
+
+
 def _kw(kwargs):
     L = kwargs.get("periodic_length")
     return dict(periodic_length=L, repulsion_strength=kwargs.get("repulsion_strength"),
@@ -98,7 +109,7 @@ def calc_blob_blob_forces_tree_hip(r_vectors, *args, precision: str = "fp32", **
 
 
 def default_zero_r_vectors(r_vectors, *args, **kwargs) -> np.ndarray:
-    """No blob-blob interaction (multi_bodies_functions.py:240-246)."""
+    """No blob-blob interaction (multi_bodies_functions.py:84-90)."""
     return np.zeros((np.asarray(r_vectors).size // 3, 3))
 
 

@@ -25,17 +25,52 @@ def _cloud(n, seed, side=14.0):
     return r, rng
 
 
+def _sym(s):
+    return _lib.Tuning(0, 0, 0, s, 0)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 @pytest.mark.parametrize("L", [None, (9.0, 8.0, 0.0), (9.0, 8.0, 3.5)])
-def test_forces_match_oracle(gpu, prec, L):
-    n = 1500
+@pytest.mark.parametrize("sym", [1, -1])
+def test_forces_match_oracle(gpu, prec, L, sym):
+    n = 1500                                   # 6 tiles, the last one partial
     r, _ = _cloud(n, 1)
     eps, b, a = 0.3, 0.08, 0.3
-    print(f"case forces n={n} L={L} {prec}", flush=True)
-    F = forces.blob_forces(r, eps, b, a, periodic_length=L, precision=prec).cpu().numpy()
+    print(f"case forces n={n} L={L} {prec} symmetric={sym}", flush=True)
+    F = forces.blob_forces(r, eps, b, a, periodic_length=L, precision=prec, tuning=_sym(sym)).cpu().numpy()
     ref = O.blob_forces(r, eps, b, a, periodic_length=L)
     assert F.shape == (n, 3)
     assert rel(F, ref) <= TOL[prec]
+
+
+def test_symmetric_kernel_choice_and_runs(gpu):
+    """Automatic choice: symmetric once the tile pairs fill the device;
+    tile-run lengths from 1 to beyond the tile count; full and ragged last
+    tiles."""
+    assert forces.describe(32768)["kernel"] == "symmetric"
+    assert forces.describe(300)["kernel"] == "ordered"
+    assert forces.describe(32768, tuning=_sym(-1))["kernel"] == "ordered"
+    for n in (512, 777):
+        r, _ = _cloud(n, 6, side=7.0)
+        ref = O.blob_forces(r, 0.3, 0.1, 0.25)
+        print(f"case forces symmetric runs n={n}", flush=True)
+        for chunk in (256, 512, 768, 4096):
+            F = forces.blob_forces(r, 0.3, 0.1, 0.25, precision="fp64",
+                                   tuning=_lib.Tuning(chunk, 0, 0, 1, 0)).cpu().numpy()
+            assert rel(F, ref) <= 1e-12, (n, chunk)
+        F = forces.blob_forces(r, 0.3, 0.1, 0.25, precision="fp64", tuning=_lib.Tuning(256, 3, 0, 1, 0))
+        assert rel(F.cpu().numpy(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("sym", [1, -1])
+def test_fp32_overlap_branch_is_exact(gpu, sym):
+    """Inside the overlap the FP32-exponential mode takes 2^0 = 1: the force
+    magnitude is eps/b to FP64 rounding."""
+    eps, b, a = 0.4, 0.05, 0.5
+    r = np.array([[0.0, 0.0, 1.0], [0.3, 0.4, 1.0]])
+    F = forces.blob_forces(r, eps, b, a, tuning=_sym(sym)).cpu().numpy()
+    assert abs(np.linalg.norm(F[0]) - eps / b) <= 1e-14 * eps / b
+    assert np.array_equal(F[0], -F[1])
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
