@@ -189,10 +189,10 @@ def gmres_leg(steps: int, epochs: int, evals: int, log):
     return out
 
 
-def load_traffic(workload: str, precision: str):
+def load_traffic(workload: str, precision: str, kernel: str = "pair_kernel_uf"):
     """HBM bytes per pair-kernel launch from the committed rocprof PMC summary
-    of the same workload and precision (None when there is none)."""
-    path = os.path.join(REPO, "profiles", f"pair_kernel_uf_{workload}_{precision}_traffic.json")
+    of the same kernel, workload and precision (None when there is none)."""
+    path = os.path.join(REPO, "profiles", f"{kernel}_{workload}_{precision}_traffic.json")
     if os.path.exists(path):
         with open(path) as fh:
             return json.load(fh).get("traffic_bytes")
@@ -301,8 +301,9 @@ def main(argv=None):
         launch = {"kernel": "radii (ordered)"}
         kname = f"radii_kernel<{args.precision}>"
     if fparams:
-        launch = {"kernel": "forces (ordered)", "mode": fprec}
-        kname = f"force_kernel<{'fp64' if fprec == 'fp64' else 'fp32 exp'}>"
+        launch = forces.describe(n, fprec)
+        kname = (f"{'force_mutual_kernel' if launch['kernel'] == 'symmetric' else 'force_kernel'}"
+                 f"<{'fp64' if fprec == 'fp64' else 'fp32 exp'}>")
     out = {
         "metric": METRIC,
         "value": value,
@@ -320,7 +321,10 @@ def main(argv=None):
                    "precision": args.precision + PRECISION_NOTE.get(args.precision, ""),
                    "device": info, "launch_shape": launch},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": load_traffic(args.workload, args.precision),
+                     "frac": achieved / peak,
+                     "traffic": load_traffic(args.workload, fprec if fparams else args.precision,
+                                             "force_kernel" if fparams else "radii_kernel" if radii
+                                             else "pair_kernel_uf"),
                      "kernel": kname,
                      "kernel_mean_ms": kernel_ms, "kernel_launches": pk.launches,
                      "flop_per_pair": FLOP_PER_PAIR[op]},
