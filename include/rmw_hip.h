@@ -172,6 +172,37 @@ int rmw_blob_forces(int prec, const double* r_vectors, const double* radius_blob
                     const double* L, const rmw_tuning* tuning, void* workspace, size_t workspace_bytes,
                     void* stream);
 
+/* ------------------------------------------------------------------------
+ * Per-roller parts of the rollers time step (FP64, n x 3 row-major device
+ * arrays; omega is a HOST array of 3 values).
+ * ---------------------------------------------------------------------- */
+/* force_z = -g m + (eps_w/b_w) exp(-(z - a)/b_w) (z > a) or + eps_w/b_w,
+ * force_x = force_y = 0; accumulate != 0 adds to force instead.  Replaces
+ * blob_external_forces, multi_bodies/multi_bodies_functions.py:151-196. */
+int rmw_rollers_one_blob_forces(const double* r, int64_t n, double blob_mass, double g, double blob_radius,
+                                double repulsion_strength_wall, double debye_length_wall, int accumulate,
+                                double* force, void* stream);
+/* Velocity (and torque, if torque != NULL) without hydrodynamic
+ * interactions: compute_deterministic_velocity_and_torque_uncorrelated,
+ * quaternion_integrator_rollers.py:985-1080 (free_kinematics != 0: the
+ * torque of get_torque, :1586-1597). */
+int rmw_rollers_uncorrelated_velocity(const double* r, const double* force, int64_t n, const double* omega,
+                                      int free_kinematics, double eta, double a, double* velocity, double* torque,
+                                      void* stream);
+/* sqrt(2 kT/dt) sqrt(mu_self) z + kT d(mu_perp)/dh e_z:
+ * compute_stochastic_linear_velocity_uncorrelated, :1263-1313. */
+int rmw_rollers_uncorrelated_noise(const double* r, const double* z, int64_t n, double eta, double a, double kT,
+                                   double dt, double* velocity, void* stream);
+/* r_new = r_old + dt (c_det det - c_prev prev + stoch) (prev, stoch may be
+ * NULL) and counts[0..1] (device int64) = number of new heights below 0
+ * and below a: the position update of the rollers schemes,
+ * quaternion_integrator_rollers.py:119-302 (Adams-Bashforth c_det = 1.5,
+ * c_prev = 0.5; forward Euler c_det = 1). */
+int rmw_rollers_advance_workspace_size(int64_t n, size_t* bytes);
+int rmw_rollers_advance(const double* r_old, const double* det_velocity, const double* prev_velocity,
+                        const double* stoch_velocity, int64_t n, double c_det, double c_prev, double dt, double a,
+                        double* r_new, int64_t* counts, void* workspace, size_t workspace_bytes, void* stream);
+
 /*
  * Optional timing of the pair kernels (wall RPY, different radii and
  * blob-blob forces) with HIP events recorded on the launch stream.  rmw_profile_collect synchronises on the recorded events,

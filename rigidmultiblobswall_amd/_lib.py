@@ -72,6 +72,15 @@ PROTOTYPES = {
     "rmw_blob_forces_describe": (ctypes.c_int, [ctypes.c_int, _i64, _tp, ctypes.POINTER(PlanInfo)]),
     "rmw_blob_forces": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, _vp, _tp, _vp, _sz, _vp]),
+    "rmw_rollers_one_blob_forces": (ctypes.c_int, [_dp, _i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _vp]),
+    "rmw_rollers_uncorrelated_velocity": (ctypes.c_int, [_dp, _dp, _i64, _hdp, ctypes.c_int, ctypes.c_double,
+                                                         ctypes.c_double, _dp, _dp, _vp]),
+    "rmw_rollers_uncorrelated_noise": (ctypes.c_int, [_dp, _dp, _i64, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_double, _dp, _vp]),
+    "rmw_rollers_advance_workspace_size": (ctypes.c_int, [_i64, _szp]),
+    "rmw_rollers_advance": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, _dp, _dp, _vp, _sz, _vp]),
     "rmw_wall_rpy_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, ctypes.c_int,
                                              ctypes.POINTER(PlanInfo)]),
     "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,
