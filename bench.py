@@ -19,9 +19,10 @@ Prints ONE JSON line (rank 0):
                GMRES from x0 = 0 against the learned warm start (network
                trained on the GPU from a short trajectory), per-solve times
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step]
        (c3r: the different-radii product on the C3 geometry; c3f: the
-       blob-blob soft-repulsion forces on it)
+       blob-blob soft-repulsion forces on it; c3step: whole rollers time
+       steps, steps/s)
 """
 from __future__ import annotations
 
@@ -189,6 +190,90 @@ def gmres_leg(steps: int, epochs: int, evals: int, log):
     return out
 
 
+def rollers_step_bench(args, log):
+    """One step = one stochastic Adams-Bashforth step of the rollers
+    integrator (rollers example parameters) on the 32768-roller strip:
+    blob-blob + one-blob forces, the fused UF+UT product, Lanczos noise to
+    the solver tolerance, the two RFD products and the position update."""
+    import torch
+    from rigidmultiblobswall_amd import wall_rpy, workloads
+    from rigidmultiblobswall_amd.rollers_integrator import RollersIntegrator
+    w = workloads.c3_rollers_strip()
+    p = w["params"]
+    r0 = w["r"]
+    n = r0.shape[0]
+    kw = dict(scheme=p["scheme"], eta=p["eta"], a=p["blob_radius"], kT=p["kT"], g=p["g"],
+              repulsion_strength_wall=p["repulsion_strength_wall"], debye_length_wall=p["debye_length_wall"],
+              repulsion_strength=p["repulsion_strength"], debye_length=p["debye_length"],
+              omega_one_roller=p["omega_one_roller"], free_kinematics=p["free_kinematics"],
+              tolerance=p["solver_tolerance"])
+    integ = RollersIntegrator(r0, blob_blob_force_implementation="hip", precision=args.precision, seed=11, **kw)
+    for _ in range(args.warmup):
+        integ.advance_time_step(p["dt"])
+    torch.cuda.synchronize()
+    prod0, it0, inv0 = dict(integ.products), integ.stoch_iterations_count, integ.invalid_configuration_count
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with wall_rpy.PairKernelTimer() as pk:
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(args.steps):
+            integ.advance_time_step(p["dt"])
+        stop.record()
+        torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    per = {k: (integ.products[k] - prod0[k]) / args.steps for k in integ.products}
+    lanczos_its = (integ.stoch_iterations_count - it0) / args.steps
+    nw, tw, mw = pk.by_kind["wall_rpy"]
+    nf, tf, mf = pk.by_kind["forces"]
+    pairs = float(n) * n
+    flop = (per["UF"] * FLOP_PER_PAIR["UF"] + per["UF+UT"] * (FLOP_PER_PAIR["UF"] + FLOP_PER_PAIR["UT"])) \
+        * args.steps * pairs
+    achieved = flop / (tw * 1e-3) / 1e12
+    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
+    log(f"rollers step: {ms:.3f} ms/step, {per} products/step, Lanczos {lanczos_its:.1f} its, "
+        f"wall-RPY kernels {tw / args.steps:.3f} ms/step, forces {tf / args.steps:.3f} ms/step")
+    out = {
+        "metric": "rollers time steps/s (stochastic Adams-Bashforth, 32768 rollers)",
+        "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32", "data": "synthetic",
+        "config": {"workload": "rollers example on 32768 rollers in its footprint (39.36 x 3224.3, z = 0.66 + "
+                               "Exp(2.25), min distance 1.38): dt 0.016, kT 0.0041419464, omega (0, 62.8, 0), "
+                               "free kinematics, blob-blob and wall repulsion, Lanczos tolerance 1e-3",
+                   "blobs": n, "precision": args.precision + PRECISION_NOTE.get(args.precision, ""),
+                   "per_step": {"products": per, "lanczos_iterations": lanczos_its,
+                                "invalid_configurations": integ.invalid_configuration_count - inv0,
+                                "wall_rpy_kernel_ms": tw / args.steps, "force_kernel_ms": tf / args.steps,
+                                "pair_kernel_share": (tw + tf) / (ms * args.steps)}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel": "wall-RPY pair kernels of the step (UF, UF+UT)",
+                     "kernel_mean_ms": mw, "kernel_launches": nw,
+                     "flop_per_pair": {"UF": FLOP_PER_PAIR["UF"], "UF+UT": FLOP_PER_PAIR["UF"] + FLOP_PER_PAIR["UT"]}},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        from oracle import rollers_oracle as RO
+        rng = np.random.default_rng(5)
+        st = RO.RollersStep(r0, p["scheme"], p["eta"], p["blob_radius"], kT=p["kT"], g=p["g"],
+                            eps_wall=p["repulsion_strength_wall"], b_wall=p["debye_length_wall"],
+                            eps=p["repulsion_strength"], b=p["debye_length"], omega=p["omega_one_roller"],
+                            free_kinematics=p["free_kinematics"], tolerance=p["solver_tolerance"], kind="fast")
+        draws = iter([(rng.standard_normal(3 * n), rng.standard_normal((n, 3))) for _ in range(4)])
+        t = time.perf_counter()
+        attempts = st.advance(p["dt"], draws)
+        dt_cpu = time.perf_counter() - t
+        threads = O.num_threads("fast")
+        out["cpu_baseline"] = {"value": 1.0 / dt_cpu, "unit": "steps/s", "cores": threads, "kind": "port",
+                               "sample": f"1 stochastic Adams-Bashforth step ({attempts} attempt(s)) of the same "
+                                         f"workload from its initial positions: FP64 C oracle products "
+                                         f"(-O3 -ffast-math, OpenMP {threads} threads) and numpy Lanczos; "
+                                         f"{dt_cpu:.1f} s",
+                               "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+        log(f"cpu baseline: {dt_cpu:.1f} s per step")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def load_traffic(workload: str, precision: str, kernel: str = "pair_kernel_uf"):
     """HBM bytes per pair-kernel launch from the committed rocprof PMC summary
     of the same kernel, workload and precision (None when there is none)."""
@@ -239,6 +324,8 @@ def main(argv=None):
         assert bool(torch.equal(x, y))
         print(json.dumps({"calibrate_copy_bytes": nbytes}), flush=True)
         return 0
+    if args.workload == "c3step":
+        return rollers_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
     dev = torch.device("cuda", 0)
     w, desc = load_workload(args.workload)
     r_h, eta, a = w["r"], w["eta"], w["a"]

@@ -211,6 +211,9 @@ int rmw_rollers_advance(const double* r_old, const double* det_velocity, const d
  */
 int rmw_profile_enable(int on);
 int rmw_profile_collect(int64_t* launches, double* total_ms, double* mean_ms);
+/* The same record split by kernel family (arrays of 3: wall RPY, different
+ * radii, blob-blob forces); also clears the record. */
+int rmw_profile_collect_kinds(int64_t* launches, double* total_ms);
 
 /* ------------------------------------------------------------------------
  * Rigid multiblob bodies (FP64).  body_offsets is a HOST array of nbodies+1

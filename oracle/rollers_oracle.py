@@ -98,8 +98,10 @@ class RollersStep:
 
     def __init__(self, r, scheme, eta, a, kT=0.0, g=1.0, blob_mass=1.0, eps_wall=1.0, b_wall=1.0,
                  blob_blob=True, eps=1.0, b=1.0, periodic_length=None, omega=(0.0, 0.0, 0.0),
-                 free_kinematics="True", hydro_interactions=1, tolerance=1e-8, rf_delta=1e-3, cutoff=0.0):
+                 free_kinematics="True", hydro_interactions=1, tolerance=1e-8, rf_delta=1e-3, cutoff=0.0,
+                 kind="strict"):
         self.r = np.asarray(r, dtype=np.float64).reshape(-1, 3).copy()
+        self.kind = kind          # oracle build of the products ("fast": the CPU-baseline build)
         self.scheme = scheme.replace("_rollers", "")
         self.eta, self.a, self.kT, self.g, self.blob_mass = eta, a, kT, g, blob_mass
         self.eps_wall, self.b_wall, self.blob_blob, self.eps, self.b = eps_wall, b_wall, blob_blob, eps, b
@@ -117,7 +119,8 @@ class RollersStep:
     def forces(self, r):
         f = one_blob_forces(r, self.blob_mass, self.g, self.a, self.eps_wall, self.b_wall)
         if self.blob_blob:
-            f = f + O.blob_forces(r, self.eps, self.b, self.a, periodic_length=self.L, cutoff=self.cutoff)
+            f = f + O.blob_forces(r, self.eps, self.b, self.a, periodic_length=self.L, cutoff=self.cutoff,
+                                  kind=self.kind)
         return f
 
     def det_velocity(self, r):
@@ -128,27 +131,27 @@ class RollersStep:
             return v.ravel(), t.ravel()
         L = self.L
         if self.free_kinematics == "False":
-            rhs = np.tile(self.omega, n) - O.wf(r, F, self.eta, self.a, periodic_length=L)
+            rhs = np.tile(self.omega, n) - O.wf(r, F, self.eta, self.a, periodic_length=L, kind=self.kind)
             nrm = np.linalg.norm(rhs)
             if nrm > 0:
                 rhs = rhs / nrm
-            sol, _ = O.gmres_right(lambda x: O.wt(r, x, self.eta, self.a, periodic_length=L), lambda x: x, rhs,
-                                   tol=self.tol, restart=20, maxiter=1000, x0=self.torque_prev)
+            sol, _ = O.gmres_right(lambda x: O.wt(r, x, self.eta, self.a, periodic_length=L, kind=self.kind),
+                                   lambda x: x, rhs, tol=self.tol, restart=20, maxiter=1000, x0=self.torque_prev)
             self.torque_prev = sol
             T = sol * nrm if nrm > 0 else sol
         else:
             T = np.tile(self.omega * 8.0 * math.pi * self.eta * self.a ** 3, n)
-        v = O.uf(r, F, self.eta, self.a, periodic_length=L)
+        v = O.uf(r, F, self.eta, self.a, periodic_length=L, kind=self.kind)
         if np.any(T):
-            v = v + O.ut(r, T, self.eta, self.a, periodic_length=L)
+            v = v + O.ut(r, T, self.eta, self.a, periodic_length=L, kind=self.kind)
         return v, T
 
     def stoch_velocity(self, r, dt, z, dx):
         if self.hydro != 1:
             return uncorrelated_noise(r, z, self.eta, self.a, self.kT, dt).ravel()
         L = self.L
-        noise, _ = O.lanczos_sqrt(lambda v: O.uf(r, v, self.eta, self.a, periodic_length=L), np.asarray(z).ravel(),
-                                  factor=np.sqrt(2 * self.kT / dt), tolerance=self.tol)
+        noise, _ = O.lanczos_sqrt(lambda v: O.uf(r, v, self.eta, self.a, periodic_length=L, kind=self.kind),
+                                  np.asarray(z).ravel(), factor=np.sqrt(2 * self.kT / dt), tolerance=self.tol)
         if self.kT > 0.0:
             return noise + self.rfd_drift(r, dx)
         return noise
@@ -159,9 +162,9 @@ class RollersStep:
         L = self.L
         dx = np.asarray(dx, dtype=np.float64).reshape(-1, 3)
         r1 = r + dx * (self.rf_delta * self.a * 0.5)
-        div = O.uf(r1, dx.ravel(), self.eta, self.a, periodic_length=L)
+        div = O.uf(r1, dx.ravel(), self.eta, self.a, periodic_length=L, kind=self.kind)
         r2 = r1 - dx * self.rf_delta * self.a
-        div = div - O.uf(r2, dx.ravel(), self.eta, self.a, periodic_length=L)
+        div = div - O.uf(r2, dx.ravel(), self.eta, self.a, periodic_length=L, kind=self.kind)
         return (self.kT / (self.rf_delta * self.a)) * div
 
     def advance(self, dt, draws: Optional[Iterator] = None):

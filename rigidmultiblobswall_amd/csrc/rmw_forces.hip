@@ -418,7 +418,7 @@ extern "C" int rmw_blob_forces(int prec, const double* r_vectors, const double* 
     hipStream_t st = (hipStream_t)stream;
     ProfRec rec{};
     bool prof = false;
-    prof_begin(rec, prof, st);
+    prof_begin(rec, prof, st, 2);
     const bool cut = cutoff > 0.0;
     if (prec == RMW_PREC_F64) {
       if (cut) launch_force<true, true>(pl.mutual, q, grid, st);

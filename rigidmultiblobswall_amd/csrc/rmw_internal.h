@@ -68,8 +68,10 @@ const DeviceProps& device_props();
 // Pair-kernel timing (rmw_profile_enable / rmw_profile_collect): when
 // enabled, prof_begin/prof_end bracket one pair-kernel launch with HIP
 // events recorded on its stream.
-struct ProfRec { hipEvent_t a, b; };
-void prof_begin(ProfRec& rec, bool& on, hipStream_t st);
+// kind: 0 wall RPY, 1 different radii, 2 blob-blob forces.
+struct ProfRec { hipEvent_t a, b; int kind; };
+constexpr int kProfKinds = 3;
+void prof_begin(ProfRec& rec, bool& on, hipStream_t st, int kind = 0);
 void prof_end(const ProfRec& rec, bool on, hipStream_t st);
 
 }  // namespace rmw

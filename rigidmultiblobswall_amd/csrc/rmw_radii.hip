@@ -245,7 +245,7 @@ extern "C" int rmw_radii_apply(int prec, int wall, const double* source, const d
     const dim3 g(grid), b(kRBlock);
     ProfRec rec{};
     bool prof = false;
-    prof_begin(rec, prof, st);
+    prof_begin(rec, prof, st, 1);
     if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
     else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
     else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);

@@ -1170,8 +1170,9 @@ static void dispatch_mutual(int mask, int prec, int K, const MutParams& q, int g
 }
 
 // Pair-kernel timing records (rmw_profile_*).
-void prof_begin(ProfRec& rec, bool& on, hipStream_t st)
+void prof_begin(ProfRec& rec, bool& on, hipStream_t st, int kind)
 {
+  rec.kind = kind;
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     on = g_prof_on;
@@ -1452,6 +1453,31 @@ extern "C" int rmw_profile_collect(int64_t* launches, double* total_ms, double* 
     if (launches) *launches = (int64_t)recs.size();
     if (total_ms) *total_ms = tot;
     if (mean_ms) *mean_ms = recs.empty() ? 0.0 : tot / recs.size();
+  });
+}
+
+extern "C" int rmw_profile_collect_kinds(int64_t* launches, double* total_ms)
+{
+  return guarded([&] {
+    RMW_REQUIRE(launches && total_ms, "launches and total_ms are required");
+    std::vector<ProfRec> recs;
+    {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      recs.swap(g_prof);
+    }
+    for (int k = 0; k < kProfKinds; ++k) {
+      launches[k] = 0;
+      total_ms[k] = 0.0;
+    }
+    for (auto& r : recs) {
+      RMW_CHECK_HIP(hipEventSynchronize(r.b));
+      float ms = 0.f;
+      RMW_CHECK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      launches[r.kind] += 1;
+      total_ms[r.kind] += ms;
+      RMW_CHECK_HIP(hipEventDestroy(r.a));
+      RMW_CHECK_HIP(hipEventDestroy(r.b));
+    }
   });
 }
 

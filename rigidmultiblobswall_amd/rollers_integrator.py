@@ -98,6 +98,8 @@ class RollersIntegrator:
         self.wall_overlaps = 0
         self.det_iterations_count = 0
         self.stoch_iterations_count = 0
+        # mobility products issued, by block mask ("UF", "UF+UT", "WF", "WT")
+        self.products = {"UF": 0, "UF+UT": 0, "WF": 0, "WT": 0}
         # device buffers
         dev, n = self.device, self.n
         self._F = torch.empty((n, 3), dtype=torch.float64, device=dev)
@@ -120,6 +122,7 @@ class RollersIntegrator:
         return torch.randn(shape, dtype=torch.float64, device=self.device, generator=self.gen)
 
     def _uf(self, r: torch.Tensor, f: torch.Tensor, out: torch.Tensor) -> None:
+        self.products["UF"] += 1
         wall_rpy.apply(RMW_UF, r, force=f, eta=self.eta, a=self.a, periodic_length=self.periodic_length,
                        precision=self.precision, out_velocity=out.reshape(1, self.n, 3))
 
@@ -154,6 +157,7 @@ class RollersIntegrator:
         F = self.blob_forces(r)
         if self.free_kinematics == "False":
             rhs = torch.from_numpy(np.tile(self.omega_one_roller, self.n)).to(self.device)
+            self.products["WF"] += 1
             _, wf = wall_rpy.apply(RMW_WF, r, force=F, eta=self.eta, a=self.a, periodic_length=self.periodic_length,
                                    precision=self.precision)
             rhs = rhs - wf.reshape(-1)
@@ -162,6 +166,7 @@ class RollersIntegrator:
                 rhs = rhs / rhs_norm
 
             def A(x, out):
+                self.products["WT"] += 1
                 wall_rpy.apply(RMW_WT, r, torque=x.reshape(self.n, 3), eta=self.eta, a=self.a,
                                periodic_length=self.periodic_length, precision=self.precision,
                                out_omega=out.reshape(1, self.n, 3))
@@ -175,6 +180,7 @@ class RollersIntegrator:
             T = self.get_torque()
         v = torch.empty((self.n, 3), dtype=torch.float64, device=self.device)
         if bool(torch.any(T != 0)):
+            self.products["UF+UT"] += 1
             wall_rpy.apply(RMW_UF | RMW_UT, r, force=F, torque=T, eta=self.eta, a=self.a,
                            periodic_length=self.periodic_length, precision=self.precision,
                            out_velocity=v.reshape(1, self.n, 3))

@@ -244,8 +244,15 @@ def device_info() -> dict:
     return dict(info)
 
 
+PROFILE_KINDS = ("wall_rpy", "radii", "forces")
+
+
 class PairKernelTimer:
-    """Context manager timing every pair-kernel launch with HIP events."""
+    """Context manager timing every pair-kernel launch with HIP events.
+
+    After the block: launches / total_ms / mean_ms over all pair kernels and
+    ``by_kind[name] = (launches, total_ms, mean_ms)`` per kernel family
+    (wall RPY, different radii, blob-blob forces)."""
 
     def __enter__(self):
         lib = _lib.load()
@@ -256,9 +263,11 @@ class PairKernelTimer:
     def __exit__(self, *exc):
         lib = _lib.load()
         _lib.check(lib.rmw_profile_enable(0))
-        n = ctypes.c_int64()
-        tot = ctypes.c_double()
-        mean = ctypes.c_double()
-        _lib.check(lib.rmw_profile_collect(ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mean)))
-        self.launches, self.total_ms, self.mean_ms = n.value, tot.value, mean.value
+        n = (ctypes.c_int64 * 3)()
+        tot = (ctypes.c_double * 3)()
+        _lib.check(lib.rmw_profile_collect_kinds(n, tot))
+        self.by_kind = {k: (n[i], tot[i], tot[i] / n[i] if n[i] else 0.0) for i, k in enumerate(PROFILE_KINDS)}
+        self.launches = sum(n)
+        self.total_ms = sum(tot)
+        self.mean_ms = self.total_ms / self.launches if self.launches else 0.0
         return False

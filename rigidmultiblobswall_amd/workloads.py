@@ -16,6 +16,10 @@ numpy arrays.
       (box side sqrt(1024 pi / 0.4)), z_c = 1.3 + N(0, 0.05^2).
   C5  64 independent roller suspensions, N_r ~ U{128..4096}, same areal
       density as C3, f ~ N(0, 1).
+  C3 strip (time-step workload): 32768 rollers in the footprint of the
+      rollers example's clones file, x ~ U[0, 39.36], y ~ U[0, 3224.3],
+      z = 0.66 + Exp(mean 2.25), centres at least 1.38 apart (the example's
+      smallest neighbour distances), with the example's parameters.
 """
 from __future__ import annotations
 
@@ -138,3 +142,33 @@ def c5_ragged_rollers(seed: int = 5, nrep: int = 64, nmin: int = 128, nmax: int 
     r = np.vstack(rs)
     f = rng.standard_normal(3 * int(offsets[-1]))
     return {"r": r, "f": f, "offsets": offsets, "eta": 1e-3, "a": 0.656, "L": np.zeros(3)}
+
+
+# Parameters of the rollers example (multi_bodies/examples/rollers/inputfile_rollers.dat).
+ROLLERS_EXAMPLE = dict(scheme="stochastic_adams_bashforth_rollers", dt=0.016, solver_tolerance=1e-3, eta=1.0e-3,
+                       g=0.0024892, blob_radius=0.656, kT=0.0041419464, omega_one_roller=(0.0, 62.8, 0.0),
+                       free_kinematics="True", repulsion_strength=0.0165677856, debye_length=0.0656,
+                       repulsion_strength_wall=0.0165677856, debye_length_wall=0.0656)
+
+
+def c3_rollers_strip(seed: int = 6, n: int = 32768, min_dist: float = 1.38):
+    """Rollers placed like the example's suspension (see the module notes),
+    by batched rejection with a k-d tree."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    Lx, Ly = 39.36, 3224.3
+    pts = np.empty((0, 3))
+    while pts.shape[0] < n:
+        m = n - pts.shape[0]
+        cand = np.column_stack([rng.uniform(0, Lx, 2 * m), rng.uniform(0, Ly, 2 * m),
+                                0.66 + rng.exponential(2.25, 2 * m)])
+        keep = np.ones(cand.shape[0], dtype=bool)
+        if pts.shape[0]:
+            d, _ = cKDTree(pts).query(cand, k=1)
+            keep &= d >= min_dist
+        for i, j in sorted(cKDTree(cand).query_pairs(min_dist)):
+            if keep[i] and keep[j]:
+                keep[j] = False
+        pts = np.vstack([pts, cand[keep][:m]])
+    return {"r": pts[:n], "eta": ROLLERS_EXAMPLE["eta"], "a": ROLLERS_EXAMPLE["blob_radius"], "L": np.zeros(3),
+            "params": dict(ROLLERS_EXAMPLE)}

@@ -133,3 +133,38 @@ def test_example_parameters_run(gpu):
     assert np.isfinite(x).all() and (x[:, 2] > 0).all()
     # free kinematics with omega_y > 0: the rollers translate along +x on average
     assert (x[:, 0] - loc[:, 0]).mean() > 0
+
+
+@pytest.mark.parametrize("save", ["one_file", "one_file_per_step"])
+def test_run_driver_writes_the_reference_layout(gpu, tmp_path, save):
+    """run_rollers on the example's input (first 256 rollers, 3 steps):
+    bodies_info, one frame per saved step in the .config / .clones layout."""
+    import os
+    import re
+    from rigidmultiblobswall_amd import read_input as RI
+    from rigidmultiblobswall_amd import run_rollers
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rollers")
+    text = open(os.path.join(gdir, "inputfile_rollers.dat")).read()
+    text = text.replace("../../Structures/blob.vertex rollers.clones",
+                        f"{gdir}/blob.vertex {gdir}/rollers_256.clones")
+    text = re.sub(r"^n_steps.*$", "n_steps 3", text, flags=re.M)
+    text = re.sub(r"^save_clones.*$", f"save_clones {save}", text, flags=re.M)
+    text = text.replace("run_rollers", str(tmp_path / "run"))
+    p = tmp_path / "in.dat"
+    p.write_text(text + "\nseed 4\n")
+    rd = RI.ReadInput(str(p))
+    assert rd.n_steps == 3 and rd.save_clones == save
+    print(f"case run_rollers driver {save}", flush=True)
+    info = run_rollers.run(str(p), log=lambda m: None)
+    assert info["steps"] == 3 and info["invalid_configurations"] == 0
+    assert (tmp_path / "run.bodies_info").read_text().splitlines()[3].split()[-1] == "256"
+    if save == "one_file":
+        lines = (tmp_path / "run.rollers_256.config").read_text().splitlines()
+        assert len(lines) == 4 * 257 and lines[0] == "256" and lines[257] == "256"
+        first = np.array([[float(v) for v in ln.split()] for ln in lines[1:257]])
+        _, loc, _ = RI.read_clones_file(os.path.join(gdir, "rollers_256.clones"))
+        assert np.array_equal(first[:, :3], loc)
+    else:
+        for step in range(4):
+            n, loc, q = RI.read_clones_file(str(tmp_path / f"run.rollers_256.{step:08d}.clones"))
+            assert n == 256 and (loc[:, 2] > 0).all()
