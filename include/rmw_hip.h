@@ -90,6 +90,37 @@ int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * The same products for the other boundaries of the fluid.  domain:
+ *   RMW_DOMAIN_WALL          = rmw_wall_rpy_apply
+ *   RMW_DOMAIN_NO_WALL       unbounded RPY, no height shift or damping, any
+ *                            mask: no_wall_mobility_{trans,rot}_times_
+ *                            {force,torque}_numba, mobility/mobility_numba.py
+ *                            :12-120, 440-545, 832-936, 1077-1186 (wrappers
+ *                            mobility.py:1119-1129, 1200-1210, 1258-1270,
+ *                            1301-1313)
+ *   RMW_DOMAIN_IN_PLANE      the wall blocks with only the x-y components of
+ *                            the sources and outputs (P M P, P = diag(1,1,0)),
+ *                            masks UF, UT, UF|UT: in_plane_mobility_trans_
+ *                            times_{force,torque}_numba, :291-437, 690-830
+ *                            (wrappers mobility.py:1166-1198, 1235-1256)
+ *   RMW_DOMAIN_FREE_SURFACE  RPY(r_i - r_j) + RPY(r_i - r_j*) D with the image
+ *                            r_j* = (x_j, y_j, -z_j), D = diag(1,1,-1), UF
+ *                            only: free_surface_mobility_trans_times_force_
+ *                            numba, :1770-1938 (wrapper mobility.py:1390-1406)
+ * One right-hand side per pass (k > 1 runs k passes); ordered kernels only.
+ */
+#define RMW_DOMAIN_WALL 0
+#define RMW_DOMAIN_NO_WALL 1
+#define RMW_DOMAIN_IN_PLANE 2
+#define RMW_DOMAIN_FREE_SURFACE 3
+int rmw_mobility_workspace_size(int domain, int mask, int prec, int64_t n, int k, const rmw_tuning* tuning,
+                                size_t* bytes);
+int rmw_mobility_apply(int domain, int mask, int prec, const double* r_vectors, const double* force,
+                       const double* torque, double* velocity, double* omega, int64_t n, int k, int64_t ld,
+                       double eta, double a, const double* L, const rmw_tuning* tuning, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/*
  * The launch plan rmw_wall_rpy_apply would use for these sizes (periodic:
  * whether L has a positive entry): which kernel, the right-hand sides per
  * pass, the source chunk and the number of work items.  No reference

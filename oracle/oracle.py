@@ -34,7 +34,11 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 UF, UT, WF, WT, UF_NOWALL = 0, 1, 2, 3, 4
-BLOCK_POWER = {UF: 1, UT: 2, WF: 2, WT: 3, UF_NOWALL: 1}
+UT_NOWALL, WF_NOWALL, WT_NOWALL, UF_INPLANE, UT_INPLANE, UF_FREE_SURFACE = 5, 6, 7, 8, 9, 10
+BLOCK_POWER = {UF: 1, UT: 2, WF: 2, WT: 3, UF_NOWALL: 1, UT_NOWALL: 2, WF_NOWALL: 2, WT_NOWALL: 3,
+               UF_INPLANE: 1, UT_INPLANE: 2, UF_FREE_SURFACE: 1}
+# blocks whose wrapper applies the wall height shift and damping
+WALL_BLOCKS = (UF, UT, WF, WT, UF_INPLANE, UT_INPLANE)
 
 _LIBS: dict = {}
 
@@ -211,7 +215,7 @@ def dense_block_matrix(block: int, r_vectors, eta: float, a: float, kind="strict
     B M(z_eff) B is applied like the product wrappers."""
     r = np.asarray(r_vectors, dtype=np.float64).reshape(-1, 3)
     n = r.shape[0]
-    if wrapper and block != UF_NOWALL:
+    if wrapper and block in WALL_BLOCKS:
         r_eff, Bd, _ = shift_and_damping(r, a)
     else:
         r_eff, Bd = r, np.ones(n)

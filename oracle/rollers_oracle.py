@@ -99,9 +99,13 @@ class RollersStep:
     def __init__(self, r, scheme, eta, a, kT=0.0, g=1.0, blob_mass=1.0, eps_wall=1.0, b_wall=1.0,
                  blob_blob=True, eps=1.0, b=1.0, periodic_length=None, omega=(0.0, 0.0, 0.0),
                  free_kinematics="True", hydro_interactions=1, tolerance=1e-8, rf_delta=1e-3, cutoff=0.0,
-                 kind="strict"):
+                 kind="strict", domain="single_wall"):
         self.r = np.asarray(r, dtype=np.float64).reshape(-1, 3).copy()
         self.kind = kind          # oracle build of the products ("fast": the CPU-baseline build)
+        self.domain = domain      # quaternion_integrator_rollers.py:59-92
+        self.B = {"single_wall": (O.UF, O.UT, O.WF, O.WT),
+                  "no_wall": (O.UF_NOWALL, O.UT_NOWALL, O.WF_NOWALL, O.WT_NOWALL),
+                  "in_plane": (O.UF_INPLANE, O.UT_INPLANE, None, None)}[domain]
         self.scheme = scheme.replace("_rollers", "")
         self.eta, self.a, self.kT, self.g, self.blob_mass = eta, a, kT, g, blob_mass
         self.eps_wall, self.b_wall, self.blob_blob, self.eps, self.b = eps_wall, b_wall, blob_blob, eps, b
@@ -115,6 +119,9 @@ class RollersStep:
         self.torque_prev = None
         self.invalid = 0
         self.wall_overlaps = 0
+
+    def _p(self, b, r, v, eta, a, periodic_length=None, kind="strict"):
+        return O.block_apply(self.B[b], r, v, eta, a, periodic_length, kind=kind)
 
     def forces(self, r):
         f = one_blob_forces(r, self.blob_mass, self.g, self.a, self.eps_wall, self.b_wall)
@@ -131,28 +138,28 @@ class RollersStep:
             return v.ravel(), t.ravel()
         L = self.L
         if self.free_kinematics == "False":
-            rhs = np.tile(self.omega, n) - O.wf(r, F, self.eta, self.a, periodic_length=L, kind=self.kind)
+            rhs = np.tile(self.omega, n) - self._p(2, r, F, self.eta, self.a, periodic_length=L, kind=self.kind)
             nrm = np.linalg.norm(rhs)
             if nrm > 0:
                 rhs = rhs / nrm
-            sol, _ = O.gmres_right(lambda x: O.wt(r, x, self.eta, self.a, periodic_length=L, kind=self.kind),
+            sol, _ = O.gmres_right(lambda x: self._p(3, r, x, self.eta, self.a, periodic_length=L, kind=self.kind),
                                    lambda x: x, rhs, tol=self.tol, restart=20, maxiter=1000, x0=self.torque_prev)
             self.torque_prev = sol
             T = sol * nrm if nrm > 0 else sol
         else:
             T = np.tile(self.omega * 8.0 * math.pi * self.eta * self.a ** 3, n)
-        v = O.uf(r, F, self.eta, self.a, periodic_length=L, kind=self.kind)
+        v = self._p(0, r, F, self.eta, self.a, periodic_length=L, kind=self.kind)
         if np.any(T):
-            v = v + O.ut(r, T, self.eta, self.a, periodic_length=L, kind=self.kind)
+            v = v + self._p(1, r, T, self.eta, self.a, periodic_length=L, kind=self.kind)
         return v, T
 
     def stoch_velocity(self, r, dt, z, dx):
         if self.hydro != 1:
             return uncorrelated_noise(r, z, self.eta, self.a, self.kT, dt).ravel()
         L = self.L
-        noise, _ = O.lanczos_sqrt(lambda v: O.uf(r, v, self.eta, self.a, periodic_length=L, kind=self.kind),
+        noise, _ = O.lanczos_sqrt(lambda v: self._p(0, r, v, self.eta, self.a, periodic_length=L, kind=self.kind),
                                   np.asarray(z).ravel(), factor=np.sqrt(2 * self.kT / dt), tolerance=self.tol)
-        if self.kT > 0.0:
+        if self.kT > 0.0 and self.domain != "no_wall":
             return noise + self.rfd_drift(r, dx)
         return noise
 
@@ -162,9 +169,9 @@ class RollersStep:
         L = self.L
         dx = np.asarray(dx, dtype=np.float64).reshape(-1, 3)
         r1 = r + dx * (self.rf_delta * self.a * 0.5)
-        div = O.uf(r1, dx.ravel(), self.eta, self.a, periodic_length=L, kind=self.kind)
+        div = self._p(0, r1, dx.ravel(), self.eta, self.a, periodic_length=L, kind=self.kind)
         r2 = r1 - dx * self.rf_delta * self.a
-        div = div - O.uf(r2, dx.ravel(), self.eta, self.a, periodic_length=L, kind=self.kind)
+        div = div - self._p(0, r2, dx.ravel(), self.eta, self.a, periodic_length=L, kind=self.kind)
         return (self.kT / (self.rf_delta * self.a)) * div
 
     def advance(self, dt, draws: Optional[Iterator] = None):
@@ -185,12 +192,13 @@ class RollersStep:
             else:
                 vel = det + stoch
             new = self.r + dt * vel.reshape(-1, 3)
-            if not (new[:, 2] < 0.0).any():
+            if self.domain != "single_wall" or not (new[:, 2] < 0.0).any():
                 if ab:
                     self.first_step = False
                     self.prev = det
                 self.r = new
-                self.wall_overlaps += int((new[:, 2] < self.a).sum())
+                if self.domain == "single_wall" or self.scheme == "stochastic_first_order":
+                    self.wall_overlaps += int((new[:, 2] < self.a).sum())
                 return attempts
             self.invalid += 1
             if not stochastic:

@@ -382,18 +382,118 @@ static inline mat3 uf_nowall_pair(double rx, double ry, double rz, int self)
   return M;
 }
 
+/* Unbounded rotational coupling, the same matrix for UT (mobility_numba.py
+ * :500-535) and WF (:893-929): M_xy = rz c, M_xz = -ry c, M_yz = rx c,
+ * antisymmetric, c = 1/r^3 (r >= 2) or (1 - 3r/8)/2; zero for the self pair. */
+static inline mat3 rot_nowall_pair(double rx, double ry, double rz, int self)
+{
+  mat3 M = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (!self) {
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double r = sqrt(r2);
+    double c = r >= 2 ? 1.0 / (r2 * r) : 0.5 * (1.0 - 0.375 * r);
+    M.xy = rz * c;
+    M.xz = -ry * c;
+    M.yz = rx * c;
+  }
+  M.yx = -M.xy; M.zx = -M.xz; M.zy = -M.yz;
+  return M;
+}
+
+/* Unbounded rotation from torque (mobility_numba.py:1141-1175). */
+static inline mat3 wt_nowall_pair(double rx, double ry, double rz, int self)
+{
+  mat3 M;
+  if (self) {
+    M.xx = 1.0; M.xy = 0; M.xz = 0; M.yy = 1.0; M.yz = 0; M.zz = 1.0;
+  } else {
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double r = sqrt(r2);
+    double r3 = r2 * r;
+    double invr = 1.0 / r, invr2 = 1.0 / r2, invr3 = 1.0 / r3;
+    if (r >= 2) {
+      double c1 = -0.5, c2 = 1.5 * invr2;
+      M.xx = (c1 + c2 * rx * rx) * invr3;
+      M.xy = (c2 * rx * ry) * invr3;
+      M.xz = (c2 * rx * rz) * invr3;
+      M.yy = (c1 + c2 * ry * ry) * invr3;
+      M.yz = (c2 * ry * rz) * invr3;
+      M.zz = (c1 + c2 * rz * rz) * invr3;
+    } else {
+      double c1 = 1.0 - 0.84375 * r + 0.078125 * r3;
+      double c2 = 0.28125 * invr - 0.046875 * r;
+      M.xx = c1 + c2 * rx * rx;
+      M.xy = c2 * rx * ry;
+      M.xz = c2 * rx * rz;
+      M.yy = c1 + c2 * ry * ry;
+      M.yz = c2 * ry * rz;
+      M.zz = c1 + c2 * rz * rz;
+    }
+  }
+  M.yx = M.xy; M.zx = M.xz; M.zy = M.yz;
+  return M;
+}
+
+/* In-plane blocks (mobility_numba.py:291-437, 690-830): the kernels differ
+ * from the single-wall ones only by dropping every entry of the z row and
+ * the z column (and the z components of the products), i.e. P M P. */
+static inline mat3 in_plane(mat3 M)
+{
+  M.xz = 0; M.yz = 0; M.zx = 0; M.zy = 0; M.zz = 0;
+  return M;
+}
+
+/* Free surface (mobility_numba.py:1840-1925): RPY of d (4/3 I for the self
+ * pair) plus the RPY of the image vector R = (rx, ry, (z_i + z_j)/a), added
+ * with the signs of M D, D = diag(1, 1, -1) (z column negated). */
+static inline mat3 uf_free_surface_pair(double rx, double ry, double rz, double zi, double zj, int self)
+{
+  mat3 M = uf_nowall_pair(rx, ry, rz, self);
+  if (self) { rx = 0; ry = 0; }
+  mat3 I = uf_nowall_pair(rx, ry, zi + zj, 0);
+  M.xx += I.xx; M.xy += I.xy; M.xz -= I.xz;
+  M.yx += I.xy; M.yy += I.yy; M.yz -= I.yz;
+  M.zx += I.xz; M.zy += I.yz; M.zz -= MUT(8, I.zz, -I.zz);
+  return M;
+}
+
 /* Block ids used by the exported functions. */
-enum { ORC_UF = 0, ORC_UT = 1, ORC_WF = 2, ORC_WT = 3, ORC_UF_NOWALL = 4 };
+enum {
+  ORC_UF = 0, ORC_UT = 1, ORC_WF = 2, ORC_WT = 3, ORC_UF_NOWALL = 4,
+  ORC_UT_NOWALL = 5, ORC_WF_NOWALL = 6, ORC_WT_NOWALL = 7,
+  ORC_UF_INPLANE = 8, ORC_UT_INPLANE = 9, ORC_UF_FREE_SURFACE = 10,
+  ORC_NBLOCKS = 11
+};
 
 /* Norm exponent p of 1/(8 pi eta a^p) per block (mobility_numba.py:133,559,949,1200). */
 static int block_power(int block)
 {
   switch (block) {
-    case ORC_UF: return 1;
-    case ORC_UT: return 2;
-    case ORC_WF: return 2;
-    case ORC_WT: return 3;
+    case ORC_UT: case ORC_WF: case ORC_UT_NOWALL: case ORC_WF_NOWALL: case ORC_UT_INPLANE: return 2;
+    case ORC_WT: case ORC_WT_NOWALL: return 3;
     default: return 1;
+  }
+}
+
+/* Blocks whose wrapper shifts heights and damps (a wall bounds the fluid). */
+static int block_has_wall(int block)
+{
+  return block <= ORC_WT || block == ORC_UF_INPLANE || block == ORC_UT_INPLANE;
+}
+
+static inline mat3 block_pair(int block, double rx, double ry, double rz, double zi, double zj, int self)
+{
+  switch (block) {
+    case ORC_UF: return uf_pair(rx, ry, rz, zi, zj, self);
+    case ORC_UT: return ut_pair(rx, ry, rz, zi, zj, self);
+    case ORC_WF: return wf_pair(rx, ry, rz, zi, zj, self);
+    case ORC_WT: return wt_pair(rx, ry, rz, zi, zj, self);
+    case ORC_UT_NOWALL: case ORC_WF_NOWALL: return rot_nowall_pair(rx, ry, rz, self);
+    case ORC_WT_NOWALL: return wt_nowall_pair(rx, ry, rz, self);
+    case ORC_UF_INPLANE: return in_plane(uf_pair(rx, ry, rz, zi, zj, self));
+    case ORC_UT_INPLANE: return in_plane(ut_pair(rx, ry, rz, zi, zj, self));
+    case ORC_UF_FREE_SURFACE: return uf_free_surface_pair(rx, ry, rz, zi, zj, self);
+    default: return uf_nowall_pair(rx, ry, rz, self);
   }
 }
 
@@ -407,7 +507,7 @@ int orc_block_kernel(int block, const double *r, const double *vec, double *out,
                      int64_t n, int64_t t0, int64_t t1, double eta, double a,
                      const double *L)
 {
-  if (block < 0 || block > 4 || n < 0 || t0 < 0 || t1 > n || t0 > t1) return -1;
+  if (block < 0 || block >= ORC_NBLOCKS || n < 0 || t0 < 0 || t1 > n || t0 > t1) return -1;
   const double inva = 1.0 / a;
   const int p = block_power(block);
   const double norm = 1.0 / (8.0 * M_PI * eta * pow(a, (double)p));
@@ -429,14 +529,7 @@ int orc_block_kernel(int block, const double *r, const double *vec, double *out,
             int self = central && (MUT(7, i == j, i == j + 1));
             rx *= inva; ry *= inva; rz *= inva;
             double zi = rzi * inva, zj = r[3 * j + 2] * inva;
-            mat3 M;
-            switch (block) {
-              case ORC_UF: M = uf_pair(rx, ry, rz, zi, zj, self); break;
-              case ORC_UT: M = ut_pair(rx, ry, rz, zi, zj, self); break;
-              case ORC_WF: M = wf_pair(rx, ry, rz, zi, zj, self); break;
-              case ORC_WT: M = wt_pair(rx, ry, rz, zi, zj, self); break;
-              default: M = uf_nowall_pair(rx, ry, rz, self); break;
-            }
+            mat3 M = block_pair(block, rx, ry, rz, zi, zj, self);
             const double fx = vec[3 * j], fy = vec[3 * j + 1], fz = vec[3 * j + 2];
             ux += (M.xx * fx + M.xy * fy + M.xz * fz) * norm;
             uy += (M.yx * fx + M.yy * fy + M.yz * fz) * norm;
@@ -480,7 +573,7 @@ int orc_block_apply(int block, const double *r, const double *vec, double *out,
                     int64_t n, int64_t t0, int64_t t1, double eta, double a,
                     const double *L, double *work, double *vec_work)
 {
-  if (block == ORC_UF_NOWALL)
+  if (!block_has_wall(block))
     return orc_block_kernel(block, r, vec, out, n, t0, t1, eta, a, L);
   double *r_eff = work;
   double *B = work + 3 * n;
@@ -510,15 +603,8 @@ int orc_block_pair_matrix(int block, const double *ri, const double *rj, int sel
   const double inva = 1.0 / a;
   double rx = (ri[0] - rj[0]) * inva, ry = (ri[1] - rj[1]) * inva, rz = (ri[2] - rj[2]) * inva;
   double zi = ri[2] * inva, zj = rj[2] * inva;
-  mat3 M;
-  switch (block) {
-    case ORC_UF: M = uf_pair(rx, ry, rz, zi, zj, self); break;
-    case ORC_UT: M = ut_pair(rx, ry, rz, zi, zj, self); break;
-    case ORC_WF: M = wf_pair(rx, ry, rz, zi, zj, self); break;
-    case ORC_WT: M = wt_pair(rx, ry, rz, zi, zj, self); break;
-    case ORC_UF_NOWALL: M = uf_nowall_pair(rx, ry, rz, self); break;
-    default: return -1;
-  }
+  if (block < 0 || block >= ORC_NBLOCKS) return -1;
+  mat3 M = block_pair(block, rx, ry, rz, zi, zj, self);
   const double norm = 1.0 / (8.0 * M_PI * eta * pow(a, (double)block_power(block)));
   double v[9] = {M.xx, M.xy, M.xz, M.yx, M.yy, M.yz, M.zx, M.zy, M.zz};
   for (int k = 0; k < 9; ++k) m9[k] = v[k] * norm;

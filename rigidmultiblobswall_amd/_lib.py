@@ -82,6 +82,11 @@ PROTOTYPES = {
     "rmw_rollers_advance": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, _dp, _dp, _vp, _sz, _vp]),
     "rmw_profile_collect_kinds": (ctypes.c_int, [_i64p, _hdp]),
+    "rmw_mobility_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int,
+                                                   _tp, _szp]),
+    "rmw_mobility_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64,
+                                          ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz,
+                                          _vp]),
     "rmw_wall_rpy_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, ctypes.c_int,
                                              ctypes.POINTER(PlanInfo)]),
     "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,

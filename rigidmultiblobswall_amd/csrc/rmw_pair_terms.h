@@ -38,6 +38,16 @@ namespace rmw {
 
 template <typename T> struct Vec3 { T x, y, z; };
 
+// Boundary of the fluid (the `domain` options of the reference):
+//   wall          RPY + Swan-Brady image (mobility_numba.py:123-287, ...)
+//   no wall       RPY alone (:12-120, 440-545, 832-936, 1077-1186)
+//   in plane      the wall blocks projected on x-y, P M P with
+//                 P = diag(1, 1, 0) (:291-437, 690-830); the caller
+//                 projects sources and outputs
+//   free surface  RPY(d) + RPY(R) D, D = diag(1, 1, -1) (:1770-1938); UF only
+enum : int { kDomWall = 0, kDomNoWall = 1, kDomInPlane = 2, kDomFreeSurface = 3 };
+RMW_HD bool dom_has_wall(int dom) { return dom == kDomWall || dom == kDomInPlane; }
+
 // Reciprocal square root.  FP32 on the device uses the hardware v_rsq_f32
 // (about 1 ulp); FP64 always uses a correctly rounded sqrt and a division.
 RMW_HD float rsq(float x)
@@ -111,20 +121,40 @@ template <typename T> struct CoefUF {
   T g4;     // f4 / |R|    -> coefficient of (R.v) z
 };
 
+// RPY translational coefficients A I + B r r^T at distance r (units of a):
+// far (r > 2) and overlapping branches as selects.
 template <typename T>
-RMW_HD CoefUF<T> coef_uf(const Geom<T>& g, T zi, T zj)
+RMW_HD void rpy_tt(T r, T invr, T& A, T& B)
 {
   const T one = T(1), two = T(2);
+  T invr2 = invr * invr;
+  T A_far = (one + T(2.0 / 3.0) * invr2) * invr;
+  T B_far = (one - two * invr2) * (invr2 * invr);
+  T A_near = T(4.0 / 3.0) - T(4.0 / 3.0 * 0.28125) * r;
+  T B_near = T(4.0 / 3.0 * 0.09375) * invr;
+  bool far = r > two;
+  A = far ? A_far : A_near;
+  B = far ? B_far : B_near;
+}
+
+template <typename T>
+RMW_HD CoefUF<T> coef_uf(const Geom<T>& g, T zi, T zj, int dom = kDomWall)
+{
+  const T one = T(1);
   CoefUF<T> c;
-  // RPY, far (r > 2) or overlapping (r <= 2) branch as selects.
-  T invr2 = g.invr * g.invr;
-  T A_far = (one + T(2.0 / 3.0) * invr2) * g.invr;
-  T B_far = (one - two * invr2) * (invr2 * g.invr);
-  T A_near = T(4.0 / 3.0) - T(4.0 / 3.0 * 0.28125) * g.r;
-  T B_near = T(4.0 / 3.0 * 0.09375) * g.invr;
-  bool far = g.r > two;
-  T A = far ? A_far : A_near;
-  c.Bc = far ? B_far : B_near;
+  T A;
+  rpy_tt(g.r, g.invr, A, c.Bc);
+  if (dom == kDomNoWall) {
+    c.s = A; c.s5 = A; c.g2 = T(0); c.g3 = T(0); c.g4 = T(0);
+    return c;
+  }
+  if (dom == kDomFreeSurface) {
+    // + RPY(R) D v = A_R (v - 2 v_z z) + B_R R (R.v) - 2 B_R R_z v_z R
+    T AR, BR;
+    rpy_tt(one / g.invR, g.invR, AR, BR);
+    c.s = A + AR; c.s5 = A - AR; c.g2 = BR; c.g3 = T(-2) * BR * g.Rz; c.g4 = T(0);
+    return c;
+  }
 
   // Swan-Brady wall correction (mobility_numba.py:254-264) regrouped as
   // polynomials in w = 1/|R|^2 with E = e_z^2 = R_z^2 w.  With
@@ -212,13 +242,17 @@ template <typename T> struct CoefRot {
 };
 
 template <typename T>
-RMW_HD CoefRot<T> coef_rot(const Geom<T>& g, T hnum)
+RMW_HD CoefRot<T> coef_rot(const Geom<T>& g, T hnum, int dom = kDomWall)
 {
   const T one = T(1), two = T(2);
   CoefRot<T> c;
   T invr3 = g.invr * g.invr * g.invr;
   T c_near = T(0.5) * (one - T(0.375) * g.r);
   c.c = (g.r >= two) ? invr3 : c_near;
+  if (!dom_has_wall(dom)) {
+    c.F1r = T(0); c.F2r = T(0); c.F3r2 = T(0); c.F4 = T(0);
+    return c;
+  }
   // h_hat = h/R_z enters only as h_hat e_z^2 = h R_z/|R|^2 and
   // h_hat e_z = h/|R| (no division needed).
   T invR = g.invR;
@@ -317,7 +351,7 @@ template <typename T> struct CoefWT {
 };
 
 template <typename T>
-RMW_HD CoefWT<T> coef_wt(const Geom<T>& g)
+RMW_HD CoefWT<T> coef_wt(const Geom<T>& g, int dom = kDomWall)
 {
   const T one = T(1), two = T(2);
   CoefWT<T> c;
@@ -331,6 +365,10 @@ RMW_HD CoefWT<T> coef_wt(const Geom<T>& g)
   bool far = g.r >= two;
   T A = far ? A_far : A_near;
   c.Bc = far ? B_far : B_near;
+  if (!dom_has_wall(dom)) {
+    c.s = A; c.G2r = T(0); c.G3r = T(0); c.G4r = T(0);
+    return c;
+  }
   T invR = g.invR;
   T invR2 = invR * invR;
   T invR3 = invR2 * invR;
@@ -506,6 +544,39 @@ RMW_HD Vec3<T> self_wt(T h, T vx, T vy, T vz)
   T pxy = T(1) - T(0.3125) * ih3;
   T pz = T(1) - T(0.125) * ih3;
   return Vec3<T>{pxy * vx, pxy * vy, pz * vz};
+}
+
+// Self terms of every domain (same units as the wall ones): no wall, the
+// bare RPY self mobility (4/3 I, 0, 0, I); free surface, 4/3 I plus the
+// image RPY(R) D at R = (0, 0, 2h); in plane, the wall terms (the caller
+// projects).
+template <typename T>
+RMW_HD Vec3<T> self_uf_dom(int dom, T h, T vx, T vy, T vz)
+{
+  const T s = T(4) / T(3);
+  if (dom == kDomNoWall) return Vec3<T>{s * vx, s * vy, s * vz};
+  if (dom == kDomFreeSurface) {
+    const T R = h + h;
+    T AR, BR;
+    rpy_tt(R, T(1) / R, AR, BR);
+    return Vec3<T>{(s + AR) * vx, (s + AR) * vy, (s - AR - R * R * BR) * vz};
+  }
+  return self_uf(h, vx, vy, vz);
+}
+template <typename T>
+RMW_HD Vec3<T> self_ut_dom(int dom, T h, T vx, T vy, T vz)
+{
+  return dom_has_wall(dom) ? self_ut(h, vx, vy, vz) : Vec3<T>{T(0), T(0), T(0)};
+}
+template <typename T>
+RMW_HD Vec3<T> self_wf_dom(int dom, T h, T vx, T vy, T vz)
+{
+  return dom_has_wall(dom) ? self_wf(h, vx, vy, vz) : Vec3<T>{T(0), T(0), T(0)};
+}
+template <typename T>
+RMW_HD Vec3<T> self_wt_dom(int dom, T h, T vx, T vy, T vz)
+{
+  return dom_has_wall(dom) ? self_wt(h, vx, vy, vz) : Vec3<T>{vx, vy, vz};
 }
 
 }  // namespace rmw

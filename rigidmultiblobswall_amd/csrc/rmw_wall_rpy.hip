@@ -97,6 +97,7 @@ struct PairParams {
   int64_t total_items;
   int periodic;           // bit d set when L_d > 0
   double L[3];            // periodic lengths / a
+  int domain;             // kDomWall, kDomNoWall, kDomInPlane, kDomFreeSurface
 };
 
 __device__ inline SysInfo find_sys_item(const PairParams& p, int64_t item)
@@ -113,17 +114,20 @@ __device__ inline SysInfo find_sys_item(const PairParams& p, int64_t item)
 // ---------------------------------------------------------------------------
 // Blob staging: scaled position (x/a, y/a, z_eff/a) with z_eff = max(z, a)
 // (shift_heights, mobility.py:52-64) and damping B = min(z/a, 1)
-// (damping_matrix_B, mobility.py:67-84).
+// (damping_matrix_B, mobility.py:67-84) when a wall bounds the fluid; the
+// no-wall and free-surface products take the heights as they are
+// (mobility.py:1119-1129, 1390-1406).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void load_blob(const double* __restrict__ r, int64_t g, double a, double inva,
-                                          double& X, double& Y, double& Z, double& B)
+                                          double& X, double& Y, double& Z, double& B, int dom)
 {
   const double x = r[3 * g], y = r[3 * g + 1], z = r[3 * g + 2];
-  const double ze = z <= a ? a : z;
+  const bool wall = dom_has_wall(dom);
+  const double ze = wall && z <= a ? a : z;
   X = x * inva;
   Y = y * inva;
   Z = ze * inva;
-  B = z < a ? z * inva : 1.0;
+  B = wall && z < a ? z * inva : 1.0;
 }
 
 // Source record of the FP32 kernels: FP64 position for the difference, FP32
@@ -164,7 +168,7 @@ template <> struct V4<double> { using type = double4; };
 
 // One (target, source) pair for all blocks of MASK and all K right-hand
 // sides.  `skip` drops the pair (the self pair of the central box).
-template <int MASK, int K, int ACC, typename T>
+template <int MASK, int K, int ACC, typename T, int DOM>
 __device__ __forceinline__ void pair_eval(T dx, T dy, T dz, T zi, T zj,
                                           const typename V4<T>::type* __restrict__ sf,
                                           const typename V4<T>::type* __restrict__ st, int j,
@@ -175,10 +179,10 @@ __device__ __forceinline__ void pair_eval(T dx, T dy, T dz, T zi, T zj,
   CoefUF<T> cuf;
   CoefRot<T> cut, cwf;
   CoefWT<T> cwt;
-  if constexpr ((MASK & kUF) != 0) cuf = coef_uf(g, zi, zj);
-  if constexpr ((MASK & kUT) != 0) cut = coef_rot(g, zi);
-  if constexpr ((MASK & kWF) != 0) cwf = coef_rot(g, zj);
-  if constexpr ((MASK & kWT) != 0) cwt = coef_wt(g);
+  if constexpr ((MASK & kUF) != 0) cuf = coef_uf(g, zi, zj, DOM);
+  if constexpr ((MASK & kUT) != 0) cut = coef_rot(g, zi, DOM);
+  if constexpr ((MASK & kWF) != 0) cwf = coef_rot(g, zj, DOM);
+  if constexpr ((MASK & kWT) != 0) cwt = coef_wt(g, DOM);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     Vec3<T> uu{T(0), T(0), T(0)}, ww{T(0), T(0), T(0)};
@@ -247,21 +251,23 @@ template <> __device__ __forceinline__ double4 make_v4<double4>(double x, double
 
 // Stage the sources of blob gj for K right-hand sides into LDS slot `slot`:
 // B f / (8 pi eta a) and B t / (8 pi eta a^2) as float4 or double4.
-template <int MASK, int K, typename V>
+template <int MASK, int K, typename V, int DOM>
 __device__ __forceinline__ void stage_sources(const PairParams& p, int64_t gj, double B, int slot, V* s_f, V* s_t)
 {
   using BT = BlockTraits<MASK>;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    // in plane: only the x-y components act (P v)
+    const double pz = DOM == kDomInPlane ? 0.0 : 1.0;
     if constexpr (BT::NF) {
       const double* v = p.f + k * p.rhs_stride + 3 * gj;
       const double c = p.c1 * B;
-      s_f[k * kBlock + slot] = make_v4<V>(c * v[0], c * v[1], c * v[2]);
+      s_f[k * kBlock + slot] = make_v4<V>(c * v[0], c * v[1], pz * c * v[2]);
     }
     if constexpr (BT::NT) {
       const double* v = p.t + k * p.rhs_stride + 3 * gj;
       const double c = p.c2 * B;
-      s_t[k * kBlock + slot] = make_v4<V>(c * v[0], c * v[1], c * v[2]);
+      s_t[k * kBlock + slot] = make_v4<V>(c * v[0], c * v[1], pz * c * v[2]);
     }
   }
 }
@@ -277,7 +283,7 @@ constexpr int pair_min_waves()
   return (K == 1 && (MASK == kUF || MASK == kUT || MASK == kWF || MASK == kWT)) ? RMW_PAIR_WAVES : 1;
 }
 
-template <int MASK, int K, int ACC>
+template <int MASK, int K, int ACC, int DOM = kDomWall>
 __global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kernel_f32(PairParams p)
 {
   using BT = BlockTraits<MASK>;
@@ -296,7 +302,7 @@ __global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kern
     const int64_t il = t0 + threadIdx.x;
     const bool valid = il < S.n;
     double Xi, Yi, Zi, Bi;
-    load_blob(p.r, S.b0 + (valid ? il : 0), p.a, p.inva, Xi, Yi, Zi, Bi);
+    load_blob(p.r, S.b0 + (valid ? il : 0), p.a, p.inva, Xi, Yi, Zi, Bi, DOM);
     const float zi = (float)Zi;
 
     Acc<ACC> u[K][3], w[K][3];
@@ -309,9 +315,9 @@ __global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kern
       if ((int)threadIdx.x < cnt) {
         const int64_t gj = S.b0 + s0 + threadIdx.x;
         double X, Y, Z, B;
-        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B);
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, DOM);
         s_pos[threadIdx.x] = SrcF32{X, Y, Z, (float)Z, 0.0f};
-        stage_sources<MASK, K, float4>(p, gj, B, threadIdx.x, s_f, s_t);
+        stage_sources<MASK, K, float4, DOM>(p, gj, B, threadIdx.x, s_f, s_t);
       }
       __syncthreads();
       // Tiles that contain this workgroup's own targets need the self mask.
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kern
           const float dx = (float)(Xi - sp.x);
           const float dy = (float)(Yi - sp.y);
           const float dz = (float)(Zi - sp.z);
-          pair_eval<MASK, K, ACC, float>(dx, dy, dz, zi, sp.zf, s_f, s_t, j, masked && j == self_j, u, w);
+          pair_eval<MASK, K, ACC, float, DOM>(dx, dy, dz, zi, sp.zf, s_f, s_t, j, masked && j == self_j, u, w);
         };
         int j = 0;
         if (diag) {
@@ -357,8 +363,8 @@ __global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kern
                 const float dy = (float)(ddy + by * p.L[1]);
                 const float dz = (float)(ddz + bz * p.L[2]);
                 const bool central = (bx == 0) && (by == 0) && (bz == 0);
-                pair_eval<MASK, K, ACC, float>(dx, dy, dz, zi, sp.zf, s_f, s_t, j,
-                                               diag && central && (j == self_j), u, w);
+                pair_eval<MASK, K, ACC, float, DOM>(dx, dy, dz, zi, sp.zf, s_f, s_t, j,
+                                                    diag && central && (j == self_j), u, w);
               }
           flush_all<K, ACC>(u, w);
         }
@@ -368,7 +374,7 @@ __global__ __launch_bounds__(kBlock, (pair_min_waves<MASK, K>())) void pair_kern
   }
 }
 
-template <int MASK, int K>
+template <int MASK, int K, int DOM = kDomWall>
 __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
 {
   using BT = BlockTraits<MASK>;
@@ -387,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
     const int64_t il = t0 + threadIdx.x;
     const bool valid = il < S.n;
     double Xi, Yi, Zi, Bi;
-    load_blob(p.r, S.b0 + (valid ? il : 0), p.a, p.inva, Xi, Yi, Zi, Bi);
+    load_blob(p.r, S.b0 + (valid ? il : 0), p.a, p.inva, Xi, Yi, Zi, Bi, DOM);
 
     Acc<RMW_PREC_F64> u[K][3], w[K][3];
     const int64_t s_begin = ch * S.chunk_len;
@@ -398,9 +404,9 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
       if ((int)threadIdx.x < cnt) {
         const int64_t gj = S.b0 + s0 + threadIdx.x;
         double X, Y, Z, B;
-        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B);
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, DOM);
         s_pos[threadIdx.x] = make_double4(X, Y, Z, 0.0);
-        stage_sources<MASK, K, double4>(p, gj, B, threadIdx.x, s_f, s_t);
+        stage_sources<MASK, K, double4, DOM>(p, gj, B, threadIdx.x, s_f, s_t);
       }
       __syncthreads();
       const bool diag = (s0 < t0 + kBlock) && (t0 < s0 + cnt);
@@ -409,7 +415,8 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
         const double4 pj = s_pos[j];
         double rx = Xi - pj.x, ry = Yi - pj.y, rz = Zi - pj.z;
         if (!p.periodic) {
-          pair_eval<MASK, K, RMW_PREC_F64, double>(rx, ry, rz, Zi, pj.z, s_f, s_t, j, diag && j == self_j, u, w);
+          pair_eval<MASK, K, RMW_PREC_F64, double, DOM>(rx, ry, rz, Zi, pj.z, s_f, s_t, j, diag && j == self_j, u,
+                                                        w);
         } else {
           if (pbox[0]) rx -= image_count(rx, p.L[0]) * p.L[0];
           if (pbox[1]) ry -= image_count(ry, p.L[1]) * p.L[1];
@@ -418,8 +425,9 @@ __global__ __launch_bounds__(kBlock) void pair_kernel_f64(PairParams p)
             for (int by = -pbox[1]; by <= pbox[1]; ++by)
               for (int bz = -pbox[2]; bz <= pbox[2]; ++bz) {
                 const bool central = (bx == 0) && (by == 0) && (bz == 0);
-                pair_eval<MASK, K, RMW_PREC_F64, double>(rx + bx * p.L[0], ry + by * p.L[1], rz + bz * p.L[2], Zi,
-                                                         pj.z, s_f, s_t, j, diag && central && (j == self_j), u, w);
+                pair_eval<MASK, K, RMW_PREC_F64, double, DOM>(rx + bx * p.L[0], ry + by * p.L[1],
+                                                              rz + bz * p.L[2], Zi, pj.z, s_f, s_t, j,
+                                                              diag && central && (j == self_j), u, w);
               }
         }
       }
@@ -508,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
     const bool valid = i < Sy.n;
     const int64_t gi = Sy.b0 + (valid ? i : 0);
     double Xi, Yi, Zi, Bi;
-    load_blob(p.r, gi, p.a, p.inva, Xi, Yi, Zi, Bi);
+    load_blob(p.r, gi, p.a, p.inva, Xi, Yi, Zi, Bi, kDomWall);
     const R zi = (R)Zi;
     R fi[BT::NF ? K : 1][3], ti[BT::NT ? K : 1][3];
 #pragma unroll
@@ -533,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       if (il < cnt) {
         const int64_t gj = Sy.b0 + s0 + il;
         double X, Y, Z, B;
-        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B);
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, kDomWall);
         s_x[il] = X; s_y[il] = Y; s_z[il] = Z; s_zr[il] = (R)Z;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -696,6 +704,7 @@ struct ReduceParams {
   int64_t ntot;
   int64_t rhs_stride;  // doubles between RHS in f/t/vel/omg
   double a, inva, c1, c2;
+  int domain;
 };
 
 __device__ inline SysInfo find_sys_blob(const ReduceParams& p, int64_t gi)
@@ -716,33 +725,35 @@ __device__ __forceinline__ void finish_output(const double* __restrict__ r, cons
                                               const double* __restrict__ t, double* __restrict__ vel,
                                               double* __restrict__ omg, int mask, int k, int64_t gi, int c,
                                               double sum, int64_t rhs_stride, double a, double inva, double c1,
-                                              double c2)
+                                              double c2, int dom)
 {
   const bool HU = (mask & (kUF | kUT)) != 0;
   const bool HW = (mask & (kWF | kWT)) != 0;
+  const bool wall = dom_has_wall(dom);
   const double z = r[3 * gi + 2];
-  const double ze = z <= a ? a : z;
-  const double B = z < a ? z * inva : 1.0;
+  const double ze = wall && z <= a ? a : z;
+  const double B = wall && z < a ? z * inva : 1.0;
   const double h = ze * inva;
+  const double pz = dom == kDomInPlane ? 0.0 : 1.0;   // in plane: P v in, P u out
   const bool is_w = HW && (!HU || c >= 3);
   const int comp = c % 3;
   double fv[3] = {0, 0, 0}, tv[3] = {0, 0, 0};
   if (f) {
     const double* v = f + k * rhs_stride + 3 * gi;
-    fv[0] = c1 * B * v[0]; fv[1] = c1 * B * v[1]; fv[2] = c1 * B * v[2];
+    fv[0] = c1 * B * v[0]; fv[1] = c1 * B * v[1]; fv[2] = pz * c1 * B * v[2];
   }
   if (t) {
     const double* v = t + k * rhs_stride + 3 * gi;
-    tv[0] = c2 * B * v[0]; tv[1] = c2 * B * v[1]; tv[2] = c2 * B * v[2];
+    tv[0] = c2 * B * v[0]; tv[1] = c2 * B * v[1]; tv[2] = pz * c2 * B * v[2];
   }
   double self = 0.0;
   if (!is_w) {
-    if (mask & kUF) { Vec3<double> sv = self_uf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    if (mask & kUT) { Vec3<double> sv = self_ut(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    vel[k * rhs_stride + 3 * gi + comp] = B * (sum + self);
+    if (mask & kUF) { Vec3<double> sv = self_uf_dom(dom, h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (mask & kUT) { Vec3<double> sv = self_ut_dom(dom, h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    vel[k * rhs_stride + 3 * gi + comp] = (comp == 2 ? pz : 1.0) * B * (sum + self);
   } else {
-    if (mask & kWF) { Vec3<double> sv = self_wf(h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
-    if (mask & kWT) { Vec3<double> sv = self_wt(h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (mask & kWF) { Vec3<double> sv = self_wf_dom(dom, h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
+    if (mask & kWT) { Vec3<double> sv = self_wt_dom(dom, h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
     omg[k * rhs_stride + 3 * gi + comp] = B * inva * (sum + self);
   }
 }
@@ -775,7 +786,8 @@ __global__ __launch_bounds__(256) void reduce_kernel(ReduceParams p)
   for (; ch < S.n_chunks; ++ch) s0 += P[ch * cstride];
   double sum = (s0 + s1) + (s2 + s3);
 
-  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2);
+  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2,
+                p.domain);
 }
 
 // Reduce of the symmetric path: for blob gi of tile X of its system, the
@@ -797,6 +809,7 @@ struct MutReduceParams {
   int64_t ntot, chunk_tiles;
   int64_t rhs_stride;
   double a, inva, c1, c2;
+  int domain;
 };
 
 __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
@@ -840,13 +853,15 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
   }
   if (m < X + 1) s2 += Pj[m * step];
   const double sum = (s0 + s1) + (s2 + s3);
-  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2);
+  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2,
+                p.domain);
 }
 
 // ---------------------------------------------------------------------------
 // Host side: plan, workspace layout, dispatch.
 // ---------------------------------------------------------------------------
 struct Plan {
+  int domain = kDomWall;
   int mask = 0, prec = 0, kmax = 1;
   int slots = 0;             // resident pair-kernel workgroups on the device
   int64_t ntot = 0;
@@ -939,9 +954,13 @@ static int64_t default_chunk(double pairs, int cu_count)
 }
 
 static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, int k,
-                      const rmw_tuning* tuning, bool ragged)
+                      const rmw_tuning* tuning, bool ragged, int domain = kDomWall)
 {
   RMW_REQUIRE(valid_mask(mask), "mask must be a non-empty subset of UF|UT|WF|WT");
+  RMW_REQUIRE(domain >= kDomWall && domain <= kDomFreeSurface, "unknown domain");
+  RMW_REQUIRE(domain != kDomInPlane || (mask & ~(kUF | kUT)) == 0,
+              "the in-plane mobility has the UF and UT blocks only");
+  RMW_REQUIRE(domain != kDomFreeSurface || mask == kUF, "the free-surface mobility has the UF block only");
   RMW_REQUIRE(prec == RMW_PREC_F32_F64ACC || prec == RMW_PREC_F32_KAHAN || prec == RMW_PREC_F64 ||
                   prec == RMW_PREC_F32_BLOCKED,
               "unknown precision mode");
@@ -949,10 +968,11 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   Plan pl;
   pl.mask = mask;
   pl.prec = prec;
+  pl.domain = domain;
   RMW_REQUIRE(!tuning || tuning->rhs_group == 0 || tuning->rhs_group == 1 || tuning->rhs_group == 2 ||
                   tuning->rhs_group == 4,
               "rhs_group must be 0, 1, 2 or 4");
-  pl.kmax = group_k(k, rhs_cap(tuning));
+  pl.kmax = group_k(k, domain == kDomWall ? rhs_cap(tuning) : 1);
   const int nsys = (int)offsets.size() - 1;
   RMW_REQUIRE(nsys >= 1 && offsets[0] == 0, "offsets must start at 0");
   for (int s = 0; s < nsys; ++s) RMW_REQUIRE(offsets[s + 1] >= offsets[s], "offsets must be non-decreasing");
@@ -993,7 +1013,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
 
   // Symmetric layout: the UF block alone or all four blocks, unless switched off.
   const int sym = tuning ? tuning->symmetric : 0;
-  pl.mut_ok = (mask == kUF || mask == (kUF | kUT | kWF | kWT)) && pl.ntot > 0 && sym >= 0;
+  pl.mut_ok = (mask == kUF || mask == (kUF | kUT | kWF | kWT)) && pl.ntot > 0 && sym >= 0 && domain == kDomWall;
   RMW_REQUIRE(!(sym > 0 && !pl.mut_ok), "symmetric evaluation needs the UF block alone or all four blocks");
   int64_t tile_pairs = 0;
   for (int s = 0; s < nsys; ++s) {
@@ -1072,6 +1092,45 @@ static void launch_pair_k(int prec, int K, const PairParams& pp, int grid, hipSt
   if (K == 4) launch_pair<MASK, 4>(prec, pp, grid, st);
   else if (K == 2) launch_pair<MASK, 2>(prec, pp, grid, st);
   else launch_pair<MASK, 1>(prec, pp, grid, st);
+}
+
+// Pair kernels of the no-wall, in-plane and free-surface products (one
+// right-hand side per pass).
+template <int MASK, int DOM>
+static void launch_pair_dom(int prec, const PairParams& pp, int grid, hipStream_t st)
+{
+  const dim3 g(grid), b(kBlock);
+  if (prec == RMW_PREC_F32_F64ACC)
+    hipLaunchKernelGGL((pair_kernel_f32<MASK, 1, RMW_PREC_F32_F64ACC, DOM>), g, b, 0, st, pp);
+  else if (prec == RMW_PREC_F32_BLOCKED)
+    hipLaunchKernelGGL((pair_kernel_f32<MASK, 1, RMW_PREC_F32_BLOCKED, DOM>), g, b, 0, st, pp);
+  else if (prec == RMW_PREC_F32_KAHAN)
+    hipLaunchKernelGGL((pair_kernel_f32<MASK, 1, RMW_PREC_F32_KAHAN, DOM>), g, b, 0, st, pp);
+  else
+    hipLaunchKernelGGL((pair_kernel_f64<MASK, 1, DOM>), g, b, 0, st, pp);
+}
+
+static void dispatch_pair_domain(int domain, int mask, int prec, const PairParams& pp, int grid, hipStream_t st)
+{
+  if (domain == kDomFreeSurface) return launch_pair_dom<kUF, kDomFreeSurface>(prec, pp, grid, st);
+  if (domain == kDomInPlane) {
+    switch (mask) {
+      case 1: return launch_pair_dom<1, kDomInPlane>(prec, pp, grid, st);
+      case 2: return launch_pair_dom<2, kDomInPlane>(prec, pp, grid, st);
+      default: return launch_pair_dom<3, kDomInPlane>(prec, pp, grid, st);
+    }
+  }
+  switch (mask) {
+    case 1: return launch_pair_dom<1, kDomNoWall>(prec, pp, grid, st);
+    case 2: return launch_pair_dom<2, kDomNoWall>(prec, pp, grid, st);
+    case 3: return launch_pair_dom<3, kDomNoWall>(prec, pp, grid, st);
+    case 4: return launch_pair_dom<4, kDomNoWall>(prec, pp, grid, st);
+    case 5: return launch_pair_dom<5, kDomNoWall>(prec, pp, grid, st);
+    case 8: return launch_pair_dom<8, kDomNoWall>(prec, pp, grid, st);
+    case 10: return launch_pair_dom<10, kDomNoWall>(prec, pp, grid, st);
+    case 12: return launch_pair_dom<12, kDomNoWall>(prec, pp, grid, st);
+    default: return launch_pair_dom<15, kDomNoWall>(prec, pp, grid, st);
+  }
 }
 
 static void dispatch_pair(int mask, int prec, int K, const PairParams& pp, int grid, hipStream_t st)
@@ -1244,6 +1303,7 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     rp.chunk_tiles = pl.chunk_tiles;
     rp.rhs_stride = stride;
     rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
+    rp.domain = pl.domain;
     const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
     const int64_t nthreads = pl.ntot * K * OC;
     hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
@@ -1302,8 +1362,9 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     grid = (int)std::min<int64_t>(grid, cap);
   }
 
-  for (int k0 = 0, K = group_k(k, rhs_cap(tuning)); k0 < k; k0 += K) {
-    K = group_k(k - k0, rhs_cap(tuning));
+  const int kcap = pl.domain == kDomWall ? rhs_cap(tuning) : 1;
+  for (int k0 = 0, K = group_k(k, kcap); k0 < k; k0 += K) {
+    K = group_k(k - k0, kcap);
     PairParams q{};
     q.r = r;
     q.f = NF ? f + (size_t)k0 * stride : nullptr;
@@ -1318,12 +1379,14 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     q.total_items = pl.total_items;
     q.periodic = periodic;
     for (int d = 0; d < 3; ++d) q.L[d] = Lp[d];
+    q.domain = pl.domain;
     // The partial layout was planned for kmax RHS; a smaller trailing group
     // reuses it with its own K (items only touch their own slots).
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st);
-    dispatch_pair(kmask, pl.prec, K, q, grid, st);
+    if (pl.domain == kDomWall) dispatch_pair(kmask, pl.prec, K, q, grid, st);
+    else dispatch_pair_domain(pl.domain, kmask, pl.prec, q, grid, st);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
 
@@ -1341,6 +1404,7 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     rp.ntot = pl.ntot;
     rp.rhs_stride = stride;
     rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
+    rp.domain = pl.domain;
     const int OCr = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
     const int64_t nthreads = pl.ntot * K * OCr;
     hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
@@ -1394,6 +1458,30 @@ extern "C" int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, c
   return guarded([&] {
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
+    run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
+             workspace_bytes, (hipStream_t)stream, false);
+  });
+}
+
+extern "C" int rmw_mobility_workspace_size(int domain, int mask, int prec, int64_t n, int k,
+                                           const rmw_tuning* tuning, size_t* bytes)
+{
+  return guarded([&] {
+    RMW_REQUIRE(bytes != nullptr, "bytes is null");
+    RMW_REQUIRE(n >= 0, "n must be >= 0");
+    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false, domain);
+    *bytes = pl.total;
+  });
+}
+
+extern "C" int rmw_mobility_apply(int domain, int mask, int prec, const double* r_vectors, const double* force,
+                                  const double* torque, double* velocity, double* omega, int64_t n, int k,
+                                  int64_t ld, double eta, double a, const double* L, const rmw_tuning* tuning,
+                                  void* workspace, size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RMW_REQUIRE(n >= 0, "n must be >= 0");
+    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false, domain);
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
   });

@@ -107,11 +107,64 @@ def single_wall_grand_mobility_hip(r_vectors, force_torque, eta, a, *args, **kwa
     return np.concatenate([_host(u), _host(w)])
 
 
+def _product(domain, mask, r_vectors, force, torque, eta, a, kwargs, out="u"):
+    L, prec, tun = _opts(kwargs)
+    u, w = wall_rpy.apply(mask, r_vectors, force=force, torque=torque, eta=eta, a=a, periodic_length=L,
+                          precision=prec, tuning=tun, domain=domain)
+    return _host(u if out == "u" else w)
+
+
+def no_wall_mobility_trans_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
+    """Unbounded RPY u = M_UF f; drop-in for mobility.py:1119-1129."""
+    return _product("no_wall", RMW_UF, r_vectors, force, None, eta, a, kwargs)
+
+
+def no_wall_mobility_trans_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
+    """Unbounded u = M_UT t; drop-in for mobility.py:1200-1210."""
+    return _product("no_wall", RMW_UT, r_vectors, None, torque, eta, a, kwargs)
+
+
+def no_wall_mobility_rot_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
+    """Unbounded w = M_WF f; drop-in for mobility.py:1258-1270."""
+    return _product("no_wall", RMW_WF, r_vectors, force, None, eta, a, kwargs, out="w")
+
+
+def no_wall_mobility_rot_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
+    """Unbounded w = M_WT t; drop-in for mobility.py:1301-1313."""
+    return _product("no_wall", RMW_WT, r_vectors, None, torque, eta, a, kwargs, out="w")
+
+
+def no_wall_mobility_trans_times_force_torque_hip(r_vectors, force, torque, eta, a, *args, **kwargs):
+    """Unbounded u = M_UF f + M_UT t in one pass; drop-in for mobility.py:413-425."""
+    return _product("no_wall", RMW_UF | RMW_UT, r_vectors, force, torque, eta, a, kwargs)
+
+
+def in_plane_mobility_trans_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
+    """u = P B M_UF(z_eff) B P f (x-y components only, u_z = 0); drop-in for
+    mobility.py:1166-1198."""
+    return _product("in_plane", RMW_UF, r_vectors, force, None, eta, a, kwargs)
+
+
+def in_plane_mobility_trans_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
+    """u = P B M_UT(z_eff) B P t; drop-in for mobility.py:1235-1256."""
+    return _product("in_plane", RMW_UT, r_vectors, None, torque, eta, a, kwargs)
+
+
+def free_surface_mobility_trans_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
+    """u = [RPY(r_i - r_j) + RPY(r_i - r_j*) D] f below a free surface at
+    z = 0 (image r_j* = (x_j, y_j, -z_j)); drop-in for mobility.py:1390-1406."""
+    return _product("free_surface", RMW_UF, r_vectors, force, None, eta, a, kwargs)
+
+
 __all__ = [
     "shift_heights", "damping_matrix_B",
     "single_wall_mobility_trans_times_force_hip", "single_wall_mobility_trans_times_torque_hip",
     "single_wall_mobility_rot_times_force_hip", "single_wall_mobility_rot_times_torque_hip",
     "single_wall_mobility_trans_times_force_torque_hip", "single_wall_grand_mobility_hip",
+    "no_wall_mobility_trans_times_force_hip", "no_wall_mobility_trans_times_torque_hip",
+    "no_wall_mobility_rot_times_force_hip", "no_wall_mobility_rot_times_torque_hip",
+    "no_wall_mobility_trans_times_force_torque_hip", "in_plane_mobility_trans_times_force_hip",
+    "in_plane_mobility_trans_times_torque_hip", "free_surface_mobility_trans_times_force_hip",
 ]
 
 

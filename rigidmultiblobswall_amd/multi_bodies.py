@@ -7,8 +7,10 @@
                                              (quaternion_integrator_rollers.py:59-92)
 
 New implementation values served by librmw_hip: 'hip' (FP32 pair terms, FP64
-per-target sums), 'hip_fp32_blocked', 'hip_fp32_kahan' and 'hip_fp64', and
-for blobs of different radii 'radii_hip[_no_wall][_<precision>]'.  None
+per-target sums), 'hip_fp32_blocked', 'hip_fp32_kahan' and 'hip_fp64', the
+other boundaries 'hip_no_wall[_<precision>]' and
+'hip_free_surface[_<precision>]', and for blobs of different radii
+'radii_hip[_no_wall][_<precision>]'.  None
 contains 'numba' or 'pycuda', so the reference's substring dispatch never
 mistakes them for its own backends.
 """
@@ -55,6 +57,12 @@ def set_mobility_vector_prod(implementation, *args, **kwargs):
             mb.single_wall_mobility_trans_times_force_source_target_hip
         return partial(mb.mobility_radii_trans_times_force_hip, radius_blobs=radius_blobs,
                        function=partial(fn, precision=prec))
+    for dom, fn in (("_no_wall", mb.no_wall_mobility_trans_times_force_hip),
+                    ("_free_surface", mb.free_surface_mobility_trans_times_force_hip)):
+        # 'hip_no_wall[_<precision>]', 'hip_free_surface[_<precision>]'
+        # (the reference's 'numba_no_wall' / 'numba_free_surface', :250-265)
+        if implementation.startswith("hip" + dom):
+            return partial(fn, precision=_precision("hip" + implementation[len("hip" + dom):]))
     return partial(mb.single_wall_mobility_trans_times_force_hip, precision=_precision(implementation))
 
 
@@ -65,11 +73,22 @@ def set_mobility_blobs(implementation):
     return mb.single_wall_fluid_mobility_hip
 
 
-def set_rollers_blocks(implementation):
-    """(UF, UT, WF, WT, UF+UT) block functions for a roller integrator."""
+def set_rollers_blocks(implementation, domain: str = "single_wall"):
+    """(UF, UT, WF, WT, UF+UT) block functions for a roller integrator, per
+    domain as quaternion_integrator_rollers.py:59-92 binds them ('in_plane'
+    has no rotational blocks: None there)."""
     p = _precision(implementation)
-    return (partial(mb.single_wall_mobility_trans_times_force_hip, precision=p),
-            partial(mb.single_wall_mobility_trans_times_torque_hip, precision=p),
-            partial(mb.single_wall_mobility_rot_times_force_hip, precision=p),
-            partial(mb.single_wall_mobility_rot_times_torque_hip, precision=p),
-            partial(mb.single_wall_mobility_trans_times_force_torque_hip, precision=p))
+    if domain == "single_wall":
+        fns = (mb.single_wall_mobility_trans_times_force_hip, mb.single_wall_mobility_trans_times_torque_hip,
+               mb.single_wall_mobility_rot_times_force_hip, mb.single_wall_mobility_rot_times_torque_hip,
+               mb.single_wall_mobility_trans_times_force_torque_hip)
+    elif domain == "no_wall":
+        fns = (mb.no_wall_mobility_trans_times_force_hip, mb.no_wall_mobility_trans_times_torque_hip,
+               mb.no_wall_mobility_rot_times_force_hip, mb.no_wall_mobility_rot_times_torque_hip,
+               mb.no_wall_mobility_trans_times_force_torque_hip)
+    elif domain == "in_plane":
+        fns = (mb.in_plane_mobility_trans_times_force_hip, mb.in_plane_mobility_trans_times_torque_hip,
+               None, None, None)
+    else:
+        raise ValueError(f"unknown domain {domain!r}")
+    return tuple(None if f is None else partial(f, precision=p) for f in fns)

@@ -55,8 +55,11 @@ class RollersIntegrator:
         self.scheme = scheme.replace("_rollers", "")
         if self.scheme not in SCHEMES:
             raise ValueError(f"scheme {scheme!r} is not provided; use one of {[s + '_rollers' for s in SCHEMES]}")
-        if domain != "single_wall":
-            raise ValueError(f"domain {domain!r} is not provided (single_wall only)")
+        if domain not in ("single_wall", "no_wall", "in_plane"):
+            raise ValueError(f"unknown domain {domain!r}; use single_wall, no_wall or in_plane")
+        if domain == "in_plane" and str(free_kinematics) == "False" and int(hydro_interactions) == 1:
+            raise ValueError("the in-plane mobility has no rotational blocks: constrained kinematics needs "
+                             "single_wall or no_wall")
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.lib = _lib.load()
         r = torch.as_tensor(r_vectors) if torch.is_tensor(r_vectors) else torch.from_numpy(
@@ -124,7 +127,7 @@ class RollersIntegrator:
     def _uf(self, r: torch.Tensor, f: torch.Tensor, out: torch.Tensor) -> None:
         self.products["UF"] += 1
         wall_rpy.apply(RMW_UF, r, force=f, eta=self.eta, a=self.a, periodic_length=self.periodic_length,
-                       precision=self.precision, out_velocity=out.reshape(1, self.n, 3))
+                       precision=self.precision, out_velocity=out.reshape(1, self.n, 3), domain=self.domain)
 
     # ------------------------------------------------------------------
     def blob_forces(self, r: torch.Tensor) -> torch.Tensor:
@@ -159,7 +162,7 @@ class RollersIntegrator:
             rhs = torch.from_numpy(np.tile(self.omega_one_roller, self.n)).to(self.device)
             self.products["WF"] += 1
             _, wf = wall_rpy.apply(RMW_WF, r, force=F, eta=self.eta, a=self.a, periodic_length=self.periodic_length,
-                                   precision=self.precision)
+                                   precision=self.precision, domain=self.domain)
             rhs = rhs - wf.reshape(-1)
             rhs_norm = float(torch.linalg.norm(rhs))
             if rhs_norm > 0:
@@ -169,7 +172,7 @@ class RollersIntegrator:
                 self.products["WT"] += 1
                 wall_rpy.apply(RMW_WT, r, torque=x.reshape(self.n, 3), eta=self.eta, a=self.a,
                                periodic_length=self.periodic_length, precision=self.precision,
-                               out_omega=out.reshape(1, self.n, 3))
+                               out_omega=out.reshape(1, self.n, 3), domain=self.domain)
             res = krylov.gmres(A, rhs.contiguous(), x0=self.deterministic_torque_previous_step, tol=self.tolerance,
                                restart=self.gmres_restart, maxiter=1000)
             self.det_iterations_count += res.iterations
@@ -183,7 +186,7 @@ class RollersIntegrator:
             self.products["UF+UT"] += 1
             wall_rpy.apply(RMW_UF | RMW_UT, r, force=F, torque=T, eta=self.eta, a=self.a,
                            periodic_length=self.periodic_length, precision=self.precision,
-                           out_velocity=v.reshape(1, self.n, 3))
+                           out_velocity=v.reshape(1, self.n, 3), domain=self.domain)
         else:
             self._uf(r, F, v)
         return v, T

@@ -77,15 +77,25 @@ def _tuning_for(tuning, mask: int, precision: str, k: int, pairs: float, ragged:
     return autotune.lookup(mask, precision if isinstance(precision, str) else str(precision), k, pairs, ragged)
 
 
+# Boundaries of the fluid (the reference's `domain` values, plus the free
+# surface of its 'numba_free_surface' implementations).
+DOMAINS = {"single_wall": 0, "no_wall": 1, "in_plane": 2, "free_surface": 3}
+
+
 def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: float = 1.0,
           periodic_length=None, precision="fp32", k: int = 1, tuning=None,
-          out_velocity: Optional[torch.Tensor] = None, out_omega: Optional[torch.Tensor] = None):
+          out_velocity: Optional[torch.Tensor] = None, out_omega: Optional[torch.Tensor] = None,
+          domain: str = "single_wall"):
     """Evaluate the blocks of ``mask`` on one suspension.
 
     r_vectors: (N, 3); force / torque: (k, N, 3) (or (N, 3) for k = 1).
     Returns (velocity, omega) as float64 device tensors of shape (k, N, 3)
     (squeezed to (N, 3) when k == 1), None for outputs the mask does not write.
+    domain: 'single_wall' (default), 'no_wall' (all blocks), 'in_plane'
+    (UF, UT) or 'free_surface' (UF).
     """
+    if domain not in DOMAINS:
+        raise ValueError(f"unknown domain {domain!r}; use one of {sorted(DOMAINS)}")
     lib = _lib.load()
     device = _device()
     r = torch.as_tensor(r_vectors)
@@ -100,12 +110,17 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     w = out_omega if out_omega is not None else (
         torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None)
     prec = _lib.precision_code(precision)
-    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
+    dom = DOMAINS[domain]
+    if dom == 0:
+        tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
+    else:
+        # the other domains run the ordered kernel; the cache holds wall shapes
+        tp = _lib.tuning_ptr(None if tuning is None or isinstance(tuning, str) else tuning)
     nbytes = ctypes.c_size_t()
-    _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
+    _lib.check(lib.rmw_mobility_workspace_size(dom, mask, prec, n, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device)
-    _lib.check(lib.rmw_wall_rpy_apply(
-        mask, prec, r.data_ptr(), f.data_ptr() if f is not None else None,
+    _lib.check(lib.rmw_mobility_apply(
+        dom, mask, prec, r.data_ptr(), f.data_ptr() if f is not None else None,
         t.data_ptr() if t is not None else None, u.data_ptr() if u is not None else None,
         w.data_ptr() if w is not None else None, n, k, 0, float(eta), float(a), _L_array(periodic_length),
         tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
