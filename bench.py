@@ -40,8 +40,11 @@ sys.path.insert(0, REPO)
 METRIC = "wall-RPY matvec blob-pairs/s; GMRES time-to-1e-8 residual (learned x0 vs plain)"
 FLOP_PER_PAIR = {"UF": 164, "UT": 95, "WF": 95, "WT": 101, "grand": 386,
                  "radii": 154,   # radii: counted on the dyadic form of the kernel (FMA = 2)
-                 "forces": 22}   # difference 3, r^2 5, sqrt + reciprocal 2, exponent 2, exp 1,
+                 "forces": 22,   # difference 3, r^2 5, sqrt + reciprocal 2, exponent 2, exp 1,
                                  # prefactor 2, radius sum 1, accumulation 6
+                 # UF of the other boundaries, same convention: RPY 36 + 3x3 product 18
+                 # (no wall); + the image RPY 36 (free surface); the wall terms as UF (in plane)
+                 "UF_no_wall": 54, "UF_free_surface": 90, "UF_in_plane": 164}
 PEAK_FP32_TFLOPS = 157.3
 PRECISION_NOTE = {"fp32": " (FP32 pair terms, FP64 per-target sums)",
                   "fp32_blocked": " (FP32 pair terms, FP32 sub-sums of <= 16 sources folded into FP64 per-target sums)",
@@ -291,6 +294,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--precision", default="fp32_blocked")
+    ap.add_argument("--domain", default="single_wall", choices=["single_wall", "no_wall", "in_plane", "free_surface"],
+                    help="boundary of the UF product (workloads c1, c2, c3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gmres", action="store_true", help="skip the learned-warm-start GMRES leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -352,7 +357,8 @@ def main(argv=None):
         elif radii:
             wall_rpy.apply_radii(r, r, f, rad, rad, eta, precision=args.precision, out=u2)
         else:
-            wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u)
+            wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u,
+                           domain=args.domain)
 
     for _ in range(args.warmup):
         step()
@@ -368,7 +374,7 @@ def main(argv=None):
     ms = start.elapsed_time(stop) / args.steps
     pairs = float(n) * n
     value = pairs / (ms * 1e-3)
-    op = "forces" if fparams else "radii" if radii else "UF"
+    op = "forces" if fparams else "radii" if radii else "UF" if args.domain == "single_wall" else f"UF_{args.domain}"
     kernel_ms = pk.mean_ms
     flop = FLOP_PER_PAIR[op] * pairs
     achieved = flop / (kernel_ms * 1e-3) / 1e12
@@ -378,12 +384,20 @@ def main(argv=None):
     from rigidmultiblobswall_amd import autotune
     shape = autotune.lookup(RMW_UF, args.precision, 1, float(n) * n, False)
     launch = wall_rpy.describe(RMW_UF, n, 1, args.precision)
+    if args.domain != "single_wall":
+        sym = float(n) * n / (256.0 * 256.0) / 2 >= info.get("cu_count", 256)
+        launch = {"kernel": "symmetric" if sym else "ordered", "domain": args.domain}
+        kname_dom = (f"{'mutual_kernel' if sym else 'pair_kernel_f32' if args.precision != 'fp64' else 'pair_kernel_f64'}"
+                     f"<UF,K=1,{args.precision},{args.domain}>")
     launch["source"] = "autotuner cache" if shape is not None else "built-in heuristic"
     if shape is not None:
         launch["max_wgs"] = shape.max_wgs
     kname = {("symmetric", True): "mutual_kernel<UF,K=1,{p}>", ("symmetric", False): "mutual_kernel<UF,K=1,fp64>",
              ("ordered", True): "pair_kernel_f32<UF,K=1,{p}>", ("ordered", False): "pair_kernel_f64<UF,K=1>"}[
         (launch["kernel"], args.precision != "fp64")].format(p=args.precision)
+    if args.domain != "single_wall" and not radii and not fparams:
+        kname = kname_dom
+        desc = desc.replace("UF block", f"UF block, {args.domain.replace('_', ' ')}")
     if radii:
         launch = {"kernel": "radii (ordered)"}
         kname = f"radii_kernel<{args.precision}>"
@@ -423,7 +437,7 @@ def main(argv=None):
         out["dtype"] = "f64"
         out["config"]["precision"] = {"fp64": "FP64 sqrt, quotient and exp; FP64 sums",
                                       "fp32": "FP64 geometry and sums, FP32 exponential"}[fprec]
-    if not radii and not fparams:
+    if not radii and not fparams and args.domain == "single_wall":
         out["end_to_end"] = end_to_end(r_h, f_h.reshape(-1, 3), eta, a, args.precision)
         log(f"end to end through the numpy signature: {out['end_to_end']['ms']:.3f} ms")
     if not args.no_gmres:

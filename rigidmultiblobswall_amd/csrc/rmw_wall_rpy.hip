@@ -490,7 +490,7 @@ __device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t it
 // Symmetric pair kernel.  R = float: FP32 pair terms (ACC: the target-side
 // accumulator; source side: FP32 sums over 16 steps folded into FP64 in the
 // blocked mode, an FP64 travelling accumulator otherwise); R = double: FP64.
-template <int MASK, int K, int ACC, typename R>
+template <int MASK, int K, int ACC, typename R, int DOM = kDomWall>
 __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
 {
   using BT = BlockTraits<MASK>;
@@ -516,7 +516,8 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
     const bool valid = i < Sy.n;
     const int64_t gi = Sy.b0 + (valid ? i : 0);
     double Xi, Yi, Zi, Bi;
-    load_blob(p.r, gi, p.a, p.inva, Xi, Yi, Zi, Bi, kDomWall);
+    load_blob(p.r, gi, p.a, p.inva, Xi, Yi, Zi, Bi, DOM);
+    constexpr double pz = DOM == kDomInPlane ? 0.0 : 1.0;   // in plane: P v
     const R zi = (R)Zi;
     R fi[BT::NF ? K : 1][3], ti[BT::NT ? K : 1][3];
 #pragma unroll
@@ -524,12 +525,12 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       if constexpr (BT::NF) {
         const double* v = p.f + k * p.rhs_stride + 3 * gi;
         const double c = valid ? p.c1 * Bi : 0.0;
-        fi[k][0] = (R)(c * v[0]); fi[k][1] = (R)(c * v[1]); fi[k][2] = (R)(c * v[2]);
+        fi[k][0] = (R)(c * v[0]); fi[k][1] = (R)(c * v[1]); fi[k][2] = (R)(pz * c * v[2]);
       }
       if constexpr (BT::NT) {
         const double* v = p.t + k * p.rhs_stride + 3 * gi;
         const double c = valid ? p.c2 * Bi : 0.0;
-        ti[k][0] = (R)(c * v[0]); ti[k][1] = (R)(c * v[1]); ti[k][2] = (R)(c * v[2]);
+        ti[k][0] = (R)(c * v[0]); ti[k][1] = (R)(c * v[1]); ti[k][2] = (R)(pz * c * v[2]);
       }
     }
     Acc<ACC> u[K][3], w[K][3];
@@ -541,19 +542,19 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       if (il < cnt) {
         const int64_t gj = Sy.b0 + s0 + il;
         double X, Y, Z, B;
-        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, kDomWall);
+        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, DOM);
         s_x[il] = X; s_y[il] = Y; s_z[il] = Z; s_zr[il] = (R)Z;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if constexpr (BT::NF) {
             const double* v = p.f + k * p.rhs_stride + 3 * gj;
             const double c = p.c1 * B;
-            s_f[k][0][il] = (R)(c * v[0]); s_f[k][1][il] = (R)(c * v[1]); s_f[k][2][il] = (R)(c * v[2]);
+            s_f[k][0][il] = (R)(c * v[0]); s_f[k][1][il] = (R)(c * v[1]); s_f[k][2][il] = (R)(pz * c * v[2]);
           }
           if constexpr (BT::NT) {
             const double* v = p.t + k * p.rhs_stride + 3 * gj;
             const double c = p.c2 * B;
-            s_t[k][0][il] = (R)(c * v[0]); s_t[k][1][il] = (R)(c * v[1]); s_t[k][2][il] = (R)(c * v[2]);
+            s_t[k][0][il] = (R)(c * v[0]); s_t[k][1][il] = (R)(c * v[1]); s_t[k][2][il] = (R)(pz * c * v[2]);
           }
         }
       }
@@ -579,9 +580,9 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
           CoefUF<R> cuf;
           CoefRot<R> cut, cwf;
           CoefWT<R> cwt;
-          if constexpr (UF) cuf = coef_uf(g, zi, zj);
-          if constexpr (UT || WF) { cut = coef_rot(g, zi); cwf = coef_rot(g, zj); }
-          if constexpr (WT) cwt = coef_wt(g);
+          if constexpr (UF) cuf = coef_uf(g, zi, zj, DOM);
+          if constexpr (UT || WF) { cut = coef_rot(g, zi, DOM); cwf = coef_rot(g, zj, DOM); }
+          if constexpr (WT) cwt = coef_wt(g, DOM);
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             // target side: u_i += UF f_j + UT t_j, w_i += WF f_j + WT t_j
@@ -1013,7 +1014,10 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
 
   // Symmetric layout: the UF block alone or all four blocks, unless switched off.
   const int sym = tuning ? tuning->symmetric : 0;
-  pl.mut_ok = (mask == kUF || mask == (kUF | kUT | kWF | kWT)) && pl.ntot > 0 && sym >= 0 && domain == kDomWall;
+  // the other boundaries: the UF block (and the unbounded grand mobility),
+  // one right-hand side per pass
+  pl.mut_ok = (mask == kUF || (mask == (kUF | kUT | kWF | kWT) && (domain == kDomWall || domain == kDomNoWall))) &&
+              pl.ntot > 0 && sym >= 0;
   RMW_REQUIRE(!(sym > 0 && !pl.mut_ok), "symmetric evaluation needs the UF block alone or all four blocks");
   int64_t tile_pairs = 0;
   for (int s = 0; s < nsys; ++s) {
@@ -1026,7 +1030,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   if (pl.mut_ok) {
     // at most 2 right-hand sides per pass (LDS of the source-side sums, registers)
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = group_k(k, rc > 0 ? std::min(rc, 2) : 2);
+    pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 2) : 2) : 1;
     if (tuning && tuning->chunk_len > 0) {
       pl.chunk_tiles = std::max<int64_t>(1, (tuning->chunk_len + kBlock - 1) / kBlock);
     } else {
@@ -1203,6 +1207,20 @@ static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
 
+template <int MASK, int DOM>
+static void dispatch_mutual_dom(int prec, const MutParams& q, int grid, hipStream_t st)
+{
+  const dim3 g(grid), b(kBlock);
+  if (prec == RMW_PREC_F32_BLOCKED)
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_BLOCKED, float, DOM>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_F64ACC)
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_F64ACC, float, DOM>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_KAHAN)
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_KAHAN, float, DOM>), g, b, 0, st, q);
+  else
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F64, double, DOM>), g, b, 0, st, q);
+}
+
 template <int MASK>
 static void dispatch_mutual_m(int prec, int K, const MutParams& q, int grid, hipStream_t st)
 {
@@ -1222,10 +1240,20 @@ static void dispatch_mutual_m(int prec, int K, const MutParams& q, int grid, hip
   }
 }
 
-static void dispatch_mutual(int mask, int prec, int K, const MutParams& q, int grid, hipStream_t st)
+static void dispatch_mutual(int domain, int mask, int prec, int K, const MutParams& q, int grid, hipStream_t st)
 {
-  if (mask == kUF) dispatch_mutual_m<kUF>(prec, K, q, grid, st);
-  else dispatch_mutual_m<kUF | kUT | kWF | kWT>(prec, K, q, grid, st);
+  if (domain == kDomNoWall) {
+    if (mask == kUF) dispatch_mutual_dom<kUF, kDomNoWall>(prec, q, grid, st);
+    else dispatch_mutual_dom<kUF | kUT | kWF | kWT, kDomNoWall>(prec, q, grid, st);
+  } else if (domain == kDomInPlane) {
+    dispatch_mutual_dom<kUF, kDomInPlane>(prec, q, grid, st);
+  } else if (domain == kDomFreeSurface) {
+    dispatch_mutual_dom<kUF, kDomFreeSurface>(prec, q, grid, st);
+  } else if (mask == kUF) {
+    dispatch_mutual_m<kUF>(prec, K, q, grid, st);
+  } else {
+    dispatch_mutual_m<kUF | kUT | kWF | kWT>(prec, K, q, grid, st);
+  }
 }
 
 // Pair-kernel timing records (rmw_profile_*).
@@ -1283,7 +1311,7 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st);
-    dispatch_mutual(pl.mask, pl.prec, K, q, grid, st);
+    dispatch_mutual(pl.domain, pl.mask, pl.prec, K, q, grid, st);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
 

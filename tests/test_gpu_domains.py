@@ -144,3 +144,23 @@ def test_rollers_in_other_domains(gpu, domain, free):
         g.advance_time_step(0.016)
         o.advance(0.016, draws)
     assert np.linalg.norm(g.locations() - o.r) <= 1e-8 * np.linalg.norm(o.r - r0)
+
+
+@pytest.mark.parametrize("domain,mask", [("no_wall", 1), ("no_wall", 15), ("in_plane", 1), ("free_surface", 1)])
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp64"])
+def test_symmetric_kernel_in_other_domains(gpu, domain, mask, prec):
+    """The symmetric kernel (each unordered pair once) serves the UF block of
+    every boundary and the unbounded grand mobility."""
+    from rigidmultiblobswall_amd import _lib
+    rng = np.random.default_rng(7)
+    n, a, eta = 777, 0.3, 0.8
+    r = np.column_stack([rng.uniform(0, 12, n), rng.uniform(0, 12, n), rng.uniform(0.1, 3.0, n)])
+    f, t = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    print(f"case symmetric domain {domain} mask {mask} {prec}", flush=True)
+    for chunk in (256, 768):
+        u, w = wall_rpy.apply(mask, r, force=f, torque=t, eta=eta, a=a, precision=prec, domain=domain,
+                              tuning=_lib.Tuning(chunk, 0, 0, 1, 0))
+        ur, wr = _ref(domain, mask, r, f, t, eta, a, None)
+        assert rel(u.cpu().numpy().ravel(), ur) <= TOL[prec]
+        if mask & RMW_WF:
+            assert rel(w.cpu().numpy().ravel(), wr) <= TOL[prec]
