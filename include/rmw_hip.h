@@ -279,6 +279,22 @@ int rmw_rigid_operator_post(const double* r_rel, const int64_t* body_offsets, in
  * back to back (block b at offset sum_{b'<b} (3 n_b')^2).  Replaces
  * single_wall_fluid_mobility, mobility/mobility.py:1018-1116, as selected by
  * mobility_blobs_implementation (multi_bodies/multi_bodies.py:207-230). */
+/* force_torque (B, 6) = [sum_i f_i, sum_i r_rel_i x f_i] per body from the
+ * blob forces (N, 3): force_torque_calculator_sort_by_bodies,
+ * multi_bodies/multi_bodies_functions.py:411-445. */
+int rmw_rigid_force_torque(const double* r_rel, const int64_t* body_offsets, int nbodies, const double* blob_force,
+                           double* force_torque, void* workspace, size_t workspace_bytes, void* stream);
+/* Body update of the deterministic rigid schemes: V = c_det U - c_prev U_prev
+ * (U_prev may be NULL), centre += V_v dt, quaternion = q(V_w dt) * quaternion
+ * (quaternion_integrator_multi_bodies.py:75-186, quaternion.py:15-49). */
+int rmw_bodies_advance(const double* centres, const double* quats, const double* velocities,
+                       const double* prev_velocities, int nbodies, double c_det, double c_prev, double dt,
+                       double* centres_new, double* quats_new, void* stream);
+/* count (device int64) = number of rows of r (n, 3) with r_z < level
+ * (body.check_function, body/body.py:118-140). */
+int rmw_count_below_workspace_size(int64_t n, size_t* bytes);
+int rmw_count_below(const double* r, int64_t n, double level, int64_t* count, void* workspace,
+                    size_t workspace_bytes, void* stream);
 int rmw_dense_wall_mobility(const double* r_blobs, const int64_t* body_offsets, int nbodies,
                             double eta, double a, double* out, void* workspace,
                             size_t workspace_bytes, void* stream);

@@ -87,6 +87,11 @@ PROTOTYPES = {
     "rmw_mobility_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64,
                                           ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz,
                                           _vp]),
+    "rmw_rigid_force_torque": (ctypes.c_int, [_dp, _i64p, ctypes.c_int, _dp, _dp, _vp, _sz, _vp]),
+    "rmw_bodies_advance": (ctypes.c_int, [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, _dp, _dp, _vp]),
+    "rmw_count_below_workspace_size": (ctypes.c_int, [_i64, _szp]),
+    "rmw_count_below": (ctypes.c_int, [_dp, _i64, ctypes.c_double, _dp, _vp, _sz, _vp]),
     "rmw_wall_rpy_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _tp, ctypes.c_int,
                                              ctypes.POINTER(PlanInfo)]),
     "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,

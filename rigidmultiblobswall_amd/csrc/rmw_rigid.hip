@@ -112,6 +112,99 @@ __global__ __launch_bounds__(kRigidBlock) void rigid_post_kernel(const double* _
   if (threadIdx.x < 6) y[3 * nblobs + 6 * (int64_t)b + threadIdx.x] = -red[threadIdx.x][0];
 }
 
+// Body force-torque from blob forces: F_b = sum f_i, T_b = sum r_i x f_i
+// over the blobs of body b (force_torque_calculator_sort_by_bodies,
+// multi_bodies_functions.py:411-445, T = rot^T f).  One workgroup per body,
+// fixed-order tree reduction.
+__global__ __launch_bounds__(kRigidBlock) void body_force_torque_kernel(const double* __restrict__ rrel,
+                                                                        const int64_t* __restrict__ offs,
+                                                                        const double* __restrict__ f,
+                                                                        double* __restrict__ ft)
+{
+  const int b = blockIdx.x;
+  const int64_t i0 = offs[b], i1 = offs[b + 1];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const double rx = rrel[3 * i], ry = rrel[3 * i + 1], rz = rrel[3 * i + 2];
+    const double fx = f[3 * i], fy = f[3 * i + 1], fz = f[3 * i + 2];
+    acc[0] += fx; acc[1] += fy; acc[2] += fz;
+    acc[3] += ry * fz - rz * fy;
+    acc[4] += rz * fx - rx * fz;
+    acc[5] += rx * fy - ry * fx;
+  }
+  __shared__ double red[6][kRigidBlock];
+  for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = kRigidBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) ft[6 * (int64_t)b + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Rigid-body position update of the deterministic schemes
+// (quaternion_integrator_multi_bodies.py:75-186): with V = c_det U - c_prev
+// U_prev, location += V_v dt and orientation = q(V_w dt) * orientation, q(phi)
+// = (cos(|phi|/2), sin(|phi|/2) phi/|phi|) (quaternion.py:15-38, product
+// :41-49; no renormalisation).
+__global__ void bodies_advance_kernel(const double* __restrict__ centres, const double* __restrict__ quats,
+                                      const double* __restrict__ U, const double* __restrict__ Uprev, int nbodies,
+                                      double c_det, double c_prev, double dt, double* __restrict__ cnew,
+                                      double* __restrict__ qnew)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbodies) return;
+  double V[6];
+  for (int c = 0; c < 6; ++c) {
+    V[c] = c_det * U[6 * b + c];
+    if (Uprev) V[c] = V[c] - c_prev * Uprev[6 * b + c];
+  }
+  for (int c = 0; c < 3; ++c) cnew[3 * b + c] = centres[3 * b + c] + V[c] * dt;
+  const double ph0 = V[3] * dt, ph1 = V[4] * dt, ph2 = V[5] * dt;
+  const double pn = sqrt(ph0 * ph0 + ph1 * ph1 + ph2 * ph2);
+  const double s1 = cos(pn / 2.0);
+  double p1[3] = {0.0, 0.0, 0.0};
+  if (pn != 0.0) {
+    const double k = sin(pn / 2.0) / pn;
+    p1[0] = k * ph0; p1[1] = k * ph1; p1[2] = k * ph2;
+  }
+  const double s2 = quats[4 * b], p2[3] = {quats[4 * b + 1], quats[4 * b + 2], quats[4 * b + 3]};
+  qnew[4 * b] = s1 * s2 - (p1[0] * p2[0] + p1[1] * p2[1] + p1[2] * p2[2]);
+  qnew[4 * b + 1] = s1 * p2[0] + s2 * p1[0] + (p1[1] * p2[2] - p1[2] * p2[1]);
+  qnew[4 * b + 2] = s1 * p2[1] + s2 * p1[1] + (p1[2] * p2[0] - p1[0] * p2[2]);
+  qnew[4 * b + 3] = s1 * p2[2] + s2 * p1[2] + (p1[0] * p2[1] - p1[1] * p2[0]);
+}
+
+// Number of heights r[3i+2] below `level` (body.check_function,
+// body/body.py:118-140): per-workgroup counts, then one workgroup sums them.
+__global__ __launch_bounds__(256) void count_below_kernel(const double* __restrict__ r, int64_t n, double level,
+                                                          int64_t* __restrict__ block_counts)
+{
+  __shared__ int s_cnt[4];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int below = i < n && r[3 * i + 2] < level;
+  const unsigned long long m = __ballot(below);
+  if ((threadIdx.x & 63) == 0) s_cnt[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = (int64_t)s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+}
+
+__global__ __launch_bounds__(256) void count_sum_kernel(const int64_t* __restrict__ block_counts, int64_t nblocks,
+                                                        int64_t* __restrict__ count)
+{
+  __shared__ int64_t s[256];
+  int64_t a = 0;
+  for (int64_t k = threadIdx.x; k < nblocks; k += 256) a += block_counts[k];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = s[0];
+}
+
 // ---------------------------------------------------------------------------
 // Dense wall mobility entries (FP64), B M(z_eff) B, one 3x3 block.
 // ---------------------------------------------------------------------------
@@ -518,6 +611,62 @@ extern "C" int rmw_rigid_operator_post(const double* r_rel, const int64_t* body_
     char* ws = (char*)workspace;
     hipLaunchKernelGGL(rigid_post_kernel, dim3(nbodies), dim3(kRigidBlock), 0, st, r_rel,
                        (const int64_t*)(ws + L.off_offs), x, m_lambda, y, L.nblobs);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_rigid_force_torque(const double* r_rel, const int64_t* body_offsets, int nbodies,
+                                      const double* blob_force, double* force_torque, void* workspace,
+                                      size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(r_rel && blob_force && force_torque, "null pointer");
+    hipLaunchKernelGGL(body_force_torque_kernel, dim3(nbodies), dim3(kRigidBlock), 0, (hipStream_t)stream, r_rel,
+                       (const int64_t*)((char*)workspace + L.off_offs), blob_force, force_torque);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_bodies_advance(const double* centres, const double* quats, const double* velocities,
+                                  const double* prev_velocities, int nbodies, double c_det, double c_prev,
+                                  double dt, double* centres_new, double* quats_new, void* stream)
+{
+  return guarded([&] {
+    RMW_REQUIRE(nbodies >= 0, "nbodies must be >= 0");
+    if (nbodies == 0) return;
+    RMW_REQUIRE(centres && quats && velocities && centres_new && quats_new, "null pointer");
+    hipLaunchKernelGGL(bodies_advance_kernel, dim3((nbodies + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       centres, quats, velocities, prev_velocities, nbodies, c_det, c_prev, dt, centres_new,
+                       quats_new);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_count_below_workspace_size(int64_t n, size_t* bytes)
+{
+  return guarded([&] {
+    RMW_REQUIRE(bytes != nullptr && n >= 0, "bad arguments");
+    *bytes = align_up((size_t)std::max<int64_t>((n + 255) / 256, 1) * sizeof(int64_t));
+  });
+}
+
+extern "C" int rmw_count_below(const double* r, int64_t n, double level, int64_t* count, void* workspace,
+                               size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RMW_REQUIRE(n >= 0 && count != nullptr, "bad arguments");
+    const int64_t nb = (n + 255) / 256;
+    RMW_REQUIRE(workspace && workspace_bytes >= (size_t)std::max<int64_t>(nb, 1) * sizeof(int64_t),
+                "workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    if (nb > 0) {
+      RMW_REQUIRE(r != nullptr, "r is null");
+      hipLaunchKernelGGL(count_below_kernel, dim3((unsigned)nb), dim3(256), 0, st, r, n, level, (int64_t*)workspace);
+      RMW_CHECK_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(count_sum_kernel, dim3(1), dim3(256), 0, st, (const int64_t*)workspace, nb, count);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

@@ -1,0 +1,176 @@
+"""Deterministic time stepping of rigid multiblob bodies above a wall.
+
+Mirror of QuaternionIntegrator (quaternion_integrator/
+quaternion_integrator_multi_bodies.py) for its GMRES schemes
+
+  deterministic_forward_euler     :75-107
+  deterministic_adams_bashforth   :142-186
+
+with solve_mobility_problem (:1441-1548), the body forces of
+force_torque_calculator_sort_by_bodies (multi_bodies/multi_bodies_functions.py
+:411-445: one-blob gravity and wall repulsion, blob-blob soft repulsion,
+torque sum r x f) and check_positions (:1779-1829: every blob of every body
+above the wall).  Everything runs on the device: blob positions, forces,
+the body force-torque sums, the preconditioned FP64 GMRES solve and the
+position / quaternion update.  The previous step's scaled solution is the
+GMRES initial guess (the reference passes it as x0, which its
+right-preconditioned wrapper then ignores, general_application_utils.py
+:627; honouring it changes only the iteration count).
+
+Active slip: ``slip_reference`` gives each body's blob slip in its
+reference frame (as read from a .slip file); it is rotated with the body
+every step like active_body_slip (multi_bodies_functions.py:123-140).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib, forces
+from .rigid import RigidSuspension
+
+SCHEMES = ("deterministic_forward_euler", "deterministic_adams_bashforth")
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+class RigidIntegrator:
+    def __init__(self, locations, orientations, reference, scheme: str = "deterministic_forward_euler",
+                 eta: float = 1.0, a: float = 1.0, g: float = 1.0, blob_mass: float = 1.0,
+                 repulsion_strength_wall: float = 1.0, debye_length_wall: float = 1.0,
+                 blob_blob_force_implementation: str = "None", repulsion_strength: float = 1.0,
+                 debye_length: float = 1.0, periodic_length=None, tolerance: float = 1e-8,
+                 precision: str = "fp64", slip_reference=None, domain: str = "single_wall", restart: int = 60):
+        if scheme not in SCHEMES:
+            raise ValueError(f"scheme {scheme!r} is not provided; use one of {list(SCHEMES)}")
+        if domain != "single_wall":
+            raise ValueError("rigid bodies: domain single_wall only")
+        self.scheme = scheme
+        self.lib = _lib.load()
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.centres = np.ascontiguousarray(np.asarray(locations, dtype=np.float64).reshape(-1, 3))
+        self.quats = np.ascontiguousarray(np.asarray(orientations, dtype=np.float64).reshape(-1, 4))
+        self.nbodies = self.centres.shape[0]
+        self.reference = reference
+        self.eta, self.a, self.g, self.blob_mass = float(eta), float(a), float(g), float(blob_mass)
+        self.repulsion_strength_wall, self.debye_length_wall = float(repulsion_strength_wall), float(debye_length_wall)
+        self.repulsion_strength, self.debye_length = float(repulsion_strength), float(debye_length)
+        self.periodic_length = periodic_length
+        self.tolerance, self.precision, self.restart = float(tolerance), precision, int(restart)
+        self.blob_blob = None
+        if blob_blob_force_implementation != "None":
+            kw = forces.set_blob_blob_forces(blob_blob_force_implementation).keywords
+            if blob_blob_force_implementation.startswith("radii_hip"):
+                raise ValueError("use 'hip' or 'tree_hip' (one blob radius)")
+            self.blob_blob = dict(precision=kw["precision"],
+                                  cutoff=(2 * self.a + 30 * self.debye_length
+                                          if blob_blob_force_implementation.startswith("tree_hip") else 0.0))
+        self.slip_reference = slip_reference
+        self.first_step = True
+        self.first_guess: Optional[torch.Tensor] = None   # scaled solution of the previous solve
+        self.velocities_previous_step: Optional[torch.Tensor] = None
+        self.invalid_configuration_count = 0
+        self.det_iterations_count = 0
+        self._S: Optional[RigidSuspension] = None
+
+    # ------------------------------------------------------------------
+    def suspension(self, centres=None, quats=None) -> RigidSuspension:
+        return RigidSuspension(self.centres if centres is None else centres, self.quats if quats is None else quats,
+                               self.reference, self.eta, self.a, periodic_length=self.periodic_length,
+                               precision=self.precision)
+
+    def blob_forces(self, S: RigidSuspension) -> torch.Tensor:
+        F = torch.empty((S.Nblobs, 3), dtype=torch.float64, device=self.device)
+        acc = 0
+        if self.blob_blob is not None:
+            forces.blob_forces(S.r_blobs, self.repulsion_strength, self.debye_length, self.a,
+                               periodic_length=self.periodic_length, out=F, **self.blob_blob)
+            acc = 1
+        _lib.check(self.lib.rmw_rollers_one_blob_forces(S.r_blobs.data_ptr(), S.Nblobs, self.blob_mass, self.g,
+                                                        self.a, self.repulsion_strength_wall, self.debye_length_wall,
+                                                        acc, F.data_ptr(), _stream()))
+        return F
+
+    def force_torque(self, S: RigidSuspension) -> torch.Tensor:
+        """(B, 6) [F, T] per body (force_torque_calculator_sort_by_bodies)."""
+        F = self.blob_forces(S)
+        ft = torch.empty((S.nbodies, 6), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.rmw_rigid_force_torque(S.r_rel.data_ptr(), S._offp(), S.nbodies, F.data_ptr(),
+                                                   ft.data_ptr(), S.ws.data_ptr(), S.ws.numel(), _stream()))
+        return ft
+
+    def slip(self, S: RigidSuspension) -> torch.Tensor:
+        if self.slip_reference is None:
+            return torch.zeros((S.Nblobs, 3), dtype=torch.float64, device=self.device)
+        ref = (np.vstack([np.asarray(s, dtype=np.float64).reshape(-1, 3) for s in self.slip_reference])
+               if isinstance(self.slip_reference, (list, tuple)) else
+               np.tile(np.asarray(self.slip_reference, dtype=np.float64).reshape(-1, 3), (self.nbodies, 1)))
+        sref = torch.from_numpy(np.ascontiguousarray(ref)).to(self.device)
+        zero = torch.zeros_like(S.centres)
+        rot = torch.empty((S.Nblobs, 3), dtype=torch.float64, device=self.device)
+        dummy = torch.empty_like(rot)
+        # the body rotation applied to the reference-frame slip (centres at 0)
+        _lib.check(self.lib.rmw_body_blobs(zero.data_ptr(), S.quats.data_ptr(), sref.data_ptr(), S._offp(),
+                                           S.nbodies, dummy.data_ptr(), rot.data_ptr(), S.ws.data_ptr(),
+                                           S.ws.numel(), _stream()))
+        return rot
+
+    def solve_mobility_problem(self, S: RigidSuspension):
+        """Body velocities (B, 6) of the current configuration (:1441-1548)."""
+        rhs = S.rhs(self.slip(S), self.force_torque(S))
+        rnorm = float(torch.linalg.norm(rhs))
+        x0 = None if (self.first_guess is None or rnorm == 0.0) else self.first_guess * rnorm
+        res = S.solve_mobility_problem(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0)
+        if res.info != 0:
+            raise RuntimeError(f"GMRES did not reach {self.tolerance} (residual {res.final_residual:.3e})")
+        self.det_iterations_count += res.iterations
+        self.first_guess = res.x / rnorm if rnorm > 0 else torch.zeros_like(res.x)
+        return res.x[3 * S.Nblobs:].reshape(S.nbodies, 6).contiguous()
+
+    def _advance(self, dt: float, adams_bashforth: bool) -> None:
+        S = self._S if self._S is not None else self.suspension()
+        U = self.solve_mobility_problem(S)
+        c_new = torch.empty_like(S.centres)
+        q_new = torch.empty_like(S.quats)
+        ab = adams_bashforth and not self.first_step
+        _lib.check(self.lib.rmw_bodies_advance(S.centres.data_ptr(), S.quats.data_ptr(), U.data_ptr(),
+                                               self.velocities_previous_step.data_ptr() if ab else None,
+                                               S.nbodies, 1.5 if ab else 1.0, 0.5 if ab else 0.0, float(dt),
+                                               c_new.data_ptr(), q_new.data_ptr(), _stream()))
+        S_new = self.suspension(c_new.cpu().numpy(), q_new.cpu().numpy())
+        nb = ctypes.c_size_t()
+        _lib.check(self.lib.rmw_count_below_workspace_size(S_new.Nblobs, ctypes.byref(nb)))
+        ws = torch.empty(max(nb.value, 256), dtype=torch.uint8, device=self.device)
+        cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
+        _lib.check(self.lib.rmw_count_below(S_new.r_blobs.data_ptr(), S_new.Nblobs, 0.0, cnt.data_ptr(),
+                                            ws.data_ptr(), ws.numel(), _stream()))
+        if int(cnt.item()) > 0:
+            # check_positions fails; a deterministic step would be recomputed
+            # unchanged forever by the reference
+            self.invalid_configuration_count += 1
+            raise RuntimeError("the deterministic step moves a blob below the wall")
+        if adams_bashforth:
+            self.first_step = False
+            self.velocities_previous_step = U
+        self.centres = c_new.cpu().numpy()
+        self.quats = q_new.cpu().numpy()
+        self._S = S_new
+
+    def deterministic_forward_euler(self, dt: float, *args, **kwargs):
+        self._advance(dt, adams_bashforth=False)
+
+    def deterministic_adams_bashforth(self, dt: float, *args, **kwargs):
+        self._advance(dt, adams_bashforth=True)
+
+    def advance_time_step(self, dt: float, *args, **kwargs):
+        """One step of the selected scheme (:68-72)."""
+        return getattr(self, self.scheme)(dt, *args, **kwargs)
+
+    def blob_positions(self) -> np.ndarray:
+        S = self._S if self._S is not None else self.suspension()
+        return S.r_blobs.cpu().numpy()

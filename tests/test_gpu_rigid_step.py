@@ -1,0 +1,90 @@
+"""Deterministic rigid-body time steps on the GPU against the oracle's
+restatement of the reference integrator (quaternion_integrator_multi_bodies.py
+:75-186): forward Euler and Adams-Bashforth with gravity, wall and blob-blob
+forces and an active slip, the wall check, and the shared body pieces
+(force-torque sums, quaternion update, height count)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from oracle import rigid_step_oracle as RS
+from rigidmultiblobswall_amd import _lib
+from rigidmultiblobswall_amd.rigid import RigidSuspension
+from rigidmultiblobswall_amd.rigid_integrator import RigidIntegrator
+from rigidmultiblobswall_amd.workloads import icosphere_42, random_quaternions
+
+pytestmark = pytest.mark.gpu
+
+P = dict(g=0.3, repulsion_strength_wall=0.2, debye_length_wall=0.1, repulsion_strength=0.5, debye_length=0.2)
+PO = dict(g=0.3, eps_wall=0.2, b_wall=0.1, eps=0.5, b=0.2)
+
+
+def _bodies(nb, seed):
+    rng = np.random.default_rng(seed)
+    c = np.column_stack([np.arange(nb) * 2.55, rng.uniform(-0.3, 0.3, nb), rng.uniform(1.3, 2.0, nb)])
+    return c, random_quaternions(rng, nb), rng
+
+
+@pytest.mark.parametrize("scheme", ["deterministic_forward_euler", "deterministic_adams_bashforth"])
+def test_rigid_steps_match_oracle(gpu, scheme):
+    c0, q0, rng = _bodies(4, 1)
+    ref = icosphere_42(1.0)
+    slip = [rng.standard_normal((42, 3)) * 0.2 for _ in range(4)]
+    g = RigidIntegrator(c0, q0, ref, scheme=scheme, eta=1.0, a=0.243, blob_blob_force_implementation="hip_fp64",
+                        tolerance=1e-11, slip_reference=slip, **P)
+    o = RS.RigidStep(c0, q0, ref, scheme, 1.0, 0.243, blob_blob=True, tol=1e-11, slip_refs=slip, **PO)
+    print(f"case rigid step {scheme}", flush=True)
+    for _ in range(3):
+        g.advance_time_step(0.05)
+        o.advance(0.05)
+    dc = np.linalg.norm(o.c - c0)
+    assert np.linalg.norm(g.centres - o.c) <= 1e-8 * dc
+    assert np.abs(g.quats - o.q).max() <= 1e-8 * np.abs(o.q - q0).max()
+    assert g.det_iterations_count > 0
+
+
+def test_body_pieces(gpu):
+    c, q, rng = _bodies(5, 2)
+    refs = [icosphere_42(1.0)[: 10 + 6 * k] for k in range(5)]           # ragged bodies
+    S = RigidSuspension(c, q, refs, 1.0, 0.3)
+    f = torch.from_numpy(rng.standard_normal((S.Nblobs, 3))).cuda()
+    lib = _lib.load()
+    ft = torch.empty((5, 6), dtype=torch.float64, device="cuda")
+    print("case rigid body pieces", flush=True)
+    _lib.check(lib.rmw_rigid_force_torque(S.r_rel.data_ptr(), S._offp(), 5, f.data_ptr(), ft.data_ptr(),
+                                          S.ws.data_ptr(), S.ws.numel(), torch.cuda.current_stream().cuda_stream))
+    sysO = O.RigidSystem([(c[k], q[k], refs[k]) for k in range(5)], 1.0, 0.3)
+    assert np.allclose(ft.cpu().numpy().ravel(), sysO.KT_dot(f.cpu().numpy().ravel()), rtol=0, atol=1e-13)
+    U = rng.standard_normal((5, 6))
+    Up = rng.standard_normal((5, 6))
+    cn = torch.empty((5, 3), dtype=torch.float64, device="cuda")
+    qn = torch.empty((5, 4), dtype=torch.float64, device="cuda")
+    Ud, Upd = torch.from_numpy(U).cuda(), torch.from_numpy(Up).cuda()   # kept alive through the call
+    _lib.check(lib.rmw_bodies_advance(S.centres.data_ptr(), S.quats.data_ptr(), Ud.data_ptr(), Upd.data_ptr(), 5,
+                                      1.5, 0.5, 0.01, cn.data_ptr(), qn.data_ptr(),
+                                      torch.cuda.current_stream().cuda_stream))
+    V = 1.5 * U - 0.5 * Up
+    assert np.allclose(cn.cpu().numpy(), c + 0.01 * V[:, :3], rtol=0, atol=1e-15)
+    qref = np.array([RS.quat_mul(RS.quat_from_rotation(V[k, 3:] * 0.01), q[k]) for k in range(5)])
+    assert np.allclose(qn.cpu().numpy(), qref, rtol=0, atol=1e-15)
+    nb = ctypes.c_size_t()
+    _lib.check(lib.rmw_count_below_workspace_size(S.Nblobs, ctypes.byref(nb)))
+    ws = torch.empty(max(nb.value, 256), dtype=torch.uint8, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    level = float(np.median(S.r_blobs[:, 2].cpu().numpy()))
+    _lib.check(lib.rmw_count_below(S.r_blobs.data_ptr(), S.Nblobs, level, cnt.data_ptr(), ws.data_ptr(), ws.numel(),
+                                   torch.cuda.current_stream().cuda_stream))
+    assert int(cnt.item()) == int((S.r_blobs[:, 2].cpu().numpy() < level).sum())
+
+
+def test_wall_check_stops_a_deterministic_step(gpu):
+    c, q, _ = _bodies(2, 3)
+    c[:, 2] = 1.05
+    g = RigidIntegrator(c, q, icosphere_42(1.0), eta=1.0, a=0.243, g=50.0, repulsion_strength_wall=0.0)
+    print("case rigid wall check", flush=True)
+    with pytest.raises(RuntimeError, match="below the wall"):
+        g.advance_time_step(1.0)
+    assert g.invalid_configuration_count == 1
