@@ -5,6 +5,7 @@ loop.
   ReadInput(path)        read_input/read_input.py:9-159
   read_vertex_file(p)    read_input/read_vertex_file.py:7-31
   read_clones_file(p)    read_input/read_clones_file.py:11-52
+  read_slip_file(p)      read_input/read_slip_file.py:8-35
   write_clones(...)      multi_bodies/multi_bodies.py:1430-1456 (one frame)
 
 The option names, defaults and parsing rules (comments after '#', the first
@@ -167,6 +168,24 @@ def read_clones_file(name_file: str) -> Tuple[int, np.ndarray, np.ndarray]:
             if i == number_of_bodies + 1:
                 break
     return number_of_bodies, np.array(locations).reshape(-1, 3), np.array(orientations).reshape(-1, 4)
+
+
+def read_slip_file(name_file: str) -> np.ndarray:
+    """Blob slip (Nblobs, 3) in the body's reference frame
+    (read_input/read_slip_file.py:8-35): a count line, then three numbers
+    per blob."""
+    slip = []
+    i = 0
+    with open(name_file, "r") as f:
+        for line in f:
+            line = _strip(line)
+            if line == "":
+                continue
+            if i > 0:
+                d = line.split()
+                slip.append([float(d[0]), float(d[1]), float(d[2])])
+            i += 1
+    return np.array(slip).reshape(-1, 3)
 
 
 def write_clones(f, locations: np.ndarray, orientations: np.ndarray) -> None:

@@ -88,3 +88,38 @@ def test_wall_check_stops_a_deterministic_step(gpu):
     with pytest.raises(RuntimeError, match="below the wall"):
         g.advance_time_step(1.0)
     assert g.invalid_configuration_count == 1
+
+
+def test_run_driver_rigid_bodies(gpu, tmp_path):
+    """run_simulation on a reference-style input: three 42-blob shells with a
+    .slip file, deterministic forward Euler, 2 steps, against the oracle."""
+    import re
+    from rigidmultiblobswall_amd import read_input as RI
+    from rigidmultiblobswall_amd import run_simulation
+    ref = icosphere_42(1.0)
+    c0, q0, rng = _bodies(3, 5)
+    slip = rng.standard_normal((42, 3)) * 0.1
+    (tmp_path / "shell.vertex").write_text("42\n" + "\n".join(" ".join(repr(float(v)) for v in row) for row in ref) + "\n")
+    with open(tmp_path / "shells.clones", "w") as f:
+        RI.write_clones(f, c0, q0)
+    (tmp_path / "shell.slip").write_text("42\n" + "\n".join(" ".join(repr(float(v)) for v in row) for row in slip) + "\n")
+    (tmp_path / "in.dat").write_text(
+        "scheme deterministic_forward_euler\nmobility_vector_prod_implementation numba\n"
+        "blob_blob_force_implementation numba\nrepulsion_strength 0.5\ndebye_length 0.2\n"
+        "dt 0.05\nn_steps 2\nn_save 1\neta 1.0\ng 0.3\nblob_radius 0.243\nsolver_tolerance 1e-10\n"
+        "repulsion_strength_wall 0.2\ndebye_length_wall 0.1\nsave_clones one_file\n"
+        f"output_name {tmp_path}/run\n"
+        f"structure {tmp_path}/shell.vertex {tmp_path}/shells.clones {tmp_path}/shell.slip\n")
+    print("case run_simulation rigid bodies", flush=True)
+    info = run_simulation.run(str(tmp_path / "in.dat"), log=lambda m: None)
+    assert info["steps"] == 2
+    lines = (tmp_path / "run.shells.config").read_text().splitlines()
+    assert len(lines) == 3 * 4 and lines[0] == "3"
+    last = np.array([[float(v) for v in ln.split()] for ln in lines[9:12]])
+    o = RS.RigidStep(c0, q0, ref, "deterministic_forward_euler", 1.0, 0.243, blob_blob=True, tol=1e-10,
+                     slip_refs=[slip] * 3, **PO)
+    o.advance(0.05)
+    o.advance(0.05)
+    assert np.abs(last[:, :3] - o.c).max() <= 1e-8 * np.abs(o.c - c0).max()
+    assert np.abs(last[:, 3:] - o.q).max() <= 1e-8
+    assert re.search(r"num_blobs\s+126", (tmp_path / "run.bodies_info").read_text())

@@ -66,3 +66,9 @@ def test_clones_round_trip(tmp_path):
     p = tmp_path / "r.vertex"
     p.write_text("2\n0 0 0 0.5\n1 0 0 0.25 # blob two\n")
     assert np.array_equal(RI.read_vertex_file(str(p)), [[0, 0, 0, 0.5], [1, 0, 0, 0.25]])
+
+
+def test_slip_file(tmp_path):
+    p = tmp_path / "a.slip"
+    p.write_text("# slip of two blobs\n2\n0.1 0.2 0.3\n-1 0 2.5 # second\n")
+    assert np.array_equal(RI.read_slip_file(str(p)), [[0.1, 0.2, 0.3], [-1.0, 0.0, 2.5]])
