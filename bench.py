@@ -19,10 +19,10 @@ Prints ONE JSON line (rank 0):
                GMRES from x0 = 0 against the learned warm start (network
                trained on the GPU from a short trajectory), per-solve times
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step|c2step]
        (c3r: the different-radii product on the C3 geometry; c3f: the
        blob-blob soft-repulsion forces on it; c3step: whole rollers time
-       steps, steps/s)
+       steps, steps/s; c2step: rigid-body time steps of the C2 suspension)
 """
 from __future__ import annotations
 
@@ -277,6 +277,79 @@ def rollers_step_bench(args, log):
     return 0
 
 
+def rigid_step_bench(args, log):
+    """One step = one deterministic forward Euler step of the rigid-body
+    integrator on the C2 suspension (256 spheres x 42 blobs) with gravity,
+    wall and blob-blob repulsion: blob positions, forces and body
+    force-torque sums, the block-diagonal preconditioner, the FP64 GMRES
+    solve to 1e-8 (warm-started from the previous step) and the body update."""
+    import torch
+    from rigidmultiblobswall_amd import wall_rpy, workloads
+    from rigidmultiblobswall_amd.rigid_integrator import RigidIntegrator
+    w = workloads.c2_rigid_spheres()
+    fp = dict(g=0.02, repulsion_strength_wall=0.02, debye_length_wall=0.1, repulsion_strength=0.02,
+              debye_length=0.1)
+    kw = dict(scheme="deterministic_forward_euler", eta=w["eta"], a=w["a"], blob_blob_force_implementation="hip_fp64",
+              tolerance=1e-8, **fp)
+    prec = "fp64" if args.precision == "fp32_blocked" else args.precision
+    integ = RigidIntegrator(w["centres"], w["quats"], w["ref"], precision=prec, **kw)
+    dt = 1e-3
+    for _ in range(args.warmup):
+        integ.advance_time_step(dt)
+    torch.cuda.synchronize()
+    it0 = integ.det_iterations_count
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with wall_rpy.PairKernelTimer() as pk:
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(args.steps):
+            integ.advance_time_step(dt)
+        stop.record()
+        torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    its = (integ.det_iterations_count - it0) / args.steps
+    nw, tw, mw = pk.by_kind["wall_rpy"]
+    n = w["r"].shape[0]
+    pairs = float(n) * n
+    peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
+    achieved = FLOP_PER_PAIR["UF"] * pairs * nw / (tw * 1e-3) / 1e12 if tw else 0.0
+    log(f"rigid step: {ms:.3f} ms/step, {its:.1f} GMRES iterations/step, wall-RPY kernels "
+        f"{tw / args.steps:.3f} ms/step ({nw / args.steps:.1f} launches)")
+    out = {
+        "metric": "rigid-body time steps/s (deterministic forward Euler, GMRES to 1e-8)",
+        "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f32", "data": "synthetic",
+        "config": {"workload": "C2 suspension: 256 spheres x 42 blobs = 10752 blobs, a = 0.243, eta = 1, dt 1e-3, "
+                               "gravity 0.02 per blob, wall and blob-blob repulsion (0.02, 0.1)",
+                   "blobs": n, "precision": prec + PRECISION_NOTE.get(prec, ""),
+                   "per_step": {"gmres_iterations": its, "wall_rpy_launches": nw / args.steps,
+                                "wall_rpy_kernel_ms": tw / args.steps,
+                                "pair_kernel_share": tw / (ms * args.steps)}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": "wall-RPY UF pair kernel of the GMRES operator",
+                     "kernel_mean_ms": mw, "kernel_launches": nw, "flop_per_pair": FLOP_PER_PAIR["UF"]},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        from oracle import rigid_step_oracle as RS
+        st = RS.RigidStep(w["centres"], w["quats"], w["ref"], "deterministic_forward_euler", w["eta"], w["a"],
+                          g=fp["g"], eps_wall=fp["repulsion_strength_wall"], b_wall=fp["debye_length_wall"],
+                          blob_blob=True, eps=fp["repulsion_strength"], b=fp["debye_length"], tol=1e-8)
+        t = time.perf_counter()
+        st.advance(dt)
+        dt_cpu = time.perf_counter() - t
+        threads = O.num_threads("strict")
+        out["cpu_baseline"] = {"value": 1.0 / dt_cpu, "unit": "steps/s", "cores": threads, "kind": "port",
+                               "sample": f"1 forward Euler step of the same suspension: FP64 C oracle products "
+                                         f"(OpenMP {threads} threads), numpy preconditioner and GMRES; "
+                                         f"{dt_cpu:.1f} s",
+                               "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+        log(f"cpu baseline: {dt_cpu:.1f} s per step")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def load_traffic(workload: str, precision: str, kernel: str = "pair_kernel_uf"):
     """HBM bytes per pair-kernel launch from the committed rocprof PMC summary
     of the same kernel, workload and precision (None when there is none)."""
@@ -331,6 +404,8 @@ def main(argv=None):
         return 0
     if args.workload == "c3step":
         return rollers_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
+    if args.workload == "c2step":
+        return rigid_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
     dev = torch.device("cuda", 0)
     w, desc = load_workload(args.workload)
     r_h, eta, a = w["r"], w["eta"], w["a"]
