@@ -350,13 +350,17 @@ def rigid_step_bench(args, log):
     return 0
 
 
-def load_traffic(workload: str, precision: str, kernel: str = "pair_kernel_uf"):
+def load_traffic(workload: str, precision: str, kernel: str = "pair_kernel_uf", launched: str = ""):
     """HBM bytes per pair-kernel launch from the committed rocprof PMC summary
-    of the same kernel, workload and precision (None when there is none)."""
+    of the same kernel, workload and precision (None when there is none, or
+    when the summary counted another kernel variant than `launched`)."""
     path = os.path.join(REPO, "profiles", f"{kernel}_{workload}_{precision}_traffic.json")
     if os.path.exists(path):
         with open(path) as fh:
-            return json.load(fh).get("traffic_bytes")
+            rec = json.load(fh)
+        if launched and not launched.startswith(rec.get("kernel", "") + "<"):
+            return None
+        return rec.get("traffic_bytes")
     return None
 
 
@@ -500,7 +504,7 @@ def main(argv=None):
                      "frac": achieved / peak,
                      "traffic": load_traffic(args.workload, fprec if fparams else args.precision,
                                              "force_kernel" if fparams else "radii_kernel" if radii
-                                             else "pair_kernel_uf"),
+                                             else "pair_kernel_uf", kname),
                      "kernel": kname,
                      "kernel_mean_ms": kernel_ms, "kernel_launches": pk.launches,
                      "flop_per_pair": FLOP_PER_PAIR[op]},

@@ -6,6 +6,7 @@ import torch
 
 from oracle import oracle as O
 import golden_cases as G
+from rigidmultiblobswall_amd import krylov
 from rigidmultiblobswall_amd import mobility as mob
 from rigidmultiblobswall_amd import workloads
 from rigidmultiblobswall_amd.multi_bodies import set_mobility_blobs, set_mobility_vector_prod
@@ -97,6 +98,54 @@ def test_c2_solve_residual_and_warm_start(gpu):
     pert = out.x * (1 + 1e-4 * torch.from_numpy(rng.standard_normal(S.size)).to(gpu))
     warm = S.solve_mobility_problem(slip, w["force_torque"], tol=1e-8, x0=pert)
     assert warm.info == 0 and warm.iterations < out.iterations
+
+
+def test_c2_mixed_solve_reaches_fp64_tolerance(gpu):
+    """FP32-pair inner GMRES with FP64 residual refinement: the FP64 residual
+    of the result meets 1e-8 from a zero start and from warm starts (whose
+    first residual is formed with the FP32-pair operator)."""
+    w = workloads.c2_rigid_spheres()
+    print("case C2 256 spheres x 42 blobs: mixed-precision solve to 1e-8", flush=True)
+    S = RigidSuspension(w["centres"], w["quats"], w["ref"], w["eta"], w["a"], precision="fp64")
+    slip = np.zeros((S.Nblobs, 3))
+    rhs = S.rhs(slip, w["force_torque"])
+
+    def true_res(x):
+        r = torch.empty_like(rhs)
+        S.operator(x, r, precision="fp64")
+        return float(torch.linalg.norm(r - rhs) / torch.linalg.norm(rhs))
+
+    ref = S.solve_mobility_problem(rhs=rhs, tol=1e-8)
+    cold = S.solve_mixed(rhs=rhs, tol=1e-8)
+    print(f"  fp64 iterations {ref.iterations}, mixed iterations {cold.iterations}, matvecs {cold.matvecs}")
+    assert cold.info == 0 and true_res(cold.x) <= 1.5e-8
+    assert abs(cold.final_residual - true_res(cold.x)) <= 1e-9
+    assert float(torch.linalg.norm(cold.x - ref.x) / torch.linalg.norm(ref.x)) < 1e-5
+    rng = np.random.default_rng(8)
+    pert = ref.x * (1 + 1e-3 * torch.from_numpy(rng.standard_normal(S.size)).to(gpu))
+    warm = S.solve_mixed(rhs=rhs, tol=1e-8, x0=pert)
+    assert warm.info == 0 and true_res(warm.x) <= 1.5e-8 and warm.iterations < cold.iterations
+    exact = S.solve_mixed(rhs=rhs, tol=1e-8, x0=ref.x)
+    assert exact.info == 0 and true_res(exact.x) <= 1.5e-8 and exact.iterations <= 4
+    with pytest.raises(ValueError):
+        S.solve_mixed(rhs=rhs, max_outer=0)
+
+
+def test_gmres_unverified_inner_solve(gpu):
+    """verify=False stops on the Arnoldi estimate without the product that
+    recomputes b - A x; the estimate then bounds the true residual."""
+    w = workloads.c2_rigid_spheres()
+    S = RigidSuspension(w["centres"], w["quats"], w["ref"], w["eta"], w["a"], precision="fp64")
+    rhs = S.rhs(np.zeros((S.Nblobs, 3)), w["force_torque"])
+    b = rhs / torch.linalg.norm(rhs)
+    S.build_preconditioner()
+    full = krylov.gmres(lambda v, o: S.operator(v, o), b, Pinv=S.pc_apply, tol=1e-6, restart=60)
+    fast = krylov.gmres(lambda v, o: S.operator(v, o), b, Pinv=S.pc_apply, tol=1e-6, restart=60, verify=False)
+    assert fast.info == 0 and fast.matvecs == full.matvecs - 1 and fast.iterations == full.iterations
+    r = torch.empty_like(b)
+    S.operator(fast.x, r)
+    assert float(torch.linalg.norm(r - b)) <= 2e-6
+    assert abs(fast.final_residual - float(torch.linalg.norm(r - b))) < 1e-8
 
 
 def test_dispatch_names(gpu):
