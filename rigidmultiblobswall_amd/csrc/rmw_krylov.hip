@@ -18,7 +18,8 @@ constexpr int kKBlock = 256;
 constexpr int kMaxDotBlocks = 512;
 constexpr int kMaxVec = 1024;
 
-// partial[j * G + g] = sum over the g-th slice of V_j . w
+// partial[j * G + g] = sum over the g-th slice of V_j . w  (long vectors:
+// each block walks a slice of (n + G - 1) / G elements once per vector)
 __global__ __launch_bounds__(kKBlock) void dot_many_partial(const double* __restrict__ V, int64_t ldv, int nvec,
                                                             const double* __restrict__ w, int64_t n,
                                                             double* __restrict__ partial)
@@ -45,16 +46,67 @@ __global__ __launch_bounds__(kKBlock) void dot_many_partial(const double* __rest
   }
 }
 
-// out[j] = scale * sum_g partial[j * G + g] (+ out[j] when accumulate)
-__global__ void dot_many_final(const double* __restrict__ partial, int G, int nvec, double* __restrict__ out,
-                               int accumulate, int sqrt_out)
+// Same partials when every block owns one slice of kKBlock * E elements:
+// the slice of w stays in registers across the vectors, each vector's
+// wave sums go to LDS and one barrier per group of 64 vectors publishes
+// them (fixed summation order, as above).
+constexpr int kDotGroup = 64;
+
+template <int E>
+__global__ __launch_bounds__(kKBlock) void dot_many_partial_reg(const double* __restrict__ V, int64_t ldv, int nvec,
+                                                                const double* __restrict__ w, int64_t n,
+                                                                double* __restrict__ partial)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[kDotGroup][kKBlock / 64];
+  const int G = gridDim.x;
+  const int64_t base = (int64_t)blockIdx.x * (kKBlock * E) + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double wr[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int64_t i = base + (int64_t)e * kKBlock;
+    wr[e] = i < n ? w[i] : 0.0;
+  }
+  for (int j0 = 0; j0 < nvec; j0 += kDotGroup) {
+    const int jn = nvec - j0 < kDotGroup ? nvec - j0 : kDotGroup;
+    for (int jj = 0; jj < jn; ++jj) {
+      const double* v = V + (int64_t)(j0 + jj) * ldv;
+      double acc = 0.0;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int64_t i = base + (int64_t)e * kKBlock;
+        if (i < n) acc = fma(v[i], wr[e], acc);
+      }
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+      if (lane == 0) red[jj][wv] = acc;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < jn) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < kKBlock / 64; ++k) s += red[threadIdx.x][k];
+      partial[(int64_t)(j0 + threadIdx.x) * G + blockIdx.x] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// out[j] = sum_g partial[j * G + g] (sqrt when sqrt_out; + out[j] when
+// accumulate): one wavefront per output, lanes stride the partials, then a
+// fixed shuffle tree.
+__global__ __launch_bounds__(64) void dot_many_final(const double* __restrict__ partial, int G, int nvec,
+                                                     double* __restrict__ out, int accumulate, int sqrt_out)
+{
+  const int j = blockIdx.x;
   if (j >= nvec) return;
+  const double* p = partial + (int64_t)j * G;
   double s = 0.0;
-  for (int g = 0; g < G; ++g) s += partial[(int64_t)j * G + g];
-  if (sqrt_out) s = sqrt(s);
-  out[j] = accumulate ? out[j] + s : s;
+  for (int g = threadIdx.x; g < G; g += 64) s += p[g];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (threadIdx.x == 0) {
+    if (sqrt_out) s = sqrt(s);
+    out[j] = accumulate ? out[j] + s : s;
+  }
 }
 
 // w -= sum_j coef[j] V_j     (coef on the device)
@@ -109,11 +161,41 @@ __global__ void add_vec(const double* __restrict__ a, double* __restrict__ b, in
   if (j < m) b[j] += a[j];
 }
 
-static int dot_blocks(int64_t n)
+// Dot-product launch: elements per thread E of the register-slice kernel
+// (0: the slice-walking kernel) and the number of blocks G <= kMaxDotBlocks.
+struct DotLaunch {
+  int E, G;
+};
+
+static DotLaunch dot_launch(int64_t n)
 {
   const DeviceProps& dp = device_props();
+  // aim for about two blocks per CU with the fewest elements per thread
+  const int64_t want = std::min<int64_t>(kMaxDotBlocks, 2 * (int64_t)dp.cu_count);
+  for (int E : {1, 2, 4, 8}) {
+    const int64_t g = (n + (int64_t)kKBlock * E - 1) / ((int64_t)kKBlock * E);
+    if (g <= want || (E == 8 && g <= kMaxDotBlocks)) return {E, (int)std::max<int64_t>(1, g)};
+  }
   int64_t g = (n + 4 * kKBlock - 1) / (4 * kKBlock);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(g, std::min(kMaxDotBlocks, 2 * dp.cu_count)));
+  return {0, (int)std::max<int64_t>(1, std::min<int64_t>(g, kMaxDotBlocks))};
+}
+
+static void launch_dot_partial(const DotLaunch& L, hipStream_t st, const double* V, int64_t ldv, int nvec,
+                               const double* w, int64_t n, double* partial)
+{
+  switch (L.E) {
+    case 1: hipLaunchKernelGGL(dot_many_partial_reg<1>, dim3(L.G), dim3(kKBlock), 0, st, V, ldv, nvec, w, n, partial); break;
+    case 2: hipLaunchKernelGGL(dot_many_partial_reg<2>, dim3(L.G), dim3(kKBlock), 0, st, V, ldv, nvec, w, n, partial); break;
+    case 4: hipLaunchKernelGGL(dot_many_partial_reg<4>, dim3(L.G), dim3(kKBlock), 0, st, V, ldv, nvec, w, n, partial); break;
+    case 8: hipLaunchKernelGGL(dot_many_partial_reg<8>, dim3(L.G), dim3(kKBlock), 0, st, V, ldv, nvec, w, n, partial); break;
+    default: hipLaunchKernelGGL(dot_many_partial, dim3(L.G), dim3(kKBlock), 0, st, V, ldv, nvec, w, n, partial);
+  }
+}
+
+static void launch_dot_final(hipStream_t st, const double* partial, int G, int nvec, double* out, int accumulate,
+                             int sqrt_out)
+{
+  hipLaunchKernelGGL(dot_many_final, dim3(nvec), dim3(64), 0, st, partial, G, nvec, out, accumulate, sqrt_out);
 }
 
 static int stream_blocks(int64_t n)
@@ -141,12 +223,12 @@ extern "C" int rmw_dot_many(const double* V, int64_t ldv, int nvec, const double
 {
   return guarded([&] {
     RMW_REQUIRE(nvec >= 1 && nvec <= kMaxVec && n >= 0, "bad sizes");
-    const int G = dot_blocks(n);
-    RMW_REQUIRE(workspace_bytes >= (size_t)G * nvec * sizeof(double), "workspace too small");
+    const DotLaunch L = dot_launch(n);
+    RMW_REQUIRE(workspace_bytes >= (size_t)L.G * nvec * sizeof(double), "workspace too small");
     hipStream_t st = (hipStream_t)stream;
     double* partial = (double*)workspace;
-    hipLaunchKernelGGL(dot_many_partial, dim3(G), dim3(kKBlock), 0, st, V, ldv, nvec, w, n, partial);
-    hipLaunchKernelGGL(dot_many_final, dim3((nvec + 63) / 64), dim3(64), 0, st, partial, G, nvec, out, 0, sqrt_out);
+    launch_dot_partial(L, st, V, ldv, nvec, w, n, partial);
+    launch_dot_final(st, partial, L.G, nvec, out, 0, sqrt_out);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }
@@ -197,23 +279,23 @@ extern "C" int rmw_arnoldi_cgs2(double* V, int64_t ldv, int j, double* w, int64_
   return guarded([&] {
     const int m = j + 1;
     RMW_REQUIRE(m >= 1 && m + 1 <= kMaxVec, "bad Arnoldi index");
-    const int G = dot_blocks(n);
-    RMW_REQUIRE(workspace_bytes >= (size_t)G * (m + 1) * sizeof(double), "workspace too small");
+    const DotLaunch L = dot_launch(n);
+    RMW_REQUIRE(workspace_bytes >= (size_t)L.G * (m + 1) * sizeof(double), "workspace too small");
     hipStream_t st = (hipStream_t)stream;
     double* partial = (double*)workspace;
     const int sb = stream_blocks(n);
     // pass 1
-    hipLaunchKernelGGL(dot_many_partial, dim3(G), dim3(kKBlock), 0, st, V, ldv, m, w, n, partial);
-    hipLaunchKernelGGL(dot_many_final, dim3((m + 63) / 64), dim3(64), 0, st, partial, G, m, h, 0, 0);
+    launch_dot_partial(L, st, V, ldv, m, w, n, partial);
+    launch_dot_final(st, partial, L.G, m, h, 0, 0);
     hipLaunchKernelGGL(sub_combination, dim3(sb), dim3(kKBlock), 0, st, V, ldv, m, h, w, n);
     // pass 2 (re-orthogonalisation)
-    hipLaunchKernelGGL(dot_many_partial, dim3(G), dim3(kKBlock), 0, st, V, ldv, m, w, n, partial);
-    hipLaunchKernelGGL(dot_many_final, dim3((m + 63) / 64), dim3(64), 0, st, partial, G, m, hscratch, 0, 0);
+    launch_dot_partial(L, st, V, ldv, m, w, n, partial);
+    launch_dot_final(st, partial, L.G, m, hscratch, 0, 0);
     hipLaunchKernelGGL(sub_combination, dim3(sb), dim3(kKBlock), 0, st, V, ldv, m, hscratch, w, n);
     hipLaunchKernelGGL(add_vec, dim3((m + 63) / 64), dim3(64), 0, st, hscratch, h, m);
     // norm and normalisation
-    hipLaunchKernelGGL(dot_many_partial, dim3(G), dim3(kKBlock), 0, st, w, (int64_t)0, 1, w, n, partial);
-    hipLaunchKernelGGL(dot_many_final, dim3(1), dim3(64), 0, st, partial, G, 1, h + m, 0, 1);
+    launch_dot_partial(L, st, w, (int64_t)0, 1, w, n, partial);
+    launch_dot_final(st, partial, L.G, 1, h + m, 0, 1);
     hipLaunchKernelGGL(axpby_kernel, dim3(sb), dim3(kKBlock), 0, st, 1.0, w, 0.0, V + (int64_t)m * ldv, n,
                        (const double*)(h + m));
     RMW_CHECK_HIP(hipGetLastError());
