@@ -434,7 +434,7 @@ def main(argv=None):
         if fparams:
             forces.blob_forces(r, precision=fprec, out=u2, **fparams)
         elif radii:
-            wall_rpy.apply_radii(r, r, f, rad, rad, eta, precision=args.precision, out=u2)
+            wall_rpy.apply_radii_self(r, f, rad, eta, precision=args.precision, out=u2)
         else:
             wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u,
                            domain=args.domain)
@@ -478,8 +478,8 @@ def main(argv=None):
         kname = kname_dom
         desc = desc.replace("UF block", f"UF block, {args.domain.replace('_', ' ')}")
     if radii:
-        launch = {"kernel": "radii (ordered)"}
-        kname = f"radii_kernel<{args.precision}>"
+        launch = wall_rpy.describe_radii_self(n, args.precision)
+        kname = f"{'radii_mutual_kernel' if launch['kernel'] == 'symmetric' else 'radii_kernel'}<{args.precision}>"
     if fparams:
         launch = forces.describe(n, fprec)
         kname = (f"{'force_mutual_kernel' if launch['kernel'] == 'symmetric' else 'force_kernel'}"

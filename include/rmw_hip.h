@@ -173,6 +173,26 @@ int rmw_radii_apply(int prec, int wall, const double* source, const double* targ
                     int64_t nt, double eta, const double* L, const rmw_tuning* tuning, void* workspace,
                     size_t workspace_bytes, void* stream);
 
+/*
+ * Blobs of different radii, sources = targets: velocity (n, 3) =
+ * B M(r, r) B force with the conventions of rmw_radii_apply.  The pair
+ * tensor is its own transpose under exchange of the two blobs, so when the
+ * unordered tile pairs fill the device (tuning->symmetric > 0 forces it,
+ * < 0 forbids it) and no L[d] > 0, each unordered pair is evaluated once
+ * and applied to both blobs; otherwise this is rmw_radii_apply with
+ * source = target.  describe reports the choice (symmetric 1/0), the work
+ * items and the workspace bytes.  Replaces
+ *   mobility_radii_trans_times_force  mobility/mobility.py:1369-1374
+ * (sources = targets of the source-target kernels above).
+ */
+int rmw_radii_self_workspace_size(int prec, int64_t n, const double* L, const rmw_tuning* tuning,
+                                  size_t* bytes);
+int rmw_radii_self_describe(int prec, int64_t n, const double* L, const rmw_tuning* tuning, int* symmetric,
+                            int64_t* work_items, size_t* bytes);
+int rmw_radii_self_apply(int prec, int wall, const double* r_vectors, const double* force,
+                         const double* radius, double* velocity, int64_t n, double eta, const double* L,
+                         const rmw_tuning* tuning, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Blob-blob soft repulsion (FP64 positions, radii and forces, n x 3 row
  * major).  force_i = sum_{j != i} f0(r) (r_j - r_i), dr taken to its

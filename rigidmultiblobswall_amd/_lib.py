@@ -68,6 +68,11 @@ PROTOTYPES = {
     "rmw_radii_workspace_size": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _tp, _szp]),
     "rmw_radii_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _i64, _i64,
                                        ctypes.c_double, _vp, _tp, _vp, _sz, _vp]),
+    "rmw_radii_self_workspace_size": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _tp, _szp]),
+    "rmw_radii_self_describe": (ctypes.c_int, [ctypes.c_int, _i64, _vp, _tp, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int64), _szp]),
+    "rmw_radii_self_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _i64, ctypes.c_double,
+                                            _vp, _tp, _vp, _sz, _vp]),
     "rmw_blob_forces_workspace_size": (ctypes.c_int, [ctypes.c_int, _i64, _tp, _szp]),
     "rmw_blob_forces_describe": (ctypes.c_int, [ctypes.c_int, _i64, _tp, ctypes.POINTER(PlanInfo)]),
     "rmw_blob_forces": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double,

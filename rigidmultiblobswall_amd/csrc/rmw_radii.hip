@@ -22,6 +22,7 @@
 #include "rmw_accum.h"
 #include "rmw_internal.h"
 #include "rmw_pair_terms.h"
+#include "rmw_tile_pairs.h"
 
 namespace rmw {
 
@@ -194,6 +195,219 @@ static RadiiPlan radii_plan(int prec, int64_t ns, int64_t nt, const rmw_tuning* 
   return pl;
 }
 
+// ---------------------------------------------------------------------------
+// Sources = targets (mobility.py:1369-1374): the tensor of a pair is its own
+// transpose under exchange (M_ji = M_ij^T: the far, overlap and engulfed RPY
+// coefficients and the wall's al, be, de are symmetric in (a_i, a_j), while
+// exchanging the blobs swaps g1 and g2), so each unordered pair is
+// evaluated once and applied to both blobs.
+// ---------------------------------------------------------------------------
+
+// M_ij^T v (the coefficients as computed for the ordered pair (i, j)).
+template <typename T>
+__device__ __forceinline__ Vec3<T> apply_radii_t(T dx, T dy, T dz, T Rz, const CoefRadii<T>& c, T vx, T vy, T vz)
+{
+  const T p = dx * vx + dy * vy;
+  const T dv = p + dz * vz;
+  const T Rv = p + Rz * vz;               // R . v
+  const T h = c.be * Rv + c.g2 * vz;      // coefficient of -D R
+  const T q = c.C2 * dv;
+  Vec3<T> u;
+  u.x = (c.C1 - c.al) * vx + (q - h) * dx;
+  u.y = (c.C1 - c.al) * vy + (q - h) * dy;
+  u.z = (c.C1 + c.al) * vz + q * dz + h * Rz + c.g1 * Rv + c.de * vz;
+  return u;
+}
+
+struct RadiiMutParams {
+  const double* r;
+  const double* f;
+  const double* rad;
+  int64_t n;
+  double norm;            // 1 / (8 pi eta)
+  int wall;
+  double* part_i;         // [item][kRBlock][3] target-side sums
+  double* part_j;         // [tri_index(S, T)][kRBlock][3] source-side sums
+  int64_t n_t, chunk_tiles, total_items;
+};
+
+// Work item = (target tile T, run of source tiles S >= T); on the diagonal
+// tile only sources above the target are taken, plus the self pair on the
+// target side (it is the inner RPY regime, no special case).  Target-side sums use the
+// precision's accumulator; source-side sums rotate one lane per step
+// (DPP) so that lane l ends with source (l + 63) mod 64 of the block, in
+// FP32 sub-sums of kSub steps folded into FP64 in the blocked mode.
+template <int ACC, typename T>
+__global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
+{
+  constexpr bool kFoldJ = ACC == RMW_PREC_F32_BLOCKED;
+  using JT = typename std::conditional<kFoldJ, float, double>::type;
+  __shared__ double s_x[kRBlock], s_y[kRBlock], s_z[kRBlock];
+  __shared__ T s_zr[kRBlock], s_b[kRBlock], s_ib[kRBlock];
+  __shared__ T s_f[3][kRBlock];
+  __shared__ double s_jd[kRBlock / 64][3][64];   // per-wave source-side sums of one block
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int il = threadIdx.x;
+
+  for (int64_t item = blockIdx.x; item < p.total_items; item += gridDim.x) {
+    const int64_t Tt = pair_target_tile(item, p.n_t, p.chunk_tiles);
+    const int64_t S0 = Tt + (item - items_before(Tt, p.n_t, p.chunk_tiles)) * p.chunk_tiles;
+    const int64_t S1 = S0 + p.chunk_tiles < p.n_t ? S0 + p.chunk_tiles : p.n_t;
+    const int64_t i = Tt * kRBlock + il;
+    const bool valid = i < p.n;
+    const int64_t gi = valid ? i : 0;
+    const double xi = p.r[3 * gi], yi = p.r[3 * gi + 1];
+    const double a = p.rad[gi];
+    double zi, Bi;
+    wall_shift(p.r[3 * gi + 2], a, p.wall, zi, Bi);
+    const T ar = (T)a, inva = (T)(1.0 / a), zir = (T)zi;
+    T fi[3];
+    {
+      const double c = valid ? p.norm * Bi : 0.0;
+      fi[0] = (T)(c * p.f[3 * gi]); fi[1] = (T)(c * p.f[3 * gi + 1]); fi[2] = (T)(c * p.f[3 * gi + 2]);
+    }
+    Acc<ACC> u[3];
+
+    for (int64_t S = S0; S < S1; ++S) {
+      const int64_t s0 = S * kRBlock;
+      const int cnt = (int)(p.n - s0 < kRBlock ? p.n - s0 : kRBlock);
+      __syncthreads();  // the previous tile has been consumed
+      if (il < cnt) {
+        const int64_t j = s0 + il;
+        const double b = p.rad[j];
+        double zj, Bj;
+        wall_shift(p.r[3 * j + 2], b, p.wall, zj, Bj);
+        s_x[il] = p.r[3 * j]; s_y[il] = p.r[3 * j + 1]; s_z[il] = zj;
+        s_zr[il] = (T)zj; s_b[il] = (T)b; s_ib[il] = (T)(1.0 / b);
+        const double c = p.norm * Bj;
+        s_f[0][il] = (T)(c * p.f[3 * j]); s_f[1][il] = (T)(c * p.f[3 * j + 1]); s_f[2][il] = (T)(c * p.f[3 * j + 2]);
+      } else {
+        // finite stand-ins beyond n (masked below)
+        s_x[il] = 0.0; s_y[il] = 0.0; s_z[il] = 1.0; s_zr[il] = T(1); s_b[il] = T(1); s_ib[il] = T(1);
+        s_f[0][il] = T(0); s_f[1][il] = T(0); s_f[2][il] = T(0);
+      }
+      __syncthreads();
+      const bool diag = S == Tt;
+      for (int b = 0; b < kRBlock / 64 && 64 * b < cnt; ++b) {
+        JT ja[3] = {JT(0), JT(0), JT(0)};
+        for (int st = 0; st < 64; ++st) {
+          const int jl = 64 * b + ((lane + st) & 63);
+          // the self pair (the RPY self mobility and its image) once, on the target side
+          const bool on = valid && jl < cnt && (!diag || jl >= il);
+          const bool on_j = on && (!diag || jl != il);
+          T dx, dy, dz;
+          if constexpr (sizeof(T) == 4) {
+            dx = (T)(xi - s_x[jl]); dy = (T)(yi - s_y[jl]); dz = (T)(zi - s_z[jl]);
+          } else {
+            dx = xi - s_x[jl]; dy = yi - s_y[jl]; dz = zi - s_z[jl];
+          }
+          const T zj = s_zr[jl];
+          const T Rz = zir + zj;
+          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[jl], inva, s_ib[jl], p.wall != 0);
+          const Vec3<T> vi = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][jl], s_f[1][jl], s_f[2][jl]);
+          const Vec3<T> vj = apply_radii_t<T>(dx, dy, dz, Rz, c, fi[0], fi[1], fi[2]);
+          u[0].add(on ? vi.x : T(0));
+          u[1].add(on ? vi.y : T(0));
+          u[2].add(on ? vi.z : T(0));
+          ja[0] = rol1(ja[0]) + (on_j ? (JT)vj.x : JT(0));
+          ja[1] = rol1(ja[1]) + (on_j ? (JT)vj.y : JT(0));
+          ja[2] = rol1(ja[2]) + (on_j ? (JT)vj.z : JT(0));
+          if ((st & (kSub - 1)) == kSub - 1) {
+            u[0].flush(); u[1].flush(); u[2].flush();
+            if constexpr (kFoldJ) {
+              // lane l holds the sub-sum of source (l + st) mod 64 of the block
+              const int jw = (lane + st) & 63;
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+                const double prev = st < kSub ? 0.0 : s_jd[wv][k][jw];
+                s_jd[wv][k][jw] = prev + (double)ja[k];
+                ja[k] = JT(0);
+              }
+            }
+          }
+        }
+        if constexpr (!kFoldJ) {
+          const int jw = (lane + 63) & 63;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) s_jd[wv][k][jw] = (double)ja[k];
+        }
+        __syncthreads();  // every wave's sums of this block are in LDS
+        if (il < 64 && 64 * b + il < cnt) {
+          double* P = p.part_j + (tri_index(S, Tt) * kRBlock + 64 * b + il) * 3;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) P[k] = ((s_jd[0][k][il] + s_jd[1][k][il]) + s_jd[2][k][il]) + s_jd[3][k][il];
+        }
+        __syncthreads();  // s_jd is free for the next block
+      }
+      u[0].flush(); u[1].flush(); u[2].flush();
+    }
+    if (valid) {
+      double* P = p.part_i + (item * kRBlock + il) * 3;
+      P[0] = u[0].get(); P[1] = u[1].get(); P[2] = u[2].get();
+    }
+  }
+}
+
+// Blob i of tile X: the target-side sums of the items of tile X, then the
+// source-side sums (X, T) for T = 0..X, in that order; then the damping.
+__global__ __launch_bounds__(256) void radii_mutual_reduce_kernel(const double* __restrict__ part_i,
+                                                                  const double* __restrict__ part_j,
+                                                                  const double* __restrict__ r,
+                                                                  const double* __restrict__ rad, int wall, int64_t n,
+                                                                  int64_t n_t, int64_t chunk_tiles,
+                                                                  double* __restrict__ out)
+{
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n * 3) return;
+  const int64_t i = tid / 3;
+  const int c = (int)(tid - 3 * i);
+  const int64_t X = i / kRBlock, l = i - X * kRBlock;
+  const int64_t it0 = items_before(X, n_t, chunk_tiles), it1 = items_before(X + 1, n_t, chunk_tiles);
+  double s = 0.0;
+  for (int64_t it = it0; it < it1; ++it) s += part_i[(it * kRBlock + l) * 3 + c];
+  const double* Pj = part_j + (tri_index(X, 0) * kRBlock + l) * 3 + c;
+  for (int64_t T = 0; T <= X; ++T) s += Pj[T * kRBlock * 3];
+  double ze, B;
+  wall_shift(r[3 * i + 2], rad[i], wall, ze, B);
+  out[tid] = B * s;
+}
+
+constexpr size_t kRMutAutoBytes = size_t(2) << 30;
+
+struct RadiiSelfPlan {
+  bool mutual = false;
+  int64_t n_t = 0, chunk_tiles = 0, total_items = 0;
+  size_t off_pj = 0, bytes = 0;
+  RadiiPlan ordered;
+};
+
+// Symmetric kernel when the tile pairs fill the device (or tuning->symmetric
+// > 0), the partial sums fit kRMutAutoBytes and there is no periodic box;
+// otherwise the ordered kernel with sources = targets.
+static RadiiSelfPlan radii_self_plan(int prec, int64_t n, bool periodic, const rmw_tuning* tuning)
+{
+  RadiiSelfPlan pl;
+  pl.ordered = radii_plan(prec, n, n, tuning);
+  const int sym = tuning ? tuning->symmetric : 0;
+  const int cu = std::max(device_props().cu_count, 1);
+  pl.n_t = (n + kRBlock - 1) / kRBlock;
+  const int64_t tile_pairs = pl.n_t * (pl.n_t + 1) / 2;
+  RMW_REQUIRE(!(sym > 0 && periodic), "the symmetric radii kernel has no periodic form");
+  pl.mutual = n > 0 && !periodic && sym >= 0 && (sym > 0 || tile_pairs >= cu);
+  if (pl.mutual) {
+    if (tuning && tuning->chunk_len > 0)
+      pl.chunk_tiles = std::max<int64_t>(1, (tuning->chunk_len + kRBlock - 1) / kRBlock);
+    else
+      pl.chunk_tiles = std::max<int64_t>(1, tile_pairs / (16 * (int64_t)cu));
+    pl.total_items = items_before(pl.n_t, pl.n_t, pl.chunk_tiles);
+    pl.off_pj = align_up((size_t)pl.total_items * kRBlock * 3 * sizeof(double));
+    pl.bytes = pl.off_pj + align_up((size_t)tile_pairs * kRBlock * 3 * sizeof(double));
+    if (sym == 0 && pl.bytes > kRMutAutoBytes) pl.mutual = false;
+  }
+  if (!pl.mutual) pl.bytes = pl.ordered.bytes;
+  return pl;
+}
+
 }  // namespace rmw
 
 using namespace rmw;
@@ -204,6 +418,48 @@ extern "C" int rmw_radii_workspace_size(int prec, int64_t ns, int64_t nt, const 
     RMW_REQUIRE(bytes != nullptr, "bytes is null");
     *bytes = radii_plan(prec, ns, nt, tuning).bytes;
   });
+}
+
+// The ordered pair kernel and its reduce, source -> target.
+static void radii_ordered(const RadiiPlan& pl, int prec, int wall, const double* source, const double* target,
+                          const double* force, const double* radius_source, const double* radius_target,
+                          double* velocity, int64_t ns, int64_t nt, double eta, const double* L,
+                          const rmw_tuning* tuning, void* workspace, hipStream_t st)
+{
+  RadiiParams q{};
+  q.src = source; q.trg = target; q.f = force; q.rs = radius_source; q.rt = radius_target;
+  q.ns = ns; q.nt = nt;
+  q.norm = 1.0 / (8.0 * M_PI * eta);
+  q.wall = wall != 0;
+  for (int d = 0; d < 3; ++d) {
+    const double Ld = L ? L[d] : 0.0;
+    q.L[d] = Ld > 0.0 ? Ld : 0.0;
+    if (Ld > 0.0) q.periodic |= 1 << d;
+  }
+  q.partial = (double*)workspace;
+  q.n_chunks = pl.n_chunks;
+  q.chunk_len = pl.chunk_len;
+  q.total_items = pl.total_items;
+  int grid = (int)std::min<int64_t>(pl.total_items, (int64_t)INT32_MAX);
+  if (tuning && tuning->max_wgs > 0) grid = std::min(grid, tuning->max_wgs);
+  const dim3 g(grid), b(kRBlock);
+  ProfRec rec{};
+  bool prof = false;
+  prof_begin(rec, prof, st, 1);
+  if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
+  else hipLaunchKernelGGL((radii_kernel<RMW_PREC_F64, double>), g, b, 0, st, q);
+  RMW_CHECK_HIP(hipGetLastError());
+  prof_end(rec, prof, st);
+  hipLaunchKernelGGL(radii_reduce_kernel, dim3((unsigned)((nt * 3 + 255) / 256)), dim3(256), 0, st, q.partial,
+                     pl.n_chunks, nt, target, radius_target, q.wall, velocity);
+  RMW_CHECK_HIP(hipGetLastError());
+}
+
+static bool any_periodic(const double* L)
+{
+  return L && (L[0] > 0.0 || L[1] > 0.0 || L[2] > 0.0);
 }
 
 extern "C" int rmw_radii_apply(int prec, int wall, const double* source, const double* target, const double* force,
@@ -226,34 +482,68 @@ extern "C" int rmw_radii_apply(int prec, int wall, const double* source, const d
       return;
     }
     RMW_REQUIRE(source && force && radius_source && workspace, "source, force, radius_source and workspace are required");
-    RadiiParams q{};
-    q.src = source; q.trg = target; q.f = force; q.rs = radius_source; q.rt = radius_target;
-    q.ns = ns; q.nt = nt;
+    radii_ordered(pl, prec, wall, source, target, force, radius_source, radius_target, velocity, ns, nt, eta, L,
+                  tuning, workspace, st);
+  });
+}
+
+extern "C" int rmw_radii_self_workspace_size(int prec, int64_t n, const double* L, const rmw_tuning* tuning,
+                                             size_t* bytes)
+{
+  return guarded([&] {
+    RMW_REQUIRE(bytes != nullptr, "bytes is null");
+    *bytes = radii_self_plan(prec, n, any_periodic(L), tuning).bytes;
+  });
+}
+
+extern "C" int rmw_radii_self_describe(int prec, int64_t n, const double* L, const rmw_tuning* tuning,
+                                       int* symmetric, int64_t* work_items, size_t* bytes)
+{
+  return guarded([&] {
+    const RadiiSelfPlan pl = radii_self_plan(prec, n, any_periodic(L), tuning);
+    if (symmetric) *symmetric = pl.mutual ? 1 : 0;
+    if (work_items) *work_items = pl.mutual ? pl.total_items : pl.ordered.total_items;
+    if (bytes) *bytes = pl.bytes;
+  });
+}
+
+extern "C" int rmw_radii_self_apply(int prec, int wall, const double* r, const double* force, const double* radius,
+                                    double* velocity, int64_t n, double eta, const double* L,
+                                    const rmw_tuning* tuning, void* workspace, size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    const RadiiSelfPlan pl = radii_self_plan(prec, n, any_periodic(L), tuning);
+    RMW_REQUIRE(workspace_bytes >= pl.bytes, "workspace too small: need " + std::to_string(pl.bytes) + " bytes");
+    RMW_REQUIRE(eta > 0.0 && std::isfinite(eta), "eta must be positive");
+    if (n == 0) return;
+    RMW_REQUIRE(r && force && radius && velocity && workspace, "r, force, radius, velocity and workspace are required");
+    hipStream_t st = (hipStream_t)stream;
+    if (!pl.mutual) {
+      radii_ordered(pl.ordered, prec, wall, r, r, force, radius, radius, velocity, n, n, eta, L, tuning, workspace,
+                    st);
+      return;
+    }
+    RadiiMutParams q{};
+    q.r = r; q.f = force; q.rad = radius; q.n = n;
     q.norm = 1.0 / (8.0 * M_PI * eta);
     q.wall = wall != 0;
-    for (int d = 0; d < 3; ++d) {
-      const double Ld = L ? L[d] : 0.0;
-      q.L[d] = Ld > 0.0 ? Ld : 0.0;
-      if (Ld > 0.0) q.periodic |= 1 << d;
-    }
-    q.partial = (double*)workspace;
-    q.n_chunks = pl.n_chunks;
-    q.chunk_len = pl.chunk_len;
-    q.total_items = pl.total_items;
+    q.part_i = (double*)workspace;
+    q.part_j = (double*)((char*)workspace + pl.off_pj);
+    q.n_t = pl.n_t; q.chunk_tiles = pl.chunk_tiles; q.total_items = pl.total_items;
     int grid = (int)std::min<int64_t>(pl.total_items, (int64_t)INT32_MAX);
     if (tuning && tuning->max_wgs > 0) grid = std::min(grid, tuning->max_wgs);
     const dim3 g(grid), b(kRBlock);
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st, 1);
-    if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
-    else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
-    else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
-    else hipLaunchKernelGGL((radii_kernel<RMW_PREC_F64, double>), g, b, 0, st, q);
+    if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
+    else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
+    else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
+    else hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F64, double>), g, b, 0, st, q);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
-    hipLaunchKernelGGL(radii_reduce_kernel, dim3((unsigned)((nt * 3 + 255) / 256)), dim3(256), 0, st, q.partial,
-                       pl.n_chunks, nt, target, radius_target, q.wall, velocity);
+    hipLaunchKernelGGL(radii_mutual_reduce_kernel, dim3((unsigned)((n * 3 + 255) / 256)), dim3(256), 0, st, q.part_i,
+                       q.part_j, r, radius, q.wall, n, pl.n_t, pl.chunk_tiles, velocity);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

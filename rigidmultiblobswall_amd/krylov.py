@@ -16,6 +16,7 @@ on the residual r0 = b - A x0 and returns x0 + P^-1 y.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass, field
 from typing import Callable, List, Optional
 
@@ -100,6 +101,10 @@ def gmres(A: Callable[[torch.Tensor, torch.Tensor], None], b: torch.Tensor,
     h = torch.empty(m + 2, dtype=torch.float64, device=device)
     hs = torch.empty(m + 2, dtype=torch.float64, device=device)
     ycoef = torch.empty(m + 1, dtype=torch.float64, device=device)
+    # pinned host copy of the Arnoldi column: one short copy and a stream
+    # synchronisation per step
+    h_host = torch.empty(m + 2, dtype=torch.float64, pin_memory=True)
+    stream = torch.cuda.current_stream(device)
     x = torch.zeros(n, dtype=torch.float64, device=device) if x0 is None else x0.clone()
     ident = Pinv is None
     bnorm = ops.norm(b)
@@ -128,10 +133,10 @@ def gmres(A: Callable[[torch.Tensor, torch.Tensor], None], b: torch.Tensor,
             break
         ops.axpby(1.0 / beta, r, 0.0, V[0])
         H = np.zeros((m + 1, m))
-        g = np.zeros(m + 1)
+        g = [0.0] * (m + 1)        # plain floats: the Givens sweep runs between GPU steps
         g[0] = beta
-        cs = np.zeros(m)
-        sn = np.zeros(m)
+        cs = [0.0] * m
+        sn = [0.0] * m
         jj = 0
         for j in range(m):
             if ident:
@@ -141,13 +146,16 @@ def gmres(A: Callable[[torch.Tensor, torch.Tensor], None], b: torch.Tensor,
                 A(z, w)
             res.matvecs += 1
             ops.arnoldi(V, j, w, h, hs)
-            col = h[: j + 2].cpu().numpy()
-            hnext = float(col[j + 1])
+            h_host[: j + 2].copy_(h[: j + 2], non_blocking=True)
+            stream.synchronize()
+            col = h_host[: j + 2].tolist()
+            hnext = col[j + 1]
             for i in range(j):
-                t = cs[i] * col[i] + sn[i] * col[i + 1]
-                col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1]
+                c, s_ = cs[i], sn[i]
+                t = c * col[i] + s_ * col[i + 1]
+                col[i + 1] = -s_ * col[i] + c * col[i + 1]
                 col[i] = t
-            den = float(np.hypot(col[j], col[j + 1]))
+            den = math.hypot(col[j], col[j + 1])
             if den == 0.0:
                 cs[j], sn[j] = 1.0, 0.0
             else:
@@ -167,7 +175,7 @@ def gmres(A: Callable[[torch.Tensor, torch.Tensor], None], b: torch.Tensor,
                 break
         # x += P^-1 V y
         R = np.triu(H[:jj, :jj])
-        y = np.linalg.solve(R, g[:jj]) if jj > 0 else np.zeros(0)
+        y = np.linalg.solve(R, np.asarray(g[:jj])) if jj > 0 else np.zeros(0)
         ycoef[:jj].copy_(torch.from_numpy(y))
         if ident:
             ops.combination(V, jj, ycoef, x, accumulate=True)

@@ -18,6 +18,8 @@ host is the only transfer).
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 import torch
 
@@ -220,6 +222,18 @@ def no_wall_mobility_trans_times_force_source_target_hip(source, target, force, 
 
 def mobility_radii_trans_times_force_hip(r_vectors, force, eta, a, radius_blobs, function=None, *args, **kwargs):
     """M f with per-blob radii, sources = targets; drop-in for
-    mobility.py:1369-1374 (``function`` defaults to the wall kernel)."""
+    mobility.py:1369-1374 (``function`` defaults to the wall kernel).  With
+    the wall or unbounded kernel of this module the product runs as one
+    sources-equal-targets call (each unordered pair evaluated once)."""
     fn = function or single_wall_mobility_trans_times_force_source_target_hip
+    own = {single_wall_mobility_trans_times_force_source_target_hip: True,
+           no_wall_mobility_trans_times_force_source_target_hip: False}
+    base, opts = fn, dict(kwargs)
+    if isinstance(fn, functools.partial) and not fn.args:
+        base, opts = fn.func, {**fn.keywords, **kwargs}
+    if base in own and not args:
+        L, prec, tun = _opts(opts)
+        u = wall_rpy.apply_radii_self(r_vectors, force, radius_blobs, eta, periodic_length=L, wall=own[base],
+                                      precision=prec, tuning=tun)
+        return _host(u)
     return fn(r_vectors, r_vectors, force, radius_blobs, radius_blobs, eta, *args, **kwargs)

@@ -53,9 +53,9 @@ class RigidIntegrator:
         self.scheme = scheme
         self.lib = _lib.load()
         self.device = torch.device("cuda", torch.cuda.current_device())
-        self.centres = np.ascontiguousarray(np.asarray(locations, dtype=np.float64).reshape(-1, 3))
-        self.quats = np.ascontiguousarray(np.asarray(orientations, dtype=np.float64).reshape(-1, 4))
-        self.nbodies = self.centres.shape[0]
+        self._centres = np.ascontiguousarray(np.asarray(locations, dtype=np.float64).reshape(-1, 3))
+        self._quats = np.ascontiguousarray(np.asarray(orientations, dtype=np.float64).reshape(-1, 4))
+        self.nbodies = self._centres.shape[0]
         self.reference = reference
         self.eta, self.a, self.g, self.blob_mass = float(eta), float(a), float(g), float(blob_mass)
         self.repulsion_strength_wall, self.debye_length_wall = float(repulsion_strength_wall), float(debye_length_wall)
@@ -76,13 +76,45 @@ class RigidIntegrator:
         self.velocities_previous_step: Optional[torch.Tensor] = None
         self.invalid_configuration_count = 0
         self.det_iterations_count = 0
-        self._S: Optional[RigidSuspension] = None
+        self._S: Optional[RigidSuspension] = None       # the current configuration
+        self._S_next: Optional[RigidSuspension] = None  # buffers of the trial configuration
+        self._slip_ref_dev: Optional[torch.Tensor] = None
+
+    # host copies of the configuration, read back from the device on demand
+    @property
+    def centres(self) -> np.ndarray:
+        if self._centres is None:
+            self._centres = self._S.centres.cpu().numpy()
+        return self._centres
+
+    @centres.setter
+    def centres(self, value):
+        self._centres = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(-1, 3))
+        if self._S is not None:
+            self._S.set_configuration(self._centres, self._S.quats)
+
+    @property
+    def quats(self) -> np.ndarray:
+        if self._quats is None:
+            self._quats = self._S.quats.cpu().numpy()
+        return self._quats
+
+    @quats.setter
+    def quats(self, value):
+        self._quats = np.ascontiguousarray(np.asarray(value, dtype=np.float64).reshape(-1, 4))
+        if self._S is not None:
+            self._S.set_configuration(self._S.centres, self._quats)
 
     # ------------------------------------------------------------------
     def suspension(self, centres=None, quats=None) -> RigidSuspension:
         return RigidSuspension(self.centres if centres is None else centres, self.quats if quats is None else quats,
                                self.reference, self.eta, self.a, periodic_length=self.periodic_length,
                                precision=self.precision)
+
+    def _current(self) -> RigidSuspension:
+        if self._S is None:
+            self._S = self.suspension()
+        return self._S
 
     def blob_forces(self, S: RigidSuspension) -> torch.Tensor:
         F = torch.empty((S.Nblobs, 3), dtype=torch.float64, device=self.device)
@@ -107,10 +139,12 @@ class RigidIntegrator:
     def slip(self, S: RigidSuspension) -> torch.Tensor:
         if self.slip_reference is None:
             return torch.zeros((S.Nblobs, 3), dtype=torch.float64, device=self.device)
-        ref = (np.vstack([np.asarray(s, dtype=np.float64).reshape(-1, 3) for s in self.slip_reference])
-               if isinstance(self.slip_reference, (list, tuple)) else
-               np.tile(np.asarray(self.slip_reference, dtype=np.float64).reshape(-1, 3), (self.nbodies, 1)))
-        sref = torch.from_numpy(np.ascontiguousarray(ref)).to(self.device)
+        if self._slip_ref_dev is None:
+            ref = (np.vstack([np.asarray(s, dtype=np.float64).reshape(-1, 3) for s in self.slip_reference])
+                   if isinstance(self.slip_reference, (list, tuple)) else
+                   np.tile(np.asarray(self.slip_reference, dtype=np.float64).reshape(-1, 3), (self.nbodies, 1)))
+            self._slip_ref_dev = torch.from_numpy(np.ascontiguousarray(ref)).to(self.device)
+        sref = self._slip_ref_dev
         zero = torch.zeros_like(S.centres)
         rot = torch.empty((S.Nblobs, 3), dtype=torch.float64, device=self.device)
         dummy = torch.empty_like(rot)
@@ -133,23 +167,26 @@ class RigidIntegrator:
         return res.x[3 * S.Nblobs:].reshape(S.nbodies, 6).contiguous()
 
     def _advance(self, dt: float, adams_bashforth: bool) -> None:
-        S = self._S if self._S is not None else self.suspension()
+        S = self._current()
         U = self.solve_mobility_problem(S)
-        c_new = torch.empty_like(S.centres)
-        q_new = torch.empty_like(S.quats)
+        if self._S_next is None:
+            self._S_next = self.suspension()
+            nb = ctypes.c_size_t()
+            _lib.check(self.lib.rmw_count_below_workspace_size(S.Nblobs, ctypes.byref(nb)))
+            self._count_ws = torch.empty(max(nb.value, 256), dtype=torch.uint8, device=self.device)
+            self._count = torch.empty(1, dtype=torch.int64, device=self.device)
+        Sn = self._S_next
         ab = adams_bashforth and not self.first_step
+        # the new centres and quaternions go straight into the trial suspension
         _lib.check(self.lib.rmw_bodies_advance(S.centres.data_ptr(), S.quats.data_ptr(), U.data_ptr(),
                                                self.velocities_previous_step.data_ptr() if ab else None,
                                                S.nbodies, 1.5 if ab else 1.0, 0.5 if ab else 0.0, float(dt),
-                                               c_new.data_ptr(), q_new.data_ptr(), _stream()))
-        S_new = self.suspension(c_new.cpu().numpy(), q_new.cpu().numpy())
-        nb = ctypes.c_size_t()
-        _lib.check(self.lib.rmw_count_below_workspace_size(S_new.Nblobs, ctypes.byref(nb)))
-        ws = torch.empty(max(nb.value, 256), dtype=torch.uint8, device=self.device)
-        cnt = torch.zeros(1, dtype=torch.int64, device=self.device)
-        _lib.check(self.lib.rmw_count_below(S_new.r_blobs.data_ptr(), S_new.Nblobs, 0.0, cnt.data_ptr(),
-                                            ws.data_ptr(), ws.numel(), _stream()))
-        if int(cnt.item()) > 0:
+                                               Sn.centres.data_ptr(), Sn.quats.data_ptr(), _stream()))
+        Sn.set_configuration(Sn.centres, Sn.quats)
+        self._count.zero_()
+        _lib.check(self.lib.rmw_count_below(Sn.r_blobs.data_ptr(), Sn.Nblobs, 0.0, self._count.data_ptr(),
+                                            self._count_ws.data_ptr(), self._count_ws.numel(), _stream()))
+        if int(self._count.item()) > 0:
             # check_positions fails; a deterministic step would be recomputed
             # unchanged forever by the reference
             self.invalid_configuration_count += 1
@@ -157,9 +194,9 @@ class RigidIntegrator:
         if adams_bashforth:
             self.first_step = False
             self.velocities_previous_step = U
-        self.centres = c_new.cpu().numpy()
-        self.quats = q_new.cpu().numpy()
-        self._S = S_new
+        self._S, self._S_next = Sn, S
+        self._centres = None
+        self._quats = None
 
     def deterministic_forward_euler(self, dt: float, *args, **kwargs):
         self._advance(dt, adams_bashforth=False)
@@ -172,5 +209,4 @@ class RigidIntegrator:
         return getattr(self, self.scheme)(dt, *args, **kwargs)
 
     def blob_positions(self) -> np.ndarray:
-        S = self._S if self._S is not None else self.suspension()
-        return S.r_blobs.cpu().numpy()
+        return self._current().r_blobs.cpu().numpy()

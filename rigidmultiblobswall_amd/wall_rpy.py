@@ -229,6 +229,45 @@ def apply_radii(source, target, force, radius_source, radius_target, eta: float,
     return u
 
 
+def apply_radii_self(r_vectors, force, radius, eta: float, periodic_length=None, wall: bool = True,
+                     precision="fp32_blocked", tuning=None, out: Optional[torch.Tensor] = None):
+    """Different radii, sources = targets: u (N, 3) = B M B f (wall) or the
+    unbounded RPY of different radii (wall=False).  Each unordered pair is
+    evaluated once when the symmetric kernel is chosen (no periodic box)."""
+    lib = _lib.load()
+    device = _device()
+    r = torch.as_tensor(r_vectors)
+    n = r.numel() // 3
+    r = _as_device_f64(r, n, None, device, "r_vectors")
+    f = _as_device_f64(force, n, None, device, "force")
+    rad = torch.as_tensor(radius).to(device=device, dtype=torch.float64).contiguous().reshape(-1)
+    if rad.numel() != n:
+        raise ValueError("one radius per blob is required")
+    u = out if out is not None else torch.empty((n, 3), dtype=torch.float64, device=device)
+    prec = _lib.precision_code(precision)
+    tp = _lib.tuning_ptr(None if isinstance(tuning, str) else tuning)
+    L = _L_array(periodic_length)
+    nbytes = ctypes.c_size_t()
+    _lib.check(lib.rmw_radii_self_workspace_size(prec, n, L, tp, ctypes.byref(nbytes)))
+    ws = workspace(nbytes.value, device, slot="radii")
+    _lib.check(lib.rmw_radii_self_apply(prec, int(bool(wall)), r.data_ptr(), f.data_ptr(), rad.data_ptr(),
+                                        u.data_ptr(), n, float(eta), L, tp, ws.data_ptr(), ws.numel(),
+                                        _stream_ptr(device)))
+    return u
+
+
+def describe_radii_self(n: int, precision="fp32_blocked", periodic_length=None, tuning=None) -> dict:
+    """The launch apply_radii_self would use."""
+    lib = _lib.load()
+    _device()
+    tp = _lib.tuning_ptr(None if isinstance(tuning, str) else tuning)
+    sym, items, nbytes = ctypes.c_int(), ctypes.c_int64(), ctypes.c_size_t()
+    _lib.check(lib.rmw_radii_self_describe(_lib.precision_code(precision), n, _L_array(periodic_length), tp,
+                                           ctypes.byref(sym), ctypes.byref(items), ctypes.byref(nbytes)))
+    return {"kernel": "symmetric" if sym.value else "ordered", "work_items": items.value,
+            "workspace_bytes": nbytes.value}
+
+
 def describe(mask: int, n: int, k: int = 1, precision="fp32", periodic_length=None, tuning=None) -> dict:
     """The launch plan ``apply`` would use (kernel, RHS per pass, chunk,
     work items), with the same tuning resolution as ``apply``."""

@@ -88,3 +88,73 @@ def test_radii_dropin_signatures(gpu):
     assert rel(fn(r, f, 0.7, 0.0), ref) <= 1e-12
     fn0 = multi_bodies.set_mobility_vector_prod("radii_hip_no_wall", radius_blobs=rad)
     assert rel(fn0(r, f, 0.7, 0.0), O.radii_apply(r, r, f, rad, rad, 0.7, wall=False)) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
+@pytest.mark.parametrize("wall", [True, False])
+def test_radii_symmetric_kernel_matches_oracle(gpu, prec, wall):
+    """Sources = targets through the symmetric kernel (forced on at this
+    size): 1100 blobs = 4 full tiles and a ragged one, radii 0.1-0.6 so the
+    far, overlapping and engulfed regimes all occur, blobs below their
+    radius (wall damping)."""
+    n = 1100
+    r, rad, rng = _blobs(n, 11, side=8.0)
+    f = rng.standard_normal((n, 3))
+    ref = O.radii_apply(r, r, f, rad, rad, 0.9, wall=wall)
+    print(f"case radii symmetric n={n} wall={wall} {prec}", flush=True)
+    for chunk in (0, 512):
+        t = _lib.Tuning(chunk, 0, 0, 1, 0)
+        assert wall_rpy.describe_radii_self(n, prec, tuning=t)["kernel"] == "symmetric"
+        u = wall_rpy.apply_radii_self(r, f, rad, 0.9, wall=wall, precision=prec, tuning=t).cpu().numpy().ravel()
+        assert rel(u, ref) <= TOL[prec], chunk
+    # the ordered kernel through the same entry point
+    t = _lib.Tuning(0, 0, 0, -1, 0)
+    assert wall_rpy.describe_radii_self(n, prec, tuning=t)["kernel"] == "ordered"
+    u = wall_rpy.apply_radii_self(r, f, rad, 0.9, wall=wall, precision=prec, tuning=t).cpu().numpy().ravel()
+    assert rel(u, ref) <= TOL[prec]
+
+
+def test_radii_symmetric_edges_and_choice(gpu):
+    r, rad, rng = _blobs(300, 12)
+    f = rng.standard_normal((300, 3))
+    print("case radii symmetric edges: grid caps, tiny n, periodic, default choice", flush=True)
+    ref = O.radii_apply(r, r, f, rad, rad, 1.0)
+    for wgs in (1, 3):
+        u = wall_rpy.apply_radii_self(r, f, rad, 1.0, precision="fp64", tuning=_lib.Tuning(0, wgs, 0, 1, 0))
+        assert rel(u.cpu().numpy().ravel(), ref) <= 1e-12, wgs
+    for n in (1, 2, 65):
+        u = wall_rpy.apply_radii_self(r[:n], f[:n], rad[:n], 1.0, precision="fp64", tuning=_lib.Tuning(0, 0, 0, 1, 0))
+        assert rel(u.cpu().numpy().ravel(), O.radii_apply(r[:n], r[:n], f[:n], rad[:n], rad[:n], 1.0)) <= 1e-12, n
+    # periodic boxes take the ordered kernel; forcing the symmetric one is an error
+    L = np.array([7.0, 6.0, 0.0])
+    assert wall_rpy.describe_radii_self(300, "fp64", periodic_length=L)["kernel"] == "ordered"
+    u = wall_rpy.apply_radii_self(r, f, rad, 1.0, periodic_length=L, precision="fp64")
+    assert rel(u.cpu().numpy().ravel(), O.radii_apply(r, r, f, rad, rad, 1.0, periodic_length=L)) <= 1e-12
+    with pytest.raises(RuntimeError):
+        wall_rpy.apply_radii_self(r, f, rad, 1.0, periodic_length=L, tuning=_lib.Tuning(0, 0, 0, 1, 0))
+    # default: ordered below one tile pair per CU, symmetric from there on
+    assert wall_rpy.describe_radii_self(300)["kernel"] == "ordered"
+    assert wall_rpy.describe_radii_self(32768)["kernel"] == "symmetric"
+    # the drop-in routes sources = targets through this entry point
+    fn = multi_bodies.set_mobility_vector_prod("radii_hip_fp64", radius_blobs=rad)
+    assert rel(fn(r, f.ravel(), 1.0, 0.0), ref) <= 1e-12
+
+
+def test_radii_symmetric_large_matches_ordered(gpu):
+    """At the benchmark's size (32768 blobs): symmetric and ordered kernels
+    agree to the FP32 tolerance and the FP64 symmetric product is
+    reproducible bit for bit."""
+    from rigidmultiblobswall_amd import workloads
+    w = workloads.c3_rollers()
+    r = w["r"]
+    n = r.shape[0]
+    rad = np.random.default_rng(33).uniform(0.5, 0.8, n)
+    f = np.random.default_rng(13).standard_normal((n, 3))
+    print("case radii symmetric vs ordered, 32768 blobs", flush=True)
+    us = wall_rpy.apply_radii_self(r, f, rad, 1e-3, tuning=_lib.Tuning(0, 0, 0, 1, 0)).cpu().numpy().ravel()
+    uo = wall_rpy.apply_radii_self(r, f, rad, 1e-3, tuning=_lib.Tuning(0, 0, 0, -1, 0)).cpu().numpy().ravel()
+    assert rel(us, uo) <= 1e-6
+    a1 = wall_rpy.apply_radii_self(r, f, rad, 1e-3, precision="fp64", tuning=_lib.Tuning(0, 0, 0, 1, 0))
+    a2 = wall_rpy.apply_radii_self(r, f, rad, 1e-3, precision="fp64", tuning=_lib.Tuning(0, 0, 0, 1, 0))
+    assert torch.equal(a1, a2)
+    assert rel(a1.cpu().numpy().ravel(), us) <= 1e-6
