@@ -83,6 +83,7 @@ class RigidSuspension:
                                            self.r_rel.data_ptr(), self.ws.data_ptr(), self.ws.numel(), _stream()))
         self._mlam = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
         self.minv = None
+        self.pc_r_rel = None     # body-frame blob offsets the preconditioner was built with
 
     def _offp(self):
         return self.body_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
@@ -100,6 +101,7 @@ class RigidSuspension:
                                            self._offp(), self.nbodies, self.r_blobs.data_ptr(),
                                            self.r_rel.data_ptr(), self.ws.data_ptr(), self.ws.numel(), _stream()))
         self.minv = None
+        self.pc_r_rel = None
 
     # -- blob mobility and operator -------------------------------------------------
     def mobility_times(self, lam: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
@@ -130,16 +132,30 @@ class RigidSuspension:
                                                self.pc_scratch_doubles, self.ws.data_ptr(), self.ws.numel(),
                                                _stream()))
         del scratch
+        self.pc_r_rel = self.r_rel
         bad = int(status.sum().item())
         if bad:
             raise RuntimeError(f"{bad} body mobility blocks are not positive definite")
+
+    def preconditioner_state(self):
+        """The built preconditioner (M_b^-1, M_b^-1 K_b, N_b and the K_b
+        offsets it used), for reuse on a later configuration."""
+        if self.minv is None:
+            self.build_preconditioner()
+        return (self.minv, self.minvk, self.nbody, self.pc_r_rel.clone())
+
+    def adopt_preconditioner(self, state):
+        """Apply a preconditioner built on an earlier configuration of the
+        same bodies (multi_bodies.py:766-819 with update_PC > 1: the stale
+        M_b^-1, N_b and K_b are used until the next rebuild)."""
+        self.minv, self.minvk, self.nbody, self.pc_r_rel = state
 
     def pc_apply(self, x: torch.Tensor, out: torch.Tensor):
         if self.minv is None:
             self.build_preconditioner()
         _lib.check(self.lib.rmw_rigid_pc_apply(self._offp(), self.nbodies, self.minv.data_ptr(),
                                                self.minvk.data_ptr(), self.nbody.data_ptr(),
-                                               self.r_rel.data_ptr(), x.data_ptr(), out.data_ptr(),
+                                               self.pc_r_rel.data_ptr(), x.data_ptr(), out.data_ptr(),
                                                self.ws.data_ptr(), self.ws.numel(), _stream()))
 
     # -- solve ------------------------------------------------------------------------

@@ -45,7 +45,8 @@ class RigidIntegrator:
                  repulsion_strength_wall: float = 1.0, debye_length_wall: float = 1.0,
                  blob_blob_force_implementation: str = "None", repulsion_strength: float = 1.0,
                  debye_length: float = 1.0, periodic_length=None, tolerance: float = 1e-8,
-                 precision: str = "fp64", slip_reference=None, domain: str = "single_wall", restart: int = 60):
+                 precision: str = "fp64", slip_reference=None, domain: str = "single_wall", restart: int = 60,
+                 update_PC: int = 1):
         if scheme not in SCHEMES:
             raise ValueError(f"scheme {scheme!r} is not provided; use one of {list(SCHEMES)}")
         if domain != "single_wall":
@@ -71,6 +72,12 @@ class RigidIntegrator:
                                   cutoff=(2 * self.a + 30 * self.debye_length
                                           if blob_blob_force_implementation.startswith("tree_hip") else 0.0))
         self.slip_reference = slip_reference
+        if int(update_PC) < 1:
+            raise ValueError("update_PC must be >= 1")
+        self.update_PC = int(update_PC)   # rebuild the preconditioner every update_PC steps
+        self._pc_state = None
+        self.pc_builds = 0
+        self.step_count = 0
         self.first_step = True
         self.first_guess: Optional[torch.Tensor] = None   # scaled solution of the previous solve
         self.velocities_previous_step: Optional[torch.Tensor] = None
@@ -154,8 +161,21 @@ class RigidIntegrator:
                                            S.ws.numel(), _stream()))
         return rot
 
-    def solve_mobility_problem(self, S: RigidSuspension):
+    def _preconditioner(self, S: RigidSuspension, step: int) -> None:
+        """Rebuilt when step % update_PC == 0 or none exists, else the last
+        one is reused (multi_bodies.py:766-819)."""
+        if self._pc_state is None or step % self.update_PC == 0:
+            S.build_preconditioner()
+            self.pc_builds += 1
+            self._pc_state = S.preconditioner_state() if self.update_PC > 1 else None
+            if self.update_PC > 1:
+                S.adopt_preconditioner(self._pc_state)
+        else:
+            S.adopt_preconditioner(self._pc_state)
+
+    def solve_mobility_problem(self, S: RigidSuspension, step: Optional[int] = None):
         """Body velocities (B, 6) of the current configuration (:1441-1548)."""
+        self._preconditioner(S, self.step_count if step is None else step)
         rhs = S.rhs(self.slip(S), self.force_torque(S))
         rnorm = float(torch.linalg.norm(rhs))
         x0 = None if (self.first_guess is None or rnorm == 0.0) else self.first_guess * rnorm
@@ -166,9 +186,9 @@ class RigidIntegrator:
         self.first_guess = res.x / rnorm if rnorm > 0 else torch.zeros_like(res.x)
         return res.x[3 * S.Nblobs:].reshape(S.nbodies, 6).contiguous()
 
-    def _advance(self, dt: float, adams_bashforth: bool) -> None:
+    def _advance(self, dt: float, adams_bashforth: bool, step: Optional[int] = None) -> None:
         S = self._current()
-        U = self.solve_mobility_problem(S)
+        U = self.solve_mobility_problem(S, step)
         if self._S_next is None:
             self._S_next = self.suspension()
             nb = ctypes.c_size_t()
@@ -197,12 +217,13 @@ class RigidIntegrator:
         self._S, self._S_next = Sn, S
         self._centres = None
         self._quats = None
+        self.step_count += 1
 
     def deterministic_forward_euler(self, dt: float, *args, **kwargs):
-        self._advance(dt, adams_bashforth=False)
+        self._advance(dt, adams_bashforth=False, step=kwargs.get("step"))
 
     def deterministic_adams_bashforth(self, dt: float, *args, **kwargs):
-        self._advance(dt, adams_bashforth=True)
+        self._advance(dt, adams_bashforth=True, step=kwargs.get("step"))
 
     def advance_time_step(self, dt: float, *args, **kwargs):
         """One step of the selected scheme (:68-72)."""

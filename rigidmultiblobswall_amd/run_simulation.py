@@ -136,7 +136,7 @@ def build_rigid(read: ReadInput, precision: str):
         repulsion_strength=read.repulsion_strength, debye_length=read.debye_length,
         periodic_length=read.periodic_length, tolerance=read.solver_tolerance,
         precision=mobility_precision(read.mobility_vector_prod_implementation, precision),
-        slip_reference=slips if have_slip else None, domain=read.domain)
+        slip_reference=slips if have_slip else None, domain=read.domain, update_PC=read.update_PC)
     return _RigidRun(integ), counts, None
 
 

@@ -46,6 +46,28 @@ def test_rigid_steps_match_oracle(gpu, scheme):
     assert g.det_iterations_count > 0
 
 
+def test_update_pc_reuses_a_stale_preconditioner(gpu):
+    """update_PC = 3 (multi_bodies.py:766-819): the preconditioner is built
+    on steps 0 and 3 only and reused in between; GMRES to the same
+    tolerance gives the same trajectory as rebuilding every step."""
+    c0, q0, _ = _bodies(4, 3)
+    ref = icosphere_42(1.0)
+    print("case rigid step update_PC 3 vs 1", flush=True)
+    runs = {}
+    for upd in (1, 3):
+        g = RigidIntegrator(c0, q0, ref, scheme="deterministic_adams_bashforth", eta=1.0, a=0.243,
+                            blob_blob_force_implementation="hip_fp64", tolerance=1e-11, update_PC=upd, **P)
+        for step in range(5):
+            g.advance_time_step(0.05, step=step)
+        runs[upd] = g
+    assert runs[1].pc_builds == 5 and runs[3].pc_builds == 2
+    dc = np.linalg.norm(runs[1].centres - c0)
+    assert np.linalg.norm(runs[3].centres - runs[1].centres) <= 1e-8 * dc
+    assert np.abs(runs[3].quats - runs[1].quats).max() <= 1e-8
+    with pytest.raises(ValueError):
+        RigidIntegrator(c0, q0, ref, update_PC=0)
+
+
 def test_body_pieces(gpu):
     c, q, rng = _bodies(5, 2)
     refs = [icosphere_42(1.0)[: 10 + 6 * k] for k in range(5)]           # ragged bodies
