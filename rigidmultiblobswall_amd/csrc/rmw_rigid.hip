@@ -437,6 +437,177 @@ __global__ __launch_bounds__(kRigidBlock) void pc_build_kernel(PcBuildParams p)
   }
 }
 
+// Preconditioner build for bodies that fit LDS (n = 3 nb <= kLdsMaxN): M_b
+// assembled in LDS, then inverted in registers by Gauss-Jordan elimination
+// without pivoting (M_b is symmetric positive definite: every pivot is a
+// Schur-complement diagonal, positive; a non-positive one flags the body).
+// Thread (ty, tx) of the 16 x 16 grid holds elements (ty + 16 a, tx + 16 c),
+// a, c < R, of the matrix; each elimination step broadcasts the pivot row
+// and column through LDS (double-buffered, one barrier per step) and
+// applies the rank-one update to the R x R register block:
+//   A'_ij = [i != k][j != k] A_ij - c'_i r'_j / A_kk,
+//   c'_i = A_ik (c'_k = -1),  r'_j = A_kj (r'_k = 1),
+// which also writes the scaled pivot row, column and 1/A_kk in place.
+// Then M^-1 K from LDS, K^T M^-1 K and its pseudo-inverse.
+template <int R>
+__global__ __launch_bounds__(kRigidBlock) void pc_build_gj_kernel(PcBuildParams p)
+{
+  extern __shared__ double smem[];   // A (n x n) | pivot rows/cols during the elimination, then M^-1 K (n x 6)
+  const int b = blockIdx.x;
+  const int64_t i0 = p.offs[b], nb = p.offs[b + 1] - i0;
+  const int n = (int)(3 * nb);
+  if (n > kLdsMaxN || (n + 15) / 16 > R) return;   // another launch owns this body
+  double* A = smem;
+  double* prow = smem + (size_t)n * n;               // [2][16 R]
+  double* pcol = prow + 2 * 16 * R;                  // [2][16 R]
+  double* MK = smem + (size_t)n * n;                 // n x 6, after the elimination
+  __shared__ double a6[36];
+  __shared__ int bad;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const double inva = 1.0 / p.a, norm = 1.0 / (8.0 * M_PI * p.eta * p.a);
+  if (tid == 0) bad = 0;
+
+  // 1. assemble M_b
+  for (int64_t idx = tid; idx < nb * nb; idx += nt) {
+    const int64_t i = idx / nb, j = idx - (idx / nb) * nb;
+    double m[9];
+    wall_uf_block(p.r + 3 * (i0 + i), p.r + 3 * (i0 + j), i == j, p.a, inva, norm, m);
+    for (int q = 0; q < 3; ++q)
+      for (int s = 0; s < 3; ++s) A[(3 * i + q) * n + 3 * j + s] = m[q * 3 + s];
+  }
+  __syncthreads();
+  double E[R][R];
+#pragma unroll
+  for (int ra = 0; ra < R; ++ra)
+#pragma unroll
+    for (int cb = 0; cb < R; ++cb) {
+      const int i = ty + 16 * ra, j = tx + 16 * cb;
+      E[ra][cb] = (i < n && j < n) ? A[i * n + j] : 0.0;
+    }
+
+  // 2. in-place Gauss-Jordan inverse
+  for (int k = 0; k < n; ++k) {
+    double* pr = prow + (k & 1) * 16 * R;
+    double* pc = pcol + (k & 1) * 16 * R;
+    const int ka = k >> 4, kt = k & 15;
+    if (ty == kt) {
+#pragma unroll
+      for (int ra = 0; ra < R; ++ra)
+        if (ra == ka) {
+#pragma unroll
+          for (int cb = 0; cb < R; ++cb) pr[tx + 16 * cb] = E[ra][cb];
+        }
+    }
+    if (tx == kt) {
+#pragma unroll
+      for (int cb = 0; cb < R; ++cb)
+        if (cb == ka) {
+#pragma unroll
+          for (int ra = 0; ra < R; ++ra) pc[ty + 16 * ra] = E[ra][cb];
+        }
+    }
+    __syncthreads();
+    double piv = pr[k];
+    if (!(piv > 0.0)) {
+      if (tid == 0) bad = 1;
+      piv = 1.0;
+    }
+    const double ip = 1.0 / piv;
+    double c[R], r[R];
+#pragma unroll
+    for (int ra = 0; ra < R; ++ra) {
+      const int i = ty + 16 * ra;
+      c[ra] = i == k ? -ip : (i < n ? pc[i] * ip : 0.0);
+    }
+#pragma unroll
+    for (int cb = 0; cb < R; ++cb) {
+      const int j = tx + 16 * cb;
+      r[cb] = j == k ? 1.0 : (j < n ? pr[j] : 0.0);
+    }
+#pragma unroll
+    for (int ra = 0; ra < R; ++ra) {
+      const bool rowk = ty + 16 * ra == k;
+#pragma unroll
+      for (int cb = 0; cb < R; ++cb) {
+        const bool keep = !rowk && tx + 16 * cb != k;
+        E[ra][cb] = fma(-c[ra], r[cb], keep ? E[ra][cb] : 0.0);
+      }
+    }
+  }
+  __syncthreads();   // the pivot buffers are free; A is rewritten below
+
+  // 3. M^-1 to LDS and to the output blocks
+  double* Minv = p.minv + p.moff[b];
+#pragma unroll
+  for (int ra = 0; ra < R; ++ra)
+#pragma unroll
+    for (int cb = 0; cb < R; ++cb) {
+      const int i = ty + 16 * ra, j = tx + 16 * cb;
+      if (i < n && j < n) {
+        A[i * n + j] = E[ra][cb];
+        Minv[(int64_t)i * n + j] = E[ra][cb];
+      }
+    }
+  __syncthreads();
+
+  // 4. M^-1 K with K rows [e_c | rot_c] (body.py:92-99: rot_i x = -(r_i x x))
+  double* MKg = p.minvk + 3 * i0 * 6;
+  for (int idx = tid; idx < n * 6; idx += nt) {
+    const int i = idx / 6, c = idx - (idx / 6) * 6;
+    const double* row = A + (int64_t)i * n;
+    double s = 0.0;
+    if (c < 3) {
+      for (int m = 0; m < (int)nb; ++m) s += row[3 * m + c];
+    } else {
+      const int d = c - 3;
+      for (int m = 0; m < (int)nb; ++m) {
+        const double* rr = p.rrel + 3 * (i0 + m);
+        const double rx = rr[0], ry = rr[1], rz = rr[2];
+        // column d of rot = [[0, rz, -ry], [-rz, 0, rx], [ry, -rx, 0]]
+        const double k0 = d == 0 ? 0.0 : (d == 1 ? rz : -ry);
+        const double k1 = d == 0 ? -rz : (d == 1 ? 0.0 : rx);
+        const double k2 = d == 0 ? ry : (d == 1 ? -rx : 0.0);
+        s += row[3 * m] * k0 + row[3 * m + 1] * k1 + row[3 * m + 2] * k2;
+      }
+    }
+    MK[idx] = s;
+    MKg[idx] = s;
+  }
+  __syncthreads();
+  // K^T M^-1 K
+  if (tid < 36) {
+    const int r6 = tid / 6, c6 = tid - (tid / 6) * 6;
+    double s = 0.0;
+    if (r6 < 3) {
+      for (int m = 0; m < (int)nb; ++m) s += MK[(3 * m + r6) * 6 + c6];
+    } else {
+      const int d = r6 - 3;
+      for (int m = 0; m < (int)nb; ++m) {
+        const double* rr = p.rrel + 3 * (i0 + m);
+        const double rx = rr[0], ry = rr[1], rz = rr[2];
+        const double k0 = d == 0 ? 0.0 : (d == 1 ? rz : -ry);
+        const double k1 = d == 0 ? -rz : (d == 1 ? 0.0 : rx);
+        const double k2 = d == 0 ? ry : (d == 1 ? -rx : 0.0);
+        s += k0 * MK[(3 * m) * 6 + c6] + k1 * MK[(3 * m + 1) * 6 + c6] + k2 * MK[(3 * m + 2) * 6 + c6];
+      }
+    }
+    a6[tid] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // symmetrise the rounding of K^T M^-1 K before the pseudo-inverse
+    for (int r6 = 0; r6 < 6; ++r6)
+      for (int c6 = r6 + 1; c6 < 6; ++c6) {
+        const double v = 0.5 * (a6[r6 * 6 + c6] + a6[c6 * 6 + r6]);
+        a6[r6 * 6 + c6] = v;
+        a6[c6 * 6 + r6] = v;
+      }
+    pinv6(a6, p.nbody + 36 * (int64_t)b);
+    p.status[b] = bad;
+  }
+}
+
 // Preconditioner apply: y = P^-1 x, one workgroup per body.
 __global__ __launch_bounds__(kRigidBlock) void pc_apply_kernel(const int64_t* __restrict__ offs,
                                                                const int64_t* __restrict__ moff,
@@ -557,6 +728,31 @@ static void upload_offsets(const RigidLayout& L, char* ws, hipStream_t st)
 // Every rigid entry point below reads the body offsets from the workspace;
 // rmw_rigid_setup must have been called on that workspace (stream-ordered)
 // with the same body_offsets.
+
+template <int R>
+static void launch_pc_gj_r(const PcBuildParams& p, int nbodies, size_t lds, hipStream_t st)
+{
+  RMW_CHECK_HIP(hipFuncSetAttribute((const void*)pc_build_gj_kernel<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+  hipLaunchKernelGGL(pc_build_gj_kernel<R>, dim3(nbodies), dim3(kRigidBlock), lds, st, p);
+  RMW_CHECK_HIP(hipGetLastError());
+}
+
+static void launch_pc_gj(int R, const PcBuildParams& p, int nbodies, size_t lds, hipStream_t st)
+{
+  switch (R) {
+    case 1: launch_pc_gj_r<1>(p, nbodies, lds, st); break;
+    case 2: launch_pc_gj_r<2>(p, nbodies, lds, st); break;
+    case 3: launch_pc_gj_r<3>(p, nbodies, lds, st); break;
+    case 4: launch_pc_gj_r<4>(p, nbodies, lds, st); break;
+    case 5: launch_pc_gj_r<5>(p, nbodies, lds, st); break;
+    case 6: launch_pc_gj_r<6>(p, nbodies, lds, st); break;
+    case 7: launch_pc_gj_r<7>(p, nbodies, lds, st); break;
+    case 8: launch_pc_gj_r<8>(p, nbodies, lds, st); break;
+    case 9: launch_pc_gj_r<9>(p, nbodies, lds, st); break;
+    default: RMW_REQUIRE(false, "body too large for the LDS preconditioner build");
+  }
+}
 
 }  // namespace rmw
 
@@ -723,11 +919,12 @@ extern "C" int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, co
       if (n <= kLdsMaxN) { ++small; max_small = std::max(max_small, n); } else { ++big; }
     }
     if (small) {
-      const size_t lds = ((size_t)max_small * max_small + 8 * (size_t)max_small) * sizeof(double);
-      RMW_CHECK_HIP(hipFuncSetAttribute((const void*)pc_build_kernel<true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(pc_build_kernel<true>, dim3(nbodies), dim3(kRigidBlock), lds, st, p);
-      RMW_CHECK_HIP(hipGetLastError());
+      // Gauss-Jordan in registers, R x R elements per thread for the largest body
+      const int R = std::max(1, (max_small + 15) / 16);
+      const size_t lds = ((size_t)max_small * max_small + std::max<size_t>(6 * (size_t)max_small, 4 * 16 * (size_t)R)) *
+                         sizeof(double);
+      RMW_REQUIRE(lds <= 160 * 1024, "preconditioner build exceeds LDS");
+      launch_pc_gj(R, p, nbodies, lds, st);
     }
     if (big) {
       hipLaunchKernelGGL(pc_build_kernel<false>, dim3(nbodies), dim3(kRigidBlock), 0, st, p);

@@ -61,6 +61,42 @@ def test_body_blobs_operator_and_pc(gpu):
     assert np.abs(S.nbody[0].cpu().numpy() - Og.N[0]).max() <= 1e-10 * np.abs(Og.N[0]).max()
 
 
+def test_preconditioner_ragged_bodies(gpu):
+    """Bodies of 1, 7, 16, 42 and 50 blobs in one suspension: the register
+    Gauss-Jordan build (up to 45 blobs, the elements of the largest body per
+    thread) and the global-memory Cholesky build (50 blobs) against the
+    oracle's Cholesky-based M_b^-1 and N_b, and the applied preconditioner."""
+    rng = np.random.default_rng(9)
+    ico = workloads.icosphere_42(1.0)
+    sizes = [1, 7, 16, 42, 50]
+    refs = []
+    for m in sizes:
+        if m <= 42:
+            refs.append(ico[:m].copy())
+        else:
+            extra = rng.standard_normal((m - 42, 3))
+            extra = 0.6 * extra / np.linalg.norm(extra, axis=1, keepdims=True)   # inside the shell
+            refs.append(np.vstack([ico, extra]))
+    c = np.column_stack([np.arange(len(sizes)) * 3.0, np.zeros(len(sizes)), np.full(len(sizes), 1.6)])
+    q = workloads.random_quaternions(rng, len(sizes))
+    print("case preconditioner build, ragged bodies 1..50 blobs", flush=True)
+    S = RigidSuspension(c, q, refs, 0.8, 0.2, precision="fp64")
+    S.build_preconditioner()
+    Og = O.RigidSystem([(c[k], q[k], refs[k]) for k in range(len(sizes))], 0.8, 0.2)
+    Og.build_pc()
+    off = 0
+    for k, m in enumerate(sizes):
+        n = 3 * m
+        Mk = S.minv[off: off + n * n].reshape(n, n).cpu().numpy()
+        off += n * n
+        assert np.abs(Mk - Og.Minv[k]).max() <= 1e-9 * np.abs(Og.Minv[k]).max(), m
+        assert np.abs(S.nbody[k].cpu().numpy() - Og.N[k]).max() <= 1e-9 * np.abs(Og.N[k]).max(), m
+    x = rng.standard_normal(S.size)
+    y = torch.empty(S.size, dtype=torch.float64, device=gpu)
+    S.pc_apply(torch.from_numpy(x).to(gpu), y)
+    assert rel(y.cpu().numpy(), Og.pc_apply(x)) <= 1e-9
+
+
 @pytest.mark.parametrize("res,tol", [("low", 1e-8), ("mid", 1e-8), ("high", 5e-8)])
 def test_pair_active_rods_golden_gpu(gpu, res, tol):
     c = G.load_case(res)
