@@ -13,6 +13,8 @@
 // B[4 (l / 16) + 0..3][l % 16] and receives D[4 (l / 16) + i][l % 16].
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -226,25 +228,70 @@ __device__ inline void quat_to_rot(const double* q, double R[9])
 }
 
 // 8 nearest neighbours of every body by centre distance (ties by index)
-__global__ void knn_kernel(const double* __restrict__ c, int nb, int deg, int* __restrict__ nbr)
+// k nearest neighbours of each body (ascending distance, ties by index,
+// the body itself excluded): one workgroup per body.  Each thread keeps a
+// sorted list of its nearest candidates (bodies j = tid, tid + 256, ...)
+// in registers -- a compare-and-swap insertion over compile-time slots --
+// then deg rounds of a block-wide (distance, index) minimum over the list
+// heads take the overall nearest in order.
+constexpr int kKnnBlock = 256;
+constexpr int kKnnMax = 16;
+
+__device__ __forceinline__ bool knn_less(double da, int ia, double db, int ib)
 {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  double bd[16];
-  int bi[16];
-  for (int e = 0; e < deg; ++e) { bd[e] = 1e300; bi[e] = b; }
+  return da < db || (da == db && ia < ib);
+}
+
+__global__ __launch_bounds__(kKnnBlock) void knn_kernel(const double* __restrict__ c, int nb, int deg,
+                                                        int* __restrict__ nbr)
+{
+  __shared__ double s_d[kKnnBlock / 64];
+  __shared__ int s_i[kKnnBlock / 64];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const double x = c[3 * b], y = c[3 * b + 1], z = c[3 * b + 2];
-  for (int j = 0; j < nb; ++j) {
+  double bd[kKnnMax];
+  int bi[kKnnMax];
+#pragma unroll
+  for (int e = 0; e < kKnnMax; ++e) { bd[e] = INFINITY; bi[e] = INT_MAX; }
+  for (int j = tid; j < nb; j += kKnnBlock) {
     if (j == b) continue;
     const double dx = c[3 * j] - x, dy = c[3 * j + 1] - y, dz = c[3 * j + 2] - z;
-    const double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 >= bd[deg - 1]) continue;
-    int e = deg - 1;
-    while (e > 0 && bd[e - 1] > d2) { bd[e] = bd[e - 1]; bi[e] = bi[e - 1]; --e; }
-    bd[e] = d2;
-    bi[e] = j;
+    double d = dx * dx + dy * dy + dz * dz;
+    int id = j;
+    // insertion: the candidate sinks to its slot, the slot's entry moves down
+#pragma unroll
+    for (int e = 0; e < kKnnMax; ++e) {
+      if (e < deg && knn_less(d, id, bd[e], bi[e])) {
+        const double td = bd[e]; const int ti = bi[e];
+        bd[e] = d; bi[e] = id;
+        d = td; id = ti;
+      }
+    }
   }
-  for (int e = 0; e < deg; ++e) nbr[b * deg + e] = bi[e];
+  for (int r = 0; r < deg; ++r) {
+    double d = bd[0];
+    int id = bi[0];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double od = __shfl_xor(d, o, 64);
+      const int oi = __shfl_xor(id, o, 64);
+      if (knn_less(od, oi, d, id)) { d = od; id = oi; }
+    }
+    if (lane == 0) { s_d[wv] = d; s_i[wv] = id; }
+    __syncthreads();
+    d = s_d[0]; id = s_i[0];
+#pragma unroll
+    for (int w2 = 1; w2 < kKnnBlock / 64; ++w2)
+      if (knn_less(s_d[w2], s_i[w2], d, id)) { d = s_d[w2]; id = s_i[w2]; }
+    __syncthreads();   // s_d / s_i are rewritten next round
+    if (tid == 0) nbr[(int64_t)b * deg + r] = id == INT_MAX ? b : id;
+    if (bi[0] == id) {
+      // the winner's list pops its head
+#pragma unroll
+      for (int e = 0; e < kKnnMax - 1; ++e) { bd[e] = bd[e + 1]; bi[e] = bi[e + 1]; }
+      bd[kKnnMax - 1] = INFINITY; bi[kKnnMax - 1] = INT_MAX;
+    }
+  }
 }
 
 // Node features (10): z-hat in the body frame (3), centre height (1),
@@ -439,7 +486,7 @@ extern "C" int rmw_gnn_knn(const double* centres, int nb, int deg, int* nbr, voi
 {
   return guarded([&] {
     RMW_REQUIRE(deg >= 1 && deg <= 16 && nb > deg, "deg must be in [1, 16] and below the body count");
-    hipLaunchKernelGGL(knn_kernel, dim3((nb + 127) / 128), dim3(128), 0, (hipStream_t)stream, centres, nb, deg, nbr);
+    hipLaunchKernelGGL(knn_kernel, dim3(nb), dim3(kKnnBlock), 0, (hipStream_t)stream, centres, nb, deg, nbr);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

@@ -101,6 +101,32 @@ def test_training_reduces_loss(gpu):
     assert l1 < 0.1 * l0
 
 
+@pytest.mark.parametrize("nb,deg,lattice", [(9, 8, False), (300, 8, False), (1024, 8, False), (400, 16, True),
+                                            (700, 3, True)])
+def test_knn_matches_sorted_distances(gpu, nb, deg, lattice):
+    """Neighbour lists: ascending squared distance, ties by index, the body
+    itself excluded; a square lattice gives exact ties."""
+    rng = np.random.default_rng(nb + deg)
+    if lattice:
+        side = int(np.ceil(np.sqrt(nb)))
+        g = np.stack(np.meshgrid(np.arange(side), np.arange(side), indexing="ij"), -1).reshape(-1, 2)[:nb]
+        c = np.column_stack([g * 2.0, np.full(nb, 1.3)])
+    else:
+        c = np.column_stack([rng.uniform(0, 60, nb), rng.uniform(0, 60, nb), rng.uniform(1.2, 1.4, nb)])
+    cd = torch.from_numpy(np.ascontiguousarray(c)).to(gpu)
+    nbr = torch.empty(nb * deg, dtype=torch.int32, device=gpu)
+    lib = _lib.load()
+    print(f"case knn nb={nb} deg={deg} lattice={lattice}", flush=True)
+    _lib.check(lib.rmw_gnn_knn(cd.data_ptr(), nb, deg, nbr.data_ptr(), torch.cuda.current_stream().cuda_stream))
+    got = nbr.cpu().numpy().reshape(nb, deg)
+    d2 = ((c[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+    idx = np.arange(nb)
+    for i in range(nb):
+        keys = np.lexsort((idx, d2[i]))          # by distance, then index
+        keys = keys[keys != i][:deg]
+        assert np.array_equal(got[i], keys), i
+
+
 def test_features_and_body_frame_round_trip(gpu):
     w = workloads.c4_monolayer(nbodies=64, seed=7)
     net = gnn.GraphNet(42, gpu)
