@@ -378,7 +378,7 @@ def main(argv=None):
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--gmres-steps", type=int, default=64)
-    ap.add_argument("--gmres-epochs", type=int, default=40)
+    ap.add_argument("--gmres-epochs", type=int, default=120)
     ap.add_argument("--gmres-evals", type=int, default=3)
     args = ap.parse_args(argv)
     if args.gpus != 1:

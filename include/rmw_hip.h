@@ -426,6 +426,14 @@ int rmw_lanczos_sqrt_coeffs(const double* H, int64_t h_rhs_stride, int64_t ldh, 
 int rmw_gemm_bf16(int transA, int transB, int M, int N, int K, const uint16_t* A, int64_t lda,
                   const uint16_t* B, int64_t ldb, float* C, int64_t ldc, float beta, uint16_t* H,
                   int64_t ldh, const float* bias, int act, void* stream);
+/* C = op(A) op(B) (+ beta C) with K split into `splits` slices of whole
+ * 32-wide steps, each slice's FP32 sums written to `partial` ([slice][M][N],
+ * rmw_gemm_bf16_splitk_workspace bytes) and added in slice order: the
+ * weight gradients, whose K is the batch (M, N <= 128, K up to 16384). */
+int rmw_gemm_bf16_splitk_workspace(int M, int N, int K, int splits, size_t* bytes);
+int rmw_gemm_bf16_splitk(int transA, int transB, int M, int N, int K, const uint16_t* A, int64_t lda,
+                         const uint16_t* B, int64_t ldb, float* C, int64_t ldc, float beta, int splits,
+                         float* partial, size_t partial_bytes, void* stream);
 int rmw_f32_to_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
 /* dZ = dH * SiLU'(Z), BF16 out */
 int rmw_silu_backward(const float* dH, const float* Z, uint16_t* dZ, int64_t n, void* stream);

@@ -136,6 +136,9 @@ PROTOTYPES = {
     # learned warm start
     "rmw_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _i64,
                                      _dp, _i64, _dp, _i64, ctypes.c_float, _dp, _i64, _dp, ctypes.c_int, _vp]),
+    "rmw_gemm_bf16_splitk_workspace": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _szp]),
+    "rmw_gemm_bf16_splitk": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                            _i64, _dp, _i64, _dp, _i64, ctypes.c_float, ctypes.c_int, _dp, _sz, _vp]),
     "rmw_f32_to_bf16": (ctypes.c_int, [_dp, _dp, _i64, _vp]),
     "rmw_silu_backward": (ctypes.c_int, [_dp, _dp, _dp, _i64, _vp]),
     "rmw_colsum_bf16": (ctypes.c_int, [_dp, _i64, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_float, _vp]),

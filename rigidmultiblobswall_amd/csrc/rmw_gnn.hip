@@ -47,6 +47,8 @@ struct GemmParams {
   const float* bias;                      // optional, length N
   int act;                                // 0 none, 1 SiLU
   int M, N, K;
+  int kchunk;                             // K range per blockIdx.z (split K)
+  float* partial;                         // split K: raw sums [z][M][N], no epilogue
 };
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmParams p)
@@ -59,18 +61,26 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmParams p)
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kb = blockIdx.z * p.kchunk;
+  const int ke = kb + p.kchunk < p.K ? kb + p.kchunk : p.K;
 
-  for (int k0 = 0; k0 < p.K; k0 += GK) {
-    // stage A tile (rows m, cols k) and B tile (rows n, cols k) as [row][k]
+  for (int k0 = kb; k0 < ke; k0 += GK) {
+    // stage A tile (rows m, cols k) and B tile (rows n, cols k) as [row][k];
+    // consecutive threads take consecutive addresses of the operand in memory
     for (int e = tid; e < GM * GK; e += 256) {
-      const int r = e / GK, c = e - (e / GK) * GK;
+      int r, c;
+      if (p.transA) { c = e / GM; r = e - c * GM; } else { r = e / GK; c = e - r * GK; }
       const int m = m0 + r, k = k0 + c;
       uint16_t va = 0;
-      if (m < p.M && k < p.K) va = p.transA ? p.A[(int64_t)k * p.lda + m] : p.A[(int64_t)m * p.lda + k];
+      if (m < p.M && k < ke) va = p.transA ? p.A[(int64_t)k * p.lda + m] : p.A[(int64_t)m * p.lda + k];
       As[r * GKP + c] = va;
-      const int n = n0 + r;
+    }
+    for (int e = tid; e < GN * GK; e += 256) {
+      int r, c;
+      if (!p.transB) { c = e / GN; r = e - c * GN; } else { r = e / GK; c = e - r * GK; }
+      const int n = n0 + r, k = k0 + c;
       uint16_t vb = 0;
-      if (n < p.N && k < p.K) vb = p.transB ? p.B[(int64_t)n * p.ldb + k] : p.B[(int64_t)k * p.ldb + n];
+      if (n < p.N && k < ke) vb = p.transB ? p.B[(int64_t)n * p.ldb + k] : p.B[(int64_t)k * p.ldb + n];
       Bs[r * GKP + c] = vb;
     }
     __syncthreads();
@@ -91,6 +101,20 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmParams p)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
+  }
+  if (p.partial) {
+    // split K: this slice's raw sums
+    float* P = p.partial + (int64_t)blockIdx.z * p.M * p.N;
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (n >= p.N) continue;
+        for (int q = 0; q < 4; ++q) {
+          const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + q;
+          if (m < p.M) P[(int64_t)m * p.N + n] = acc[i][j][q];
+        }
+      }
+    return;
   }
   // epilogue
   for (int i = 0; i < 2; ++i)
@@ -135,15 +159,48 @@ __global__ void silu_backward_kernel(const float* __restrict__ dH, const float* 
   }
 }
 
-// column sums: out[j] (+)= sum_i X[i * ldx + j] over i < rows (bf16 input)
-__global__ void colsum_bf16_kernel(const uint16_t* __restrict__ X, int64_t ldx, int rows, int cols,
-                                   float* __restrict__ out, float beta)
+// split K: C = sum_z partial[z] (fixed order) + beta C
+__global__ void gemm_splitk_reduce_kernel(const float* __restrict__ partial, int splits, int M, int N,
+                                          float* __restrict__ C, int64_t ldc, float beta)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cols) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)M * N) return;
+  const int m = (int)(t / N), n = (int)(t - (int64_t)m * N);
   float s = 0.f;
-  for (int i = 0; i < rows; ++i) s += bf2f(X[(int64_t)i * ldx + j]);
-  out[j] = beta != 0.f ? s + beta * out[j] : s;
+  for (int z = 0; z < splits; ++z) s += partial[(int64_t)z * M * N + t];
+  float* c = C + (int64_t)m * ldc + n;
+  *c = beta != 0.f ? s + beta * *c : s;
+}
+
+// column sums: out[j] (+)= sum_i X[i * ldx + j] over i < rows (bf16 input).
+// One 1024-thread workgroup per 64 columns: thread t sums columns
+// 4 (t mod 16) .. +3 over rows t/16, t/16 + 64, ...; then the 64 row phases
+// are added in order.
+constexpr int kColsBlock = 1024, kColsPhases = kColsBlock / 16;
+
+__global__ __launch_bounds__(kColsBlock) void colsum_bf16_kernel(const uint16_t* __restrict__ X, int64_t ldx,
+                                                                 int rows, int cols, float* __restrict__ out,
+                                                                 float beta)
+{
+  __shared__ float part[kColsPhases][64];
+  const int t = threadIdx.x, cg = t & 15, ph = t >> 4;
+  const int j0 = blockIdx.x * 64 + 4 * cg;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int i = ph; i < rows; i += kColsPhases) {
+    const uint16_t* row = X + (int64_t)i * ldx;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (j0 + q < cols) s[q] += bf2f(row[j0 + q]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) part[ph][4 * cg + q] = s[q];
+  __syncthreads();
+  if (t < 64) {
+    const int j = blockIdx.x * 64 + t;
+    float v = 0.f;
+    for (int k = 0; k < kColsPhases; ++k) v += part[k][t];
+    if (j < cols) out[j] = beta != 0.f ? v + beta * out[j] : v;
+  }
 }
 
 // node input: Xc[b] = [Xn[b] (nin) | sum_{e<deg} M[b*deg+e] (nmsg) | 0 pad], bf16
@@ -402,10 +459,44 @@ extern "C" int rmw_gemm_bf16(int transA, int transB, int M, int N, int K, const 
     RMW_REQUIRE(C || H, "no output");
     RMW_REQUIRE(act == 0 || act == 1, "act must be 0 (none) or 1 (SiLU)");
     if (M == 0 || N == 0) return;
-    GemmParams p{A, lda, transA, B, ldb, transB, C, ldc, beta, H, ldh, bias, act, M, N, K};
+    GemmParams p{A, lda, transA, B, ldb, transB, C, ldc, beta, H, ldh, bias, act, M, N, K, K > 0 ? K : GK, nullptr};
     dim3 grid((N + GN - 1) / GN, (M + GM - 1) / GM);
     RMW_REQUIRE(grid.y <= 65535, "M too large");
     hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, p);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_gemm_bf16_splitk_workspace(int M, int N, int K, int splits, size_t* bytes)
+{
+  return guarded([&] {
+    RMW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && splits >= 1 && bytes, "bad sizes");
+    *bytes = (size_t)splits * M * N * sizeof(float);
+  });
+}
+
+extern "C" int rmw_gemm_bf16_splitk(int transA, int transB, int M, int N, int K, const uint16_t* A, int64_t lda,
+                                    const uint16_t* B, int64_t ldb, float* C, int64_t ldc, float beta, int splits,
+                                    float* partial, size_t partial_bytes, void* stream)
+{
+  return guarded([&] {
+    RMW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && splits >= 1 && splits <= 65535, "bad sizes");
+    RMW_REQUIRE(C != nullptr, "no output");
+    if (M == 0 || N == 0) return;
+    // K slices of whole GK steps
+    int kchunk = (K + splits - 1) / splits;
+    kchunk = (kchunk + GK - 1) / GK * GK;
+    if (kchunk <= 0) kchunk = GK;
+    const int used = K > 0 ? (K + kchunk - 1) / kchunk : 1;
+    RMW_REQUIRE(partial && partial_bytes >= (size_t)used * M * N * sizeof(float), "split-K workspace too small");
+    GemmParams p{A, lda, transA, B, ldb, transB, C, ldc, beta, nullptr, 0, nullptr, 0, M, N, K, kchunk, partial};
+    dim3 grid((N + GN - 1) / GN, (M + GM - 1) / GM, used);
+    RMW_REQUIRE(grid.y <= 65535, "M too large");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, st, p);
+    const int64_t mn = (int64_t)M * N;
+    hipLaunchKernelGGL(gemm_splitk_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, partial, used,
+                       M, N, C, ldc, beta);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }
@@ -430,8 +521,8 @@ extern "C" int rmw_colsum_bf16(const uint16_t* X, int64_t ldx, int rows, int col
                                void* stream)
 {
   return guarded([&] {
-    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((cols + 127) / 128), dim3(128), 0, (hipStream_t)stream, X, ldx, rows,
-                       cols, out, beta);
+    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((cols + 63) / 64), dim3(kColsBlock), 0, (hipStream_t)stream, X, ldx,
+                       rows, cols, out, beta);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

@@ -127,6 +127,20 @@ class GraphNet:
                                           H.data_ptr() if H is not None else None, ldh,
                                           bias.data_ptr() if bias is not None else None, act, _stream()))
 
+    def _gemm_wgrad(self, M, N, K, A, lda, B, ldb, C, ldc):
+        """C = A^T B for a weight gradient (K = batch rows): K split so that
+        about 256 workgroups share it."""
+        tiles = ((M + 63) // 64) * ((N + 63) // 64)
+        splits = max(1, min(256 // tiles, K // 128))
+        if splits == 1:
+            self._gemm(1, 0, M, N, K, A, lda, B, ldb, C=C, ldc=ldc)
+            return
+        nbytes = ctypes.c_size_t()
+        _lib.check(self.lib.rmw_gemm_bf16_splitk_workspace(M, N, K, splits, ctypes.byref(nbytes)))
+        part = torch.empty(max(1, nbytes.value // 4), dtype=torch.float32, device=self.dev)
+        _lib.check(self.lib.rmw_gemm_bf16_splitk(1, 0, M, N, K, A.data_ptr(), lda, B.data_ptr(), ldb, C.data_ptr(),
+                                                 ldc, 0.0, splits, part.data_ptr(), part.numel() * 4, _stream()))
+
     def _bf16(self, x: torch.Tensor) -> torch.Tensor:
         y = torch.empty(x.shape, dtype=torch.int16, device=self.dev)
         _lib.check(self.lib.rmw_f32_to_bf16(x.data_ptr(), y.data_ptr(), x.numel(), _stream()))
@@ -170,14 +184,14 @@ class GraphNet:
         B, E = a["B"], a["E"]
         no = self.nout
         # node layer 3
-        self._gemm(1, 0, no, HN, B, dY_b, no, a["Hn2"], HN, C=self.G("Wn3"), ldc=HN)
+        self._gemm_wgrad(no, HN, B, dY_b, no, a["Hn2"], HN, self.G("Wn3"), HN)
         _lib.check(self.lib.rmw_colsum_bf16(dY_b.data_ptr(), no, B, no, self.G("bn3").data_ptr(), 0.0, _stream()))
         dH = self._empty((B, HN))
         self._gemm(0, 0, B, HN, no, dY_b, no, self.Wb("Wn3"), HN, C=dH, ldc=HN)
         # node layer 2
         dZ = self._empty((B, HN), torch.int16)
         _lib.check(self.lib.rmw_silu_backward(dH.data_ptr(), a["Zn2"].data_ptr(), dZ.data_ptr(), B * HN, _stream()))
-        self._gemm(1, 0, HN, HN, B, dZ, HN, a["Hn1"], HN, C=self.G("Wn2"), ldc=HN)
+        self._gemm_wgrad(HN, HN, B, dZ, HN, a["Hn1"], HN, self.G("Wn2"), HN)
         _lib.check(self.lib.rmw_colsum_bf16(dZ.data_ptr(), HN, B, HN, self.G("bn2").data_ptr(), 0.0, _stream()))
         dH1 = self._empty((B, HN))
         self._gemm(0, 0, B, HN, HN, dZ, HN, self.Wb("Wn2"), HN, C=dH1, ldc=HN)
@@ -185,14 +199,14 @@ class GraphNet:
         dZ1 = self._empty((B, HN), torch.int16)
         _lib.check(self.lib.rmw_silu_backward(dH1.data_ptr(), a["Zn1"].data_ptr(), dZ1.data_ptr(), B * HN,
                                               _stream()))
-        self._gemm(1, 0, HN, NC, B, dZ1, HN, a["Xc"], NC, C=self.G("Wn1"), ldc=NC)
+        self._gemm_wgrad(HN, NC, B, dZ1, HN, a["Xc"], NC, self.G("Wn1"), NC)
         _lib.check(self.lib.rmw_colsum_bf16(dZ1.data_ptr(), HN, B, HN, self.G("bn1").data_ptr(), 0.0, _stream()))
         dXc = self._empty((B, NC))
         self._gemm(0, 0, B, NC, HN, dZ1, HN, self.Wb("Wn1"), NC, C=dXc, ldc=NC)
         # messages -> edge layer 2
         dM = self._empty((E, NMSG), torch.int16)
         _lib.check(self.lib.rmw_gnn_scatter(dXc.data_ptr(), NC, NIN, NMSG, DEG, B, dM.data_ptr(), _stream()))
-        self._gemm(1, 0, NMSG, HE, E, dM, NMSG, a["He1"], HE, C=self.G("We2"), ldc=HE)
+        self._gemm_wgrad(NMSG, HE, E, dM, NMSG, a["He1"], HE, self.G("We2"), HE)
         _lib.check(self.lib.rmw_colsum_bf16(dM.data_ptr(), NMSG, E, NMSG, self.G("be2").data_ptr(), 0.0, _stream()))
         dHe = self._empty((E, HE))
         self._gemm(0, 0, E, HE, NMSG, dM, NMSG, self.Wb("We2"), HE, C=dHe, ldc=HE)
@@ -200,7 +214,7 @@ class GraphNet:
         dZe = self._empty((E, HE), torch.int16)
         _lib.check(self.lib.rmw_silu_backward(dHe.data_ptr(), a["Ze1"].data_ptr(), dZe.data_ptr(), E * HE,
                                               _stream()))
-        self._gemm(1, 0, HE, EIN, E, dZe, HE, a["Xe_b"], EIN, C=self.G("We1"), ldc=EIN)
+        self._gemm_wgrad(HE, EIN, E, dZe, HE, a["Xe_b"], EIN, self.G("We1"), EIN)
         _lib.check(self.lib.rmw_colsum_bf16(dZe.data_ptr(), HE, E, HE, self.G("be1").data_ptr(), 0.0, _stream()))
         # the padding columns of Wn1 stay zero
         self.G("Wn1")[:, NIN + NMSG:].zero_()

@@ -198,7 +198,7 @@ def main(argv=None):
     ap.add_argument("--bodies", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--train-frac", type=float, default=0.75)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--eval", type=int, default=5, help="held-out steps evaluated")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)

@@ -1,6 +1,8 @@
 """GPU checks of the learned warm start: BF16 MFMA GEMM, network forward and
 gradients against the FP64 CPU oracle, graph features, and an end-to-end
 warm start on a small monolayer."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -44,6 +46,52 @@ def test_gemm_bf16_matches_fp64(gpu, tA, tB, M, N, K):
     hs = (H.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
     silu = ref / (1 + np.exp(-ref))
     assert np.abs(hs - silu).max() <= 1e-2 * np.abs(silu).max()
+
+
+@pytest.mark.parametrize("M,N,K,splits", [(64, 16, 16384, 128), (128, 80, 2048, 16), (132, 128, 2048, 3),
+                                          (64, 64, 1000, 7), (5, 9, 40, 4)])
+def test_gemm_bf16_splitk_matches_fp64(gpu, M, N, K, splits):
+    """Weight-gradient form C = A^T B with K split into slices (summed in
+    slice order), with beta accumulation; K not a multiple of the 32-wide
+    step and more slices than 32-wide steps exercised."""
+    lib = _lib.load()
+    rng = np.random.default_rng(M + N + K + splits)
+    Am = GO.bf16_round(rng.standard_normal((M, K)))
+    Bm = GO.bf16_round(rng.standard_normal((K, N)))
+    A = _bf16_tensor(Am.T.copy(), gpu)          # stored K x M
+    B = _bf16_tensor(Bm, gpu)                   # stored K x N
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    C = torch.from_numpy(C0.copy()).to(gpu)
+    nb = ctypes.c_size_t()
+    _lib.check(lib.rmw_gemm_bf16_splitk_workspace(M, N, K, splits, ctypes.byref(nb)))
+    part = torch.empty(max(1, nb.value // 4), dtype=torch.float32, device=gpu)
+    print(f"case gemm split-K M={M} N={N} K={K} splits={splits}", flush=True)
+    _lib.check(lib.rmw_gemm_bf16_splitk(1, 0, M, N, K, A.data_ptr(), M, B.data_ptr(), N, C.data_ptr(), N, 0.5,
+                                        splits, part.data_ptr(), part.numel() * 4,
+                                        torch.cuda.current_stream().cuda_stream))
+    ref = Am @ Bm + 0.5 * C0
+    assert np.abs(C.cpu().numpy() - ref).max() <= 1e-5 * np.abs(ref).max() * max(1.0, np.sqrt(K / 2048))
+    # the unsplit kernel agrees
+    C1 = torch.empty((M, N), dtype=torch.float32, device=gpu)
+    _lib.check(lib.rmw_gemm_bf16(1, 0, M, N, K, A.data_ptr(), M, B.data_ptr(), N, C1.data_ptr(), N, 0.0, None, 0,
+                                 None, 0, torch.cuda.current_stream().cuda_stream))
+    assert np.abs(C1.cpu().numpy() - Am @ Bm).max() <= 1e-5 * np.abs(Am @ Bm).max() * max(1.0, np.sqrt(K / 2048))
+
+
+@pytest.mark.parametrize("rows,cols,ldx", [(16384, 64, 64), (2048, 132, 132), (2048, 16, 80), (3, 200, 201),
+                                           (1, 1, 1)])
+def test_colsum_bf16_matches_fp64(gpu, rows, cols, ldx):
+    lib = _lib.load()
+    rng = np.random.default_rng(rows + cols)
+    X = GO.bf16_round(rng.standard_normal((rows, ldx)))
+    Xd = _bf16_tensor(X, gpu)
+    out0 = rng.standard_normal(cols).astype(np.float32)
+    out = torch.from_numpy(out0.copy()).to(gpu)
+    print(f"case colsum rows={rows} cols={cols} ldx={ldx}", flush=True)
+    _lib.check(lib.rmw_colsum_bf16(Xd.data_ptr(), ldx, rows, cols, out.data_ptr(), 2.0,
+                                   torch.cuda.current_stream().cuda_stream))
+    ref = X[:, :cols].sum(axis=0) + 2.0 * out0
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max()) * max(1.0, np.sqrt(rows / 256))
 
 
 def _net_and_batch(gpu, B=37, seed=3):
