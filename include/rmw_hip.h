@@ -333,6 +333,19 @@ int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, const int64_t
 
 /* y = P^-1 x, the body preconditioner apply of multi_bodies/multi_bodies.py:
  * 825-849 (x != y). */
+/* Velocity (nblobs, 3) at every blob induced by the net forces (first three
+ * entries of each force_stride-long row of `force`) of all other bodies,
+ * each body a sphere of radius `radius` at its centre, through the
+ * different-radii wall RPY of rmw_radii_apply (wall != 0) or the unbounded
+ * one: u_i = B_i sum_{c != body(i)} M(r_i, centre_c; a, radius) B_c F_c.
+ * The far field of the off-diagonal blocks of the rigid operator
+ * (multi_bodies.py:424-477) on any iterate whose body force sums are F:
+ * P^-1 (b - [u; 0]) is a block-Jacobi step from P^-1 b without the blob
+ * product.  No reference counterpart (an initial-guess helper). */
+int rmw_rigid_point_field(const double* r_blobs, const double* centres, const double* force,
+                          int64_t force_stride, const int64_t* body_offsets, int nbodies, double radius,
+                          double a, double eta, int wall, double* velocity, void* workspace,
+                          size_t workspace_bytes, void* stream);
 int rmw_rigid_pc_apply(const int64_t* body_offsets, int nbodies, const double* minv,
                        const double* minvk, const double* nbody, const double* r_rel,
                        const double* x, double* y, void* workspace, size_t workspace_bytes,

@@ -119,6 +119,8 @@ PROTOTYPES = {
                                                _vp, _sz, _vp]),
     "rmw_rigid_pc_build": (ctypes.c_int, [_dp, _dp, _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
                                           _dp, _dp, _vp, _dp, _sz, _vp, _sz, _vp]),
+    "rmw_rigid_point_field": (ctypes.c_int, [_dp, _dp, _dp, _i64, _i64p, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _vp, _sz, _vp]),
     "rmw_rigid_pc_apply": (ctypes.c_int, [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _sz, _vp]),
     # Krylov
     "rmw_krylov_workspace_size": (ctypes.c_int, [_i64, ctypes.c_int, _szp]),

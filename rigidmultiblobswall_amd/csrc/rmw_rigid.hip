@@ -608,6 +608,74 @@ __global__ __launch_bounds__(kRigidBlock) void pc_build_gj_kernel(PcBuildParams 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Point field of the other bodies: the velocity at every blob induced by the
+// net forces of all other bodies, each body a sphere of radius R at its
+// centre -- the different-radii wall RPY (coef_radii) between the blob
+// (radius a) and the sphere, heights shifted to max(z, radius) and damped
+// below it as the radii product does:
+//   u_i = B_i sum_{c != body(i)} M(r_i, R_c; a, R) B_c F_c.
+// It is the far-field part of the off-diagonal blocks of the rigid
+// operator acting on any iterate whose body force sums are F, so that
+// P^-1 (b - [u; 0]) is a block-Jacobi step from P^-1 b without the product.
+// ---------------------------------------------------------------------------
+constexpr int kPfBlock = 256;
+
+__global__ __launch_bounds__(kPfBlock) void point_field_kernel(const double* __restrict__ r, int64_t nblobs,
+                                                               const int64_t* __restrict__ offs, int nbodies,
+                                                               const double* __restrict__ centres,
+                                                               const double* __restrict__ force, int64_t fstride,
+                                                               double R, double a, double norm, int wall,
+                                                               double* __restrict__ out)
+{
+  __shared__ double s_x[kPfBlock], s_y[kPfBlock], s_z[kPfBlock];
+  __shared__ double s_f[3][kPfBlock];
+  const int64_t i = (int64_t)blockIdx.x * kPfBlock + threadIdx.x;
+  const bool valid = i < nblobs;
+  const int64_t gi = valid ? i : 0;
+  int lo = 0, hi = nbodies - 1;   // own body: the last b with offs[b] <= gi
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= gi) lo = mid; else hi = mid - 1;
+  }
+  const int own = lo;
+  const double xi = r[3 * gi], yi = r[3 * gi + 1], z0 = r[3 * gi + 2];
+  const double zi = (wall && !(z0 > a)) ? a : z0;
+  const double Bi = (wall && z0 < a) ? z0 / a : 1.0;
+  const double inva = 1.0 / a, invR = 1.0 / R;
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  for (int c0 = 0; c0 < nbodies; c0 += kPfBlock) {
+    const int cnt = nbodies - c0 < kPfBlock ? nbodies - c0 : kPfBlock;
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+      const int c = c0 + threadIdx.x;
+      const double zc = centres[3 * c + 2];
+      const double Bc = (wall && zc < R) ? zc / R : 1.0;
+      s_x[threadIdx.x] = centres[3 * c];
+      s_y[threadIdx.x] = centres[3 * c + 1];
+      s_z[threadIdx.x] = (wall && !(zc > R)) ? R : zc;
+      const double k = norm * Bc;
+      s_f[0][threadIdx.x] = k * force[c * fstride];
+      s_f[1][threadIdx.x] = k * force[c * fstride + 1];
+      s_f[2][threadIdx.x] = k * force[c * fstride + 2];
+    }
+    __syncthreads();
+    for (int k = 0; k < cnt; ++k) {
+      if (c0 + k == own) continue;
+      const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
+      const double Rz = zi + s_z[k];
+      const CoefRadii<double> co = coef_radii<double>(dx, dy, dz, Rz, zi, s_z[k], a, R, inva, invR, wall != 0);
+      const Vec3<double> v = apply_radii<double>(dx, dy, dz, Rz, co, s_f[0][k], s_f[1][k], s_f[2][k]);
+      ux += v.x; uy += v.y; uz += v.z;
+    }
+  }
+  if (valid) {
+    out[3 * i] = Bi * ux;
+    out[3 * i + 1] = Bi * uy;
+    out[3 * i + 2] = Bi * uz;
+  }
+}
+
 // Preconditioner apply: y = P^-1 x, one workgroup per body.
 __global__ __launch_bounds__(kRigidBlock) void pc_apply_kernel(const int64_t* __restrict__ offs,
                                                                const int64_t* __restrict__ moff,
@@ -930,6 +998,27 @@ extern "C" int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, co
       hipLaunchKernelGGL(pc_build_kernel<false>, dim3(nbodies), dim3(kRigidBlock), 0, st, p);
       RMW_CHECK_HIP(hipGetLastError());
     }
+  });
+}
+
+extern "C" int rmw_rigid_point_field(const double* r_blobs, const double* centres, const double* force,
+                                     int64_t force_stride, const int64_t* body_offsets, int nbodies, double radius,
+                                     double a, double eta, int wall, double* velocity, void* workspace,
+                                     size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(r_blobs && centres && force && velocity, "null pointer");
+    RMW_REQUIRE(force_stride >= 3, "force_stride must be >= 3");
+    RMW_REQUIRE(radius > 0.0 && a > 0.0 && eta > 0.0, "radius, a and eta must be positive");
+    if (L.nblobs == 0) return;
+    const char* ws = (const char*)workspace;
+    const unsigned grid = (unsigned)((L.nblobs + kPfBlock - 1) / kPfBlock);
+    hipLaunchKernelGGL(point_field_kernel, dim3(grid), dim3(kPfBlock), 0, (hipStream_t)stream, r_blobs, L.nblobs,
+                       (const int64_t*)(ws + L.off_offs), nbodies, centres, force, force_stride, radius, a,
+                       1.0 / (8.0 * M_PI * eta), wall, velocity);
+    RMW_CHECK_HIP(hipGetLastError());
   });
 }
 

@@ -1,12 +1,14 @@
 """Learned GMRES warm start: a per-body graph network (BF16 MFMA kernels).
 
 For every rigid body b the network predicts the part of the solution that
-the block-diagonal preconditioner misses, in the body's own frame:
+a base guess misses, in the body's own frame:
 
-    x0 = P^-1 b + R_b (network output) * fscale
+    x0 = base + R_b (network output) * fscale
 
-(P^-1 b is the exact solution of the isolated-body problem; the network
-adds the correction due to the 8 nearest neighbours.)  The prediction is
+(the base is P^-1 b, the exact solution of the isolated-body problem, or
+the point-field block-Jacobi step P^-1 (b - [u; 0]) that adds the flow of
+the other bodies' net forces; the network adds the correction due to the
+8 nearest neighbours.)  The prediction is
 only the GMRES initial guess: GMRES to the same tolerance remains the judge,
 and a non-finite prediction is discarded (x0 = 0).
 
@@ -256,7 +258,7 @@ class GraphFeatures:
         return Batch(Xn=Xn, Xe=Xe)
 
     def targets(self, quats, offs_dev, nblobs, sol, pc, raw: bool = False) -> torch.Tensor:
-        """Body-frame (sol - pc)/fscale, standardised with the net's mean/sd
+        """Body-frame (sol - base)/fscale (base passed as `pc`), standardised with the net's mean/sd
         (raw=True: mean 0, sd 1)."""
         nb = quats.shape[0]
         T = torch.empty((nb, self.net.nout), dtype=torch.float32, device=self.dev)
@@ -268,7 +270,7 @@ class GraphFeatures:
         return T
 
     def initial_guess(self, quats, offs_dev, nblobs, pred: torch.Tensor, pc: torch.Tensor) -> torch.Tensor:
-        """x0 = pc + R_b (pred sd + mean) fscale."""
+        """x0 = base + R_b (pred sd + mean) fscale (base passed as `pc`)."""
         nb = quats.shape[0]
         x0 = torch.empty_like(pc)
         _lib.check(self.lib.rmw_gnn_body_frame(quats.data_ptr(), offs_dev.data_ptr(), nb, nblobs, self.net.nbb,

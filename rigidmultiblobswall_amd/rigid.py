@@ -158,6 +158,32 @@ class RigidSuspension:
                                                self.pc_r_rel.data_ptr(), x.data_ptr(), out.data_ptr(),
                                                self.ws.data_ptr(), self.ws.numel(), _stream()))
 
+    # -- initial guesses -----------------------------------------------------------------
+    def point_field(self, force_torque, radius: float = 0.9, wall: bool = True) -> torch.Tensor:
+        """(Nblobs, 3) velocity at every blob from the net forces of the other
+        bodies, each a sphere of `radius` at its centre (different-radii
+        wall RPY)."""
+        ft = torch.as_tensor(force_torque, dtype=torch.float64).to(self.device).reshape(self.nbodies, -1).contiguous()
+        if ft.shape[1] < 3:
+            raise ValueError("force_torque must hold at least the three force components per body")
+        u = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.rmw_rigid_point_field(self.r_blobs.data_ptr(), self.centres.data_ptr(), ft.data_ptr(),
+                                                  ft.shape[1], self._offp(), self.nbodies, float(radius), self.a,
+                                                  self.eta, int(bool(wall)), u.data_ptr(), self.ws.data_ptr(),
+                                                  self.ws.numel(), _stream()))
+        return u
+
+    def point_field_guess(self, rhs: torch.Tensor, force_torque, radius: float = 0.9) -> torch.Tensor:
+        """Block-Jacobi step from P^-1 b with the other bodies as point
+        forces: x = P^-1 (b - [u; 0]), u = point_field(F).  Costs one
+        bodies-to-blobs product and a preconditioner application instead of
+        a blob-blob product."""
+        b = rhs.clone()
+        b[: 3 * self.Nblobs] -= self.point_field(force_torque, radius).reshape(-1)
+        x = torch.empty_like(rhs)
+        self.pc_apply(b, x)
+        return x
+
     # -- solve ------------------------------------------------------------------------
     def rhs(self, slip, force_torque) -> torch.Tensor:
         """[slip; -force_torque] as in solve_mobility_problem (no constraints)."""

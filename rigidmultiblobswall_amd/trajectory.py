@@ -93,6 +93,7 @@ class Snapshot:
     pc: torch.Tensor           # P^-1 b on the device
     iterations: int
     solve_s: float
+    base: Optional[torch.Tensor] = None   # point-field block-Jacobi guess P^-1 (b - [u; 0])
 
 
 @dataclass
@@ -102,7 +103,8 @@ class Trajectory:
 
 
 def run_c4_trajectory(centres, quats, ref, a, eta, nsteps=200, dt=1e-3, kT=0.1, seed=4, tol=1e-8,
-                      fp: Optional[ForceParams] = None, precision="fp64", log=None) -> Trajectory:
+                      fp: Optional[ForceParams] = None, precision="fp64", log=None,
+                      field_radius: float = 0.9) -> Trajectory:
     fp = fp or ForceParams()
     rng = np.random.default_rng(seed + 1000)
     centres = np.array(centres, dtype=np.float64)
@@ -117,6 +119,7 @@ def run_c4_trajectory(centres, quats, ref, a, eta, nsteps=200, dt=1e-3, kT=0.1, 
         rhs = S.rhs(np.zeros((S.Nblobs, 3)), ft)
         pc = torch.empty_like(rhs)
         S.pc_apply(rhs, pc)
+        base = S.point_field_guess(rhs, ft, radius=field_radius)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = S.solve_mobility_problem(rhs=rhs, tol=tol, x0=prev)
@@ -124,7 +127,7 @@ def run_c4_trajectory(centres, quats, ref, a, eta, nsteps=200, dt=1e-3, kT=0.1, 
         ts = time.perf_counter() - t0
         if res.info != 0:
             raise RuntimeError(f"GMRES did not converge at step {step}")
-        traj.snapshots.append(Snapshot(centres.copy(), quats.copy(), ft, res.x.clone(), pc, res.iterations, ts))
+        traj.snapshots.append(Snapshot(centres.copy(), quats.copy(), ft, res.x.clone(), pc, res.iterations, ts, base))
         if log:
             log(f"step {step}: GMRES iterations {res.iterations}, {ts * 1e3:.1f} ms")
         prev = res.x

@@ -3,13 +3,17 @@
   python -m rigidmultiblobswall_amd.warmstart [--steps 200] [--out results.json]
 
 1. Brownian trajectory of 1024 42-blob spheres (C4, trajectory.py); every
-   step's FP64 GMRES solution and preconditioner guess P^-1 b are kept.
+   step's FP64 GMRES solution, the preconditioner guess P^-1 b and the
+   point-field guess P^-1 (b - [u; 0]) (u: velocity at each blob from the
+   other bodies' net forces) are kept.
 2. The graph network is trained (BF16 MFMA kernels, Adam) on the body-frame
-   correction (x - P^-1 b) of the first `train_frac` of the steps.
+   correction (x - point-field guess) of the first `train_frac` of the
+   steps.
 3. On held-out steps (and on a fresh configuration) the rigid solve to 1e-8
-   is timed for x0 in {0, previous step, P^-1 b, network}, each with FP64
-   GMRES and with mixed precision (FP32-pair inner GMRES, FP64 residual
-   refinement).  GMRES to the same FP64 tolerance judges every variant.
+   is timed for x0 in {0, previous step, P^-1 b, point field, network},
+   each with FP64 GMRES and with mixed precision (FP32-pair inner GMRES,
+   FP64 residual refinement); the guesses' own cost is in their times.
+   GMRES to the same FP64 tolerance judges every variant.
 """
 from __future__ import annotations
 
@@ -17,7 +21,7 @@ import argparse
 import json
 import math
 import time
-from typing import Dict, List
+from typing import Dict, List, Optional
 
 import numpy as np
 import torch
@@ -32,8 +36,14 @@ def _sync():
 
 
 class WarmStart:
-    def __init__(self, ref, a, eta, device=None, seed=0):
+    """The network's x0 = base + R_b (prediction) fscale, where the base is
+    the point-field block-Jacobi guess P^-1 (b - [u; 0]) (u: velocity at
+    each blob from the other bodies' net forces, spheres of `field_radius`)
+    and the network predicts the rest of the solution in each body's frame."""
+
+    def __init__(self, ref, a, eta, device=None, seed=0, field_radius: float = 0.9):
         self.ref, self.a, self.eta = ref, a, eta
+        self.field_radius = float(field_radius)
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         self.nbb = ref.shape[0]
         self.net = gnn.GraphNet(self.nbb, self.dev, seed=seed)
@@ -53,7 +63,12 @@ class WarmStart:
 
     def raw_targets(self, snap: Snapshot) -> torch.Tensor:
         nb = snap.centres.shape[0]
-        return self.feats.targets(self._dev(snap.quats), self._offs(nb), nb * self.nbb, snap.sol, snap.pc, raw=True)
+        return self.feats.targets(self._dev(snap.quats), self._offs(nb), nb * self.nbb, snap.sol, self._base(snap),
+                                  raw=True)
+
+    @staticmethod
+    def _base(snap: Snapshot) -> torch.Tensor:
+        return snap.base if snap.base is not None else snap.pc
 
     # -- training -----------------------------------------------------------------
     def fit(self, snaps: List[Snapshot], epochs=40, batch=2048, lr=2e-3, lr_min=2e-5, seed=0, log=None) -> Dict:
@@ -104,10 +119,11 @@ class WarmStart:
                 "steps": it, "parameters": self.net.nparam}
 
     # -- prediction ---------------------------------------------------------------
-    def predict_x0(self, snap: Snapshot) -> torch.Tensor:
+    def predict_x0(self, snap: Snapshot, base: Optional[torch.Tensor] = None) -> torch.Tensor:
         nb = snap.centres.shape[0]
         pred = self.net.forward(self.inputs(snap))
-        x0 = self.feats.initial_guess(self._dev(snap.quats), self._offs(nb), nb * self.nbb, pred, snap.pc)
+        x0 = self.feats.initial_guess(self._dev(snap.quats), self._offs(nb), nb * self.nbb, pred,
+                                      self._base(snap) if base is None else base)
         if not bool(torch.isfinite(x0).all().item()):
             return None   # discarded: GMRES starts from zero
         return x0
@@ -143,13 +159,17 @@ def evaluate(ws: WarmStart, snaps: List[Snapshot], prev_snaps: List, tol=1e-8, r
         xref = snap.sol
         _sync()
         t0 = time.perf_counter()
-        x_gnn = ws.predict_x0(snap)
+        x_pt = S.point_field_guess(rhs, snap.force_torque, radius=ws.field_radius)
+        _sync()
+        t_pt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        x_gnn = ws.predict_x0(snap, base=x_pt)
         _sync()
         t_gnn = time.perf_counter() - t0
-        guesses = {"zero": (None, 0.0), "pc": (snap.pc, 0.0), "gnn": (x_gnn, t_gnn)}
+        guesses = {"zero": (None, 0.0), "pc": (snap.pc, 0.0), "point": (x_pt, t_pt), "gnn": (x_gnn, t_pt + t_gnn)}
         if prev is not None:
             guesses["previous"] = (prev.sol, 0.0)
-        row = {"pc_build_ms": t_pc * 1e3, "gnn_inference_ms": t_gnn * 1e3}
+        row = {"pc_build_ms": t_pc * 1e3, "point_field_ms": t_pt * 1e3, "gnn_inference_ms": t_gnn * 1e3}
         for name, (x0, t_extra) in guesses.items():
             err = _rel(x0, xref) if x0 is not None else 1.0
             for mode in ("fp64", "mixed"):

@@ -211,8 +211,9 @@ def test_features_and_body_frame_round_trip(gpu):
 
 def test_small_warm_start_end_to_end(gpu):
     """64 bodies, 16 Brownian steps: the trained network's x0 beats the
-    preconditioner guess on held-out steps, and GMRES from it converges to
-    the same FP64 tolerance in no more iterations."""
+    preconditioner guess and its own point-field base on held-out steps, and
+    GMRES from it converges to the same FP64 tolerance in no more
+    iterations."""
     from rigidmultiblobswall_amd.trajectory import run_c4_trajectory
     from rigidmultiblobswall_amd.warmstart import WarmStart, evaluate
     w = workloads.c4_monolayer(nbodies=64, seed=11)
@@ -227,3 +228,6 @@ def test_small_warm_start_end_to_end(gpu):
         assert row["gnn/fp64"]["converged"] and row["zero/fp64"]["converged"] and row["gnn/mixed"]["converged"]
         assert row["gnn/fp64"]["x0_rel_err"] < row["pc/fp64"]["x0_rel_err"]
         assert row["gnn/fp64"]["iterations"] <= row["pc/fp64"]["iterations"]
+        # the point-field base beats P^-1 b, and the network improves on its base
+        assert row["point/fp64"]["x0_rel_err"] < row["pc/fp64"]["x0_rel_err"]
+        assert row["gnn/fp64"]["x0_rel_err"] < row["point/fp64"]["x0_rel_err"]

@@ -61,6 +61,55 @@ def test_body_blobs_operator_and_pc(gpu):
     assert np.abs(S.nbody[0].cpu().numpy() - Og.N[0]).max() <= 1e-10 * np.abs(Og.N[0]).max()
 
 
+@pytest.mark.parametrize("wall", [True, False])
+def test_point_field_matches_oracle(gpu, wall):
+    """Velocity at every blob from the net forces of all other bodies (each
+    a sphere of radius R at its centre) = the oracle's different-radii
+    product with the blob's own body left out; one centre below R (damped
+    source)."""
+    w = _small_spheres(nb=16, seed=31)
+    c = w["centres"].copy()
+    c[3, 2] = 1.1                       # below R = 1.2: the source is damped
+    R = 1.2
+    S = RigidSuspension(c, w["quats"], w["ref"], w["eta"], w["a"], precision="fp64")
+    rng = np.random.default_rng(31)
+    ft = rng.standard_normal((16, 6))
+    print(f"case point field 16 spheres wall={wall}", flush=True)
+    u = S.point_field(ft, radius=R, wall=wall).cpu().numpy()
+    rb = S.r_blobs.cpu().numpy()
+    ref = np.empty_like(rb)
+    for b in range(16):
+        others = np.arange(16) != b
+        ref[42 * b: 42 * (b + 1)] = O.radii_apply(c[others], rb[42 * b: 42 * (b + 1)], ft[others, :3],
+                                                  np.full(15, R), np.full(42, w["a"]), w["eta"],
+                                                  wall=wall).reshape(42, 3)
+    assert np.abs(u - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_point_field_guess_is_a_block_jacobi_step(gpu):
+    """P^-1 (b - [u_point; 0]) against P^-1 b and the exact block-Jacobi step
+    P^-1 (2 b - A P^-1 b): most of the exact step's gain without the blob
+    product."""
+    w = workloads.c4_monolayer(nbodies=64, seed=5)
+    S = RigidSuspension(w["centres"], w["quats"], w["ref"], w["eta"], w["a"], precision="fp64")
+    rng = np.random.default_rng(5)
+    ft = np.column_stack([rng.normal(0, 0.1, (64, 2)), rng.normal(-0.8, 0.1, 64), rng.normal(0, 0.1, (64, 3))])
+    rhs = S.rhs(np.zeros((S.Nblobs, 3)), ft)
+    print("case point-field guess, 64-sphere monolayer", flush=True)
+    x = S.solve_mobility_problem(rhs=rhs, tol=1e-10).x
+    pc = torch.empty_like(rhs)
+    S.pc_apply(rhs, pc)
+    Apc = torch.empty_like(rhs)
+    S.operator(pc, Apc)
+    ex = torch.empty_like(rhs)
+    S.pc_apply(2 * rhs - Apc, ex)
+    pt = S.point_field_guess(rhs, ft, radius=0.9)
+    err = lambda v: float(torch.linalg.norm(v - x) / torch.linalg.norm(x))   # noqa: E731
+    print(f"  errors: P^-1 b {err(pc):.3f}, exact block-Jacobi {err(ex):.3f}, point field {err(pt):.3f}")
+    assert err(pt) < 0.8 * err(pc)
+    assert (err(pc) - err(pt)) >= 0.5 * (err(pc) - err(ex))    # at least half the exact step's gain
+
+
 def test_preconditioner_ragged_bodies(gpu):
     """Bodies of 1, 7, 16, 42 and 50 blobs in one suspension: the register
     Gauss-Jordan build (up to 45 blobs, the elements of the largest body per
