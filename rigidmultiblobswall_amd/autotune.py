@@ -132,11 +132,13 @@ def lookup(mask: int, precision: str, k: int, pairs: float, ragged: bool) -> Opt
 
 def chunk_candidates(nmax: int, max_chunks: int = 48) -> List[int]:
     """Chunk lengths to try (explicit, so a cached entry does not depend on
-    the library's built-in rule): powers of two from 256 and every distinct
-    length ceil(n / c) rounded to 64 for c = 1..max_chunks chunks."""
+    the library's built-in rule): 64 and 128 (for the symmetric kernel a
+    quarter or a half of a tile pair per work item), powers of two from 256
+    and every distinct length ceil(n / c) rounded to 64 for c = 1..max_chunks
+    chunks."""
     q = 64
     top = max(q, (nmax + q - 1) // q * q)
-    out = set()
+    out = {c for c in (64, 128) if c < top}
     c = 256
     while c < top:
         out.add(c)

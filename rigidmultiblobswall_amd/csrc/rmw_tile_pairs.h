@@ -38,6 +38,29 @@ __device__ __forceinline__ double rol1(double v)
   return __hiloint2double(hi, lo);
 }
 
+// Work items counted in runs of C 64-source blocks (C = 1, 2 or a multiple
+// of 4; four blocks to a tile): target tile t has ceil(4 (n_t - t) / C)
+// runs of blocks from its own tile on.  sum_{m=1}^{M} ceil(4 m / C):
+__host__ __device__ __forceinline__ int64_t block_ceil_sum(int64_t M, int64_t C)
+{
+  return C % 4 == 0 ? ceil_sum(M, C / 4) : (4 / C) * M * (M + 1) / 2;
+}
+
+__host__ __device__ __forceinline__ int64_t items_before_blocks(int64_t T, int64_t n_t, int64_t C)
+{
+  return block_ceil_sum(n_t, C) - block_ceil_sum(n_t - T, C);
+}
+
+__device__ __forceinline__ int64_t pair_target_tile_blocks(int64_t local, int64_t n_t, int64_t C)
+{
+  int64_t lo = 0, hi = n_t - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (items_before_blocks(mid, n_t, C) <= local) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ int64_t pair_target_tile(int64_t local, int64_t n_t, int64_t C)
 {
   int64_t lo = 0, hi = n_t - 1;

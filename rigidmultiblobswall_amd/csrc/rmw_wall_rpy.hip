@@ -473,7 +473,7 @@ struct MutParams {
   const MutSys* table;      // nullptr -> single
   MutSys single;
   int nsys;
-  int64_t n_items, chunk_tiles;
+  int64_t n_items, chunk_blocks;   // work item: a run of chunk_blocks 64-source blocks
 };
 
 __device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t item)
@@ -509,9 +509,10 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
   for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
     const MutSys Sy = mut_sys_of_item(p, item);
     const int64_t local = item - Sy.item0;
-    const int64_t T = pair_target_tile(local, Sy.n_t, p.chunk_tiles);
-    const int64_t S0 = T + (local - items_before(T, Sy.n_t, p.chunk_tiles)) * p.chunk_tiles;
-    const int64_t S1 = S0 + p.chunk_tiles < Sy.n_t ? S0 + p.chunk_tiles : Sy.n_t;
+    const int64_t T = pair_target_tile_blocks(local, Sy.n_t, p.chunk_blocks);
+    const int64_t blk0 = 4 * T + (local - items_before_blocks(T, Sy.n_t, p.chunk_blocks)) * p.chunk_blocks;
+    const int64_t blk1 = blk0 + p.chunk_blocks < 4 * Sy.n_t ? blk0 + p.chunk_blocks : 4 * Sy.n_t;
+    const int64_t S0 = blk0 >> 2, S1 = (blk1 + 3) >> 2;
     const int64_t i = T * kBlock + il;  // within the system
     const bool valid = i < Sy.n;
     const int64_t gi = Sy.b0 + (valid ? i : 0);
@@ -567,7 +568,9 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       // the masked one on the same device.
       constexpr bool kUnmasked = DOM == kDomNoWall || DOM == kDomFreeSurface;
       const bool full = kUnmasked && !diag && cnt == kBlock && (T + 1) * kBlock <= Sy.n;
-      for (int b = 0; b < kBlock / 64 && 64 * b < cnt; ++b) {
+      const int bb0 = S == S0 ? (int)(blk0 - 4 * S) : 0;
+      const int bb1 = blk1 - 4 * S < kBlock / 64 ? (int)(blk1 - 4 * S) : kBlock / 64;
+      for (int b = bb0; b < bb1 && 64 * b < cnt; ++b) {
         JT ja[K][OC];
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -822,7 +825,7 @@ struct MutReduceParams {
   int nsys;
   int mask;
   int K;
-  int64_t ntot, chunk_tiles;
+  int64_t ntot, chunk_blocks;
   int64_t rhs_stride;
   double a, inva, c1, c2;
   int domain;
@@ -851,7 +854,8 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
   }
   const int64_t il = gi - Sy.b0;
   const int64_t X = il / kBlock, l = il - X * kBlock;
-  const int64_t it0 = items_before(X, Sy.n_t, p.chunk_tiles), it1 = items_before(X + 1, Sy.n_t, p.chunk_tiles);
+  const int64_t it0 = items_before_blocks(X, Sy.n_t, p.chunk_blocks);
+  const int64_t it1 = items_before_blocks(X + 1, Sy.n_t, p.chunk_blocks);
   const int64_t step = (int64_t)p.K * kBlock * OC;
   const double* Pi = p.part_i + (((Sy.pi0 + it0) * p.K + k) * kBlock + l) * OC + c;
   const double* Pj = p.part_j + (((Sy.pj0 + tri_index(X, 0)) * p.K + k) * kBlock + l) * OC + c;
@@ -888,7 +892,7 @@ struct Plan {
   // symmetric path (UF or all four blocks, no periodic images)
   bool mut_ok = false;
   int kmax_mut = 1;
-  int64_t mut_items = 0, chunk_tiles = 1;
+  int64_t mut_items = 0, chunk_blocks = 4;   // symmetric work item: chunk_blocks 64-source blocks
   std::vector<MutSys> msys;  // non-empty systems only
   size_t off_pi = 0, off_pj = 0, off_mtable = 0;
 };
@@ -1047,10 +1051,13 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
     const int rc = rhs_cap(tuning);
     pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 2) : 2) : 1;
     if (tuning && tuning->chunk_len > 0) {
-      pl.chunk_tiles = std::max<int64_t>(1, (tuning->chunk_len + kBlock - 1) / kBlock);
+      // 64 or 128 sources: a quarter or half of a tile pair per work item
+      // (finer work for suspensions of few tiles); else whole source tiles
+      const int64_t blocks = (tuning->chunk_len + 63) / 64;
+      pl.chunk_blocks = blocks <= 2 ? blocks : (blocks + 3) / 4 * 4;
     } else {
       const DeviceProps& dp = device_props();
-      pl.chunk_tiles = std::max<int64_t>(1, tile_pairs / (16 * (int64_t)std::max(dp.cu_count, 1)));
+      pl.chunk_blocks = 4 * std::max<int64_t>(1, tile_pairs / (16 * (int64_t)std::max(dp.cu_count, 1)));
     }
     int64_t item = 0, pi = 0, pj = 0;
     for (int s = 0; s < nsys; ++s) {
@@ -1062,7 +1069,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
       M.item0 = item;
       M.pi0 = pi;
       M.pj0 = pj;
-      const int64_t its = items_before(M.n_t, M.n_t, pl.chunk_tiles);
+      const int64_t its = items_before_blocks(M.n_t, M.n_t, pl.chunk_blocks);
       item += its;
       pi += its;
       pj += M.n_t * (M.n_t + 1) / 2;
@@ -1322,7 +1329,7 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     q.single = pl.msys[0];
     q.nsys = (int)pl.msys.size();
     q.n_items = pl.mut_items;
-    q.chunk_tiles = pl.chunk_tiles;
+    q.chunk_blocks = pl.chunk_blocks;
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st);
@@ -1343,7 +1350,7 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     rp.mask = pl.mask;
     rp.K = K;
     rp.ntot = pl.ntot;
-    rp.chunk_tiles = pl.chunk_tiles;
+    rp.chunk_blocks = pl.chunk_blocks;
     rp.rhs_stride = stride;
     rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
     rp.domain = pl.domain;
@@ -1487,7 +1494,7 @@ extern "C" int rmw_wall_rpy_describe(int mask, int prec, int64_t n, int k, const
     const bool mut = pl.mut_ok && !periodic;
     info->symmetric = mut ? 1 : 0;
     info->rhs_group = mut ? pl.kmax_mut : pl.kmax;
-    info->chunk_len = mut ? pl.chunk_tiles * kBlock : (pl.sys.empty() ? 0 : pl.sys[0].chunk_len);
+    info->chunk_len = mut ? pl.chunk_blocks * 64 : (pl.sys.empty() ? 0 : pl.sys[0].chunk_len);
     info->work_items = mut ? pl.mut_items : pl.total_items;
     info->workspace_bytes = (int64_t)pl.total;
   });

@@ -38,6 +38,7 @@ def test_cache_round_trip_and_nearest(tmp_path):
 def test_candidates():
     ch = autotune.chunk_candidates(10752)
     assert 0 not in ch and 256 in ch and 10752 in ch and 448 in ch   # 448 = 10752 / 24
+    assert 64 in ch and 128 in ch                                    # sub-tile symmetric items
     assert all(c % 64 == 0 for c in ch)
     assert autotune.chunk_candidates(100) == [64, 128]
     cs = autotune.candidates(nmax=10752, k=1, cu=256, ragged=False)

@@ -60,7 +60,8 @@ def test_symmetric_k_rhs(gpu, prec, k):
         assert rel(sym[j], one) <= TOL[prec], j
 
 
-@pytest.mark.parametrize("chunk,wgs", [(256, 0), (512, 0), (768, 0), (1024, 0), (256, 7), (512, 3)])
+@pytest.mark.parametrize("chunk,wgs", [(64, 0), (128, 0), (192, 0), (256, 0), (512, 0), (768, 0), (1024, 0),
+                                       (64, 5), (256, 7), (512, 3)])
 def test_symmetric_work_split(gpu, chunk, wgs):
     n = 1900                                       # 8 tiles, last one ragged
     r = _geom(n, 9)
@@ -174,6 +175,10 @@ def test_symmetric_ragged_matches_ordered_and_oracle(gpu, prec, mask):
     split = run(1, 512, 5)                      # multi-tile items, small persistent grid
     for a_, b_ in zip(split, ordered):
         assert rel(a_, b_) <= TOL[prec]
+    for chunk in (64, 128):                     # quarter- and half-tile items
+        fine = run(1, chunk, 0)
+        for a_, b_ in zip(fine, ordered):
+            assert rel(a_, b_) <= TOL[prec], chunk
     # per system against the oracle
     for s in range(len(SIZES_R)):
         b0, b1 = int(off[s]), int(off[s + 1])
@@ -186,6 +191,24 @@ def test_symmetric_ragged_matches_ordered_and_oracle(gpu, prec, mask):
             ow = O.block_apply(O.WF, rr, f[b0:b1], 0.9, 0.3) + O.block_apply(O.WT, rr, t[b0:b1], 0.9, 0.3)
             assert rel(sym[1][b0:b1].ravel(), ow) <= (1e-12 if prec == "fp64" else 2e-6), s
         assert rel(sym[0][b0:b1].ravel(), ou) <= (1e-12 if prec == "fp64" else 2e-6), s
+
+
+@pytest.mark.parametrize("chunk", [64, 128, 256])
+def test_symmetric_sub_tile_items_c2(gpu, chunk):
+    """The rigid-solve geometry (42 full tiles) with work items of a quarter,
+    a half and a whole tile pair: the same product as the ordered kernel."""
+    w = workloads.c2_rigid_spheres()
+    r = w["r"]
+    f = np.random.default_rng(61).standard_normal(r.shape)
+    print(f"case symmetric C2 chunk={chunk}", flush=True)
+    base = _uf(r, f, "fp64", 1, -1)
+    got = _uf(r, f, "fp64", 1, 1, chunk, 0)
+    assert rel(got, base) <= 1e-13
+    g32 = _uf(r, f, "fp32_blocked", 1, 1, chunk, 0)
+    assert rel(g32, base) <= 1e-6
+    d = wall_rpy.describe(RMW_UF, r.shape[0], 1, "fp32_blocked", tuning=_lib.Tuning(chunk, 0, 0, 1, 0))
+    assert d["kernel"] == "symmetric" and d["chunk_len"] == chunk
+    assert d["work_items"] == 903 * 256 // chunk
 
 
 def test_symmetric_c5_matches_ordered(gpu):
