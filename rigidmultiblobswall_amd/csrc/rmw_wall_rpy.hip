@@ -831,50 +831,62 @@ struct MutReduceParams {
   int domain;
 };
 
+// Output o (component c of blob gi, RHS k) sums the target-side slots of the
+// work items of its tile and the source-side slots (X, T), T <= X.  Eight
+// groups of 32 threads share a block's 32 outputs: group g takes slots
+// g, g + 8, ... of both lists (consecutive threads read consecutive
+// outputs of one slot), then group 0 adds the eight partial sums in order.
+constexpr int kRedGroups = 8, kRedOut = 256 / kRedGroups;
+
 __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
 {
+  __shared__ double part[kRedGroups][kRedOut];
   const bool HU = (p.mask & (kUF | kUT)) != 0;
   const bool HW = (p.mask & (kWF | kWT)) != 0;
   const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = threadIdx.x / kRedOut, ol = threadIdx.x - g * kRedOut;
+  const int64_t tid = (int64_t)blockIdx.x * kRedOut + ol;
   const int64_t per_k = p.ntot * OC;
-  if (tid >= per_k * p.K) return;
-  const int k = (int)(tid / per_k);
-  const int64_t rem = tid - (int64_t)k * per_k;
-  const int64_t gi = rem / OC;
-  const int c = (int)(rem - gi * OC);
-  MutSys Sy = p.single;
-  if (p.table) {
-    int lo = 0, hi = p.nsys - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (p.table[mid].b0 <= gi) lo = mid; else hi = mid - 1;
+  const bool valid = tid < per_k * p.K;
+  double sum = 0.0;
+  int k = 0;
+  int64_t gi = 0;
+  int c = 0;
+  if (valid) {
+    k = (int)(tid / per_k);
+    const int64_t rem = tid - (int64_t)k * per_k;
+    gi = rem / OC;
+    c = (int)(rem - gi * OC);
+    MutSys Sy = p.single;
+    if (p.table) {
+      int lo = 0, hi = p.nsys - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.table[mid].b0 <= gi) lo = mid; else hi = mid - 1;
+      }
+      Sy = p.table[lo];
     }
-    Sy = p.table[lo];
+    const int64_t il = gi - Sy.b0;
+    const int64_t X = il / kBlock, l = il - X * kBlock;
+    const int64_t it0 = items_before_blocks(X, Sy.n_t, p.chunk_blocks);
+    const int64_t it1 = items_before_blocks(X + 1, Sy.n_t, p.chunk_blocks);
+    const int64_t step = (int64_t)p.K * kBlock * OC;
+    const double* Pi = p.part_i + (((Sy.pi0 + it0) * p.K + k) * kBlock + l) * OC + c;
+    const double* Pj = p.part_j + (((Sy.pj0 + tri_index(X, 0)) * p.K + k) * kBlock + l) * OC + c;
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t m = g; m < it1 - it0; m += kRedGroups) s0 += Pi[m * step];
+    for (int64_t m = g; m <= X; m += kRedGroups) s1 += Pj[m * step];
+    sum = s0 + s1;
   }
-  const int64_t il = gi - Sy.b0;
-  const int64_t X = il / kBlock, l = il - X * kBlock;
-  const int64_t it0 = items_before_blocks(X, Sy.n_t, p.chunk_blocks);
-  const int64_t it1 = items_before_blocks(X + 1, Sy.n_t, p.chunk_blocks);
-  const int64_t step = (int64_t)p.K * kBlock * OC;
-  const double* Pi = p.part_i + (((Sy.pi0 + it0) * p.K + k) * kBlock + l) * OC + c;
-  const double* Pj = p.part_j + (((Sy.pj0 + tri_index(X, 0)) * p.K + k) * kBlock + l) * OC + c;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  const int64_t ni = it1 - it0;
-  int64_t m = 0;
-  for (; m + 2 <= ni; m += 2) {
-    s0 += Pi[m * step];
-    s1 += Pi[(m + 1) * step];
+  part[g][ol] = sum;
+  __syncthreads();
+  if (g == 0 && valid) {
+    double s = part[0][ol];
+#pragma unroll
+    for (int q = 1; q < kRedGroups; ++q) s += part[q][ol];
+    finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, s, p.rhs_stride, p.a, p.inva, p.c1, p.c2,
+                  p.domain);
   }
-  if (m < ni) s0 += Pi[m * step];
-  for (m = 0; m + 2 <= X + 1; m += 2) {
-    s2 += Pj[m * step];
-    s3 += Pj[(m + 1) * step];
-  }
-  if (m < X + 1) s2 += Pj[m * step];
-  const double sum = (s0 + s1) + (s2 + s3);
-  finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, sum, p.rhs_stride, p.a, p.inva, p.c1, p.c2,
-                p.domain);
 }
 
 // ---------------------------------------------------------------------------
@@ -1355,8 +1367,8 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
     rp.domain = pl.domain;
     const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
-    const int64_t nthreads = pl.ntot * K * OC;
-    hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
+    const int64_t nout = pl.ntot * K * OC;
+    hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nout + kRedOut - 1) / kRedOut)), dim3(256), 0, st, rp);
     RMW_CHECK_HIP(hipGetLastError());
   }
 }
