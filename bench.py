@@ -19,10 +19,12 @@ Prints ONE JSON line (rank 0):
                GMRES from x0 = 0 against the learned warm start (network
                trained on the GPU from a short trajectory), per-solve times
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step|c2step]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step|c2step|c3lz|c5lz]
        (c3r: the different-radii product on the C3 geometry; c3f: the
        blob-blob soft-repulsion forces on it; c3step: whole rollers time
-       steps, steps/s; c2step: rigid-body time steps of the C2 suspension)
+       steps, steps/s; c2step: rigid-body time steps of the C2 suspension;
+       c3lz / c5lz: Lanczos Brownian-slip solves of the C3 block of 4 RHS
+       and of the C5 ragged batch, solves/s)
 """
 from __future__ import annotations
 
@@ -281,6 +283,110 @@ def rollers_step_bench(args, log):
     return 0
 
 
+def lanczos_bench(args, log, ragged: bool):
+    """One step = one Brownian-slip solve factor N^1/2 z by Lanczos to the
+    tolerance 1e-3 with the grand (trans + rot) mobility: the C3 block of 4
+    right-hand sides on 32768 rollers sharing each fused grand product
+    (ragged=False), or one right-hand side per replica of the C5 batch (64
+    replicas, 128-4096 rollers each) in one persistent ragged launch per
+    product (ragged=True)."""
+    import torch
+    from rigidmultiblobswall_amd import wall_rpy, workloads
+    from rigidmultiblobswall_amd.rollers import RaggedRollers, Rollers
+    prec = args.precision
+    if ragged:
+        w = workloads.c5_ragged_rollers()
+        off = w["offsets"]
+        n = int(off[-1])
+        Z = torch.from_numpy(np.random.default_rng(9).standard_normal((1, 6 * n))).cuda()
+        R = RaggedRollers(w["r"], off, w["eta"], w["a"], precision=prec)
+        pairs = float(np.sum(np.diff(off).astype(np.float64) ** 2))
+        k = 1
+        desc = (f"C5 ragged batch: {len(off) - 1} roller replicas of 128-4096 blobs ({n} blobs), a=0.656, eta=1e-3, "
+                f"grand mobility, 1 RHS per replica, Lanczos tol 1e-3")
+    else:
+        w = workloads.c3_rollers()
+        n = w["r"].shape[0]
+        Z = torch.from_numpy(w["z"]).cuda()
+        R = Rollers(w["r"], w["eta"], w["a"], precision=prec)
+        pairs = float(n) * n
+        k = Z.shape[0]
+        desc = (f"C3 rollers: N={n}, a=0.656, eta=1e-3, grand mobility (trans + rot), block of {k} RHS sharing "
+                f"each product, Lanczos tol 1e-3")
+    solve = lambda: R.brownian_noise(Z, factor=1.0, tolerance=1e-3)   # noqa: E731
+    for _ in range(args.warmup):
+        solve()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with wall_rpy.PairKernelTimer() as pk:
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(args.steps):
+            _, its = solve()
+        stop.record()
+        torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    nw, tw, mw = pk.by_kind["wall_rpy"]
+    # algorithmic FLOP of one k-RHS grand product (314 + 72 k per pair)
+    flop_product = (314 + 72 * k) * pairs
+    launches_per_product = 1
+    if not ragged:
+        d = wall_rpy.describe(15, n, k, prec)
+        launches_per_product = max(1, -(-k // max(1, d["rhs_group"])))
+    nprod = nw / launches_per_product
+    achieved = flop_product * nprod / (tw * 1e-3) / 1e12 if tw else 0.0
+    peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
+    its_np = np.asarray(its.cpu().numpy() if torch.is_tensor(its) else its)
+    log(f"lanczos: {ms:.3f} ms per solve, {nprod / args.steps:.1f} products per solve, pair kernels "
+        f"{tw / args.steps:.3f} ms per solve")
+    out = {
+        "metric": ("Lanczos Brownian slip solves/s (C5 ragged batch)" if ragged else
+                   "Lanczos Brownian slip solves/s (block of 4 RHS, 32768 rollers)"),
+        "value": 1e3 / ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f32", "data": "synthetic",
+        "config": {"workload": desc, "blobs": n, "pairs_per_product": pairs,
+                   "precision": prec + PRECISION_NOTE.get(prec, ""),
+                   "per_solve": {"grand_products": nprod / args.steps, "pair_launches": nw / args.steps,
+                                 "pair_kernel_ms": tw / args.steps,
+                                 "lanczos_iterations_max": int(its_np.max()) if its_np.size else 0}},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "wall-RPY grand pair kernel" + (" (ragged)" if ragged else f" ({k} RHS)"),
+                     "kernel_mean_ms": mw, "kernel_launches": nw, "flop_per_pair": 314 + 72 * k},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = O.num_threads("fast")
+        r_h = w["r"]
+        nn = r_h.shape[0]
+        f_h = np.random.default_rng(3).standard_normal((nn, 3))
+        # bounded sample: the oracle's four grand blocks (one RHS) on a row block of the
+        # largest system (C3: the whole suspension), scaled to one solve's products x RHS
+        r_s = r_h if not ragged else r_h[int(off[np.argmax(np.diff(off))]): int(off[np.argmax(np.diff(off)) + 1])]
+        ns = r_s.shape[0]
+        f_s = f_h[:ns]
+        rows = min(ns, 512)
+        t = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t < 10.0:
+            for blk in (O.UF, O.UT, O.WF, O.WT):
+                O.block_apply(blk, r_s, f_s, w["eta"], w["a"], rows=(0, rows), kind="fast")
+            reps += 1
+        dt_cpu = time.perf_counter() - t
+        rate = reps * rows * ns / dt_cpu          # grand pairs/s, one RHS
+        sec = nprod / args.steps * k * pairs / rate
+        out["cpu_baseline"] = {"value": 1.0 / sec, "unit": "solves/s", "cores": threads, "kind": "port",
+                               "sample": f"{reps} x the four grand blocks on rows 0..{rows} of {ns} blobs (one RHS), "
+                                         f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads, {dt_cpu:.1f} s: "
+                                         f"{rate:.3g} grand pairs/s, scaled to the solve's "
+                                         f"{nprod / args.steps:.1f} products x {k} RHS (Lanczos algebra not timed)",
+                               "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+        log(f"cpu baseline: {rate:.3g} grand pairs/s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def rigid_step_bench(args, log):
     """One step = one deterministic forward Euler step of the rigid-body
     integrator on the C2 suspension (256 spheres x 42 blobs) with gravity,
@@ -414,6 +520,8 @@ def main(argv=None):
         return rollers_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
     if args.workload == "c2step":
         return rigid_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
+    if args.workload in ("c3lz", "c5lz"):
+        return lanczos_bench(args, lambda m: print(m, file=sys.stderr, flush=True), ragged=args.workload == "c5lz")
     dev = torch.device("cuda", 0)
     w, desc = load_workload(args.workload)
     r_h, eta, a = w["r"], w["eta"], w["a"]

@@ -154,7 +154,7 @@ def candidates(nmax: int, k: int, cu: int, ragged: bool, symmetric_possible: boo
     per CU, plus 16 per CU for ragged batches) x RHS groups x (where the
     symmetric UF kernel applies) symmetric on / off."""
     wgs = [0, 4 * cu, 8 * cu] + ([16 * cu] if ragged else [])
-    groups = ([0] + [g for g in (1, 2) if g < min(k, 4)]) if k > 1 else [0]
+    groups = ([0] + [g for g in (1, 2) if g < min(k, 4)] + ([4] if k >= 4 else [])) if k > 1 else [0]
     syms = [0, -1] if symmetric_possible else [0]
     return [{"max_wgs": w, "rhs_group": g, "symmetric": sy} for w in wgs for g in groups for sy in syms]
 

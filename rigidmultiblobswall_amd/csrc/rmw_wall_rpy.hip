@@ -1059,9 +1059,10 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   if (pl.mut_ok && sym == 0) pl.mut_ok = tile_pairs >= device_props().cu_count;
   size_t mut_total = 0;
   if (pl.mut_ok) {
-    // at most 2 right-hand sides per pass (LDS of the source-side sums, registers)
+    // 2 right-hand sides per pass unless the tuning asks for 4 (the source-side
+    // sums of 4 take 49 KB of LDS and ~220 VGPRs: two workgroups per CU)
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 2) : 2) : 1;
+    pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 4) : 2) : 1;
     if (tuning && tuning->chunk_len > 0) {
       // 64 or 128 sources: a quarter or half of a tile pair per work item
       // (finer work for suspensions of few tiles); else whole source tiles
@@ -1260,16 +1261,20 @@ static void dispatch_mutual_m(int prec, int K, const MutParams& q, int grid, hip
 {
   const dim3 g(grid), b(kBlock);
   if (prec == RMW_PREC_F32_BLOCKED) {
-    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
+    if (K == 4) hipLaunchKernelGGL((mutual_kernel<MASK, 4, RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
+    else if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
     else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
   } else if (prec == RMW_PREC_F32_F64ACC) {
-    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
+    if (K == 4) hipLaunchKernelGGL((mutual_kernel<MASK, 4, RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
+    else if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
     else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
   } else if (prec == RMW_PREC_F32_KAHAN) {
-    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
+    if (K == 4) hipLaunchKernelGGL((mutual_kernel<MASK, 4, RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
+    else if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
     else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
   } else {
-    if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F64, double>), g, b, 0, st, q);
+    if (K == 4) hipLaunchKernelGGL((mutual_kernel<MASK, 4, RMW_PREC_F64, double>), g, b, 0, st, q);
+    else if (K == 2) hipLaunchKernelGGL((mutual_kernel<MASK, 2, RMW_PREC_F64, double>), g, b, 0, st, q);
     else hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F64, double>), g, b, 0, st, q);
   }
 }

@@ -45,7 +45,7 @@ def test_candidates():
     assert {c["rhs_group"] for c in cs} == {0}
     assert {c["max_wgs"] for c in cs} == {0, 1024, 2048}
     cs4 = autotune.candidates(nmax=32768, k=4, cu=256, ragged=False)
-    assert {c["rhs_group"] for c in cs4} == {0, 1, 2}
+    assert {c["rhs_group"] for c in cs4} == {0, 1, 2, 4}
     cs2 = autotune.candidates(nmax=100, k=2, cu=256, ragged=True)
     assert {c["rhs_group"] for c in cs2} == {0, 1}
     assert {c["max_wgs"] for c in cs2} == {0, 1024, 2048, 4096}

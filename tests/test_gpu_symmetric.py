@@ -123,7 +123,7 @@ def _grand(r, f, t, prec, k, sym, offsets=None, chunk=0, wgs=0):
 
 
 @pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
-@pytest.mark.parametrize("n,k", [(300, 1), (1000, 2), (513, 3)])
+@pytest.mark.parametrize("n,k", [(300, 1), (1000, 2), (513, 3), (700, 4)])
 def test_symmetric_grand_matches_ordered(gpu, prec, n, k):
     r = _geom(n, 20 + n)
     rng = np.random.default_rng(n)
@@ -134,6 +134,18 @@ def test_symmetric_grand_matches_ordered(gpu, prec, n, k):
     ordered = _grand(r, f, t, prec, k, -1)
     for j in range(k):
         assert rel(sym[j], ordered[j]) <= TOL[prec], j
+    if k == 4:
+        # four right-hand sides in one symmetric pass (rhs_group 4)
+        tun = _lib.Tuning(0, 0, 4, 1, 0)
+        assert wall_rpy.describe(GRAND, n, 4, prec, tuning=tun)["rhs_group"] == 4
+        u, w = wall_rpy.apply(GRAND, r, force=f, torque=t, eta=0.9, a=0.3, precision=prec, k=4, tuning=tun)
+        one = np.concatenate([u.cpu().numpy().reshape(4, -1), w.cpu().numpy().reshape(4, -1)], axis=1)
+        for j in range(4):
+            assert rel(one[j], ordered[j]) <= TOL[prec], j
+        uf4, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=0.9, a=0.3, precision=prec, k=4, tuning=tun)
+        uf1, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=0.9, a=0.3, precision=prec, k=4,
+                                tuning=_lib.Tuning(0, 0, 0, -1, 0))
+        assert rel(uf4.cpu().numpy(), uf1.cpu().numpy()) <= TOL[prec]
 
 
 def test_symmetric_grand_vs_oracle(gpu):
