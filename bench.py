@@ -20,7 +20,8 @@ Prints ONE JSON line (rank 0):
                trained on the GPU from a short trajectory), per-solve times
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step|c2step|c3lz|c5lz]
-       (c3r: the different-radii product on the C3 geometry; c3f: the
+       (c3r: the different-radii product on the C3 geometry, above the wall,
+       or with --domain no_wall / free_surface; c3f: the
        blob-blob soft-repulsion forces on it; c3step: whole rollers time
        steps, steps/s; c2step: rigid-body time steps of the C2 suspension;
        c3lz / c5lz: Lanczos Brownian-slip solves of the C3 block of 4 RHS
@@ -46,7 +47,11 @@ FLOP_PER_PAIR = {"UF": 164, "UT": 95, "WF": 95, "WT": 101, "grand": 386,
                                  # prefactor 2, radius sum 1, accumulation 6
                  # UF of the other boundaries, same convention: RPY 36 + 3x3 product 18
                  # (no wall); + the image RPY 36 (free surface); the wall terms as UF (in plane)
-                 "UF_no_wall": 54, "UF_free_surface": 90, "UF_in_plane": 164}
+                 "UF_no_wall": 54, "UF_free_surface": 90, "UF_in_plane": 164,
+                 # different radii, same convention as the equal-radii domains (the
+                 # radius products and regime branches of the radii form not counted)
+                 "radii_no_wall": 54, "radii_free_surface": 90}
+RADII_BOUNDARY = {"single_wall": "wall", "no_wall": "unbounded", "free_surface": "free_surface"}
 PEAK_FP32_TFLOPS = 157.3
 PRECISION_NOTE = {"fp32": " (FP32 pair terms, FP64 per-target sums)",
                   "fp32_blocked": " (FP32 pair terms, FP32 sub-sums of <= 16 sources folded into FP64 per-target sums)",
@@ -92,7 +97,7 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(r, f, eta, a, target_seconds=16.0, radii=None, force_params=None):
+def cpu_baseline(r, f, eta, a, target_seconds=16.0, radii=None, force_params=None, boundary="wall"):
     """Time the FP64 oracle (fast-math build, all usable cores) on a
     bounded sample of the same matvec (whole products repeated, or a row
     block when one product is longer than the budget); returns pairs/s,
@@ -108,7 +113,7 @@ def cpu_baseline(r, f, eta, a, target_seconds=16.0, radii=None, force_params=Non
     elif radii is not None:
         # different radii: the oracle's radii product on a block of rows
         def run(rows):
-            O.radii_apply(r, r, f, radii, radii, eta, rows=rows, kind="fast")
+            O.radii_apply(r, r, f, radii, radii, eta, rows=rows, kind="fast", boundary=boundary)
     else:
         def run(rows):
             O.block_apply(O.UF, r, f, eta, a, rows=rows, kind="fast")
@@ -482,7 +487,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--precision", default="fp32_blocked")
     ap.add_argument("--domain", default="single_wall", choices=["single_wall", "no_wall", "in_plane", "free_surface"],
-                    help="boundary of the UF product (workloads c1, c2, c3)")
+                    help="boundary of the UF product (workloads c1, c2, c3) or of the radii product "
+                         "(c3r: single_wall, no_wall, free_surface)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gmres", action="store_true", help="skip the learned-warm-start GMRES leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -537,6 +543,9 @@ def main(argv=None):
     if radii or fparams:
         u2 = torch.empty((n, 3), dtype=torch.float64, device=dev)
     if radii:
+        if args.domain not in RADII_BOUNDARY:
+            raise SystemExit(f"the radii product has no {args.domain} form")
+        rbnd = RADII_BOUNDARY[args.domain]
         rad = torch.from_numpy(w["radii"]).to(dev)
     if fparams:
         from rigidmultiblobswall_amd import forces
@@ -546,7 +555,7 @@ def main(argv=None):
         if fparams:
             forces.blob_forces(r, precision=fprec, out=u2, **fparams)
         elif radii:
-            wall_rpy.apply_radii_self(r, f, rad, eta, precision=args.precision, out=u2)
+            wall_rpy.apply_radii_self(r, f, rad, eta, precision=args.precision, out=u2, boundary=rbnd)
         else:
             wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u,
                            domain=args.domain)
@@ -565,7 +574,8 @@ def main(argv=None):
     ms = start.elapsed_time(stop) / args.steps
     pairs = float(n) * n
     value = pairs / (ms * 1e-3)
-    op = "forces" if fparams else "radii" if radii else "UF" if args.domain == "single_wall" else f"UF_{args.domain}"
+    op = ("forces" if fparams else ("radii" if args.domain == "single_wall" else f"radii_{args.domain}") if radii
+          else "UF" if args.domain == "single_wall" else f"UF_{args.domain}")
     kernel_ms = pk.mean_ms
     flop = FLOP_PER_PAIR[op] * pairs
     achieved = flop / (kernel_ms * 1e-3) / 1e12
@@ -592,6 +602,10 @@ def main(argv=None):
     if radii:
         launch = wall_rpy.describe_radii_self(n, args.precision)
         kname = f"{'radii_mutual_kernel' if launch['kernel'] == 'symmetric' else 'radii_kernel'}<{args.precision}>"
+        if args.domain != "single_wall":
+            launch["boundary"] = rbnd
+            kname = kname[:-1] + f",{rbnd}>"
+            desc = desc.replace("different-radii wall RPY", f"different-radii RPY, {rbnd.replace('_', ' ')}")
     if fparams:
         launch = forces.describe(n, fprec)
         kname = (f"{'force_mutual_kernel' if launch['kernel'] == 'symmetric' else 'force_kernel'}"
@@ -614,9 +628,11 @@ def main(argv=None):
                    "device": info, "launch_shape": launch},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
+                     # the committed PMC summaries count the single-wall kernels only
                      "traffic": load_traffic(args.workload, fprec if fparams else args.precision,
                                              "force_kernel" if fparams else "radii_kernel" if radii
-                                             else "pair_kernel_uf", kname),
+                                             else "pair_kernel_uf", kname)
+                                if args.domain == "single_wall" else None,
                      "kernel": kname,
                      "kernel_mean_ms": kernel_ms, "kernel_launches": pk.launches,
                      "flop_per_pair": FLOP_PER_PAIR[op]},
@@ -634,7 +650,8 @@ def main(argv=None):
     if not args.no_gmres:
         out["gmres"] = gmres_leg(args.gmres_steps, args.gmres_epochs, args.gmres_evals, log)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(r_h, f_h, eta, a, radii=w.get("radii"), force_params=fparams)
+        out["cpu_baseline"] = cpu_baseline(r_h, f_h, eta, a, radii=w.get("radii"), force_params=fparams,
+                                           boundary=rbnd if radii else "wall")
     print(json.dumps(out), flush=True)
     return 0
 

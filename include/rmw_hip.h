@@ -153,22 +153,33 @@ int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vectors, const
                               double a, const double* L, const rmw_tuning* tuning, void* workspace,
                               size_t workspace_bytes, void* stream);
 
+/* boundary argument of the different-radii products */
+#define RMW_RADII_UNBOUNDED 0
+#define RMW_RADII_WALL 1
+#define RMW_RADII_FREE_SURFACE 2
+
 /*
  * Blobs of different radii, source -> target: velocity (nt, 3) =
  * B_t M(x, y) B_s force, force (ns, 3), radii per blob, lengths in absolute
- * units (not scaled by a radius), normalisation 1/(8 pi eta).  wall != 0:
- * single-wall Rotne-Prager-Blake with heights shifted to max(z, radius)
- * and damping B = z/radius below the radius; wall == 0: unbounded RPY of
- * different radii.  L: pseudo-periodic lengths as rmw_wall_rpy_apply.
- * Sources and targets may be the same arrays (then the self pair is the
- * RPY self mobility 1/(6 pi eta a) plus its wall image).  Replaces
+ * units (not scaled by a radius), normalisation 1/(8 pi eta).  boundary
+ * RMW_RADII_WALL: single-wall Rotne-Prager-Blake with heights shifted to
+ * max(z, radius) and damping B = z/radius below the radius;
+ * RMW_RADII_UNBOUNDED: unbounded RPY of different radii;
+ * RMW_RADII_FREE_SURFACE: RPY of d plus the RPY of the image vector
+ * (d_x, d_y, z_i + z_j) times diag(1, 1, -1), heights as given; other
+ * values return RMW_E_ARG.  L: pseudo-periodic lengths as
+ * rmw_wall_rpy_apply.  Sources and targets may be the same arrays (then
+ * the self pair is the RPY self mobility 1/(6 pi eta a) plus its image).
+ * Replaces
  *   mobility_trans_times_force_source_target_numba  mobility/mobility_numba.py:1479-1658
  *   single_wall_mobility_trans_times_force_source_target_numba  mobility/mobility.py:551-591
  *   no_wall_mobility_trans_times_force_source_target_numba      mobility/mobility.py:593-615
+ *   free_surface_mobility_trans_times_force_source_target_numba mobility/mobility.py:1409-1429
+ *     (kernel mobility/mobility_numba.py:1941-2093)
  * as selected by the 'radii*' options of multi_bodies/multi_bodies.py:266-287.
  */
 int rmw_radii_workspace_size(int prec, int64_t ns, int64_t nt, const rmw_tuning* tuning, size_t* bytes);
-int rmw_radii_apply(int prec, int wall, const double* source, const double* target, const double* force,
+int rmw_radii_apply(int prec, int boundary, const double* source, const double* target, const double* force,
                     const double* radius_source, const double* radius_target, double* velocity, int64_t ns,
                     int64_t nt, double eta, const double* L, const rmw_tuning* tuning, void* workspace,
                     size_t workspace_bytes, void* stream);
@@ -189,7 +200,7 @@ int rmw_radii_self_workspace_size(int prec, int64_t n, const double* L, const rm
                                   size_t* bytes);
 int rmw_radii_self_describe(int prec, int64_t n, const double* L, const rmw_tuning* tuning, int* symmetric,
                             int64_t* work_items, size_t* bytes);
-int rmw_radii_self_apply(int prec, int wall, const double* r_vectors, const double* force,
+int rmw_radii_self_apply(int prec, int boundary, const double* r_vectors, const double* force,
                          const double* radius, double* velocity, int64_t n, double eta, const double* L,
                          const rmw_tuning* tuning, void* workspace, size_t workspace_bytes, void* stream);
 

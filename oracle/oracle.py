@@ -156,11 +156,23 @@ def pair_matrix(block: int, ri, rj, self_term: bool, eta: float, a: float, kind=
     return m.reshape(3, 3)
 
 
+RADII_BOUNDARIES = {"unbounded": 0, "wall": 1, "free_surface": 2}
+
+
+def _radii_boundary(wall, boundary) -> int:
+    if boundary is None:
+        return 1 if wall else 0
+    return RADII_BOUNDARIES[boundary]
+
+
 def radii_apply(source, target, force, radius_source, radius_target, eta: float, periodic_length=None,
-                wall: bool = True, rows: Optional[tuple] = None, kind: str = "strict") -> np.ndarray:
+                wall: bool = True, rows: Optional[tuple] = None, kind: str = "strict",
+                boundary: Optional[str] = None) -> np.ndarray:
     """u = B_t M B_s f for blobs of different radii, source -> target
     (mobility_numba.py:1479-1658 with the wrapper mobility.py:551-591;
-    wall=False: the unbounded kernel of mobility.py:593-615).  (3 N_t,)."""
+    wall=False: the unbounded kernel of mobility.py:593-615;
+    boundary='free_surface': mobility_numba.py:1941-2093 behind
+    mobility.py:1409-1429, heights as given).  (3 N_t,)."""
     src = np.ascontiguousarray(np.asarray(source, dtype=np.float64).reshape(-1, 3))
     trg = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(-1, 3))
     f = np.ascontiguousarray(np.asarray(force, dtype=np.float64).reshape(-1, 3))
@@ -174,7 +186,7 @@ def radii_apply(source, target, force, radius_source, radius_target, eta: float,
     out = np.zeros((nt, 3))
     t0, t1 = (0, nt) if rows is None else rows
     rc = _lib(kind).orc_radii_apply(_ptr(src), _ptr(trg), _ptr(f), _ptr(rs), _ptr(rt), ns, nt, t0, t1, float(eta),
-                                    _ptr(L), int(bool(wall)), _ptr(out))
+                                    _ptr(L), _radii_boundary(wall, boundary), _ptr(out))
     if rc:
         raise ValueError("bad row range")
     return out.reshape(-1)
@@ -199,12 +211,14 @@ def blob_forces(r_vectors, eps: float, debye: float, a: float, periodic_length=N
     return out
 
 
-def radii_pair_matrix(ri, rj, a: float, b: float, eta: float, wall: bool = True, kind="strict") -> np.ndarray:
+def radii_pair_matrix(ri, rj, a: float, b: float, eta: float, wall: bool = True, kind="strict",
+                      boundary: Optional[str] = None) -> np.ndarray:
     """3x3 block of one (target radius a, source radius b) pair, no wrapper."""
     m = np.empty(9)
     ri = np.ascontiguousarray(ri, dtype=np.float64)
     rj = np.ascontiguousarray(rj, dtype=np.float64)
-    _lib(kind).orc_radii_pair_matrix(_ptr(ri), _ptr(rj), float(a), float(b), float(eta), int(bool(wall)), _ptr(m))
+    _lib(kind).orc_radii_pair_matrix(_ptr(ri), _ptr(rj), float(a), float(b), float(eta),
+                                     _radii_boundary(wall, boundary), _ptr(m))
     return m.reshape(3, 3)
 
 

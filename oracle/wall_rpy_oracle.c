@@ -641,9 +641,12 @@ void orc_set_num_threads(int n)
 /* mobility_numba.py:1479-1658 (mobility_trans_times_force_source_target_  */
 /* numba) in its expression order; lengths are NOT scaled by a radius, the */
 /* normalisation is 1/(8 pi eta).  a = target radius, b = source radius;   */
-/* x3, y3 = target / source heights.  wall = 0: the RPY part only.         */
+/* x3, y3 = target / source heights.  wall = 0: the RPY part only;         */
+/* wall = 1: plus the single-wall image; wall = 2: plus the free-surface   */
+/* image of mobility_numba.py:1941-2093 (the RPY of R = (rx, ry, x3 + y3)  */
+/* with its z column negated).                                             */
 /* ---------------------------------------------------------------------- */
-static inline mat3 radii_pair(double rx, double ry, double rz, double x3, double y3, double a, double b, int wall)
+static inline mat3 radii_rpy(double rx, double ry, double rz, double a, double b)
 {
   const double fourOverThree = 4.0 / 3.0;
   mat3 M;
@@ -672,8 +675,26 @@ static inline mat3 radii_pair(double rx, double ry, double rz, double x3, double
   M.yx = M.xy;
   M.zx = M.xz;
   M.zy = M.yz;
-  if (wall) {
-    rz = x3 + y3;
+  return M;
+}
+
+static inline mat3 radii_pair(double rx, double ry, double rz, double x3, double y3, double a, double b, int wall)
+{
+  mat3 M = radii_rpy(rx, ry, rz, a, b);
+  if (wall == 2) {
+    mat3 I = radii_rpy(rx, ry, x3 + y3, a, b);
+    M.xx = M.xx + I.xx;
+    M.xy = M.xy + I.xy;
+    M.xz = M.xz - I.xz;
+    M.yx = M.yx + I.xy;
+    M.yy = M.yy + I.yy;
+    M.yz = M.yz - I.yz;
+    M.zx = M.zx + I.xz;
+    M.zy = M.zy + I.yz;
+    M.zz = M.zz - MUT(9, I.zz, -I.zz);
+  } else if (wall) {
+    double rz = x3 + y3;
+    double r2, r;
     r2 = rx * rx + ry * ry + rz * rz;
     r = sqrt(r2);
     double a2 = a * a, b2 = b * b;
@@ -736,17 +757,19 @@ static inline mat3 radii_pair(double rx, double ry, double rz, double x3, double
   return M;
 }
 
-/* u = B_t M(x, y) B_s f for targets [t0, t1) (wall != 0: heights shifted to
+/* u = B_t M(x, y) B_s f for targets [t0, t1) (wall == 1: heights shifted to
  * max(z, radius) and damping B = z / radius below the radius, as the
  * wrapper mobility.py:551-591 with shift_heights_different_radius :87-99
  * and damping_matrix_B_different_radius :102-119; wall == 0: the bare
  * kernel as no_wall_mobility_trans_times_force_source_target_numba
- * :593-615).  out has nt*3 entries; rows outside [t0, t1) are untouched. */
+ * :593-615; wall == 2: the free-surface kernel with the heights as given,
+ * as the wrapper mobility.py:1409-1429).  out has nt*3 entries; rows
+ * outside [t0, t1) are untouched. */
 int orc_radii_apply(const double *src, const double *trg, const double *force, const double *rs,
                     const double *rt, int64_t ns, int64_t nt, int64_t t0, int64_t t1, double eta,
                     const double *L, int wall, double *out)
 {
-  if (t0 < 0 || t1 > nt || t0 > t1) return -1;
+  if (t0 < 0 || t1 > nt || t0 > t1 || wall < 0 || wall > 2) return -1;
   const double norm_fact_f = 1.0 / (8.0 * M_PI * eta);
   const double Lx = L ? L[0] : 0.0, Ly = L ? L[1] : 0.0, Lz = L ? L[2] : 0.0;
   const int px = Lx > 0, py = Ly > 0, pz = Lz > 0;
@@ -755,7 +778,7 @@ int orc_radii_apply(const double *src, const double *trg, const double *force, c
     double a = rt[i];
     double xi = trg[3 * i], yi = trg[3 * i + 1], zi = trg[3 * i + 2];
     double Bt = 1.0;
-    if (wall) {
+    if (wall == 1) {
       if (!(zi > a)) { Bt = zi < a ? zi / a : 1.0; zi = a; }
     }
     double ux = 0, uy = 0, uz = 0;
@@ -765,7 +788,7 @@ int orc_radii_apply(const double *src, const double *trg, const double *force, c
           for (int64_t j = 0; j < ns; ++j) {
             double b = rs[j];
             double zj = src[3 * j + 2], Bs = 1.0;
-            if (wall) {
+            if (wall == 1) {
               if (!(zj > b)) { Bs = zj < b ? zj / b : 1.0; zj = b; }
             }
             double rx = project_image(xi - src[3 * j], Lx, bx);

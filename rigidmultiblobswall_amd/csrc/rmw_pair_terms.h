@@ -434,17 +434,21 @@ RMW_HD Vec3<T> apply_wt_t(const Geom<T>& g, const CoefWT<T>& c, T vx, T vy, T vz
 //         S = alpha I + beta R R^T + g1 R z^T + g2 z R^T + delta z z^T,
 //   the image Stokeslet, its doublets weighted by ca = 2a^2/3, cb = 2b^2/3
 //   and the quadrupole weighted by cab = 2a^2 b^2/3 collected per dyad.
+//   free surface (mobility_numba.py:1941-2093): + (C1' I + C2' R R^T) D v,
+//         the RPY coefficients of the three regimes evaluated at rho, i.e.
+//         S = -(C1' I + C2' R R^T) in the wall's form.
 // ---------------------------------------------------------------------------
+enum : int { kRadiiUnbounded = 0, kRadiiWall = 1, kRadiiFreeSurface = 2 };
+
 template <typename T> struct CoefRadii {
   T C1, C2;                 // RPY
-  T al, be, g1, g2, de;     // wall dyads
+  T al, be, g1, g2, de;     // wall (or image) dyads
 };
 
+// RPY coefficients C1, C2 of radii a, b at squared distance r2.
 template <typename T>
-RMW_HD CoefRadii<T> coef_radii(T dx, T dy, T dz, T Rz, T x3, T y3, T a, T b, T inva, T invb, bool wall)
+RMW_HD void rpy_radii(T r2, T a, T b, T inva, T invb, T& C1, T& C2)
 {
-  CoefRadii<T> c;
-  const T r2 = dx * dx + dy * dy + dz * dz;
   const T invr = rsq(r2);
   const T r = r2 > T(0) ? r2 * invr : T(0);
   const T w = invr * invr;
@@ -462,9 +466,19 @@ RMW_HD CoefRadii<T> coef_radii(T dx, T dy, T dz, T Rz, T x3, T y3, T a, T b, T i
   const T C1_in = T(4.0 / 3.0) * (inva < invb ? inva : invb);
   const bool far = r > a + b;
   const bool mid = r > (dab < T(0) ? -dab : dab);
-  c.C1 = far ? C1_far : (mid ? C1_mid : C1_in);
-  c.C2 = far ? C2_far : (mid ? C2_mid : T(0));
-  if (wall) {
+  C1 = far ? C1_far : (mid ? C1_mid : C1_in);
+  C2 = far ? C2_far : (mid ? C2_mid : T(0));
+}
+
+// bnd: kRadiiUnbounded, kRadiiWall or kRadiiFreeSurface.
+template <typename T>
+RMW_HD CoefRadii<T> coef_radii(T dx, T dy, T dz, T Rz, T x3, T y3, T a, T b, T inva, T invb, int bnd)
+{
+  CoefRadii<T> c;
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  rpy_radii(r2, a, b, inva, invb, c.C1, c.C2);
+  const T s = a * a + b * b;
+  if (bnd == kRadiiWall) {
     const T rho2 = dx * dx + dy * dy + Rz * Rz;
     const T q = rsq(rho2);
     const T qw = q * q;
@@ -482,6 +496,12 @@ RMW_HD CoefRadii<T> coef_radii(T dx, T dy, T dz, T Rz, T x3, T y3, T a, T b, T i
     c.g1 = x3 * (e - T(2) * q3) + z10;
     c.g2 = y3 * (-e - T(2) * q3) + z10;
     c.de = -T(2) * q - (ca + cb) * q3 - T(2) * cab * q5;
+  } else if (bnd == kRadiiFreeSurface) {
+    T I1, I2;
+    rpy_radii(dx * dx + dy * dy + Rz * Rz, a, b, inva, invb, I1, I2);
+    c.al = -I1;
+    c.be = -I2;
+    c.g1 = c.g2 = c.de = T(0);
   } else {
     c.al = c.be = c.g1 = c.g2 = c.de = T(0);
   }

@@ -1,12 +1,15 @@
 // Blobs of different radii: u = B_t M(x, y) B_s f from source blobs to
-// target blobs, with or without the wall (gfx950).
+// target blobs, above a wall, unbounded or below a free surface (gfx950).
 //
 // Reference: mobility_numba.py:1479-1658 (mobility_trans_times_force_
 // source_target_numba) behind the wrappers mobility.py:551-591 (wall:
 // heights shifted to max(z, radius), damping B = z/radius below the
 // radius, shift_heights_different_radius :87-99 and damping_matrix_B_
-// different_radius :102-119) and :593-615 (unbounded, no wrapper), as
-// selected by the 'radii*' implementations of set_mobility_vector_prod
+// different_radius :102-119) and :593-615 (unbounded, no wrapper);
+// mobility_numba.py:1941-2093 (free surface: the RPY of d plus the RPY of
+// the image vector R = (d_x, d_y, z_i + z_j) times D = diag(1, 1, -1), no
+// height shift or damping) behind mobility.py:1409-1429; as selected by
+// the 'radii*' implementations of set_mobility_vector_prod
 // (multi_bodies/multi_bodies.py:266-287).
 //
 // Two launches: the pair kernel (256 targets per workgroup in registers,
@@ -36,7 +39,7 @@ struct RadiiParams {
   const double* rt;
   int64_t ns, nt;
   double norm;            // 1 / (8 pi eta)
-  int wall;
+  int bnd;                // kRadiiUnbounded, kRadiiWall or kRadiiFreeSurface
   int periodic;           // bit d set when L_d > 0
   double L[3];
   double* partial;        // [chunk][nt][3]
@@ -44,12 +47,13 @@ struct RadiiParams {
 };
 
 // Height and damping of a blob of radius rad: z_eff = max(z, rad) and
-// B = z / rad below the radius (wall only).
-__device__ __forceinline__ void wall_shift(double z, double rad, int wall, double& ze, double& B)
+// B = z / rad below the radius (wall only; the free surface and the
+// unbounded kernel use the heights as they are).
+__device__ __forceinline__ void wall_shift(double z, double rad, int bnd, double& ze, double& B)
 {
   ze = z;
   B = 1.0;
-  if (wall && !(z > rad)) {
+  if (bnd == kRadiiWall && !(z > rad)) {
     B = z < rad ? z / rad : 1.0;
     ze = rad;
   }
@@ -74,7 +78,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
     const double xi = p.trg[3 * gi], yi = p.trg[3 * gi + 1];
     const double a = p.rt[gi];
     double zi, Bi;
-    wall_shift(p.trg[3 * gi + 2], a, p.wall, zi, Bi);
+    wall_shift(p.trg[3 * gi + 2], a, p.bnd, zi, Bi);
     const T ar = (T)a, inva = (T)(1.0 / a), zir = (T)zi;
     Acc<ACC> u[3];
 
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
         const int64_t j = s0 + il;
         const double b = p.rs[j];
         double zj, Bj;
-        wall_shift(p.src[3 * j + 2], b, p.wall, zj, Bj);
+        wall_shift(p.src[3 * j + 2], b, p.bnd, zj, Bj);
         s_x[il] = p.src[3 * j];
         s_y[il] = p.src[3 * j + 1];
         s_z[il] = zj;
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
         const T Rz = zir + zj;
         if (!p.periodic) {
           const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], p.wall != 0);
+          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], p.bnd);
           const Vec3<T> v = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][j], s_f[1][j], s_f[2][j]);
           u[0].add(v.x);
           u[1].add(v.y);
@@ -115,7 +119,9 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
             u[0].flush(); u[1].flush(); u[2].flush();
           }
         } else {
-          // minimum image of the FP64 difference, then the neighbouring boxes
+          // minimum image of the FP64 difference, then the neighbouring boxes;
+          // R_z = z_i + z_j is not shifted, so with L_z > 0 the image term is
+          // taken once per z box, as the reference kernels do
           if (pbox[0]) ddx -= image_count(ddx, p.L[0]) * p.L[0];
           if (pbox[1]) ddy -= image_count(ddy, p.L[1]) * p.L[1];
           if (pbox[2]) ddz -= image_count(ddz, p.L[2]) * p.L[2];
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
               for (int bz = -pbox[2]; bz <= pbox[2]; ++bz) {
                 const T dx = (T)(ddx + bx * p.L[0]), dy = (T)(ddy + by * p.L[1]), dz = (T)(ddz + bz * p.L[2]);
                 const CoefRadii<T> c =
-                    coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], p.wall != 0);
+                    coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], p.bnd);
                 const Vec3<T> v = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][j], s_f[1][j], s_f[2][j]);
                 u[0].add(v.x);
                 u[1].add(v.y);
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
 
 __global__ __launch_bounds__(256) void radii_reduce_kernel(const double* __restrict__ partial, int64_t n_chunks,
                                                            int64_t nt, const double* __restrict__ trg,
-                                                           const double* __restrict__ rt, int wall,
+                                                           const double* __restrict__ rt, int bnd,
                                                            double* __restrict__ out)
 {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(256) void radii_reduce_kernel(const double* __restr
   }
   if (ch < n_chunks) s0 += partial[ch * nt * 3 + tid];
   double ze, B;
-  wall_shift(trg[3 * i + 2], rt[i], wall, ze, B);
+  wall_shift(trg[3 * i + 2], rt[i], bnd, ze, B);
   out[tid] = B * (s0 + s1);
 }
 
@@ -198,9 +204,10 @@ static RadiiPlan radii_plan(int prec, int64_t ns, int64_t nt, const rmw_tuning* 
 // ---------------------------------------------------------------------------
 // Sources = targets (mobility.py:1369-1374): the tensor of a pair is its own
 // transpose under exchange (M_ji = M_ij^T: the far, overlap and engulfed RPY
-// coefficients and the wall's al, be, de are symmetric in (a_i, a_j), while
-// exchanging the blobs swaps g1 and g2), so each unordered pair is
-// evaluated once and applied to both blobs.
+// coefficients, the wall's al, be, de and the free surface's image
+// coefficients are symmetric in (a_i, a_j), while exchanging the blobs
+// swaps g1 and g2, which the free surface does not have), so each unordered
+// pair is evaluated once and applied to both blobs.
 // ---------------------------------------------------------------------------
 
 // M_ij^T v (the coefficients as computed for the ordered pair (i, j)).
@@ -225,7 +232,7 @@ struct RadiiMutParams {
   const double* rad;
   int64_t n;
   double norm;            // 1 / (8 pi eta)
-  int wall;
+  int bnd;                // kRadiiUnbounded, kRadiiWall or kRadiiFreeSurface
   double* part_i;         // [item][kRBlock][3] target-side sums
   double* part_j;         // [tri_index(S, T)][kRBlock][3] source-side sums
   int64_t n_t, chunk_tiles, total_items;
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
     const double xi = p.r[3 * gi], yi = p.r[3 * gi + 1];
     const double a = p.rad[gi];
     double zi, Bi;
-    wall_shift(p.r[3 * gi + 2], a, p.wall, zi, Bi);
+    wall_shift(p.r[3 * gi + 2], a, p.bnd, zi, Bi);
     const T ar = (T)a, inva = (T)(1.0 / a), zir = (T)zi;
     T fi[3];
     {
@@ -276,7 +283,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
         const int64_t j = s0 + il;
         const double b = p.rad[j];
         double zj, Bj;
-        wall_shift(p.r[3 * j + 2], b, p.wall, zj, Bj);
+        wall_shift(p.r[3 * j + 2], b, p.bnd, zj, Bj);
         s_x[il] = p.r[3 * j]; s_y[il] = p.r[3 * j + 1]; s_z[il] = zj;
         s_zr[il] = (T)zj; s_b[il] = (T)b; s_ib[il] = (T)(1.0 / b);
         const double c = p.norm * Bj;
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
           }
           const T zj = s_zr[jl];
           const T Rz = zir + zj;
-          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[jl], inva, s_ib[jl], p.wall != 0);
+          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[jl], inva, s_ib[jl], p.bnd);
           const Vec3<T> vi = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][jl], s_f[1][jl], s_f[2][jl]);
           const Vec3<T> vj = apply_radii_t<T>(dx, dy, dz, Rz, c, fi[0], fi[1], fi[2]);
           u[0].add(on ? vi.x : T(0));
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
 __global__ __launch_bounds__(256) void radii_mutual_reduce_kernel(const double* __restrict__ part_i,
                                                                   const double* __restrict__ part_j,
                                                                   const double* __restrict__ r,
-                                                                  const double* __restrict__ rad, int wall, int64_t n,
+                                                                  const double* __restrict__ rad, int bnd, int64_t n,
                                                                   int64_t n_t, int64_t chunk_tiles,
                                                                   double* __restrict__ out)
 {
@@ -368,7 +375,7 @@ __global__ __launch_bounds__(256) void radii_mutual_reduce_kernel(const double* 
   const double* Pj = part_j + (tri_index(X, 0) * kRBlock + l) * 3 + c;
   for (int64_t T = 0; T <= X; ++T) s += Pj[T * kRBlock * 3];
   double ze, B;
-  wall_shift(r[3 * i + 2], rad[i], wall, ze, B);
+  wall_shift(r[3 * i + 2], rad[i], bnd, ze, B);
   out[tid] = B * s;
 }
 
@@ -421,7 +428,7 @@ extern "C" int rmw_radii_workspace_size(int prec, int64_t ns, int64_t nt, const 
 }
 
 // The ordered pair kernel and its reduce, source -> target.
-static void radii_ordered(const RadiiPlan& pl, int prec, int wall, const double* source, const double* target,
+static void radii_ordered(const RadiiPlan& pl, int prec, int bnd, const double* source, const double* target,
                           const double* force, const double* radius_source, const double* radius_target,
                           double* velocity, int64_t ns, int64_t nt, double eta, const double* L,
                           const rmw_tuning* tuning, void* workspace, hipStream_t st)
@@ -430,7 +437,7 @@ static void radii_ordered(const RadiiPlan& pl, int prec, int wall, const double*
   q.src = source; q.trg = target; q.f = force; q.rs = radius_source; q.rt = radius_target;
   q.ns = ns; q.nt = nt;
   q.norm = 1.0 / (8.0 * M_PI * eta);
-  q.wall = wall != 0;
+  q.bnd = bnd;
   for (int d = 0; d < 3; ++d) {
     const double Ld = L ? L[d] : 0.0;
     q.L[d] = Ld > 0.0 ? Ld : 0.0;
@@ -453,7 +460,7 @@ static void radii_ordered(const RadiiPlan& pl, int prec, int wall, const double*
   RMW_CHECK_HIP(hipGetLastError());
   prof_end(rec, prof, st);
   hipLaunchKernelGGL(radii_reduce_kernel, dim3((unsigned)((nt * 3 + 255) / 256)), dim3(256), 0, st, q.partial,
-                     pl.n_chunks, nt, target, radius_target, q.wall, velocity);
+                     pl.n_chunks, nt, target, radius_target, q.bnd, velocity);
   RMW_CHECK_HIP(hipGetLastError());
 }
 
@@ -462,13 +469,20 @@ static bool any_periodic(const double* L)
   return L && (L[0] > 0.0 || L[1] > 0.0 || L[2] > 0.0);
 }
 
-extern "C" int rmw_radii_apply(int prec, int wall, const double* source, const double* target, const double* force,
+static void check_boundary(int boundary)
+{
+  RMW_REQUIRE(boundary == kRadiiUnbounded || boundary == kRadiiWall || boundary == kRadiiFreeSurface,
+              "boundary must be 0 (unbounded), 1 (wall) or 2 (free surface)");
+}
+
+extern "C" int rmw_radii_apply(int prec, int boundary, const double* source, const double* target, const double* force,
                                const double* radius_source, const double* radius_target, double* velocity,
                                int64_t ns, int64_t nt, double eta, const double* L, const rmw_tuning* tuning,
                                void* workspace, size_t workspace_bytes, void* stream)
 {
   return guarded([&] {
     const RadiiPlan pl = radii_plan(prec, ns, nt, tuning);
+    check_boundary(boundary);
     RMW_REQUIRE(workspace_bytes >= pl.bytes, "workspace too small: need " + std::to_string(pl.bytes) + " bytes");
     RMW_REQUIRE(eta > 0.0 && std::isfinite(eta), "eta must be positive");
     if (nt == 0) return;
@@ -482,7 +496,7 @@ extern "C" int rmw_radii_apply(int prec, int wall, const double* source, const d
       return;
     }
     RMW_REQUIRE(source && force && radius_source && workspace, "source, force, radius_source and workspace are required");
-    radii_ordered(pl, prec, wall, source, target, force, radius_source, radius_target, velocity, ns, nt, eta, L,
+    radii_ordered(pl, prec, boundary, source, target, force, radius_source, radius_target, velocity, ns, nt, eta, L,
                   tuning, workspace, st);
   });
 }
@@ -507,26 +521,27 @@ extern "C" int rmw_radii_self_describe(int prec, int64_t n, const double* L, con
   });
 }
 
-extern "C" int rmw_radii_self_apply(int prec, int wall, const double* r, const double* force, const double* radius,
+extern "C" int rmw_radii_self_apply(int prec, int boundary, const double* r, const double* force, const double* radius,
                                     double* velocity, int64_t n, double eta, const double* L,
                                     const rmw_tuning* tuning, void* workspace, size_t workspace_bytes, void* stream)
 {
   return guarded([&] {
     const RadiiSelfPlan pl = radii_self_plan(prec, n, any_periodic(L), tuning);
+    check_boundary(boundary);
     RMW_REQUIRE(workspace_bytes >= pl.bytes, "workspace too small: need " + std::to_string(pl.bytes) + " bytes");
     RMW_REQUIRE(eta > 0.0 && std::isfinite(eta), "eta must be positive");
     if (n == 0) return;
     RMW_REQUIRE(r && force && radius && velocity && workspace, "r, force, radius, velocity and workspace are required");
     hipStream_t st = (hipStream_t)stream;
     if (!pl.mutual) {
-      radii_ordered(pl.ordered, prec, wall, r, r, force, radius, radius, velocity, n, n, eta, L, tuning, workspace,
+      radii_ordered(pl.ordered, prec, boundary, r, r, force, radius, radius, velocity, n, n, eta, L, tuning, workspace,
                     st);
       return;
     }
     RadiiMutParams q{};
     q.r = r; q.f = force; q.rad = radius; q.n = n;
     q.norm = 1.0 / (8.0 * M_PI * eta);
-    q.wall = wall != 0;
+    q.bnd = boundary;
     q.part_i = (double*)workspace;
     q.part_j = (double*)((char*)workspace + pl.off_pj);
     q.n_t = pl.n_t; q.chunk_tiles = pl.chunk_tiles; q.total_items = pl.total_items;
@@ -543,7 +558,7 @@ extern "C" int rmw_radii_self_apply(int prec, int wall, const double* r, const d
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
     hipLaunchKernelGGL(radii_mutual_reduce_kernel, dim3((unsigned)((n * 3 + 255) / 256)), dim3(256), 0, st, q.part_i,
-                       q.part_j, r, radius, q.wall, n, pl.n_t, pl.chunk_tiles, velocity);
+                       q.part_j, r, radius, q.bnd, n, pl.n_t, pl.chunk_tiles, velocity);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

@@ -664,7 +664,8 @@ __global__ __launch_bounds__(kPfBlock) void point_field_kernel(const double* __r
       if (c0 + k == own) continue;
       const double dx = xi - s_x[k], dy = yi - s_y[k], dz = zi - s_z[k];
       const double Rz = zi + s_z[k];
-      const CoefRadii<double> co = coef_radii<double>(dx, dy, dz, Rz, zi, s_z[k], a, R, inva, invR, wall != 0);
+      const CoefRadii<double> co = coef_radii<double>(dx, dy, dz, Rz, zi, s_z[k], a, R, inva, invR,
+                                                      wall != 0 ? kRadiiWall : kRadiiUnbounded);
       const Vec3<double> v = apply_radii<double>(dx, dy, dz, Rz, co, s_f[0][k], s_f[1][k], s_f[2][k]);
       ux += v.x; uy += v.y; uz += v.z;
     }

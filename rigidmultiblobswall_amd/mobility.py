@@ -220,20 +220,34 @@ def no_wall_mobility_trans_times_force_source_target_hip(source, target, force, 
     return _host(u)
 
 
+def free_surface_mobility_trans_times_force_source_target_hip(source, target, force, radius_source, radius_target,
+                                                              eta, *args, **kwargs):
+    """Blobs of different radii below a free surface: RPY of d plus the RPY
+    of the image vector (d_x, d_y, z_i + z_j) times diag(1, 1, -1), heights
+    as given; drop-in for mobility.py:1409-1429 (kernel
+    mobility_numba.py:1941-2093)."""
+    L, prec, tun = _opts(kwargs)
+    u = wall_rpy.apply_radii(source, target, force, radius_source, radius_target, eta, periodic_length=L,
+                             boundary="free_surface", precision=prec, tuning=tun)
+    return _host(u)
+
+
 def mobility_radii_trans_times_force_hip(r_vectors, force, eta, a, radius_blobs, function=None, *args, **kwargs):
     """M f with per-blob radii, sources = targets; drop-in for
     mobility.py:1369-1374 (``function`` defaults to the wall kernel).  With
-    the wall or unbounded kernel of this module the product runs as one
-    sources-equal-targets call (each unordered pair evaluated once)."""
+    the wall, unbounded or free-surface kernel of this module the product
+    runs as one sources-equal-targets call (each unordered pair evaluated
+    once)."""
     fn = function or single_wall_mobility_trans_times_force_source_target_hip
-    own = {single_wall_mobility_trans_times_force_source_target_hip: True,
-           no_wall_mobility_trans_times_force_source_target_hip: False}
+    own = {single_wall_mobility_trans_times_force_source_target_hip: "wall",
+           no_wall_mobility_trans_times_force_source_target_hip: "unbounded",
+           free_surface_mobility_trans_times_force_source_target_hip: "free_surface"}
     base, opts = fn, dict(kwargs)
     if isinstance(fn, functools.partial) and not fn.args:
         base, opts = fn.func, {**fn.keywords, **kwargs}
     if base in own and not args:
         L, prec, tun = _opts(opts)
-        u = wall_rpy.apply_radii_self(r_vectors, force, radius_blobs, eta, periodic_length=L, wall=own[base],
+        u = wall_rpy.apply_radii_self(r_vectors, force, radius_blobs, eta, periodic_length=L, boundary=own[base],
                                       precision=prec, tuning=tun)
         return _host(u)
     return fn(r_vectors, r_vectors, force, radius_blobs, radius_blobs, eta, *args, **kwargs)

@@ -10,7 +10,7 @@ New implementation values served by librmw_hip: 'hip' (FP32 pair terms, FP64
 per-target sums), 'hip_fp32_blocked', 'hip_fp32_kahan' and 'hip_fp64', the
 other boundaries 'hip_no_wall[_<precision>]' and
 'hip_free_surface[_<precision>]', and for blobs of different radii
-'radii_hip[_no_wall][_<precision>]'.  None
+'radii_hip[_no_wall|_free_surface][_<precision>]'.  None
 contains 'numba' or 'pycuda', so the reference's substring dispatch never
 mistakes them for its own backends.
 """
@@ -40,21 +40,23 @@ def _precision(implementation: str) -> str:
 def set_mobility_vector_prod(implementation, *args, **kwargs):
     """M*F at the blob level, signature f(r_vectors, force, eta, a, *args, **kwargs).
 
-    'radii_hip' / 'radii_hip_no_wall' (optionally with a precision suffix,
-    e.g. 'radii_hip_fp64') select the blobs-of-different-radii product as the
-    reference's 'radii*' options do (multi_bodies.py:266-287); the radii come
-    from kwargs['radius_blobs'] (the reference reads them from the bodies)."""
+    'radii_hip', 'radii_hip_no_wall' and 'radii_hip_free_surface'
+    (optionally with a precision suffix, e.g. 'radii_hip_fp64') select the
+    blobs-of-different-radii product as the reference's 'radii_numba',
+    'radii_numba_no_wall' and 'radii_numba_free_surface' do
+    (multi_bodies.py:266-287); the radii come from kwargs['radius_blobs']
+    (the reference reads them from the bodies)."""
     if implementation.startswith("radii_hip"):
         rest = implementation[len("radii_hip"):]
-        no_wall = rest.startswith("_no_wall")
-        if no_wall:
-            rest = rest[len("_no_wall"):]
+        fn = mb.single_wall_mobility_trans_times_force_source_target_hip
+        for dom, dom_fn in (("_no_wall", mb.no_wall_mobility_trans_times_force_source_target_hip),
+                            ("_free_surface", mb.free_surface_mobility_trans_times_force_source_target_hip)):
+            if rest.startswith(dom):
+                rest, fn = rest[len(dom):], dom_fn
         prec = _precision("hip" + rest)
         radius_blobs = kwargs.get("radius_blobs")
         if radius_blobs is None:
             raise ValueError("radii implementations need radius_blobs=")
-        fn = mb.no_wall_mobility_trans_times_force_source_target_hip if no_wall else \
-            mb.single_wall_mobility_trans_times_force_source_target_hip
         return partial(mb.mobility_radii_trans_times_force_hip, radius_blobs=radius_blobs,
                        function=partial(fn, precision=prec))
     for dom, fn in (("_no_wall", mb.no_wall_mobility_trans_times_force_hip),

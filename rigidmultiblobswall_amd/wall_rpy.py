@@ -200,10 +200,27 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int
                                              tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
 
 
+# boundary argument of the different-radii products (rmw_radii_apply)
+RADII_BOUNDARIES = {"unbounded": 0, "wall": 1, "free_surface": 2}
+
+
+def _radii_boundary(wall, boundary) -> int:
+    if boundary is None:
+        return RADII_BOUNDARIES["wall" if wall else "unbounded"]
+    if boundary not in RADII_BOUNDARIES:
+        raise ValueError(f"unknown boundary {boundary!r}; use one of {sorted(RADII_BOUNDARIES)}")
+    return RADII_BOUNDARIES[boundary]
+
+
 def apply_radii(source, target, force, radius_source, radius_target, eta: float, periodic_length=None,
-                wall: bool = True, precision="fp32_blocked", tuning=None, out: Optional[torch.Tensor] = None):
+                wall: bool = True, precision="fp32_blocked", tuning=None, out: Optional[torch.Tensor] = None,
+                boundary: Optional[str] = None):
     """Different radii, source -> target: u (N_t, 3) = B_t M B_s f (wall) or
-    the unbounded RPY of different radii (wall=False)."""
+    the unbounded RPY of different radii (wall=False).  boundary
+    ('wall', 'unbounded' or 'free_surface') overrides wall; 'free_surface'
+    is the RPY of d plus the RPY of the image vector (d_x, d_y, z_i + z_j)
+    times diag(1, 1, -1), heights used as given."""
+    bnd = _radii_boundary(wall, boundary)
     lib = _lib.load()
     device = _device()
     src = torch.as_tensor(source)
@@ -223,17 +240,20 @@ def apply_radii(source, target, force, radius_source, radius_target, eta: float,
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_radii_workspace_size(prec, ns, nt, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device, slot="radii")
-    _lib.check(lib.rmw_radii_apply(prec, int(bool(wall)), src.data_ptr(), trg.data_ptr(), f.data_ptr(),
+    _lib.check(lib.rmw_radii_apply(prec, bnd, src.data_ptr(), trg.data_ptr(), f.data_ptr(),
                                    rs.data_ptr(), rt.data_ptr(), u.data_ptr(), ns, nt, float(eta),
                                    _L_array(periodic_length), tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
     return u
 
 
 def apply_radii_self(r_vectors, force, radius, eta: float, periodic_length=None, wall: bool = True,
-                     precision="fp32_blocked", tuning=None, out: Optional[torch.Tensor] = None):
+                     precision="fp32_blocked", tuning=None, out: Optional[torch.Tensor] = None,
+                     boundary: Optional[str] = None):
     """Different radii, sources = targets: u (N, 3) = B M B f (wall) or the
-    unbounded RPY of different radii (wall=False).  Each unordered pair is
-    evaluated once when the symmetric kernel is chosen (no periodic box)."""
+    unbounded RPY of different radii (wall=False); boundary as apply_radii.
+    Each unordered pair is evaluated once when the symmetric kernel is
+    chosen (no periodic box)."""
+    bnd = _radii_boundary(wall, boundary)
     lib = _lib.load()
     device = _device()
     r = torch.as_tensor(r_vectors)
@@ -250,7 +270,7 @@ def apply_radii_self(r_vectors, force, radius, eta: float, periodic_length=None,
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_radii_self_workspace_size(prec, n, L, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device, slot="radii")
-    _lib.check(lib.rmw_radii_self_apply(prec, int(bool(wall)), r.data_ptr(), f.data_ptr(), rad.data_ptr(),
+    _lib.check(lib.rmw_radii_self_apply(prec, bnd, r.data_ptr(), f.data_ptr(), rad.data_ptr(),
                                         u.data_ptr(), n, float(eta), L, tp, ws.data_ptr(), ws.numel(),
                                         _stream_ptr(device)))
     return u

@@ -1,7 +1,7 @@
 """Blobs of different radii on the GPU against the FP64 oracle (source ->
-target, wall and unbounded, overlaps with the wall, periodic images, every
-precision mode), the equal-radii case against the single-radius kernels,
-and the drop-in signatures."""
+target, wall, unbounded and free surface, overlaps with the wall, periodic
+images, every precision mode), the equal-radii case against the
+single-radius kernels, and the drop-in signatures."""
 import numpy as np
 import pytest
 import torch
@@ -25,15 +25,18 @@ def _blobs(n, seed, side=12.0, zlo=0.05):
     return r, rng.uniform(0.1, 0.6, n), rng
 
 
+BOUNDARIES = ["wall", "unbounded", "free_surface"]
+
+
 @pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
-@pytest.mark.parametrize("wall", [True, False])
-def test_radii_self_interaction_matches_oracle(gpu, prec, wall):
+@pytest.mark.parametrize("bnd", BOUNDARIES)
+def test_radii_self_interaction_matches_oracle(gpu, prec, bnd):
     n = 700
     r, rad, rng = _blobs(n, 1)
     f = rng.standard_normal((n, 3))
-    print(f"case radii n={n} wall={wall} {prec}", flush=True)
-    u = wall_rpy.apply_radii(r, r, f, rad, rad, 0.9, wall=wall, precision=prec).cpu().numpy().ravel()
-    ref = O.radii_apply(r, r, f, rad, rad, 0.9, wall=wall)
+    print(f"case radii n={n} {bnd} {prec}", flush=True)
+    u = wall_rpy.apply_radii(r, r, f, rad, rad, 0.9, boundary=bnd, precision=prec).cpu().numpy().ravel()
+    ref = O.radii_apply(r, r, f, rad, rad, 0.9, boundary=bnd)
     assert rel(u, ref) <= TOL[prec]
 
 
@@ -42,13 +45,13 @@ def test_radii_source_target_sets_and_periodic(gpu, prec):
     src, rs, rng = _blobs(333, 2)
     trg, rt, _ = _blobs(517, 3)
     f = rng.standard_normal((333, 3))
-    for L in (None, np.array([7.0, 6.0, 0.0])):
+    for L in (None, np.array([7.0, 6.0, 0.0]), np.array([7.0, 6.0, 9.0])):
         print(f"case radii source/target 333 -> 517 L={L} {prec}", flush=True)
-        for wall in (True, False):
-            u = wall_rpy.apply_radii(src, trg, f, rs, rt, 1.2, periodic_length=L, wall=wall,
+        for bnd in BOUNDARIES:
+            u = wall_rpy.apply_radii(src, trg, f, rs, rt, 1.2, periodic_length=L, boundary=bnd,
                                      precision=prec).cpu().numpy().ravel()
-            ref = O.radii_apply(src, trg, f, rs, rt, 1.2, periodic_length=L, wall=wall)
-            assert rel(u, ref) <= TOL[prec], (L, wall)
+            ref = O.radii_apply(src, trg, f, rs, rt, 1.2, periodic_length=L, boundary=bnd)
+            assert rel(u, ref) <= TOL[prec], (L, bnd)
 
 
 def test_radii_work_split_and_edges(gpu):
@@ -76,6 +79,9 @@ def test_equal_radii_matches_single_radius_kernel(gpu):
     u_r = wall_rpy.apply_radii(r, r, f, rad, rad, 1.0, precision="fp64").cpu().numpy().ravel()
     u_uf, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=1.0, a=a, precision="fp64")
     assert rel(u_r, u_uf.cpu().numpy().ravel()) <= 1e-12
+    u_r = wall_rpy.apply_radii(r, r, f, rad, rad, 1.0, boundary="free_surface", precision="fp64")
+    u_fs, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=1.0, a=a, precision="fp64", domain="free_surface")
+    assert rel(u_r.cpu().numpy().ravel(), u_fs.cpu().numpy().ravel()) <= 1e-12
 
 
 def test_radii_dropin_signatures(gpu):
@@ -88,11 +94,23 @@ def test_radii_dropin_signatures(gpu):
     assert rel(fn(r, f, 0.7, 0.0), ref) <= 1e-12
     fn0 = multi_bodies.set_mobility_vector_prod("radii_hip_no_wall", radius_blobs=rad)
     assert rel(fn0(r, f, 0.7, 0.0), O.radii_apply(r, r, f, rad, rad, 0.7, wall=False)) <= 1e-6
+    ref_fs = O.radii_apply(r, r, f, rad, rad, 0.7, boundary="free_surface")
+    u = mobility.free_surface_mobility_trans_times_force_source_target_hip(r, r, f, rad, rad, 0.7, precision="fp64")
+    assert isinstance(u, np.ndarray) and u.shape == (900,) and rel(u, ref_fs) <= 1e-12
+    for name, tol in (("radii_hip_free_surface", 1e-6), ("radii_hip_free_surface_fp64", 1e-12)):
+        fn = multi_bodies.set_mobility_vector_prod(name, radius_blobs=rad)
+        assert rel(fn(r, f, 0.7, 0.0), ref_fs) <= tol, name
+    # an unknown boundary is refused by the library itself
+    lib = _lib.load()
+    with pytest.raises(ValueError):
+        wall_rpy.apply_radii(r, r, f, rad, rad, 0.7, boundary="two_walls")
+    assert lib.rmw_radii_apply(0, 3, None, None, None, None, None, None, 0, 0, 1.0, None, None, None, 0,
+                               None) == -1  # RMW_E_ARG
 
 
 @pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
-@pytest.mark.parametrize("wall", [True, False])
-def test_radii_symmetric_kernel_matches_oracle(gpu, prec, wall):
+@pytest.mark.parametrize("bnd", BOUNDARIES)
+def test_radii_symmetric_kernel_matches_oracle(gpu, prec, bnd):
     """Sources = targets through the symmetric kernel (forced on at this
     size): 1100 blobs = 4 full tiles and a ragged one, radii 0.1-0.6 so the
     far, overlapping and engulfed regimes all occur, blobs below their
@@ -100,17 +118,17 @@ def test_radii_symmetric_kernel_matches_oracle(gpu, prec, wall):
     n = 1100
     r, rad, rng = _blobs(n, 11, side=8.0)
     f = rng.standard_normal((n, 3))
-    ref = O.radii_apply(r, r, f, rad, rad, 0.9, wall=wall)
-    print(f"case radii symmetric n={n} wall={wall} {prec}", flush=True)
+    ref = O.radii_apply(r, r, f, rad, rad, 0.9, boundary=bnd)
+    print(f"case radii symmetric n={n} {bnd} {prec}", flush=True)
     for chunk in (0, 512):
         t = _lib.Tuning(chunk, 0, 0, 1, 0)
         assert wall_rpy.describe_radii_self(n, prec, tuning=t)["kernel"] == "symmetric"
-        u = wall_rpy.apply_radii_self(r, f, rad, 0.9, wall=wall, precision=prec, tuning=t).cpu().numpy().ravel()
+        u = wall_rpy.apply_radii_self(r, f, rad, 0.9, boundary=bnd, precision=prec, tuning=t).cpu().numpy().ravel()
         assert rel(u, ref) <= TOL[prec], chunk
     # the ordered kernel through the same entry point
     t = _lib.Tuning(0, 0, 0, -1, 0)
     assert wall_rpy.describe_radii_self(n, prec, tuning=t)["kernel"] == "ordered"
-    u = wall_rpy.apply_radii_self(r, f, rad, 0.9, wall=wall, precision=prec, tuning=t).cpu().numpy().ravel()
+    u = wall_rpy.apply_radii_self(r, f, rad, 0.9, boundary=bnd, precision=prec, tuning=t).cpu().numpy().ravel()
     assert rel(u, ref) <= TOL[prec]
 
 

@@ -8,6 +8,12 @@ RPY, mobility_numba.py:1479-1658 with the wrapper mobility.py:551-591).
   mathematics;
 * continuity of the RPY regimes at r = a + b and r = |a - b|;
 * positive definiteness of the dense matrix.
+
+The free-surface form (mobility_numba.py:1941-2093 behind
+mobility.py:1409-1429) is pinned by the method of images: it equals the
+unbounded product over the sources and their mirror images (x, y, -z)
+carrying D f, D = diag(1, 1, -1); by the equal-radii free-surface oracle;
+by reciprocity and by positive definiteness.
 """
 import numpy as np
 import pytest
@@ -48,9 +54,9 @@ def test_reciprocity_all_regimes():
         a, b = rng.uniform(0.05, 0.8, 2)
         d = np.linalg.norm(ri - rj)
         regimes.add(0 if d > a + b else (1 if d > abs(a - b) else 2))
-        for wall in (True, False):
-            A = O.radii_pair_matrix(ri, rj, a, b, 1.0, wall)
-            B = O.radii_pair_matrix(rj, ri, b, a, 1.0, wall)
+        for bnd in ("wall", "unbounded", "free_surface"):
+            A = O.radii_pair_matrix(ri, rj, a, b, 1.0, boundary=bnd)
+            B = O.radii_pair_matrix(rj, ri, b, a, 1.0, boundary=bnd)
             worst = max(worst, np.abs(A - B.T).max() / np.abs(A).max())
     assert regimes == {0, 1, 2}
     assert worst < 1e-13
@@ -90,13 +96,13 @@ def test_dense_matrix_is_spd():
     r, rng = _config(40, 5, zlo=0.9)
     rad = rng.uniform(0.1, 0.45, 40)
     n = 40
-    for wall in (True, False):
+    for bnd in ("wall", "unbounded", "free_surface"):
         M = np.zeros((3 * n, 3 * n))
         for j in range(n):
             for c in range(3):
                 e = np.zeros((n, 3))
                 e[j, c] = 1.0
-                M[:, 3 * j + c] = O.radii_apply(r, r, e, rad, rad, 1.0, wall=wall)
+                M[:, 3 * j + c] = O.radii_apply(r, r, e, rad, rad, 1.0, boundary=bnd)
         assert np.abs(M - M.T).max() <= 1e-13 * np.abs(M).max()
         assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
 
@@ -122,3 +128,58 @@ def test_source_target_sets_and_rows():
     part = O.radii_apply(src, trg, f, rs, rt, 0.8, rows=(5, 9)).reshape(17, 3)
     assert np.allclose(part[5:9], u.reshape(17, 3)[5:9], rtol=0, atol=0)
     assert not part[:5].any() and not part[9:].any()
+
+
+def _mirror_failures(kind="strict"):
+    """Free surface = unbounded product over sources + mirrored sources."""
+    fails = []
+    rng = np.random.default_rng(8)
+    for L in (None, (5.0, 4.0, 0.0)):
+        src = np.column_stack([rng.uniform(0, 4, 40), rng.uniform(0, 4, 40), rng.uniform(0.05, 1.5, 40)])
+        trg = np.column_stack([rng.uniform(0, 4, 25), rng.uniform(0, 4, 25), rng.uniform(0.05, 1.5, 25)])
+        rs, rt = rng.uniform(0.05, 0.6, 40), rng.uniform(0.05, 0.6, 25)
+        f = rng.standard_normal((40, 3))
+        u = O.radii_apply(src, trg, f, rs, rt, 0.7, periodic_length=L, boundary="free_surface", kind=kind)
+        img = src * [1.0, 1.0, -1.0]
+        ref = O.radii_apply(np.vstack([src, img]), trg, np.vstack([f, f * [1.0, 1.0, -1.0]]),
+                            np.concatenate([rs, rs]), rt, 0.7, periodic_length=L, wall=False, kind=kind)
+        if rel(u, ref) > 1e-13:
+            fails.append(f"mirror L={L}")
+    return fails
+
+
+def test_free_surface_is_the_mirrored_unbounded_product():
+    assert _mirror_failures() == []
+
+
+def test_free_surface_planted_mistake_is_caught():
+    O.set_mutation(9)
+    try:
+        assert _mirror_failures("mutant")
+    finally:
+        O.set_mutation(0)
+    assert _mirror_failures("mutant") == []
+
+
+@pytest.mark.parametrize("L", [None, (5.0, 4.0, 0.0), (5.0, 4.0, 6.0)])
+def test_free_surface_equal_radii_is_the_single_radius_oracle(L):
+    r, rng = _config(50, 9, zlo=0.05)
+    f = rng.standard_normal((50, 3))
+    a = 0.3
+    rad = np.full(50, a)
+    u = O.radii_apply(r, r, f, rad, rad, 1.1, periodic_length=L, boundary="free_surface")
+    ref = O.block_apply(O.UF_FREE_SURFACE, r, f, 1.1, a, periodic_length=L)
+    assert rel(u, ref) < 1e-13
+
+
+def test_free_surface_point_particles_at_the_surface():
+    """Point particles: no normal velocity on the surface and twice the
+    unbounded tangential velocity there."""
+    rng = np.random.default_rng(10)
+    for _ in range(20):
+        y = np.array([*rng.uniform(-1, 1, 2), rng.uniform(0.5, 2)])
+        x = np.array([*rng.uniform(-1, 1, 2), 0.0])
+        M = O.radii_pair_matrix(x, y, 1e-4, 2e-4, 1.0, boundary="free_surface")
+        M0 = O.radii_pair_matrix(x, y, 1e-4, 2e-4, 1.0, boundary="unbounded")
+        assert np.abs(M[2]).max() <= 1e-12 * np.abs(M).max()
+        assert np.abs(M[:2] - 2 * M0[:2]).max() <= 1e-7 * np.abs(M0).max()
