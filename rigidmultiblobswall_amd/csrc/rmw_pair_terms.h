@@ -419,6 +419,97 @@ RMW_HD Vec3<T> apply_wt_t(const Geom<T>& g, const CoefWT<T>& c, T vx, T vy, T vz
 }
 
 // ---------------------------------------------------------------------------
+// Explicit 3x3 blocks, m[r][c] = e_r . B e_c, of the four dyadic forms above
+// (read off apply_uf / apply_ut / apply_wf / apply_wt at the unit vectors).
+// A kernel that applies one pair's blocks to several right-hand sides forms
+// them once and then spends 9 FMAs per block, side and right-hand side
+// (the transposed blocks are the same entries read by columns).
+// ---------------------------------------------------------------------------
+template <typename T> struct Mat3 { T m[3][3]; };
+
+template <typename T>
+RMW_HD Mat3<T> mat_uf(const Geom<T>& g, const CoefUF<T>& c)
+{
+  Mat3<T> M;
+  const T P = c.Bc + c.g2;
+  const T h = fma_(c.Bc, g.dz, c.g2 * g.Rz);
+  const T a3 = h + c.g3, b3 = h + c.g4;
+  const T Pdx = P * g.dx;
+  M.m[0][0] = fma_(Pdx, g.dx, c.s);
+  M.m[0][1] = Pdx * g.dy;
+  M.m[0][2] = a3 * g.dx;
+  M.m[1][0] = M.m[0][1];
+  M.m[1][1] = fma_(P * g.dy, g.dy, c.s);
+  M.m[1][2] = a3 * g.dy;
+  M.m[2][0] = b3 * g.dx;
+  M.m[2][1] = b3 * g.dy;
+  M.m[2][2] = fma_(c.Bc * g.dz, g.dz, fma_(fma_(c.g2, g.Rz, c.g3 + c.g4), g.Rz, c.s5));
+  return M;
+}
+
+// UT (coefficients of coef_rot(g, z_i)); m[2][2] = 0.
+template <typename T>
+RMW_HD Mat3<T> mat_ut(const Geom<T>& g, const CoefRot<T>& c)
+{
+  Mat3<T> M;
+  const T G = fma_(c.c, g.dz, fma_(c.F1r, g.Rz, c.F4));
+  const T K = c.c - c.F1r - fma_(c.F3r2, g.Rz, c.F2r);
+  const T cf = c.c - c.F1r;
+  const T Fx = c.F3r2 * g.dx;
+  M.m[0][0] = Fx * g.dy;
+  M.m[0][1] = fma_(-Fx, g.dx, G);
+  M.m[0][2] = -cf * g.dy;
+  M.m[1][0] = fma_(c.F3r2 * g.dy, g.dy, -G);
+  M.m[1][1] = -M.m[0][0];
+  M.m[1][2] = cf * g.dx;
+  M.m[2][0] = K * g.dy;
+  M.m[2][1] = -K * g.dx;
+  M.m[2][2] = T(0);
+  return M;
+}
+
+// WF (coefficients of coef_rot(g, z_j)); m[2][2] = 0.
+template <typename T>
+RMW_HD Mat3<T> mat_wf(const Geom<T>& g, const CoefRot<T>& c)
+{
+  Mat3<T> M;
+  const T G = fma_(c.c, g.dz, -fma_(c.F1r, g.Rz, c.F4));
+  const T K = c.c - c.F1r - fma_(c.F3r2, g.Rz, c.F2r);
+  const T cf = c.c - c.F1r;
+  const T Fx = c.F3r2 * g.dx;
+  M.m[0][0] = Fx * g.dy;
+  M.m[0][1] = fma_(c.F3r2 * g.dy, g.dy, G);
+  M.m[0][2] = -K * g.dy;
+  M.m[1][0] = -fma_(Fx, g.dx, G);
+  M.m[1][1] = -M.m[0][0];
+  M.m[1][2] = K * g.dx;
+  M.m[2][0] = cf * g.dy;
+  M.m[2][1] = -cf * g.dx;
+  M.m[2][2] = T(0);
+  return M;
+}
+
+template <typename T>
+RMW_HD Mat3<T> mat_wt(const Geom<T>& g, const CoefWT<T>& c)
+{
+  Mat3<T> M;
+  const T P = c.Bc + c.G2r;
+  const T h = fma_(c.Bc, g.dz, c.G2r * g.Rz);
+  const T h3 = h + c.G3r;
+  const T dxy = g.dx * g.dy;
+  M.m[0][0] = fma_(P * g.dx, g.dx, fma_(c.G4r * g.dy, g.dy, c.s));
+  M.m[0][1] = (P - c.G4r) * dxy;
+  M.m[0][2] = h * g.dx;
+  M.m[1][0] = M.m[0][1];
+  M.m[1][1] = fma_(P * g.dy, g.dy, fma_(c.G4r * g.dx, g.dx, c.s));
+  M.m[1][2] = h * g.dy;
+  M.m[2][0] = h3 * g.dx;
+  M.m[2][1] = h3 * g.dy;
+  M.m[2][2] = fma_(c.Bc * g.dz, g.dz, fma_(fma_(c.G2r, g.Rz, c.G3r), g.Rz, c.s));
+  return M;
+}
+
+// ---------------------------------------------------------------------------
 // Blobs of different radii (target radius a, source radius b), lengths in
 // absolute units, normalisation 1/(8 pi eta) folded in by the caller.
 // mobility_numba.py:1479-1658.  The self pair (d = 0) needs no special

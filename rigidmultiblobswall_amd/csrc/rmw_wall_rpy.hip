@@ -497,6 +497,8 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
   constexpr bool UF = (MASK & kUF) != 0, UT = (MASK & kUT) != 0, WF = (MASK & kWF) != 0, WT = (MASK & kWT) != 0;
   constexpr int OC = BT::OC;
   constexpr bool kFoldJ = ACC == RMW_PREC_F32_BLOCKED;
+  // several right-hand sides: form each pair's blocks once as 3x3 matrices
+  constexpr bool kMat = K >= 2;
   using JT = typename std::conditional<kFoldJ, float, double>::type;
   __shared__ double s_x[kBlock], s_y[kBlock], s_z[kBlock];
   __shared__ R s_zr[kBlock];
@@ -558,6 +560,16 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
             s_t[k][0][il] = (R)(c * v[0]); s_t[k][1][il] = (R)(c * v[1]); s_t[k][2][il] = (R)(pz * c * v[2]);
           }
         }
+      } else if constexpr (kMat) {
+        // zero source vectors beyond the system: the matrix form multiplies
+        // them by the masked (zero) blocks, so they must be finite
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            if constexpr (BT::NF) s_f[k][c][il] = R(0);
+            if constexpr (BT::NT) s_t[k][c][il] = R(0);
+          }
       }
       __syncthreads();
       const bool diag = S == T;
@@ -595,6 +607,66 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
           if constexpr (UF) cuf = coef_uf(g, zi, zj, DOM);
           if constexpr (UT || WF) { cut = coef_rot(g, zi, DOM); cwf = coef_rot(g, zj, DOM); }
           if constexpr (WT) cwt = coef_wt(g, DOM);
+          if constexpr (kMat) {
+            // the pair's blocks as 3x3 matrices, zeroed for pairs not taken
+            Mat3<R> Mu, Mc, Md, Mw;   // UF, UT, WF, WT of (i, j)
+            if constexpr (UF) Mu = mat_uf(g, cuf);
+            if constexpr (UT) Mc = mat_ut(g, cut);
+            if constexpr (WF) Md = mat_wf(g, cwf);
+            if constexpr (WT) Mw = mat_wt(g, cwt);
+            if constexpr (MASKED) {
+#pragma unroll
+              for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  if constexpr (UF) Mu.m[r][c] = on ? Mu.m[r][c] : R(0);
+                  if constexpr (UT) Mc.m[r][c] = on ? Mc.m[r][c] : R(0);
+                  if constexpr (WF) Md.m[r][c] = on ? Md.m[r][c] : R(0);
+                  if constexpr (WT) Mw.m[r][c] = on ? Mw.m[r][c] : R(0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              R fj[3], tj[3];
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                if constexpr (BT::NF) fj[c] = s_f[k][c][jl];
+                if constexpr (BT::NT) tj[c] = s_t[k][c][jl];
+              }
+#pragma unroll
+              for (int r = 0; r < 3; ++r) {
+                // target side: u_i += UF f_j + UT t_j, w_i += WF f_j + WT t_j
+                // source side: u_j += UF^T f_i + WF^T t_i, w_j += UT^T f_i + WT^T t_i
+                if constexpr (BT::HU) {
+                  R x = R(0), y = R(0);
+                  if constexpr (UF) {
+                    x = fma_(Mu.m[r][0], fj[0], fma_(Mu.m[r][1], fj[1], Mu.m[r][2] * fj[2]));
+                    y = fma_(Mu.m[0][r], fi[k][0], fma_(Mu.m[1][r], fi[k][1], Mu.m[2][r] * fi[k][2]));
+                  }
+                  if constexpr (UT)
+                    x = fma_(Mc.m[r][0], tj[0], fma_(Mc.m[r][1], tj[1], r == 2 ? x : fma_(Mc.m[r][2], tj[2], x)));
+                  if constexpr (WF)
+                    y = fma_(Md.m[0][r], ti[k][0], fma_(Md.m[1][r], ti[k][1], r == 2 ? y : fma_(Md.m[2][r], ti[k][2], y)));
+                  u[k][r].add(x);
+                  ja[k][r] = rol1(ja[k][r]) + (JT)y;
+                }
+                if constexpr (BT::HW) {
+                  constexpr int o = BT::HU ? 3 : 0;
+                  R x = R(0), y = R(0);
+                  if constexpr (WT) {
+                    x = fma_(Mw.m[r][0], tj[0], fma_(Mw.m[r][1], tj[1], Mw.m[r][2] * tj[2]));
+                    y = fma_(Mw.m[0][r], ti[k][0], fma_(Mw.m[1][r], ti[k][1], Mw.m[2][r] * ti[k][2]));
+                  }
+                  if constexpr (WF)
+                    x = fma_(Md.m[r][0], fj[0], fma_(Md.m[r][1], fj[1], r == 2 ? x : fma_(Md.m[r][2], fj[2], x)));
+                  if constexpr (UT)
+                    y = fma_(Mc.m[0][r], fi[k][0], fma_(Mc.m[1][r], fi[k][1], r == 2 ? y : fma_(Mc.m[2][r], fi[k][2], y)));
+                  w[k][r].add(x);
+                  ja[k][o + r] = rol1(ja[k][o + r]) + (JT)y;
+                }
+              }
+            }
+          } else {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             // target side: u_i += UF f_j + UT t_j, w_i += WF f_j + WT t_j
@@ -642,6 +714,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
               ja[k][o + 1] = rol1(ja[k][o + 1]) + (on ? (JT)wj.y : JT(0));
               ja[k][o + 2] = rol1(ja[k][o + 2]) + (on ? (JT)wj.z : JT(0));
             }
+          }
           }
           if ((st & (kSub - 1)) == kSub - 1) {
             flush_all<K, ACC>(u, w);
@@ -1059,10 +1132,10 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   if (pl.mut_ok && sym == 0) pl.mut_ok = tile_pairs >= device_props().cu_count;
   size_t mut_total = 0;
   if (pl.mut_ok) {
-    // 2 right-hand sides per pass unless the tuning asks for 4 (the source-side
-    // sums of 4 take 49 KB of LDS and ~220 VGPRs: two workgroups per CU)
+    // up to 4 right-hand sides per pass (each pair's blocks formed once as
+    // 3x3 matrices and applied to all of them), fewer when the tuning caps it
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 4) : 2) : 1;
+    pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 4) : 4) : 1;
     if (tuning && tuning->chunk_len > 0) {
       // 64 or 128 sources: a quarter or half of a tile pair per work item
       // (finer work for suspensions of few tiles); else whole source tiles
@@ -1090,6 +1163,9 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
     }
     pl.mut_items = item;
     const int OCm = mask == kUF ? 3 : 6;
+    if (rc <= 0 && pl.kmax_mut > 2 &&
+        (size_t)(pi + pj) * pl.kmax_mut * kBlock * OCm * sizeof(double) > kMutAutoBytes)
+      pl.kmax_mut = group_k(k, 2);   // partial sums of 4 RHS too large: 2 per pass
     size_t o = 0;
     pl.off_pi = o; o = align_up(o + (size_t)pi * pl.kmax_mut * kBlock * OCm * sizeof(double));
     pl.off_pj = o; o = align_up(o + (size_t)pj * pl.kmax_mut * kBlock * OCm * sizeof(double));

@@ -35,15 +35,35 @@ static Vec3<T> apply_self(int block, T h, T vx, T vy, T vz)
   }
 }
 
+// Blocks 8-11: the explicit matrices mat_uf, mat_ut, mat_wf, mat_wt.
+template <typename T>
+static Mat3<T> explicit_block(int block, const Geom<T>& g, T zi, T zj)
+{
+  switch (block) {
+    case 8: return mat_uf(g, coef_uf(g, zi, zj));
+    case 9: return mat_ut(g, coef_rot(g, zi));
+    case 10: return mat_wf(g, coef_rot(g, zj));
+    default: return mat_wt(g, coef_wt(g));
+  }
+}
+
 template <typename T>
 static int pair_matrix(int block, const double* ri, const double* rj, int self, double eta, double a, double* m9)
 {
-  if (block < 0 || block > 7) return -1;
-  const int p = (block == 0 || block == 4) ? 1 : ((block == 3 || block == 7) ? 3 : 2);
+  if (block < 0 || block > 11) return -1;
+  const int b4 = block & 3;
+  const int p = b4 == 0 ? 1 : (b4 == 3 ? 3 : 2);
   const double norm = 1.0 / (8.0 * M_PI * eta * std::pow(a, p));
   T dx = T((ri[0] - rj[0]) / a), dy = T((ri[1] - rj[1]) / a), dz = T((ri[2] - rj[2]) / a);
   T zi = T(ri[2] / a), zj = T(rj[2] / a);
   Geom<T> g = make_geom(dx, dy, dz, zi, zj);
+  if (block >= 8) {
+    if (self) return -1;
+    const Mat3<T> M = explicit_block<T>(block, g, zi, zj);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) m9[3 * r + c] = double(M.m[r][c]) * norm;
+    return 0;
+  }
   for (int c = 0; c < 3; ++c) {
     T v[3] = {T(0), T(0), T(0)};
     v[c] = T(1);

@@ -91,6 +91,41 @@ def test_pair_term_header_matches_oracle(block):
     assert worst32 < 2e-5
 
 
+# explicit matrix forms (blocks 8-11) -> oracle block of the same pair, and
+# oracle block of the swapped pair that their transposes must equal
+EXPLICIT = {8: (O.UF, O.UF), 9: (O.UT, O.WF), 10: (O.WF, O.UT), 11: (O.WT, O.WT)}
+
+
+@pytest.mark.parametrize("eblock", sorted(EXPLICIT))
+def test_explicit_block_matrices_match_oracle(eblock):
+    """mat_uf / mat_ut / mat_wf / mat_wt (the blocks the symmetric kernels
+    form once per pair when several right-hand sides share it) equal the
+    oracle's block of (i, j), and read by columns its block of (j, i)."""
+    lib = _host_lib()
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rng = np.random.default_rng(31 + eblock)
+    a, eta = 0.3, 0.7
+    block, swapped = EXPLICIT[eblock]
+    worst = worst_t = worst32 = 0.0
+    for t in range(2000):
+        ri = np.array([*rng.uniform(-3, 3, 2), rng.uniform(a, 3)])
+        rj = np.array([*rng.uniform(-3, 3, 2), rng.uniform(a, 3)])
+        if t % 5 == 0:   # overlapping pairs (near RPY branch)
+            rj = ri + rng.uniform(-0.25, 0.25, 3)
+            rj[2] = max(rj[2], a)
+        m, m32 = np.empty(9), np.empty(9)
+        assert lib.pt_pair_matrix_f64(eblock, P(ri), P(rj), 0, eta, a, P(m)) == 0
+        assert lib.pt_pair_matrix_f32(eblock, P(ri), P(rj), 0, eta, a, P(m32)) == 0
+        ref = O.pair_matrix(block, ri, rj, False, eta, a).ravel()
+        ref_t = O.pair_matrix(swapped, rj, ri, False, eta, a).T.ravel()
+        sc = np.abs(ref).max()
+        worst = max(worst, np.abs(m - ref).max() / sc)
+        worst_t = max(worst_t, np.abs(m - ref_t).max() / sc)
+        worst32 = max(worst32, np.abs(m32 - ref).max() / sc)
+    assert worst < 1e-13 and worst_t < 1e-13
+    assert worst32 < 2e-5
+
+
 # transposed block of the pair (i, j) -> (oracle block of the pair (j, i),
 # untransposed block of (i, j) that it must differ from)
 TRANSPOSES = {4: (O.UF, 0), 5: (O.WF, 1), 6: (O.UT, 2), 7: (O.WT, 3)}
