@@ -49,9 +49,11 @@ typedef struct rmw_tuning {
   int64_t chunk_len;  /* sources per work item (rounded up to a multiple of 64; 0 = auto) */
   int32_t max_wgs;    /* cap on resident workgroups of persistent launches (0 = auto) */
   int32_t rhs_group;  /* right-hand sides per pair-kernel pass: 1, 2 or 4 (0 = auto) */
-  int32_t symmetric;  /* UF alone on one suspension without periodic images: each
-                         unordered pair evaluated once for both blobs (UF_ji = UF_ij^T).
-                         0 = when possible, 1 = required (error otherwise), -1 = off */
+  int32_t symmetric;  /* UF alone, UF + UT (wall) or all four blocks, without periodic
+                         images: each unordered pair evaluated once for both blobs
+                         (UF_ji = UF_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, WT_ji = WT_ij^T).
+                         0 = when the tile pairs fill the device, 1 = required (error
+                         otherwise), -1 = off */
   int32_t reserved;
 } rmw_tuning;
 

@@ -224,7 +224,7 @@ def tune(mask: int, precision: str, r: np.ndarray, offsets: Optional[np.ndarray]
     chunks = chunk_candidates(nmax)
     if max_candidates:
         chunks = chunks[:max_candidates]
-    sym_possible = mask in (RMW_UF, RMW_UF | RMW_UT | RMW_WF | RMW_WT) and periodic_length is None
+    sym_possible = mask in (RMW_UF, RMW_UF | RMW_UT, RMW_UF | RMW_UT | RMW_WF | RMW_WT) and periodic_length is None
     stage1 = [trial({"chunk_len": c, "max_wgs": 0, "rhs_group": 0, "symmetric": 0}) for c in chunks]
     if sym_possible:
         stage1 += [trial({"chunk_len": c, "max_wgs": 0, "rhs_group": 0, "symmetric": -1}) for c in chunks]
@@ -263,7 +263,7 @@ def tune(mask: int, precision: str, r: np.ndarray, offsets: Optional[np.ndarray]
 def _jobs(names: Iterable[str]):
     """(label, mask, precision, k, geometry) of the benchmark workloads."""
     from . import workloads
-    from ._lib import RMW_UF
+    from ._lib import RMW_UF, RMW_UT
     grand = 15
     for name in names:
         if name == "c1":
@@ -277,6 +277,8 @@ def _jobs(names: Iterable[str]):
         elif name == "c3":
             w = workloads.c3_rollers()
             yield "c3", RMW_UF, "fp32_blocked", 1, w["r"], None, w["eta"], w["a"]
+            # the rollers' fused u = UF f + UT t (constrained kinematics, RFD)
+            yield "c3", RMW_UF | RMW_UT, "fp32_blocked", 1, w["r"], None, w["eta"], w["a"]
             for prec in ("fp32_blocked", "fp64"):
                 yield "c3", grand, prec, 4, w["r"], None, w["eta"], w["a"]
         elif name == "c4":

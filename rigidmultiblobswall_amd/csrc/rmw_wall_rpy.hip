@@ -1120,9 +1120,11 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   const int sym = tuning ? tuning->symmetric : 0;
   // the other boundaries: the UF block (and the unbounded grand mobility),
   // one right-hand side per pass
-  pl.mut_ok = (mask == kUF || (mask == (kUF | kUT | kWF | kWT) && (domain == kDomWall || domain == kDomNoWall))) &&
+  pl.mut_ok = (mask == kUF || (mask == (kUF | kUT) && domain == kDomWall) ||
+               (mask == (kUF | kUT | kWF | kWT) && (domain == kDomWall || domain == kDomNoWall))) &&
               pl.ntot > 0 && sym >= 0;
-  RMW_REQUIRE(!(sym > 0 && !pl.mut_ok), "symmetric evaluation needs the UF block alone or all four blocks");
+  RMW_REQUIRE(!(sym > 0 && !pl.mut_ok),
+              "symmetric evaluation needs the UF block alone, UF + UT (wall) or all four blocks");
   int64_t tile_pairs = 0;
   for (int s = 0; s < nsys; ++s) {
     const int64_t nt = (offsets[s + 1] - offsets[s] + kBlock - 1) / kBlock;
@@ -1135,7 +1137,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
     // up to 4 right-hand sides per pass (each pair's blocks formed once as
     // 3x3 matrices and applied to all of them), fewer when the tuning caps it
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = domain == kDomWall ? group_k(k, rc > 0 ? std::min(rc, 4) : 4) : 1;
+    pl.kmax_mut = domain == kDomWall && mask != (kUF | kUT) ? group_k(k, rc > 0 ? std::min(rc, 4) : 4) : 1;
     if (tuning && tuning->chunk_len > 0) {
       // 64 or 128 sources: a quarter or half of a tile pair per work item
       // (finer work for suspensions of few tiles); else whole source tiles
@@ -1162,7 +1164,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
       pl.msys.push_back(M);
     }
     pl.mut_items = item;
-    const int OCm = mask == kUF ? 3 : 6;
+    const int OCm = (mask & (kWF | kWT)) ? 6 : 3;
     if (rc <= 0 && pl.kmax_mut > 2 &&
         (size_t)(pi + pj) * pl.kmax_mut * kBlock * OCm * sizeof(double) > kMutAutoBytes)
       pl.kmax_mut = group_k(k, 2);   // partial sums of 4 RHS too large: 2 per pass
@@ -1366,6 +1368,9 @@ static void dispatch_mutual(int domain, int mask, int prec, int K, const MutPara
     dispatch_mutual_dom<kUF, kDomFreeSurface>(prec, q, grid, st);
   } else if (mask == kUF) {
     dispatch_mutual_m<kUF>(prec, K, q, grid, st);
+  } else if (mask == (kUF | kUT)) {
+    // u = UF f + UT t: target side UF, UT; source side UF^T f_i + WF^T t_i
+    dispatch_mutual_dom<kUF | kUT, kDomWall>(prec, q, grid, st);
   } else {
     dispatch_mutual_m<kUF | kUT | kWF | kWT>(prec, K, q, grid, st);
   }

@@ -162,11 +162,36 @@ def test_symmetric_grand_vs_oracle(gpu):
         assert rel(got, ref) <= tol, prec
 
 
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
+def test_symmetric_uf_ut_vs_oracle_and_ordered(gpu, prec):
+    """u = UF f + UT t (the fused product of mobility.py:384-410) through the
+    symmetric kernel: target side UF and UT, source side UF^T f_i + WF^T t_i
+    (UT_ji = WF_ij^T); 700 blobs (a ragged last tile), two right-hand sides."""
+    n, k = 700, 2
+    r = _geom(n, 33)
+    rng = np.random.default_rng(34)
+    f, t = rng.standard_normal((k, n, 3)), rng.standard_normal((k, n, 3))
+    mask = RMW_UF | RMW_UT
+    print(f"case symmetric UF+UT n={n} k={k} {prec}", flush=True)
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    sym = _lib.Tuning(0, 0, 0, 1, 0)
+    assert wall_rpy.describe(mask, n, k, prec, tuning=sym)["kernel"] == "symmetric"
+    us, ws = wall_rpy.apply(mask, r, force=f, torque=t, eta=0.9, a=0.3, precision=prec, k=k, tuning=sym)
+    assert ws is None or ws.numel() == 0 or not torch.any(ws)
+    uo, _ = wall_rpy.apply(mask, r, force=f, torque=t, eta=0.9, a=0.3, precision=prec, k=k,
+                           tuning=_lib.Tuning(0, 0, 0, -1, 0))
+    us, uo = us.cpu().numpy().reshape(k, -1), uo.cpu().numpy().reshape(k, -1)
+    for j in range(k):
+        ref = O.block_apply(O.UF, r, f[j], 0.9, 0.3) + O.block_apply(O.UT, r, t[j], 0.9, 0.3)
+        assert rel(us[j], ref) <= tol, j
+        assert rel(uo[j], ref) <= tol, j
+
+
 SIZES_R = [0, 300, 7, 0, 600, 1, 256, 257]
 
 
 @pytest.mark.parametrize("prec", ["fp32_blocked", "fp64"])
-@pytest.mark.parametrize("mask", [RMW_UF, GRAND])
+@pytest.mark.parametrize("mask", [RMW_UF, RMW_UF | RMW_UT, GRAND])
 def test_symmetric_ragged_matches_ordered_and_oracle(gpu, prec, mask):
     off = np.concatenate([[0], np.cumsum(SIZES_R)]).astype(np.int64)
     n = int(off[-1])
@@ -176,7 +201,7 @@ def test_symmetric_ragged_matches_ordered_and_oracle(gpu, prec, mask):
     print(f"case symmetric ragged mask={mask} {prec}", flush=True)
 
     def run(sym, chunk=0, wgs=0):
-        u, w = wall_rpy.apply_ragged(mask, r, off, force=f, torque=t if mask == GRAND else None, eta=0.9, a=0.3,
+        u, w = wall_rpy.apply_ragged(mask, r, off, force=f, torque=t if mask != RMW_UF else None, eta=0.9, a=0.3,
                                      precision=prec, tuning=_lib.Tuning(chunk, wgs, 0, sym, 0))
         parts = [u.cpu().numpy()] + ([w.cpu().numpy()] if w is not None else [])
         return parts
@@ -198,8 +223,9 @@ def test_symmetric_ragged_matches_ordered_and_oracle(gpu, prec, mask):
             continue
         rr = r[b0:b1]
         ou = O.block_apply(O.UF, rr, f[b0:b1], 0.9, 0.3)
-        if mask == GRAND:
+        if mask & RMW_UT:
             ou = ou + O.block_apply(O.UT, rr, t[b0:b1], 0.9, 0.3)
+        if mask == GRAND:
             ow = O.block_apply(O.WF, rr, f[b0:b1], 0.9, 0.3) + O.block_apply(O.WT, rr, t[b0:b1], 0.9, 0.3)
             assert rel(sym[1][b0:b1].ravel(), ow) <= (1e-12 if prec == "fp64" else 2e-6), s
         assert rel(sym[0][b0:b1].ravel(), ou) <= (1e-12 if prec == "fp64" else 2e-6), s
