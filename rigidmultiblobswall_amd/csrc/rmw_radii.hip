@@ -59,7 +59,7 @@ __device__ __forceinline__ void wall_shift(double z, double rad, int bnd, double
   }
 }
 
-template <int ACC, typename T>
+template <int ACC, typename T, int BND>
 __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
 {
   __shared__ double s_x[kRBlock], s_y[kRBlock], s_z[kRBlock];
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
     const double xi = p.trg[3 * gi], yi = p.trg[3 * gi + 1];
     const double a = p.rt[gi];
     double zi, Bi;
-    wall_shift(p.trg[3 * gi + 2], a, p.bnd, zi, Bi);
+    wall_shift(p.trg[3 * gi + 2], a, BND, zi, Bi);
     const T ar = (T)a, inva = (T)(1.0 / a), zir = (T)zi;
     Acc<ACC> u[3];
 
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
         const int64_t j = s0 + il;
         const double b = p.rs[j];
         double zj, Bj;
-        wall_shift(p.src[3 * j + 2], b, p.bnd, zj, Bj);
+        wall_shift(p.src[3 * j + 2], b, BND, zj, Bj);
         s_x[il] = p.src[3 * j];
         s_y[il] = p.src[3 * j + 1];
         s_z[il] = zj;
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
         const T Rz = zir + zj;
         if (!p.periodic) {
           const T dx = (T)ddx, dy = (T)ddy, dz = (T)ddz;
-          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], p.bnd);
+          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], BND);
           const Vec3<T> v = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][j], s_f[1][j], s_f[2][j]);
           u[0].add(v.x);
           u[1].add(v.y);
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kRBlock) void radii_kernel(RadiiParams p)
               for (int bz = -pbox[2]; bz <= pbox[2]; ++bz) {
                 const T dx = (T)(ddx + bx * p.L[0]), dy = (T)(ddy + by * p.L[1]), dz = (T)(ddz + bz * p.L[2]);
                 const CoefRadii<T> c =
-                    coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], p.bnd);
+                    coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[j], inva, s_ib[j], BND);
                 const Vec3<T> v = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][j], s_f[1][j], s_f[2][j]);
                 u[0].add(v.x);
                 u[1].add(v.y);
@@ -244,7 +244,7 @@ struct RadiiMutParams {
 // precision's accumulator; source-side sums rotate one lane per step
 // (DPP) so that lane l ends with source (l + 63) mod 64 of the block, in
 // FP32 sub-sums of kSub steps folded into FP64 in the blocked mode.
-template <int ACC, typename T>
+template <int ACC, typename T, int BND>
 __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
 {
   constexpr bool kFoldJ = ACC == RMW_PREC_F32_BLOCKED;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
     const double xi = p.r[3 * gi], yi = p.r[3 * gi + 1];
     const double a = p.rad[gi];
     double zi, Bi;
-    wall_shift(p.r[3 * gi + 2], a, p.bnd, zi, Bi);
+    wall_shift(p.r[3 * gi + 2], a, BND, zi, Bi);
     const T ar = (T)a, inva = (T)(1.0 / a), zir = (T)zi;
     T fi[3];
     {
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
         const int64_t j = s0 + il;
         const double b = p.rad[j];
         double zj, Bj;
-        wall_shift(p.r[3 * j + 2], b, p.bnd, zj, Bj);
+        wall_shift(p.r[3 * j + 2], b, BND, zj, Bj);
         s_x[il] = p.r[3 * j]; s_y[il] = p.r[3 * j + 1]; s_z[il] = zj;
         s_zr[il] = (T)zj; s_b[il] = (T)b; s_ib[il] = (T)(1.0 / b);
         const double c = p.norm * Bj;
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
           }
           const T zj = s_zr[jl];
           const T Rz = zir + zj;
-          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[jl], inva, s_ib[jl], p.bnd);
+          const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[jl], inva, s_ib[jl], BND);
           const Vec3<T> vi = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][jl], s_f[1][jl], s_f[2][jl]);
           const Vec3<T> vj = apply_radii_t<T>(dx, dy, dz, Rz, c, fi[0], fi[1], fi[2]);
           u[0].add(on ? vi.x : T(0));
@@ -427,6 +427,26 @@ extern "C" int rmw_radii_workspace_size(int prec, int64_t ns, int64_t nt, const 
   });
 }
 
+// One instantiation per precision and boundary (the boundary's terms are
+// resolved at compile time).
+template <int BND>
+static void launch_radii(int prec, dim3 g, dim3 b, hipStream_t st, const RadiiParams& q)
+{
+  if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_F64ACC, float, BND>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_BLOCKED, float, BND>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_KAHAN, float, BND>), g, b, 0, st, q);
+  else hipLaunchKernelGGL((radii_kernel<RMW_PREC_F64, double, BND>), g, b, 0, st, q);
+}
+
+template <int BND>
+static void launch_radii_mutual(int prec, dim3 g, dim3 b, hipStream_t st, const RadiiMutParams& q)
+{
+  if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_F64ACC, float, BND>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_BLOCKED, float, BND>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_KAHAN, float, BND>), g, b, 0, st, q);
+  else hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F64, double, BND>), g, b, 0, st, q);
+}
+
 // The ordered pair kernel and its reduce, source -> target.
 static void radii_ordered(const RadiiPlan& pl, int prec, int bnd, const double* source, const double* target,
                           const double* force, const double* radius_source, const double* radius_target,
@@ -453,10 +473,9 @@ static void radii_ordered(const RadiiPlan& pl, int prec, int bnd, const double* 
   ProfRec rec{};
   bool prof = false;
   prof_begin(rec, prof, st, 1);
-  if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
-  else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
-  else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
-  else hipLaunchKernelGGL((radii_kernel<RMW_PREC_F64, double>), g, b, 0, st, q);
+  if (bnd == kRadiiWall) launch_radii<kRadiiWall>(prec, g, b, st, q);
+  else if (bnd == kRadiiFreeSurface) launch_radii<kRadiiFreeSurface>(prec, g, b, st, q);
+  else launch_radii<kRadiiUnbounded>(prec, g, b, st, q);
   RMW_CHECK_HIP(hipGetLastError());
   prof_end(rec, prof, st);
   hipLaunchKernelGGL(radii_reduce_kernel, dim3((unsigned)((nt * 3 + 255) / 256)), dim3(256), 0, st, q.partial,
@@ -551,10 +570,9 @@ extern "C" int rmw_radii_self_apply(int prec, int boundary, const double* r, con
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st, 1);
-    if (prec == RMW_PREC_F32_F64ACC) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_F64ACC, float>), g, b, 0, st, q);
-    else if (prec == RMW_PREC_F32_BLOCKED) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_BLOCKED, float>), g, b, 0, st, q);
-    else if (prec == RMW_PREC_F32_KAHAN) hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F32_KAHAN, float>), g, b, 0, st, q);
-    else hipLaunchKernelGGL((radii_mutual_kernel<RMW_PREC_F64, double>), g, b, 0, st, q);
+    if (boundary == kRadiiWall) launch_radii_mutual<kRadiiWall>(prec, g, b, st, q);
+    else if (boundary == kRadiiFreeSurface) launch_radii_mutual<kRadiiFreeSurface>(prec, g, b, st, q);
+    else launch_radii_mutual<kRadiiUnbounded>(prec, g, b, st, q);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
     hipLaunchKernelGGL(radii_mutual_reduce_kernel, dim3((unsigned)((n * 3 + 255) / 256)), dim3(256), 0, st, q.part_i,
