@@ -474,6 +474,8 @@ struct MutParams {
   MutSys single;
   int nsys;
   int64_t n_items, chunk_blocks;   // work item: a run of chunk_blocks 64-source blocks
+  int periodic;             // bit d set when L_d > 0 (the PER instantiations)
+  double L[3];              // periodic lengths / a
 };
 
 __device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t item)
@@ -490,7 +492,13 @@ __device__ __forceinline__ MutSys mut_sys_of_item(const MutParams& p, int64_t it
 // Symmetric pair kernel.  R = float: FP32 pair terms (ACC: the target-side
 // accumulator; source side: FP32 sums over 16 steps folded into FP64 in the
 // blocked mode, an FP64 travelling accumulator otherwise); R = double: FP64.
-template <int MASK, int K, int ACC, typename R, int DOM = kDomWall>
+// PER: pseudo-periodic images (mobility_numba.py:149-198).  The minimum
+// image is odd in d and the box set {-1, 0, 1} is symmetric, so the images
+// of (j, i) are the negated images of (i, j) and M_ji = M_ij^T holds image
+// by image; each unordered pair takes all its images on both sides, sums
+// are folded every step, and the self pair's images outside the central box
+// are added by the reduce.
+template <int MASK, int K, int ACC, typename R, int DOM = kDomWall, bool PER = false>
 __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
 {
   using BT = BlockTraits<MASK>;
@@ -507,6 +515,10 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
   __shared__ double s_jd[kBlock / 64][K][OC][64];  // per-wave source-side sums of one block
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int il = threadIdx.x;
+  // fold the FP32 sums every kSub sources (every source with the images)
+  constexpr int kFold = PER ? 0 : kSub - 1;
+  int pbox[3];
+  for (int d = 0; d < 3; ++d) pbox[d] = PER ? (p.periodic >> d) & 1 : 0;
 
   for (int64_t item = blockIdx.x; item < p.n_items; item += gridDim.x) {
     const MutSys Sy = mut_sys_of_item(p, item);
@@ -593,13 +605,14 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
         for (int st = 0; st < 64; ++st) {
           const int jl = 64 * b + ((lane + st) & 63);
           const bool on = !MASKED || (valid && jl < cnt && (!diag || jl > il));
-          R dx, dy, dz;
-          if constexpr (sizeof(R) == 4) {
-            dx = (R)(Xi - s_x[jl]); dy = (R)(Yi - s_y[jl]); dz = (R)(Zi - s_z[jl]);
-          } else {
-            dx = Xi - s_x[jl]; dy = Yi - s_y[jl]; dz = Zi - s_z[jl];
-          }
           const R zj = s_zr[jl];
+          // source-side sums move one lane per step (fused into the add; with
+          // images, once per step before them)
+          auto acc_j = [&](JT& a, JT x) {
+            if constexpr (PER) a = a + x; else a = rol1(a) + x;
+          };
+          // one image d = (dx, dy, dz) of the pair, applied on both sides
+          auto pair_apply = [&](R dx, R dy, R dz) {
           const Geom<R> g = make_geom(dx, dy, dz, zi, zj);
           CoefUF<R> cuf;
           CoefRot<R> cut, cwf;
@@ -648,7 +661,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
                   if constexpr (WF)
                     y = fma_(Md.m[0][r], ti[k][0], fma_(Md.m[1][r], ti[k][1], r == 2 ? y : fma_(Md.m[2][r], ti[k][2], y)));
                   u[k][r].add(x);
-                  ja[k][r] = rol1(ja[k][r]) + (JT)y;
+                  acc_j(ja[k][r], (JT)y);
                 }
                 if constexpr (BT::HW) {
                   constexpr int o = BT::HU ? 3 : 0;
@@ -662,7 +675,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
                   if constexpr (UT)
                     y = fma_(Mc.m[0][r], fi[k][0], fma_(Mc.m[1][r], fi[k][1], r == 2 ? y : fma_(Mc.m[2][r], fi[k][2], y)));
                   w[k][r].add(x);
-                  ja[k][o + r] = rol1(ja[k][o + r]) + (JT)y;
+                  acc_j(ja[k][o + r], (JT)y);
                 }
               }
             }
@@ -701,22 +714,43 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
               u[k][0].add(on ? ui.x : R(0));
               u[k][1].add(on ? ui.y : R(0));
               u[k][2].add(on ? ui.z : R(0));
-              ja[k][0] = rol1(ja[k][0]) + (on ? (JT)uj.x : JT(0));
-              ja[k][1] = rol1(ja[k][1]) + (on ? (JT)uj.y : JT(0));
-              ja[k][2] = rol1(ja[k][2]) + (on ? (JT)uj.z : JT(0));
+              acc_j(ja[k][0], (on ? (JT)uj.x : JT(0)));
+              acc_j(ja[k][1], (on ? (JT)uj.y : JT(0)));
+              acc_j(ja[k][2], (on ? (JT)uj.z : JT(0)));
             }
             if constexpr (BT::HW) {
               constexpr int o = BT::HU ? 3 : 0;
               w[k][0].add(on ? wi.x : R(0));
               w[k][1].add(on ? wi.y : R(0));
               w[k][2].add(on ? wi.z : R(0));
-              ja[k][o] = rol1(ja[k][o]) + (on ? (JT)wj.x : JT(0));
-              ja[k][o + 1] = rol1(ja[k][o + 1]) + (on ? (JT)wj.y : JT(0));
-              ja[k][o + 2] = rol1(ja[k][o + 2]) + (on ? (JT)wj.z : JT(0));
+              acc_j(ja[k][o], (on ? (JT)wj.x : JT(0)));
+              acc_j(ja[k][o + 1], (on ? (JT)wj.y : JT(0)));
+              acc_j(ja[k][o + 2], (on ? (JT)wj.z : JT(0)));
             }
           }
           }
-          if ((st & (kSub - 1)) == kSub - 1) {
+          };
+          if constexpr (!PER) {
+            if constexpr (sizeof(R) == 4)
+              pair_apply((R)(Xi - s_x[jl]), (R)(Yi - s_y[jl]), (R)(Zi - s_z[jl]));
+            else
+              pair_apply(Xi - s_x[jl], Yi - s_y[jl], Zi - s_z[jl]);
+          } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+              for (int c = 0; c < OC; ++c) ja[k][c] = rol1(ja[k][c]);
+            // minimum image of the FP64 difference, then the neighbouring boxes
+            double ddx = Xi - s_x[jl], ddy = Yi - s_y[jl], ddz = Zi - s_z[jl];
+            if (pbox[0]) ddx -= image_count(ddx, p.L[0]) * p.L[0];
+            if (pbox[1]) ddy -= image_count(ddy, p.L[1]) * p.L[1];
+            if (pbox[2]) ddz -= image_count(ddz, p.L[2]) * p.L[2];
+            for (int bx = -pbox[0]; bx <= pbox[0]; ++bx)
+              for (int by = -pbox[1]; by <= pbox[1]; ++by)
+                for (int bz = -pbox[2]; bz <= pbox[2]; ++bz)
+                  pair_apply((R)(ddx + bx * p.L[0]), (R)(ddy + by * p.L[1]), (R)(ddz + bz * p.L[2]));
+          }
+          if ((st & kFold) == kFold) {
             flush_all<K, ACC>(u, w);
             if constexpr (kFoldJ) {
               // lane l holds the sum of source (l + st) mod 64 of the block
@@ -725,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
               for (int k = 0; k < K; ++k)
 #pragma unroll
                 for (int c = 0; c < OC; ++c) {
-                  const double prev = st < kSub ? 0.0 : s_jd[wv][k][c][jw];
+                  const double prev = st <= kFold ? 0.0 : s_jd[wv][k][c][jw];
                   s_jd[wv][k][c][jw] = prev + (double)ja[k][c];
                   ja[k][c] = JT(0);
                 }
@@ -817,7 +851,8 @@ __device__ __forceinline__ void finish_output(const double* __restrict__ r, cons
                                               const double* __restrict__ t, double* __restrict__ vel,
                                               double* __restrict__ omg, int mask, int k, int64_t gi, int c,
                                               double sum, int64_t rhs_stride, double a, double inva, double c1,
-                                              double c2, int dom)
+                                              double c2, int dom, int periodic = 0, double L0 = 0.0,
+                                              double L1 = 0.0, double L2 = 0.0)
 {
   const bool HU = (mask & (kUF | kUT)) != 0;
   const bool HW = (mask & (kWF | kWT)) != 0;
@@ -839,6 +874,35 @@ __device__ __forceinline__ void finish_output(const double* __restrict__ r, cons
     tv[0] = c2 * B * v[0]; tv[1] = c2 * B * v[1]; tv[2] = pz * c2 * B * v[2];
   }
   double self = 0.0;
+  // images of the blob itself outside the central box (pseudo-periodic
+  // symmetric kernels): pairs at d = (b_x L_x, b_y L_y, b_z L_z), z_i = z_j
+  for (int bx = -(periodic & 1); bx <= (periodic & 1); ++bx)
+    for (int by = -((periodic >> 1) & 1); by <= ((periodic >> 1) & 1); ++by)
+      for (int bz = -((periodic >> 2) & 1); bz <= ((periodic >> 2) & 1); ++bz) {
+        if (bx == 0 && by == 0 && bz == 0) continue;
+        const Geom<double> g = make_geom(bx * L0, by * L1, bz * L2, h, h);
+        Vec3<double> v{0.0, 0.0, 0.0};
+        if (!is_w) {
+          if (mask & kUF) {
+            const Vec3<double> q = apply_uf(g, coef_uf(g, h, h, dom), fv[0], fv[1], fv[2]);
+            v.x += q.x; v.y += q.y; v.z += q.z;
+          }
+          if (mask & kUT) {
+            const Vec3<double> q = apply_ut(g, coef_rot(g, h, dom), tv[0], tv[1], tv[2]);
+            v.x += q.x; v.y += q.y; v.z += q.z;
+          }
+        } else {
+          if (mask & kWF) {
+            const Vec3<double> q = apply_wf(g, coef_rot(g, h, dom), fv[0], fv[1], fv[2]);
+            v.x += q.x; v.y += q.y; v.z += q.z;
+          }
+          if (mask & kWT) {
+            const Vec3<double> q = apply_wt(g, coef_wt(g, dom), tv[0], tv[1], tv[2]);
+            v.x += q.x; v.y += q.y; v.z += q.z;
+          }
+        }
+        self += comp == 0 ? v.x : (comp == 1 ? v.y : v.z);
+      }
   if (!is_w) {
     if (mask & kUF) { Vec3<double> sv = self_uf_dom(dom, h, fv[0], fv[1], fv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
     if (mask & kUT) { Vec3<double> sv = self_ut_dom(dom, h, tv[0], tv[1], tv[2]); self += comp == 0 ? sv.x : (comp == 1 ? sv.y : sv.z); }
@@ -902,6 +966,8 @@ struct MutReduceParams {
   int64_t rhs_stride;
   double a, inva, c1, c2;
   int domain;
+  int periodic;             // bit d set when L_d > 0: add the self pair's outer images
+  double L[3];              // periodic lengths / a
 };
 
 // Output o (component c of blob gi, RHS k) sums the target-side slots of the
@@ -958,7 +1024,7 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
 #pragma unroll
     for (int q = 1; q < kRedGroups; ++q) s += part[q][ol];
     finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, s, p.rhs_stride, p.a, p.inva, p.c1, p.c2,
-                  p.domain);
+                  p.domain, p.periodic, p.L[0], p.L[1], p.L[2]);
   }
 }
 
@@ -1357,6 +1423,28 @@ static void dispatch_mutual_m(int prec, int K, const MutParams& q, int grid, hip
   }
 }
 
+// Pseudo-periodic images (wall domain, one right-hand side per pass).
+template <int MASK>
+static void dispatch_mutual_per_m(int prec, const MutParams& q, int grid, hipStream_t st)
+{
+  const dim3 g(grid), b(kBlock);
+  if (prec == RMW_PREC_F32_BLOCKED)
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_BLOCKED, float, kDomWall, true>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_F64ACC)
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_F64ACC, float, kDomWall, true>), g, b, 0, st, q);
+  else if (prec == RMW_PREC_F32_KAHAN)
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F32_KAHAN, float, kDomWall, true>), g, b, 0, st, q);
+  else
+    hipLaunchKernelGGL((mutual_kernel<MASK, 1, RMW_PREC_F64, double, kDomWall, true>), g, b, 0, st, q);
+}
+
+static void dispatch_mutual_periodic(int mask, int prec, const MutParams& q, int grid, hipStream_t st)
+{
+  if (mask == kUF) dispatch_mutual_per_m<kUF>(prec, q, grid, st);
+  else if (mask == (kUF | kUT)) dispatch_mutual_per_m<kUF | kUT>(prec, q, grid, st);
+  else dispatch_mutual_per_m<kUF | kUT | kWF | kWT>(prec, q, grid, st);
+}
+
 static void dispatch_mutual(int domain, int mask, int prec, int K, const MutParams& q, int grid, hipStream_t st)
 {
   if (domain == kDomNoWall) {
@@ -1402,7 +1490,7 @@ void prof_end(const ProfRec& rec, bool on, hipStream_t st)
 
 static void run_mutual(const Plan& pl, const double* r, const double* f, const double* t, double* vel,
                        double* omg, int k, int64_t stride, double a, double inva, double c1, double c2,
-                       const rmw_tuning* tuning, char* base, hipStream_t st)
+                       const rmw_tuning* tuning, char* base, hipStream_t st, int periodic, const double* Lp)
 {
   const bool HU = (pl.mask & (kUF | kUT)) != 0, HW = (pl.mask & (kWF | kWT)) != 0;
   const bool NF = (pl.mask & (kUF | kWF)) != 0, NT = (pl.mask & (kUT | kWT)) != 0;
@@ -1413,8 +1501,10 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
   }
   int grid = (int)std::min<int64_t>(pl.mut_items, (int64_t)INT32_MAX);
   if (tuning && tuning->max_wgs > 0) grid = std::min(grid, tuning->max_wgs);
-  for (int k0 = 0, K = group_k(k, pl.kmax_mut); k0 < k; k0 += K) {
-    K = group_k(k - k0, pl.kmax_mut);
+  // periodic images: one right-hand side per pass
+  const int kmax = periodic ? 1 : pl.kmax_mut;
+  for (int k0 = 0, K = group_k(k, kmax); k0 < k; k0 += K) {
+    K = group_k(k - k0, kmax);
     MutParams q{};
     q.r = r;
     q.f = NF ? f + (size_t)k0 * stride : nullptr;
@@ -1428,10 +1518,13 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     q.nsys = (int)pl.msys.size();
     q.n_items = pl.mut_items;
     q.chunk_blocks = pl.chunk_blocks;
+    q.periodic = periodic;
+    for (int d = 0; d < 3; ++d) q.L[d] = Lp[d];
     ProfRec rec{};
     bool prof = false;
     prof_begin(rec, prof, st);
-    dispatch_mutual(pl.domain, pl.mask, pl.prec, K, q, grid, st);
+    if (periodic) dispatch_mutual_periodic(pl.mask, pl.prec, q, grid, st);
+    else dispatch_mutual(pl.domain, pl.mask, pl.prec, K, q, grid, st);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
 
@@ -1452,6 +1545,8 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     rp.rhs_stride = stride;
     rp.a = a; rp.inva = inva; rp.c1 = c1; rp.c2 = c2;
     rp.domain = pl.domain;
+    rp.periodic = periodic;
+    for (int d = 0; d < 3; ++d) rp.L[d] = Lp[d];
     const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
     const int64_t nout = pl.ntot * K * OC;
     hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nout + kRedOut - 1) / kRedOut)), dim3(256), 0, st, rp);
@@ -1493,8 +1588,8 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   const double c2 = c1 / a;
 
   const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
-  if (pl.mut_ok && periodic == 0) {
-    run_mutual(pl, r, f, t, vel, omg, k, stride, a, inva, c1, c2, tuning, base, st);
+  if (pl.mut_ok && (periodic == 0 || pl.domain == kDomWall)) {
+    run_mutual(pl, r, f, t, vel, omg, k, stride, a, inva, c1, c2, tuning, base, st, periodic, Lp);
     return;
   }
 
@@ -1589,9 +1684,9 @@ extern "C" int rmw_wall_rpy_describe(int mask, int prec, int64_t n, int k, const
     RMW_REQUIRE(info != nullptr, "info is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
-    const bool mut = pl.mut_ok && !periodic;
+    const bool mut = pl.mut_ok && (!periodic || pl.domain == kDomWall);
     info->symmetric = mut ? 1 : 0;
-    info->rhs_group = mut ? pl.kmax_mut : pl.kmax;
+    info->rhs_group = mut ? (periodic ? 1 : pl.kmax_mut) : pl.kmax;
     info->chunk_len = mut ? pl.chunk_blocks * 64 : (pl.sys.empty() ? 0 : pl.sys[0].chunk_len);
     info->work_items = mut ? pl.mut_items : pl.total_items;
     info->workspace_bytes = (int64_t)pl.total;

@@ -264,10 +264,68 @@ def test_symmetric_c5_matches_ordered(gpu):
 def test_describe_reports_the_plan(gpu):
     d = wall_rpy.describe(RMW_UF, 32768, 1, "fp32_blocked", tuning=_lib.Tuning(0, 0, 0, 0, 0))
     assert d["kernel"] == "symmetric" and d["work_items"] > 0 and d["workspace_bytes"] > 0
-    d = wall_rpy.describe(RMW_UF, 32768, 1, "fp32_blocked", periodic_length=[10.0, 0, 0],
+    d = wall_rpy.describe(RMW_UF, 32768, 4, "fp32_blocked", periodic_length=[10.0, 0, 0],
                           tuning=_lib.Tuning(0, 0, 0, 0, 0))
-    assert d["kernel"] == "ordered"
+    assert d["kernel"] == "symmetric" and d["rhs_group"] == 1      # images: one RHS per pass
     d = wall_rpy.describe(RMW_UF, 32768, 1, "fp32_blocked", tuning=_lib.Tuning(1024, 0, 0, -1, 0))
     assert d["kernel"] == "ordered" and d["chunk_len"] == 1024 and d["work_items"] == 128 * 32
     d = wall_rpy.describe(RMW_UF, 2048, 1, "fp64", tuning="heuristic")      # too few tile pairs
     assert d["kernel"] == "ordered"
+
+
+PERIODIC_L = [(6.0, 5.0, 0.0), (8.0, 0.0, 0.0), (6.0, 5.0, 7.0)]
+
+
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
+@pytest.mark.parametrize("mask", [RMW_UF, RMW_UF | RMW_UT, GRAND])
+def test_symmetric_periodic_vs_oracle_and_ordered(gpu, prec, mask):
+    """Pseudo-periodic images through the symmetric kernel: each unordered
+    pair's 3, 9 or 27 images on both sides, the self pair's outer images in
+    the reduce; against the oracle and the ordered kernel, 700 blobs (two
+    full tiles and a ragged one), boxes periodic in x and y, x only, and
+    x, y and z, two right-hand sides (one per pass)."""
+    n, k = 700, 2
+    rng = np.random.default_rng(71)
+    r = np.column_stack([rng.uniform(-4, 9, n), rng.uniform(0, 5, n), rng.uniform(0.25, 2.0, n)])
+    f, t = rng.standard_normal((k, n, 3)), rng.standard_normal((k, n, 3))
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    for L in PERIODIC_L:
+        print(f"case symmetric periodic mask={mask} L={L} {prec}", flush=True)
+        sym = _lib.Tuning(0, 0, 0, 1, 0)
+        assert wall_rpy.describe(mask, n, k, prec, periodic_length=L, tuning=sym)["kernel"] == "symmetric"
+        kw = dict(force=f, torque=t if mask != RMW_UF else None, eta=0.9, a=0.3, precision=prec, k=k,
+                  periodic_length=np.array(L))
+        us, ws = wall_rpy.apply(mask, r, tuning=sym, **kw)
+        uo, wo = wall_rpy.apply(mask, r, tuning=_lib.Tuning(0, 0, 0, -1, 0), **kw)
+        us, uo = us.cpu().numpy().reshape(k, -1), uo.cpu().numpy().reshape(k, -1)
+        for j in range(k):
+            ou = O.block_apply(O.UF, r, f[j], 0.9, 0.3, periodic_length=L)
+            if mask & RMW_UT:
+                ou = ou + O.block_apply(O.UT, r, t[j], 0.9, 0.3, periodic_length=L)
+            assert rel(us[j], ou) <= tol, (L, j)
+            assert rel(us[j], uo[j]) <= tol, (L, j)
+            if mask == GRAND:
+                ow = (O.block_apply(O.WF, r, f[j], 0.9, 0.3, periodic_length=L)
+                      + O.block_apply(O.WT, r, t[j], 0.9, 0.3, periodic_length=L))
+                wsj = ws.cpu().numpy().reshape(k, -1)[j]
+                assert rel(wsj, ow) <= tol, (L, j)
+                assert rel(wsj, wo.cpu().numpy().reshape(k, -1)[j]) <= tol, (L, j)
+
+
+def test_symmetric_periodic_ragged_and_items(gpu):
+    """Ragged systems and finer or coarser work items with periodic images."""
+    off = np.concatenate([[0], np.cumsum(SIZES_R)]).astype(np.int64)
+    n = int(off[-1])
+    r = np.vstack([_geom(m, 40 + m) for m in SIZES_R if m > 0])
+    rng = np.random.default_rng(72)
+    f, t = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    L = np.array([7.0, 6.0, 0.0])
+    print("case symmetric periodic ragged", flush=True)
+
+    def run(sym, chunk=0, wgs=0):
+        u, w = wall_rpy.apply_ragged(GRAND, r, off, force=f, torque=t, eta=0.9, a=0.3, precision="fp32_blocked",
+                                     periodic_length=L, tuning=_lib.Tuning(chunk, wgs, 0, sym, 0))
+        return np.concatenate([u.cpu().numpy().ravel(), w.cpu().numpy().ravel()])
+    ordered = run(-1)
+    for chunk, wgs in ((0, 0), (64, 0), (512, 5)):
+        assert rel(run(1, chunk, wgs), ordered) <= 2e-6, (chunk, wgs)
