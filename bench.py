@@ -407,7 +407,7 @@ def rigid_step_bench(args, log):
     kw = dict(scheme="deterministic_forward_euler", eta=w["eta"], a=w["a"], blob_blob_force_implementation="hip_fp64",
               tolerance=1e-8, **fp)
     prec = "fp64" if args.precision == "fp32_blocked" else args.precision
-    integ = RigidIntegrator(w["centres"], w["quats"], w["ref"], precision=prec, **kw)
+    integ = RigidIntegrator(w["centres"], w["quats"], w["ref"], precision=prec, solver=args.rigid_solver, **kw)
     dt = 1e-3
     for _ in range(args.warmup):
         integ.advance_time_step(dt)
@@ -428,13 +428,19 @@ def rigid_step_bench(args, log):
     pairs = float(n) * n
     peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
     achieved = FLOP_PER_PAIR["UF"] * pairs * nw / (tw * 1e-3) / 1e12 if tw else 0.0
-    log(f"rigid step: {ms:.3f} ms/step, {its:.1f} GMRES iterations/step, wall-RPY kernels "
-        f"{tw / args.steps:.3f} ms/step ({nw / args.steps:.1f} launches)")
+    log(f"rigid step ({args.rigid_solver} solver): {ms:.3f} ms/step, {its:.1f} GMRES iterations/step, wall-RPY "
+        f"kernels {tw / args.steps:.3f} ms/step ({nw / args.steps:.1f} launches)")
+    mixed = args.rigid_solver == "mixed"
+    if mixed:
+        # FP32-pair inner products and FP64 residuals: the FLOP rate of all
+        # launches against the FP32 peak
+        peak = PEAK_FP32_TFLOPS
     out = {
-        "metric": "rigid-body time steps/s (deterministic forward Euler, GMRES to 1e-8)",
+        "metric": "rigid-body time steps/s (deterministic forward Euler, GMRES to 1e-8"
+                  + (", mixed-precision solver)" if mixed else ")"),
         "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == "fp64" else "f32", "data": "synthetic",
+        "dtype": "f32+f64" if mixed else ("f64" if prec == "fp64" else "f32"), "data": "synthetic",
         "config": {"workload": "C2 suspension: 256 spheres x 42 blobs = 10752 blobs, a = 0.243, eta = 1, dt 1e-3, "
                                "gravity 0.02 per blob, wall and blob-blob repulsion (0.02, 0.1)",
                    "blobs": n, "precision": prec + PRECISION_NOTE.get(prec, ""),
@@ -493,6 +499,8 @@ def main(argv=None):
     ap.add_argument("--no-gmres", action="store_true", help="skip the learned-warm-start GMRES leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--rigid-solver", default="fp64", choices=["fp64", "mixed"],
+                    help="c2step: FP64 GMRES, or FP32-pair inner GMRES refined to the FP64 tolerance")
     ap.add_argument("--gmres-steps", type=int, default=64)
     ap.add_argument("--gmres-epochs", type=int, default=120)
     ap.add_argument("--gmres-evals", type=int, default=3)

@@ -20,6 +20,10 @@ right-preconditioned wrapper then ignores, general_application_utils.py
 Active slip: ``slip_reference`` gives each body's blob slip in its
 reference frame (as read from a .slip file); it is rotated with the body
 every step like active_body_slip (multi_bodies_functions.py:123-140).
+
+``solver="mixed"`` (no counterpart in the reference) solves each step with
+RigidSuspension.solve_mixed: FP32-pair inner GMRES refined against FP64
+residuals to the same FP64 tolerance.
 """
 from __future__ import annotations
 
@@ -46,9 +50,12 @@ class RigidIntegrator:
                  blob_blob_force_implementation: str = "None", repulsion_strength: float = 1.0,
                  debye_length: float = 1.0, periodic_length=None, tolerance: float = 1e-8,
                  precision: str = "fp64", slip_reference=None, domain: str = "single_wall", restart: int = 60,
-                 update_PC: int = 1):
+                 update_PC: int = 1, solver: str = "fp64"):
         if scheme not in SCHEMES:
             raise ValueError(f"scheme {scheme!r} is not provided; use one of {list(SCHEMES)}")
+        if solver not in ("fp64", "mixed"):
+            raise ValueError(f"solver must be 'fp64' or 'mixed', not {solver!r}")
+        self.solver = solver
         if domain != "single_wall":
             raise ValueError("rigid bodies: domain single_wall only")
         self.scheme = scheme
@@ -179,7 +186,10 @@ class RigidIntegrator:
         rhs = S.rhs(self.slip(S), self.force_torque(S))
         rnorm = float(torch.linalg.norm(rhs))
         x0 = None if (self.first_guess is None or rnorm == 0.0) else self.first_guess * rnorm
-        res = S.solve_mobility_problem(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0)
+        if self.solver == "mixed":
+            res = S.solve_mixed(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0)
+        else:
+            res = S.solve_mobility_problem(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0)
         if res.info != 0:
             raise RuntimeError(f"GMRES did not reach {self.tolerance} (residual {res.final_residual:.3e})")
         self.det_iterations_count += res.iterations

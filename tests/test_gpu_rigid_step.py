@@ -68,6 +68,27 @@ def test_update_pc_reuses_a_stale_preconditioner(gpu):
         RigidIntegrator(c0, q0, ref, update_PC=0)
 
 
+def test_mixed_solver_steps_match_fp64(gpu):
+    """solver='mixed' (FP32-pair inner GMRES refined to the FP64 tolerance)
+    gives the FP64 solver's trajectory to the solve tolerance."""
+    c0, q0, _ = _bodies(4, 5)
+    ref = icosphere_42(1.0)
+    print("case rigid step mixed vs fp64 solver", flush=True)
+    runs = {}
+    for solver in ("fp64", "mixed"):
+        g = RigidIntegrator(c0, q0, ref, scheme="deterministic_forward_euler", eta=1.0, a=0.243,
+                            blob_blob_force_implementation="hip_fp64", tolerance=1e-8, solver=solver, **P)
+        for step in range(3):
+            g.advance_time_step(0.05, step=step)
+        runs[solver] = g
+    dc = np.linalg.norm(runs["fp64"].centres - c0)
+    assert dc > 0
+    assert np.linalg.norm(runs["mixed"].centres - runs["fp64"].centres) <= 1e-6 * dc
+    assert np.abs(runs["mixed"].quats - runs["fp64"].quats).max() <= 1e-6
+    with pytest.raises(ValueError):
+        RigidIntegrator(c0, q0, ref, solver="fp16")
+
+
 def test_body_pieces(gpu):
     c, q, rng = _bodies(5, 2)
     refs = [icosphere_42(1.0)[: 10 + 6 * k] for k in range(5)]           # ragged bodies
