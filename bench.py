@@ -1,31 +1,40 @@
 """Benchmark: wall-RPY blob mobility matvec on one MI355X.
 
 One step = one wall-corrected RPY translational product u = B M_UF(z_eff) B f
-over the rigid-multiblob suspension of the rigid-solve configuration
-(256 spheres x 42 blobs = 10752 blobs, blob a = 0.243), FP32 pair terms
-with FP32 sub-sums folded into FP64 per-target sums (--precision), inputs
-resident in HBM.  This is the product
-that every GMRES iteration of that solve applies.
+over N = 32768 single-blob rollers (a = 0.656, eta = 1e-3, x, y ~ U[0, 300],
+z ~ U[0.8, 2.0], seed 3; f ~ N(0, 1), seed 7), FP32 pair terms with FP32
+sub-sums folded into FP64 per-target sums (--precision), inputs resident in
+HBM: the N = 32768 point of the 50x-over-the-host-CPU target.
 
 Prints ONE JSON line (rank 0):
   value        blob-pairs/s = N^2 / (time per step)
+  oracle_error relative L2 error of the timed product against the FP64 C
+               oracle (strict IEEE build) on a sample of 512 target rows
+  fp64         the same product with FP64 pair terms: time, pairs/s, fraction
+               of the FP64 vector peak and its oracle error
   roofline     pair kernel: algorithmic FLOP per launch (164 FLOP/pair x N^2)
                / mean pair-kernel duration (HIP events on the launch stream),
                against the FP32 vector peak; traffic from committed rocprof
                PMC runs when present
   cpu_baseline FP64 C oracle (-O3 -ffast-math, OpenMP over all usable host
-               cores) on a bounded sample of target rows of the same workload
+               cores; the x86-64-v3 build and, on AVX-512 hosts, the x86-64-v4
+               build, the faster reported) on a bounded sample of target rows
+               of the same workload
   gmres        rigid solve to 1e-8 on the 1024-sphere monolayer: plain FP64
                GMRES from x0 = 0 against the learned warm start (network
-               trained on the GPU from a short trajectory), per-solve times
+               trained on the GPU from a 200-step Brownian trajectory), per-solve
+               times
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3r|c3f|c3step|c2step|c3lz|c5lz]
-       (c3r: the different-radii product on the C3 geometry, above the wall,
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--workload c3|c2|c1|c3r|c3f|c3step|c2step|c3lz|c5lz|prodtiming]
+       (c2: the rigid-solve suspension, 256 spheres x 42 blobs;
+       c3r: the different-radii product on the C3 geometry, above the wall,
        or with --domain no_wall / free_surface; c3f: the
        blob-blob soft-repulsion forces on it; c3step: whole rollers time
        steps, steps/s; c2step: rigid-body time steps of the C2 suspension;
        c3lz / c5lz: Lanczos Brownian-slip solves of the C3 block of 4 RHS
-       and of the C5 ragged batch, solves/s)
+       and of the C5 ragged batch, solves/s; prodtiming: the reference's
+       four-block timing harness, --blobs-per-body 1|12|42|162)
 """
 from __future__ import annotations
 
@@ -70,7 +79,8 @@ def load_workload(name: str):
         desc = "C1: N=2048 blobs, a=0.13, eta=1, UF block"
     elif name == "c3":
         w = workloads.c3_rollers()
-        desc = "C3 rollers: N=32768 blobs, a=0.656, eta=1e-3, UF block"
+        desc = ("C3 matvec: N=32768 single-blob rollers above a wall, a=0.656, eta=1e-3, x,y ~ U[0,300], "
+                "z ~ U[0.8,2.0], f ~ N(0,1), UF block")
     elif name == "c3f":
         w = workloads.c3_rollers()
         w = dict(w, forces=dict(repulsion_strength=0.0165677856, debye_length=0.0656, blob_radius=0.656))
@@ -104,19 +114,28 @@ def cpu_baseline(r, f, eta, a, target_seconds=16.0, radii=None, force_params=Non
     with the CPU model, core count and a single-thread rate on a small
     row block."""
     from oracle import oracle as O
+    kinds = ["fast"] + (["fast_avx512"] if O.host_has_avx512() else [])
+    runs = [_cpu_baseline_kind(O, kind, r, f, eta, a, target_seconds / len(kinds), radii, force_params, boundary)
+            for kind in kinds]
+    best = max(runs, key=lambda x: x["value"])
+    best["builds_timed"] = {x["build"]: x["value"] for x in runs}
+    return best
+
+
+def _cpu_baseline_kind(O, kind, r, f, eta, a, target_seconds, radii, force_params, boundary):
     n = r.shape[0]
-    threads = O.num_threads("fast")
+    threads = O.num_threads(kind)
     if force_params is not None:
         fp = force_params
         def run(rows):
-            O.blob_forces(r, fp["repulsion_strength"], fp["debye_length"], fp["blob_radius"], rows=rows, kind="fast")
+            O.blob_forces(r, fp["repulsion_strength"], fp["debye_length"], fp["blob_radius"], rows=rows, kind=kind)
     elif radii is not None:
         # different radii: the oracle's radii product on a block of rows
         def run(rows):
-            O.radii_apply(r, r, f, radii, radii, eta, rows=rows, kind="fast", boundary=boundary)
+            O.radii_apply(r, r, f, radii, radii, eta, rows=rows, kind=kind, boundary=boundary)
     else:
         def run(rows):
-            O.block_apply(O.UF, r, f, eta, a, rows=rows, kind="fast")
+            O.block_apply(O.UF, r, f, eta, a, rows=rows, kind=kind)
     rows0 = max(1, min(n, 256))
     run((0, rows0))          # warm-up
     t = time.perf_counter()
@@ -130,18 +149,76 @@ def cpu_baseline(r, f, eta, a, target_seconds=16.0, radii=None, force_params=Non
         run((0, rows))
     dt = time.perf_counter() - t
     # one thread on a small block, for the per-core rate
-    O.set_num_threads(1, "fast")
+    O.set_num_threads(1, kind)
     rows1 = max(1, min(n, 32))
     t = time.perf_counter()
     run((0, rows1))
     single = rows1 * n / (time.perf_counter() - t)
-    O.set_num_threads(threads, "fast")
+    O.set_num_threads(threads, kind)
+    isa = "x86-64-v4 (AVX-512)" if kind == "fast_avx512" else "x86-64-v3 (AVX2)"
     return {"value": reps * rows * n / dt, "unit": "blob-pairs/s", "cores": threads, "kind": "port",
             "sample": f"{reps} x {'forces' if force_params else 'radii' if radii is not None else 'UF'} rows 0..{rows} "
                       f"of {n} (x {n} sources) "
                       f"of the same workload; "
-                      f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads; {dt:.1f} s",
+                      f"FP64 C oracle -O3 -ffast-math -march={isa.split()[0]}, OpenMP {threads} threads; {dt:.1f} s",
+            "build": kind, "isa": isa,
             "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "single_thread_pairs_per_s": single}
+
+
+def oracle_rows(n: int, blocks: int = 4, rows: int = 128):
+    """`blocks` row blocks of `rows` targets spread over [0, n)."""
+    if n <= blocks * rows:
+        return [(0, n)]
+    starts = np.linspace(0, n - rows, blocks).astype(int)
+    return [(int(s0), int(s0) + rows) for s0 in starts]
+
+
+def oracle_error(u, r_h, f_h, eta, a, domain="single_wall"):
+    """Relative L2 error of a UF product u (device (n, 3)) against the FP64
+    C oracle (strict build) on a sample of target rows."""
+    from oracle import oracle as O
+    block = {"single_wall": O.UF}.get(domain)
+    if block is None:
+        return None
+    n = r_h.shape[0]
+    got = u.reshape(n, 3).cpu().numpy()
+    num = den = 0.0
+    sample = oracle_rows(n)
+    for t0, t1 in sample:
+        ref = O.block_apply(block, r_h, f_h, eta, a, rows=(t0, t1)).reshape(n, 3)[t0:t1]
+        num += float(np.sum((got[t0:t1] - ref) ** 2))
+        den += float(np.sum(ref ** 2))
+    return {"rel_l2": float(np.sqrt(num / den)), "rows": int(sum(t1 - t0 for t0, t1 in sample)),
+            "oracle": "FP64 C restatement of mobility_numba.py:123-287 with the wrapper's height shift and damping "
+                      "(strict IEEE build)"}
+
+
+def fp64_leg(r, f, u, r_h, f_h, eta, a, steps, warmup):
+    """The same product with FP64 pair terms and sums (the 1e-12 mode)."""
+    import torch
+    from rigidmultiblobswall_amd import wall_rpy
+    from rigidmultiblobswall_amd._lib import RMW_UF
+    step = lambda: wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision="fp64", out_velocity=u)  # noqa: E731
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with wall_rpy.PairKernelTimer() as pk:
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(steps):
+            step()
+        stop.record()
+        torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    n = r_h.shape[0]
+    pairs = float(n) * n
+    achieved = FLOP_PER_PAIR["UF"] * pairs / (pk.mean_ms * 1e-3) / 1e12
+    launch = wall_rpy.describe(RMW_UF, n, 1, "fp64")
+    return {"ms_per_step": ms, "pairs_per_s": pairs / (ms * 1e-3), "kernel_mean_ms": pk.mean_ms,
+            "kernel": f"{'mutual_kernel<UF,K=1,fp64>' if launch['kernel'] == 'symmetric' else 'pair_kernel_f64<UF,K=1>'}",
+            "achieved_tflops": achieved, "peak_tflops": PEAK_FP64_TFLOPS, "frac": achieved / PEAK_FP64_TFLOPS,
+            "oracle_error": oracle_error(u, r_h, f_h, eta, a)}
 
 
 def end_to_end(r_h, f_h, eta, a, precision, reps=20):
@@ -471,6 +548,135 @@ def rigid_step_bench(args, log):
     return 0
 
 
+PROD_TIMING_VERTEX = {1: "blob.vertex", 12: "shell_N_12_Rg_0_7921_Rh_1.vertex",
+                      42: "shell_N_42_Rg_0_8913_Rh_1.vertex", 162: "shell_N_162_Rg_0_9497_Rh_1.vertex"}
+
+
+def prod_timing_workload(blobs_per_body: int):
+    """The reference's four-block timing harness, multi_bodies/examples/
+    Mobility_Prod_Timing/main.py:47-195 with Multiblob.inputfile: the 2048
+    spheres of Const_Torque_t_15.clones (particle radius 1.0155), each a
+    multiblob model of 1, 12, 42 or 162 blobs (vertex file scaled by the
+    particle radius; blob radius half the smallest blob distance, or the
+    particle radius for one blob), eta 0.957e-3, pseudo-periodic lengths
+    (128.7923, 128.7923, 0), body centres and blob positions wrapped into
+    the box, and X = N(0, 1) of 6 N_blobs values (numpy seed 0) read as rows
+    [F_i T_i] per blob."""
+    from scipy.spatial.distance import pdist
+    from rigidmultiblobswall_amd import workloads
+    from rigidmultiblobswall_amd.read_input import read_clones_file, read_vertex_file
+    d = os.path.join(REPO, "tests", "golden", "mobility_prod_timing")
+    radius = 1.0155
+    ref = read_vertex_file(os.path.join(d, PROD_TIMING_VERTEX[blobs_per_body]))[:, :3] * radius
+    a = 0.5 * float(pdist(ref).min()) if ref.shape[0] > 1 else radius
+    nb, loc, quat = read_clones_file(os.path.join(d, "Const_Torque_t_15.clones"))
+    L = np.array([128.7923, 128.7923, 0.0])
+    loc = loc.copy()
+    for i in range(2):
+        loc[:, i] = np.mod(loc[:, i], L[i])
+    r = workloads.bodies_blobs(loc, quat, ref)
+    for i in range(2):
+        r[:, i] = np.mod(r[:, i], L[i])
+    X = np.random.RandomState(0).normal(0.0, 1.0, 6 * r.shape[0])    # np.random.seed(0), main.py:189-191
+    FT = X.reshape(r.shape[0], 6)
+    return {"r": r, "F": np.ascontiguousarray(FT[:, :3]), "T": np.ascontiguousarray(FT[:, 3:]), "eta": 0.957e-3,
+            "a": a, "L": L, "bodies": nb}
+
+
+def prod_timing_bench(args, log):
+    """One step = the four separate block products UF F, UT T, WF F, WT T of
+    the reference's timing harness (as Wall_Mobility_Mult_Parts, main.py
+    :47-77) on device-resident inputs; the fused grand product (all four
+    blocks in one pass) is timed beside it."""
+    import torch
+    from rigidmultiblobswall_amd import wall_rpy
+    from rigidmultiblobswall_amd._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
+    w = prod_timing_workload(args.blobs_per_body)
+    r_h, eta, a, L = w["r"], w["eta"], w["a"], w["L"]
+    n = r_h.shape[0]
+    dev = torch.device("cuda", 0)
+    r = torch.from_numpy(r_h).to(dev)
+    F = torch.from_numpy(w["F"]).to(dev)
+    T = torch.from_numpy(w["T"]).to(dev)
+    outs = {b: torch.empty((1, n, 3), dtype=torch.float64, device=dev) for b in ("UF", "UT", "WF", "WT")}
+    prec = args.precision
+    blocks = (("UF", RMW_UF, F, None, "u"), ("UT", RMW_UT, None, T, "u"), ("WF", RMW_WF, F, None, "w"),
+              ("WT", RMW_WT, None, T, "w"))
+
+    def run_block(name, mask, f, t, side):
+        o = outs[name]
+        wall_rpy.apply(mask, r, force=f, torque=t, eta=eta, a=a, periodic_length=L, precision=prec,
+                       out_velocity=o if side == "u" else None, out_omega=o if side == "w" else None)
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        st, sp = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with wall_rpy.PairKernelTimer() as pk:
+            st.record()
+            for _ in range(steps):
+                fn()
+            sp.record()
+            torch.cuda.synchronize()
+        return st.elapsed_time(sp) / steps, pk
+
+    for _ in range(args.warmup):
+        for blk in blocks:
+            run_block(*blk)
+    per_block = {}
+    for blk in blocks:
+        ms, pk = timed(lambda blk=blk: run_block(*blk), args.steps)
+        per_block[blk[0]] = {"ms": ms, "pair_kernel_ms": pk.mean_ms}
+    step_ms = sum(v["ms"] for v in per_block.values())
+    uu = torch.empty((1, n, 3), dtype=torch.float64, device=dev)
+    ww = torch.empty((1, n, 3), dtype=torch.float64, device=dev)
+    grand = lambda: wall_rpy.apply(RMW_UF | RMW_UT | RMW_WF | RMW_WT, r, force=F, torque=T, eta=eta, a=a,  # noqa: E731
+                                   periodic_length=L, precision=prec, out_velocity=uu, out_omega=ww)
+    grand()
+    g_ms, g_pk = timed(grand, args.steps)
+    pairs = float(n) * n
+    # oracle check of each block on sampled rows (periodic images included)
+    from oracle import oracle as O
+    rows = oracle_rows(n, blocks=2, rows=64)
+    errs = {}
+    for name, ob, src in (("UF", O.UF, w["F"]), ("UT", O.UT, w["T"]), ("WF", O.WF, w["F"]), ("WT", O.WT, w["T"])):
+        got = outs[name].reshape(n, 3).cpu().numpy()
+        num = den = 0.0
+        for t0, t1 in rows:
+            ref = O.block_apply(ob, r_h, src.ravel(), eta, a, periodic_length=L, rows=(t0, t1)).reshape(n, 3)[t0:t1]
+            num += float(np.sum((got[t0:t1] - ref) ** 2))
+            den += float(np.sum(ref ** 2))
+        errs[name] = float(np.sqrt(num / den))
+    log(f"prod timing {args.blobs_per_body} blobs/body ({n} blobs): four blocks {step_ms:.3f} ms "
+        f"({ {k: round(v['ms'], 3) for k, v in per_block.items()} }), fused grand {g_ms:.3f} ms, errors {errs}")
+    out = {
+        "metric": "wall-RPY four-block mobility product (timing harness), blob-pairs/s",
+        "value": pairs / (step_ms * 1e-3), "unit": "blob-pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == "fp64" else "f32", "data": "reference example input",
+        "config": {"workload": f"Mobility_Prod_Timing harness: 2048 spheres x {args.blobs_per_body} blobs = {n} "
+                               f"blobs, blob a = {a:.6g}, eta 0.957e-3, pseudo-periodic 128.7923 x 128.7923 "
+                               f"(9 images per pair), UF F, UT T, WF F, WT T as four products",
+                   "blobs": n, "precision": prec + PRECISION_NOTE.get(prec, ""), "per_block": per_block,
+                   "fused_grand_ms": g_ms, "fused_grand_pair_kernel_ms": g_pk.mean_ms,
+                   "oracle_error_rel_l2": errs, "oracle_rows": int(sum(t1 - t0 for t0, t1 in rows))},
+    }
+    if not args.no_cpu:
+        threads = O.num_threads("fast")
+        t = time.perf_counter()
+        nrow = 8 if n > 100000 else 32
+        for ob, src in ((O.UF, w["F"]), (O.UT, w["T"]), (O.WF, w["F"]), (O.WT, w["T"])):
+            O.block_apply(ob, r_h, src.ravel(), eta, a, periodic_length=L, rows=(0, nrow), kind="fast")
+        dt_cpu = time.perf_counter() - t
+        sec = dt_cpu * n / nrow
+        out["cpu_baseline"] = {"value": pairs / sec, "unit": "blob-pairs/s", "cores": threads, "kind": "port",
+                               "sample": f"the four blocks on rows 0..{nrow} of {n} (x {n} sources x 9 images), "
+                                         f"FP64 C oracle -O3 -ffast-math, OpenMP {threads} threads, {dt_cpu:.1f} s, "
+                                         f"scaled to the whole product",
+                               "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def load_traffic(workload: str, precision: str, kernel: str = "pair_kernel_uf", launched: str = ""):
     """HBM bytes per pair-kernel launch from the committed rocprof PMC summary
     of the same kernel, workload and precision (None when there is none, or
@@ -490,20 +696,23 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c3")
     ap.add_argument("--precision", default="fp32_blocked")
     ap.add_argument("--domain", default="single_wall", choices=["single_wall", "no_wall", "in_plane", "free_surface"],
                     help="boundary of the UF product (workloads c1, c2, c3) or of the radii product "
                          "(c3r: single_wall, no_wall, free_surface)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-gmres", action="store_true", help="skip the learned-warm-start GMRES leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-mode leg of the UF workloads")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     ap.add_argument("--rigid-solver", default="fp64", choices=["fp64", "mixed"],
                     help="c2step: FP64 GMRES, or FP32-pair inner GMRES refined to the FP64 tolerance")
-    ap.add_argument("--gmres-steps", type=int, default=64)
+    ap.add_argument("--blobs-per-body", type=int, default=12, choices=[1, 12, 42, 162],
+                    help="prodtiming: the multiblob model of the timing harness")
+    ap.add_argument("--gmres-steps", type=int, default=200)
     ap.add_argument("--gmres-epochs", type=int, default=120)
-    ap.add_argument("--gmres-evals", type=int, default=3)
+    ap.add_argument("--gmres-evals", type=int, default=5)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -534,6 +743,8 @@ def main(argv=None):
         return rollers_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
     if args.workload == "c2step":
         return rigid_step_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
+    if args.workload == "prodtiming":
+        return prod_timing_bench(args, lambda m: print(m, file=sys.stderr, flush=True))
     if args.workload in ("c3lz", "c5lz"):
         return lanczos_bench(args, lambda m: print(m, file=sys.stderr, flush=True), ragged=args.workload == "c5lz")
     dev = torch.device("cuda", 0)
@@ -653,6 +864,12 @@ def main(argv=None):
         out["config"]["precision"] = {"fp64": "FP64 sqrt, quotient and exp; FP64 sums",
                                       "fp32": "FP64 geometry and sums, FP32 exponential"}[fprec]
     if not radii and not fparams and args.domain == "single_wall":
+        out["oracle_error"] = oracle_error(u, r_h, f_h, eta, a)
+        log(f"oracle error (rel L2, {out['oracle_error']['rows']} rows): {out['oracle_error']['rel_l2']:.3e}")
+        if args.precision != "fp64" and not args.no_fp64:
+            out["fp64"] = fp64_leg(r, f, u, r_h, f_h, eta, a, max(5, args.steps // 2), 2)
+            log(f"fp64 mode: {out['fp64']['ms_per_step']:.4f} ms/step, pair kernel {out['fp64']['kernel_mean_ms']:.4f} ms, "
+                f"oracle error {out['fp64']['oracle_error']['rel_l2']:.3e}")
         out["end_to_end"] = end_to_end(r_h, f_h.reshape(-1, 3), eta, a, args.precision)
         log(f"end to end through the numpy signature: {out['end_to_end']['ms']:.3f} ms")
     if not args.no_gmres:

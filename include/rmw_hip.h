@@ -54,8 +54,17 @@ typedef struct rmw_tuning {
                          (UF_ji = UF_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, WT_ji = WT_ij^T).
                          0 = when the tile pairs fill the device, 1 = required (error
                          otherwise), -1 = off */
-  int32_t reserved;
+  int32_t flags;      /* RMW_FLAG_* bits (0 = none) */
 } rmw_tuning;
+
+/* rmw_tuning.flags: after the product, scan the outputs and return
+ * RMW_E_NOTFINITE if any entry is inf or NaN (e.g. two distinct blobs at
+ * the same position, which the reference returns silently as NaN,
+ * mobility/mobility_numba.py:88).  The call then waits for the stream to
+ * finish; the workspace needs RMW_STATUS_BYTES more (the size queries
+ * include them when they are given the same tuning). */
+#define RMW_FLAG_CHECK_FINITE 1
+#define RMW_STATUS_BYTES 256
 
 const char* rmw_last_error(void);
 const char* rmw_version(void);
@@ -76,6 +85,9 @@ int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
  * or NULL when no block of the mask reads them; velocity/omega likewise
  * (same ld), or NULL when no block writes them.  L[3] are the
  * pseudo-periodic lengths (host array; NULL or entries <= 0 = unbounded).
+ *
+ * With tuning->flags & RMW_FLAG_CHECK_FINITE the outputs are scanned and a
+ * non-finite entry returns RMW_E_NOTFINITE (the outputs are still written).
  *
  * Replaces, for mask = RMW_UF / RMW_UT / RMW_WF / RMW_WT:
  *   single_wall_mobility_trans_times_force_numba   mobility/mobility.py:1132-1163
@@ -343,6 +355,26 @@ int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, const int64_t
                        int nbodies, double eta, double a, double* minv, double* minvk,
                        double* nbody, int* status, double* scratch, size_t scratch_doubles,
                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* The stochastic variant of the build: besides minv, minvk and nbody as
+ * rmw_rigid_pc_build, per body the lower Cholesky factor L_b of M_b
+ * (M_b = L_b L_b^T; chol) and L_b^-1 (chol_inv), row-major blocks packed as
+ * minv with zero upper triangles.  Replaces the factorisation loop of
+ * build_block_diagonal_preconditioners_det_stoch, multi_bodies/
+ * multi_bodies.py:480-560 (its M_factorization_blobs = L_b; its
+ * M_factorization_blobs_inv = U_b^-1 = L_b^-T for scipy's M_b = U_b^T U_b). */
+int rmw_rigid_pc_build_det_stoch(const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
+                                 int nbodies, double eta, double a, double* minv, double* minvk, double* nbody,
+                                 double* chol, double* chol_inv, int* status, double* scratch,
+                                 size_t scratch_doubles, void* workspace, size_t workspace_bytes, void* stream);
+/* y_b = B_b x_b (trans = 0) or B_b^T x_b (trans = 1) for every body b and
+ * each of k vectors ld doubles apart (0 = 3N), B packed as minv; lower != 0
+ * reads only the lower triangle of each block.  With chol / chol_inv these
+ * are the per-body products of the mobility_pc and P_inv_mult closures,
+ * multi_bodies/multi_bodies.py:591-615 (x != y). */
+int rmw_rigid_block_apply(const int64_t* body_offsets, int nbodies, const double* blocks, int trans, int lower,
+                          const double* x, double* y, int k, int64_t ld, void* workspace, size_t workspace_bytes,
+                          void* stream);
 
 /* y = P^-1 x, the body preconditioner apply of multi_bodies/multi_bodies.py:
  * 825-849 (x != y). */

@@ -44,11 +44,15 @@ _LIBS: dict = {}
 
 
 def _lib(kind: str = "strict") -> ctypes.CDLL:
-    """Load one of the oracle builds ('strict', 'fast', 'mutant')."""
+    """Load one of the oracle builds ('strict', 'fast', 'fast_avx512',
+    'mutant'); 'fast_avx512' only on a CPU with the x86-64-v4 features."""
     if kind in _LIBS:
         return _LIBS[kind]
+    if kind == "fast_avx512" and not host_has_avx512():
+        raise RuntimeError("this CPU lacks the AVX-512 features of the fast_avx512 oracle build")
     name = {"strict": "liboracle_wall_rpy.so",
             "fast": "liboracle_wall_rpy_fast.so",
+            "fast_avx512": "liboracle_wall_rpy_fast_avx512.so",
             "mutant": "liboracle_wall_rpy_mutant.so"}[kind]
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
@@ -78,6 +82,19 @@ def _lib(kind: str = "strict") -> ctypes.CDLL:
         lib.orc_set_mutation.argtypes = [ctypes.c_int]
     _LIBS[kind] = lib
     return lib
+
+
+def host_has_avx512() -> bool:
+    """Whether the CPU reports the AVX-512 subsets x86-64-v4 requires."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        pass
+    return False
 
 
 def _ptr(a: np.ndarray):
@@ -509,6 +526,42 @@ class RigidSystem:
             Minv = sla.cho_solve(c, np.eye(M.shape[0]))
             self.Minv.append(Minv)
             self.N.append(np.linalg.pinv(self.K[k].T @ sla.cho_solve(c, self.K[k])))
+
+    def build_stochastic_pc(self):
+        """build_block_diagonal_preconditioners_det_stoch, multi_bodies.py
+        :480-560: besides the deterministic factors, per body scipy's upper
+        Cholesky factor U (M_b = U^T U), P_b = U^-1 (its
+        M_factorization_blobs_inv) and P_inv_b = U^T (its
+        M_factorization_blobs)."""
+        import scipy.linalg as sla
+        self.build_pc()
+        self.P, self.P_inv = [], []
+        for k in range(self.Nbodies):
+            M = self.mblobs(self.r_bodies[k])
+            U = np.triu(sla.cho_factor(M)[0])
+            self.P_inv.append(U.T)
+            self.P.append(sla.solve_triangular(U, np.eye(M.shape[0])))
+
+    def mobility_pc(self, w):
+        """P^T M P w with P = blockdiag(P_b): the mobility_pc closure,
+        multi_bodies.py:591-606."""
+        res = np.empty_like(w)
+        for k in range(self.Nbodies):
+            s = slice(3 * self.offsets[k], 3 * self.offsets[k + 1])
+            res[s] = self.P[k] @ w[s]
+        res2 = self.mvp(self.r, res)
+        for k in range(self.Nbodies):
+            s = slice(3 * self.offsets[k], 3 * self.offsets[k + 1])
+            res[s] = self.P[k].T @ res2[s]
+        return res
+
+    def P_inv_mult(self, w):
+        """blockdiag(P_inv_b) w: the P_inv_mult closure, multi_bodies.py:608-615."""
+        out = np.empty_like(w)
+        for k in range(self.Nbodies):
+            s = slice(3 * self.offsets[k], 3 * self.offsets[k + 1])
+            out[s] = self.P_inv[k] @ w[s]
+        return out
 
     def pc_apply(self, v):
         """multi_bodies.py:825-849."""

@@ -1,4 +1,4 @@
-"""FP64 CPU restatement of the deterministic rigid-body time step.
+"""FP64 CPU restatement of the rigid-body time steps.
 
 TEST INFRASTRUCTURE ONLY (imported by tests/; never by the product).
 
@@ -6,6 +6,9 @@ Follows quaternion_integrator/quaternion_integrator_multi_bodies.py:
   deterministic_forward_euler              :75-107
   deterministic_forward_euler_dense_algebra :110-139 (the dense pin)
   deterministic_adams_bashforth            :142-186
+  stochastic_EM                            :262-300 (with the stochastic
+      preconditioner of multi_bodies.py:480-623 and stochastic_forcing_lanczos
+      with L_mult, stochastic_forcing.py:112-262)
   solve_mobility_problem                   :1441-1548
   check_positions                          :1779-1829
 with quaternion.py:15-49 (from_rotation, product), the body forces of
@@ -102,3 +105,25 @@ class RigidStep:
             self.first_step = False
             self.prev = U
         self.c, self.q = c_new, q_new
+
+
+    def advance_stochastic(self, dt, kT, draws, lanczos_tol=None):
+        """One stochastic_EM step; draws yields the Lanczos right-hand side z
+        (3N) of each attempt.  Returns the number of attempts."""
+        tol = self.tol if lanczos_tol is None else lanczos_tol
+        attempts = 0
+        while True:
+            attempts += 1
+            sys = self.system()
+            sys.build_stochastic_pc()
+            z = np.asarray(next(draws), dtype=np.float64).ravel()
+            noise, _ = O.lanczos_sqrt(sys.mobility_pc, z, factor=np.sqrt(2.0 * kT / dt), tolerance=tol)
+            noise = sys.P_inv_mult(noise)
+            rhs = sys.rhs(self.slip(sys).ravel() - noise, self.force_torque(sys))
+            x, _ = sys.solve(rhs, tol=self.tol, restart=60)
+            U = x[3 * sys.Nblobs:].reshape(-1, 6)
+            c_new = self.c + U[:, :3] * dt
+            q_new = np.array([quat_mul(quat_from_rotation(U[k, 3:] * dt), self.q[k]) for k in range(len(self.c))])
+            if (self.system(c_new, q_new).r[:, 2] >= 0.0).all():
+                self.c, self.q = c_new, q_new
+                return attempts

@@ -16,6 +16,9 @@ LIB_PATH = os.path.join(HERE, "librmw_hip.so")
 RMW_OK = 0
 RMW_UF, RMW_UT, RMW_WF, RMW_WT = 1, 2, 4, 8
 RMW_PREC_F32_F64ACC, RMW_PREC_F32_KAHAN, RMW_PREC_F64, RMW_PREC_F32_BLOCKED = 0, 1, 2, 3
+RMW_E_NOTFINITE = -4
+RMW_E_LIMIT = -5
+RMW_FLAG_CHECK_FINITE = 1
 
 PRECISIONS = {
     "fp32": RMW_PREC_F32_F64ACC,
@@ -36,7 +39,7 @@ class RmwError(RuntimeError):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("chunk_len", ctypes.c_int64), ("max_wgs", ctypes.c_int32), ("rhs_group", ctypes.c_int32),
-                ("symmetric", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("symmetric", ctypes.c_int32), ("flags", ctypes.c_int32)]
 
 
 class Segments(ctypes.Structure):
@@ -119,6 +122,11 @@ PROTOTYPES = {
                                                _vp, _sz, _vp]),
     "rmw_rigid_pc_build": (ctypes.c_int, [_dp, _dp, _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
                                           _dp, _dp, _vp, _dp, _sz, _vp, _sz, _vp]),
+    "rmw_rigid_pc_build_det_stoch": (ctypes.c_int, [_dp, _dp, _i64p, ctypes.c_int, ctypes.c_double,
+                                                    ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _vp, _dp, _sz, _vp,
+                                                    _sz, _vp]),
+    "rmw_rigid_block_apply": (ctypes.c_int, [_i64p, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                             ctypes.c_int, _i64, _vp, _sz, _vp]),
     "rmw_rigid_point_field": (ctypes.c_int, [_dp, _dp, _dp, _i64, _i64p, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _vp, _sz, _vp]),
     "rmw_rigid_pc_apply": (ctypes.c_int, [_i64p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _sz, _vp]),
@@ -210,5 +218,16 @@ def tuning_ptr(tuning):
     if isinstance(tuning, Tuning):
         return ctypes.byref(tuning)
     t = Tuning(int(tuning.get("chunk_len", 0)), int(tuning.get("max_wgs", 0)), int(tuning.get("rhs_group", 0)),
-               int(tuning.get("symmetric", 0)), 0)
+               int(tuning.get("symmetric", 0)), int(tuning.get("flags", 0)))
     return ctypes.byref(t)
+
+
+def with_flags(tuning, flags: int):
+    """A Tuning equal to `tuning` (None, a dict or a Tuning; None = the
+    built-in launch shape) with `flags` set in addition."""
+    if tuning is None:
+        return Tuning(0, 0, 0, 0, int(flags))
+    if isinstance(tuning, Tuning):
+        return Tuning(tuning.chunk_len, tuning.max_wgs, tuning.rhs_group, tuning.symmetric, tuning.flags | int(flags))
+    return Tuning(int(tuning.get("chunk_len", 0)), int(tuning.get("max_wgs", 0)), int(tuning.get("rhs_group", 0)),
+                  int(tuning.get("symmetric", 0)), int(tuning.get("flags", 0)) | int(flags))

@@ -322,6 +322,8 @@ struct PcBuildParams {
   int* status;            // out: per body 0 ok, 1 not positive definite
   double a, eta;
   int use_lds;
+  double* chol;           // out (optional): lower Cholesky factor L of M_b (M_b = L L^T), packed as minv
+  double* chol_inv;       // out (optional): L^-1 (lower), packed as minv
 };
 
 template <bool LDS>
@@ -369,6 +371,14 @@ __global__ __launch_bounds__(kRigidBlock) void pc_build_kernel(PcBuildParams p)
     }
     __syncthreads();
   }
+  if (p.chol) {
+    // L with its upper triangle zeroed (A above the diagonal still holds M_b)
+    double* Lb = p.chol + p.moff[b];
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) {
+      const int i = (int)(idx / n), j = (int)(idx - (int64_t)i * n);
+      Lb[idx] = j <= i ? A[idx] : 0.0;
+    }
+  }
 
   // 3. X = L^-1 in place, columns from last to first
   for (int j = n - 1; j >= 0; --j) {
@@ -382,6 +392,14 @@ __global__ __launch_bounds__(kRigidBlock) void pc_build_kernel(PcBuildParams p)
       A[i * n + j] = -y * xjj;
     }
     __syncthreads();
+  }
+
+  if (p.chol_inv) {
+    double* Xb = p.chol_inv + p.moff[b];
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += nt) {
+      const int i = (int)(idx / n), j = (int)(idx - (int64_t)i * n);
+      Xb[idx] = j <= i ? A[idx] : 0.0;
+    }
   }
 
   // 4. M^-1 = X^T X  (X lower: (X^T X)_ij = sum_{k >= max(i,j)} X_ki X_kj)
@@ -745,6 +763,45 @@ __global__ __launch_bounds__(kRigidBlock) void pc_apply_kernel(const int64_t* __
   }
 }
 
+// Per-body block matvec y_b = B_b x_b (trans = 0) or B_b^T x_b (trans = 1)
+// for k vectors (blockIdx.y), B_b row-major n_b x n_b packed as minv;
+// lower != 0: B_b is lower triangular and only its lower part is read.
+// trans = 0: one wavefront per row, lanes across the columns; trans = 1:
+// one thread per output entry, reading a row of B per step -- both
+// coalesced.
+__global__ __launch_bounds__(kRigidBlock) void block_apply_kernel(const int64_t* __restrict__ offs,
+                                                                  const int64_t* __restrict__ moff,
+                                                                  const double* __restrict__ B, int trans, int lower,
+                                                                  const double* __restrict__ x, double* __restrict__ y,
+                                                                  int64_t ld)
+{
+  extern __shared__ double sx[];
+  const int b = blockIdx.x;
+  const int64_t i0 = offs[b], nb = offs[b + 1] - i0;
+  const int n = (int)(3 * nb);
+  const double* xb = x + (int64_t)blockIdx.y * ld + 3 * i0;
+  double* yb = y + (int64_t)blockIdx.y * ld + 3 * i0;
+  const double* M = B + moff[b];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (int i = tid; i < n; i += blockDim.x) sx[i] = xb[i];
+  __syncthreads();
+  if (!trans) {
+    for (int i = wv; i < n; i += nw) {
+      const int jend = lower ? i + 1 : n;
+      double acc = 0.0;
+      for (int j = lane; j < jend; j += 64) acc += M[(int64_t)i * n + j] * sx[j];
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+      if (lane == 0) yb[i] = acc;
+    }
+  } else {
+    for (int i = tid; i < n; i += blockDim.x) {
+      double acc = 0.0;
+      for (int j = lower ? i : 0; j < n; ++j) acc += M[(int64_t)j * n + i] * sx[j];
+      yb[i] = acc;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -999,6 +1056,77 @@ extern "C" int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, co
       hipLaunchKernelGGL(pc_build_kernel<false>, dim3(nbodies), dim3(kRigidBlock), 0, st, p);
       RMW_CHECK_HIP(hipGetLastError());
     }
+  });
+}
+
+extern "C" int rmw_rigid_pc_build_det_stoch(const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
+                                            int nbodies, double eta, double a, double* minv, double* minvk,
+                                            double* nbody, double* chol, double* chol_inv, int* status,
+                                            double* scratch, size_t scratch_doubles, void* workspace,
+                                            size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(eta > 0 && a > 0, "eta and a must be positive");
+    RMW_REQUIRE(r_blobs && r_rel && minv && minvk && nbody && chol && chol_inv, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    PcBuildParams p{};
+    p.r = r_blobs; p.rrel = r_rel;
+    p.offs = (const int64_t*)(ws + L.off_offs);
+    p.moff = (const int64_t*)(ws + L.off_moff);
+    p.minv = minv; p.minvk = minvk; p.nbody = nbody;
+    p.chol = chol; p.chol_inv = chol_inv;
+    RMW_REQUIRE(scratch_doubles >= L.scratch_doubles && (L.scratch_doubles == 0 || scratch != nullptr),
+                "factorisation scratch too small (bodies above 45 blobs need it)");
+    p.scratch = L.scratch_doubles ? scratch : nullptr;
+    p.scratch_vec = L.scratch_doubles ? scratch + L.minv_doubles : nullptr;
+    p.status = status ? status : (int*)(ws + L.off_status);
+    p.a = a; p.eta = eta;
+    int max_small = 0;
+    bool big = false;
+    for (int b = 0; b < nbodies; ++b) {
+      const int n = (int)(3 * (L.offs[b + 1] - L.offs[b]));
+      if (n <= kLdsMaxN) max_small = std::max(max_small, n); else big = true;
+    }
+    if (max_small) {
+      // the factorisation in LDS: M_b (n^2), then n + 6 n of vectors
+      const size_t lds = ((size_t)max_small * max_small + 8 * (size_t)max_small) * sizeof(double);
+      RMW_REQUIRE(lds <= 160 * 1024, "preconditioner build exceeds LDS");
+      RMW_CHECK_HIP(hipFuncSetAttribute((const void*)pc_build_kernel<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(pc_build_kernel<true>, dim3(nbodies), dim3(kRigidBlock), lds, st, p);
+      RMW_CHECK_HIP(hipGetLastError());
+    }
+    if (big) {
+      hipLaunchKernelGGL(pc_build_kernel<false>, dim3(nbodies), dim3(kRigidBlock), 0, st, p);
+      RMW_CHECK_HIP(hipGetLastError());
+    }
+  });
+}
+
+extern "C" int rmw_rigid_block_apply(const int64_t* body_offsets, int nbodies, const double* blocks, int trans,
+                                     int lower, const double* x, double* y, int k, int64_t ld, void* workspace,
+                                     size_t workspace_bytes, void* stream)
+{
+  return guarded([&] {
+    RigidLayout L = rigid_layout(body_offsets, nbodies);
+    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
+    RMW_REQUIRE(blocks && x && y, "null pointer");
+    RMW_REQUIRE(x != y, "the block apply is out of place");
+    RMW_REQUIRE(k >= 1 && k <= 65535, "k must be in [1, 65535]");
+    RMW_REQUIRE(ld == 0 || ld >= 3 * L.nblobs, "ld must be 0 or >= 3 N");
+    const size_t lds = (size_t)L.max_n * sizeof(double);
+    RMW_REQUIRE(lds <= 160 * 1024, "body too large for the LDS block apply");
+    if (lds > 64 * 1024)
+      RMW_CHECK_HIP(hipFuncSetAttribute((const void*)block_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+    char* ws = (char*)workspace;
+    hipLaunchKernelGGL(block_apply_kernel, dim3(nbodies, k), dim3(kRigidBlock), lds, (hipStream_t)stream,
+                       (const int64_t*)(ws + L.off_offs), (const int64_t*)(ws + L.off_moff), blocks, trans ? 1 : 0,
+                       lower ? 1 : 0, x, y, ld > 0 ? ld : 3 * L.nblobs);
+    RMW_CHECK_HIP(hipGetLastError());
   });
 }
 

@@ -1029,14 +1029,69 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
 }
 
 // ---------------------------------------------------------------------------
+// 5. optional check of the outputs (RMW_FLAG_CHECK_FINITE).
+// ---------------------------------------------------------------------------
+__global__ void zero_count_kernel(unsigned int* count)
+{
+  if (threadIdx.x == 0) count[0] = 0u;
+}
+
+// count += number of inf / NaN entries among k rows of len doubles, rows
+// `stride` apart.
+__global__ __launch_bounds__(256) void count_nonfinite_kernel(const double* __restrict__ x, int64_t k, int64_t len,
+                                                              int64_t stride, unsigned int* count)
+{
+  unsigned int bad = 0;
+  const int64_t total = k * len;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = idx / len;
+    const double v = x[row * stride + (idx - row * len)];
+    bad += isfinite(v) ? 0u : 1u;
+  }
+  if (bad) atomicAdd(count, bad);
+}
+
+// ---------------------------------------------------------------------------
 // Host side: plan, workspace layout, dispatch.
 // ---------------------------------------------------------------------------
+// Vector with room for one element inside the object: the plan of a
+// single suspension (every call but the ragged ones) uses no heap memory.
+template <typename T>
+class SmallVec {
+ public:
+  void push_back(const T& v)
+  {
+    if (n_ == 0 && heap_.empty()) {
+      one_ = v;
+    } else {
+      if (heap_.empty()) heap_.push_back(one_);
+      heap_.push_back(v);
+    }
+    ++n_;
+  }
+  size_t size() const { return n_; }
+  bool empty() const { return n_ == 0; }
+  void clear()
+  {
+    n_ = 0;
+    heap_.clear();
+  }
+  const T* data() const { return heap_.empty() ? &one_ : heap_.data(); }
+  const T& operator[](size_t i) const { return data()[i]; }
+
+ private:
+  T one_{};
+  std::vector<T> heap_;
+  size_t n_ = 0;
+};
+
 struct Plan {
   int domain = kDomWall;
   int mask = 0, prec = 0, kmax = 1;
   int slots = 0;             // resident pair-kernel workgroups on the device
   int64_t ntot = 0;
-  std::vector<SysInfo> sys;  // non-empty systems only
+  SmallVec<SysInfo> sys;     // non-empty systems only
   int64_t total_items = 0;
   size_t partial_doubles = 0;
   size_t off_partial = 0, off_table = 0, total = 0;
@@ -1044,8 +1099,11 @@ struct Plan {
   bool mut_ok = false;
   int kmax_mut = 1;
   int64_t mut_items = 0, chunk_blocks = 4;   // symmetric work item: chunk_blocks 64-source blocks
-  std::vector<MutSys> msys;  // non-empty systems only
+  SmallVec<MutSys> msys;     // non-empty systems only
   size_t off_pi = 0, off_pj = 0, off_mtable = 0;
+  // RMW_FLAG_CHECK_FINITE: a counter of non-finite outputs after the rest
+  bool check_finite = false;
+  size_t off_status = 0;
 };
 
 static bool valid_mask(int mask) { return mask > 0 && mask < 16; }
@@ -1124,7 +1182,8 @@ static int64_t default_chunk(double pairs, int cu_count)
   return std::max<int64_t>(kBlock, (int64_t)std::ceil(c / kChunkQuantum) * kChunkQuantum);
 }
 
-static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, int k,
+// offsets: host array of nsys + 1 blob offsets (offsets[0] = 0).
+static Plan make_plan(int mask, int prec, const int64_t* offsets, int nsys, int k,
                       const rmw_tuning* tuning, bool ragged, int domain = kDomWall)
 {
   RMW_REQUIRE(valid_mask(mask), "mask must be a non-empty subset of UF|UT|WF|WT");
@@ -1144,8 +1203,7 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
                   tuning->rhs_group == 4,
               "rhs_group must be 0, 1, 2 or 4");
   pl.kmax = group_k(k, domain == kDomWall ? rhs_cap(tuning) : 1);
-  const int nsys = (int)offsets.size() - 1;
-  RMW_REQUIRE(nsys >= 1 && offsets[0] == 0, "offsets must start at 0");
+  RMW_REQUIRE(offsets != nullptr && nsys >= 1 && offsets[0] == 0, "offsets must start at 0");
   for (int s = 0; s < nsys; ++s) RMW_REQUIRE(offsets[s + 1] >= offsets[s], "offsets must be non-decreasing");
   pl.ntot = offsets[nsys];
 
@@ -1252,6 +1310,11 @@ static Plan make_plan(int mask, int prec, const std::vector<int64_t>& offsets, i
   pl.off_partial = o; o = align_up(o + pl.partial_doubles * sizeof(double));
   if (ragged) { pl.off_table = o; o = align_up(o + pl.sys.size() * sizeof(SysInfo)); }
   pl.total = std::max(o, mut_total);
+  pl.check_finite = tuning && (tuning->flags & RMW_FLAG_CHECK_FINITE) != 0;
+  if (pl.check_finite) {
+    pl.off_status = align_up(pl.total);
+    pl.total = pl.off_status + RMW_STATUS_BYTES;
+  }
   return pl;
 }
 
@@ -1554,6 +1617,31 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
   }
 }
 
+// RMW_FLAG_CHECK_FINITE: count the non-finite output entries on the stream,
+// wait for it and report them as RMW_E_NOTFINITE.
+static void check_outputs(const Plan& pl, const double* vel, const double* omg, int k, int64_t stride, char* base,
+                          hipStream_t st)
+{
+  if (!pl.check_finite) return;
+  unsigned int* count = (unsigned int*)(base + pl.off_status);
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, st, count);
+  RMW_CHECK_HIP(hipGetLastError());
+  const int64_t len = 3 * pl.ntot;
+  const int64_t grid = std::min<int64_t>(((int64_t)k * len + 255) / 256, 4096);
+  for (const double* x : {vel, omg}) {
+    if (!x || grid == 0) continue;
+    hipLaunchKernelGGL(count_nonfinite_kernel, dim3((unsigned)grid), dim3(256), 0, st, x, (int64_t)k, len, stride,
+                       count);
+    RMW_CHECK_HIP(hipGetLastError());
+  }
+  RMW_CHECK_HIP(hipStreamSynchronize(st));
+  unsigned int bad = 0;
+  RMW_CHECK_HIP(hipMemcpy(&bad, count, sizeof(bad), hipMemcpyDeviceToHost));
+  if (bad)
+    throw Error(RMW_E_NOTFINITE, std::to_string(bad) + " output entries are inf or NaN (coincident distinct "
+                                                       "blobs or non-finite inputs)");
+}
+
 static void run_plan(const Plan& pl, const double* r, const double* f, const double* t, double* vel,
                      double* omg, int k, int64_t ld, double eta, double a, const double* L,
                      const rmw_tuning* tuning, void* ws, size_t ws_bytes, hipStream_t st, bool ragged)
@@ -1590,6 +1678,7 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   const int64_t stride = ld > 0 ? ld : 3 * pl.ntot;
   if (pl.mut_ok && (periodic == 0 || pl.domain == kDomWall)) {
     run_mutual(pl, r, f, t, vel, omg, k, stride, a, inva, c1, c2, tuning, base, st, periodic, Lp);
+    check_outputs(pl, HU ? vel : nullptr, HW ? omg : nullptr, k, stride, base, st);
     return;
   }
 
@@ -1653,6 +1742,7 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
     hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, rp);
     RMW_CHECK_HIP(hipGetLastError());
   }
+  check_outputs(pl, HU ? vel : nullptr, HW ? omg : nullptr, k, stride, base, st);
 }
 
 }  // namespace rmw
@@ -1672,7 +1762,8 @@ extern "C" int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
   return guarded([&] {
     RMW_REQUIRE(bytes != nullptr, "bytes is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
-    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
+    const int64_t off[2] = {0, n};
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false);
     *bytes = pl.total;
   });
 }
@@ -1683,7 +1774,8 @@ extern "C" int rmw_wall_rpy_describe(int mask, int prec, int64_t n, int k, const
   return guarded([&] {
     RMW_REQUIRE(info != nullptr, "info is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
-    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
+    const int64_t off[2] = {0, n};
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false);
     const bool mut = pl.mut_ok && (!periodic || pl.domain == kDomWall);
     info->symmetric = mut ? 1 : 0;
     info->rhs_group = mut ? (periodic ? 1 : pl.kmax_mut) : pl.kmax;
@@ -1700,7 +1792,8 @@ extern "C" int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, c
 {
   return guarded([&] {
     RMW_REQUIRE(n >= 0, "n must be >= 0");
-    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false);
+    const int64_t off[2] = {0, n};
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false);
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
   });
@@ -1712,7 +1805,8 @@ extern "C" int rmw_mobility_workspace_size(int domain, int mask, int prec, int64
   return guarded([&] {
     RMW_REQUIRE(bytes != nullptr, "bytes is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
-    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false, domain);
+    const int64_t off[2] = {0, n};
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, domain);
     *bytes = pl.total;
   });
 }
@@ -1724,7 +1818,8 @@ extern "C" int rmw_mobility_apply(int domain, int mask, int prec, const double* 
 {
   return guarded([&] {
     RMW_REQUIRE(n >= 0, "n must be >= 0");
-    Plan pl = make_plan(mask, prec, {0, n}, k, tuning, false, domain);
+    const int64_t off[2] = {0, n};
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, domain);
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
   });
@@ -1735,8 +1830,7 @@ extern "C" int rmw_wall_rpy_ragged_workspace_size(int mask, int prec, const int6
 {
   return guarded([&] {
     RMW_REQUIRE(bytes != nullptr && offsets != nullptr && nsys >= 1, "bad arguments");
-    std::vector<int64_t> off(offsets, offsets + nsys + 1);
-    Plan pl = make_plan(mask, prec, off, k, tuning, true);
+    Plan pl = make_plan(mask, prec, offsets, nsys, k, tuning, true);
     *bytes = pl.total;
   });
 }
@@ -1749,8 +1843,7 @@ extern "C" int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vec
 {
   return guarded([&] {
     RMW_REQUIRE(offsets != nullptr && nsys >= 1, "bad offsets");
-    std::vector<int64_t> off(offsets, offsets + nsys + 1);
-    Plan pl = make_plan(mask, prec, off, k, tuning, true);
+    Plan pl = make_plan(mask, prec, offsets, nsys, k, tuning, true);
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, true);
   });

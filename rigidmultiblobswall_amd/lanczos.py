@@ -22,6 +22,7 @@ converged (extra steps only refine the converged ones).
 from __future__ import annotations
 
 import ctypes
+import warnings
 from typing import Callable, Optional
 
 import numpy as np
@@ -34,15 +35,33 @@ def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
+class LanczosNotConverged(RuntimeWarning):
+    """The noise difference between the last two Lanczos steps stayed above
+    the tolerance for max_iter steps.  The reference returns that last,
+    unconverged noise without a status (stochastic_forcing.py:257-262);
+    here it is returned too, with this warning, or raised as an error when
+    the caller passes strict=True."""
+
+
+def _unconverged(what: str, strict: bool):
+    msg = f"Lanczos did not reach the tolerance: {what}"
+    if strict:
+        raise RuntimeError(msg)
+    warnings.warn(msg, LanczosNotConverged, stacklevel=3)
+
+
 def lanczos_block(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch.Tensor, factor: float = 1.0,
                   tolerance: float = 1e-6, max_iter: int = 1000,
-                  L_mult: Optional[Callable[[torch.Tensor], torch.Tensor]] = None):
+                  L_mult: Optional[Callable[[torch.Tensor], torch.Tensor]] = None, strict: bool = False):
     """noise (k, dim) = factor * M^1/2 Z for Z (k, dim) on the device.
 
     matvec(X, Y) must write Y[r] = M X[r] for the k rows (X, Y (k, dim)
     contiguous float64).  Returns (noise, iterations) with iterations as the
     reference counts them (the step index at which the test passed, or
-    max_iter when it did not)."""
+    max_iter when it did not).  A row that never passes the test gets its
+    last noise and a LanczosNotConverged warning (strict=True: an error).
+    L_mult(x) maps a (dim,) device vector to L x (noise = L (P M P^T)^1/2 z
+    for a preconditioned operator, stochastic_forcing.py:253-256)."""
     lib = _lib.load()
     if Z.dim() == 1:
         Z = Z.reshape(1, -1)
@@ -108,6 +127,8 @@ def lanczos_block(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch
     for r in range(k):
         coef.copy_(C[r, :m])
         ops.combination(V[:, r], m, coef, noise[r], accumulate=False)
+    if not converged.all():
+        _unconverged(f"{int((~converged).sum())} of {k} right-hand sides after {max_iter} steps", strict)
     if L_mult is not None:
         noise = torch.stack([L_mult(noise[r]) for r in range(k)])
     return noise, int(it_conv.max())
@@ -157,7 +178,7 @@ class Segments:
 
 
 def lanczos_ragged(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch.Tensor, seg: Segments,
-                   factor: float = 1.0, tolerance: float = 1e-6, max_iter: int = 1000):
+                   factor: float = 1.0, tolerance: float = 1e-6, max_iter: int = 1000, strict: bool = False):
     """factor * M_s^1/2 z_s for every system s of a ragged batch at once.
 
     Z (k, seg.dim): k batch vectors; matvec(X, Y) writes Y[r] = M X[r] for
@@ -248,6 +269,8 @@ def lanczos_ragged(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torc
         if converged.all():
             break
     # systems that never passed keep their last coefficients (as the reference)
+    if not converged.all():
+        _unconverged(f"{int((~converged).sum())} (right-hand side, system) pairs after {max_iter} steps", strict)
     for r, s in zip(*np.nonzero(~converged)):
         c_final[r, s, :i_last + 1] = c_prev[r, s]
         m_used[r, s] = i_last + 1

@@ -10,6 +10,10 @@ Extra keyword arguments understood here:
   precision: 'fp32' (FP32 pair terms, FP64 per-target sums; default),
              'fp32_kahan' (compensated FP32 sums) or 'fp64'.
   tuning:    {'chunk_len': int, 'max_wgs': int} launch shape override.
+  check_finite: raise RmwError (code RMW_E_NOTFINITE, -4) when an output
+             entry is inf or NaN -- e.g. two distinct blobs at one position,
+             which the reference returns silently as NaN
+             (mobility/mobility_numba.py:88).  Off by default, as there.
 
 The computation always runs on the GPU through librmw_hip; inputs given as
 numpy arrays are copied to the device and results copied back.  Inputs
@@ -52,47 +56,43 @@ def _opts(kwargs):
             kwargs.get("precision", DEFAULT_PRECISION), kwargs.get("tuning"))
 
 
+def _apply_opts(kwargs):
+    """Keyword arguments of wall_rpy.apply from a drop-in's **kwargs."""
+    L, prec, tun = _opts(kwargs)
+    return dict(periodic_length=L, precision=prec, tuning=tun, check_finite=bool(kwargs.get("check_finite", False)))
+
+
 def _host(t):
     return t.reshape(-1).cpu().numpy()
 
 
 def single_wall_mobility_trans_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
     """u = B M_UF(z_eff) B f; drop-in for mobility.py:1132-1163."""
-    L, prec, tun = _opts(kwargs)
-    u, _ = wall_rpy.apply(RMW_UF, r_vectors, force=force, eta=eta, a=a, periodic_length=L,
-                          precision=prec, tuning=tun)
+    u, _ = wall_rpy.apply(RMW_UF, r_vectors, force=force, eta=eta, a=a, **_apply_opts(kwargs))
     return _host(u)
 
 
 def single_wall_mobility_trans_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
     """u = B M_UT(z_eff) B t; drop-in for mobility.py:1213-1232."""
-    L, prec, tun = _opts(kwargs)
-    u, _ = wall_rpy.apply(RMW_UT, r_vectors, torque=torque, eta=eta, a=a, periodic_length=L,
-                          precision=prec, tuning=tun)
+    u, _ = wall_rpy.apply(RMW_UT, r_vectors, torque=torque, eta=eta, a=a, **_apply_opts(kwargs))
     return _host(u)
 
 
 def single_wall_mobility_rot_times_force_hip(r_vectors, force, eta, a, *args, **kwargs):
     """w = B M_WF(z_eff) B f; drop-in for mobility.py:1273-1298."""
-    L, prec, tun = _opts(kwargs)
-    _, w = wall_rpy.apply(RMW_WF, r_vectors, force=force, eta=eta, a=a, periodic_length=L,
-                          precision=prec, tuning=tun)
+    _, w = wall_rpy.apply(RMW_WF, r_vectors, force=force, eta=eta, a=a, **_apply_opts(kwargs))
     return _host(w)
 
 
 def single_wall_mobility_rot_times_torque_hip(r_vectors, torque, eta, a, *args, **kwargs):
     """w = B M_WT(z_eff) B t; drop-in for mobility.py:1316-1342."""
-    L, prec, tun = _opts(kwargs)
-    _, w = wall_rpy.apply(RMW_WT, r_vectors, torque=torque, eta=eta, a=a, periodic_length=L,
-                          precision=prec, tuning=tun)
+    _, w = wall_rpy.apply(RMW_WT, r_vectors, torque=torque, eta=eta, a=a, **_apply_opts(kwargs))
     return _host(w)
 
 
 def single_wall_mobility_trans_times_force_torque_hip(r_vectors, force, torque, eta, a, *args, **kwargs):
     """u = M_UF f + M_UT t in one pass; drop-in for mobility.py:384-410."""
-    L, prec, tun = _opts(kwargs)
-    u, _ = wall_rpy.apply(RMW_UF | RMW_UT, r_vectors, force=force, torque=torque, eta=eta, a=a,
-                          periodic_length=L, precision=prec, tuning=tun)
+    u, _ = wall_rpy.apply(RMW_UF | RMW_UT, r_vectors, force=force, torque=torque, eta=eta, a=a, **_apply_opts(kwargs))
     return _host(u)
 
 
@@ -100,19 +100,17 @@ def single_wall_grand_mobility_hip(r_vectors, force_torque, eta, a, *args, **kwa
     """[UF UT; WF WT] [f; t] -> [u; w] in one fused pass; drop-in for the
     rollers' grand_mobility_matrix closure
     (quaternion_integrator/quaternion_integrator_rollers.py:1114-1122)."""
-    L, prec, tun = _opts(kwargs)
     ft = np.asarray(force_torque, dtype=np.float64).reshape(-1) if not torch.is_tensor(force_torque) \
         else force_torque.reshape(-1)
     half = ft.shape[0] // 2
     u, w = wall_rpy.apply(RMW_UF | RMW_UT | RMW_WF | RMW_WT, r_vectors, force=ft[:half], torque=ft[half:],
-                          eta=eta, a=a, periodic_length=L, precision=prec, tuning=tun)
+                          eta=eta, a=a, **_apply_opts(kwargs))
     return np.concatenate([_host(u), _host(w)])
 
 
 def _product(domain, mask, r_vectors, force, torque, eta, a, kwargs, out="u"):
-    L, prec, tun = _opts(kwargs)
-    u, w = wall_rpy.apply(mask, r_vectors, force=force, torque=torque, eta=eta, a=a, periodic_length=L,
-                          precision=prec, tuning=tun, domain=domain)
+    u, w = wall_rpy.apply(mask, r_vectors, force=force, torque=torque, eta=eta, a=a, domain=domain,
+                          **_apply_opts(kwargs))
     return _host(u if out == "u" else w)
 
 

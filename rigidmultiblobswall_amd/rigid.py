@@ -10,6 +10,11 @@ provides the pieces of the reference's rigid mobility solve:
   pc_apply(x, out)        block-diagonal preconditioner (multi_bodies.py:825-849)
   solve_mobility_problem  normalised-RHS right-preconditioned GMRES
                           (quaternion_integrator_multi_bodies.py:1441-1536)
+  build_stochastic_preconditioner, mobility_pc, P_inv_mult, brownian_slip
+                          the stochastic block-diagonal preconditioner and
+                          the Brownian slip L (P M P^T)^1/2 z it serves
+                          (multi_bodies.py:480-623, stochastic_forcing.py
+                          :112-262)
 
 The blob mobility product M lambda is librmw_hip's wall-RPY kernel; its
 pair arithmetic is chosen by `precision` ('fp64', 'fp32', 'fp32_blocked', ...).
@@ -84,6 +89,8 @@ class RigidSuspension:
         self._mlam = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
         self.minv = None
         self.pc_r_rel = None     # body-frame blob offsets the preconditioner was built with
+        self.chol = None         # per body L_b (M_b = L_b L_b^T), packed like minv
+        self.chol_inv = None     # per body L_b^-1
 
     def _offp(self):
         return self.body_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
@@ -102,6 +109,8 @@ class RigidSuspension:
                                            self.r_rel.data_ptr(), self.ws.data_ptr(), self.ws.numel(), _stream()))
         self.minv = None
         self.pc_r_rel = None
+        self.chol = None
+        self.chol_inv = None
 
     # -- blob mobility and operator -------------------------------------------------
     def mobility_times(self, lam: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
@@ -137,18 +146,91 @@ class RigidSuspension:
         if bad:
             raise RuntimeError(f"{bad} body mobility blocks are not positive definite")
 
+    def build_stochastic_preconditioner(self):
+        """The deterministic preconditioner (M_b^-1, M_b^-1 K_b, N_b) and,
+        per body, the lower Cholesky factor L_b of the dense wall mobility
+        M_b = L_b L_b^T with its inverse L_b^-1: the three factorisations of
+        build_block_diagonal_preconditioners_det_stoch (multi_bodies.py
+        :480-560; its `M_factorization_blobs` is L_b, its
+        `M_factorization_blobs_inv` is the upper U_b^-1 = L_b^-T of scipy's
+        M_b = U_b^T U_b, applied here as the transpose of L_b^-1)."""
+        dev = self.device
+        nd = max(self.minv_doubles, 1)
+        self.minv = torch.empty(nd, dtype=torch.float64, device=dev)
+        self.chol = torch.empty(nd, dtype=torch.float64, device=dev)
+        self.chol_inv = torch.empty(nd, dtype=torch.float64, device=dev)
+        self.minvk = torch.empty((self.Nblobs * 3, 6), dtype=torch.float64, device=dev)
+        self.nbody = torch.empty((self.nbodies, 6, 6), dtype=torch.float64, device=dev)
+        status = torch.zeros(self.nbodies, dtype=torch.int32, device=dev)
+        scratch = (torch.empty(self.pc_scratch_doubles, dtype=torch.float64, device=dev)
+                   if self.pc_scratch_doubles else None)
+        _lib.check(self.lib.rmw_rigid_pc_build_det_stoch(
+            self.r_blobs.data_ptr(), self.r_rel.data_ptr(), self._offp(), self.nbodies, self.eta, self.a,
+            self.minv.data_ptr(), self.minvk.data_ptr(), self.nbody.data_ptr(), self.chol.data_ptr(),
+            self.chol_inv.data_ptr(), status.data_ptr(), scratch.data_ptr() if scratch is not None else None,
+            self.pc_scratch_doubles, self.ws.data_ptr(), self.ws.numel(), _stream()))
+        del scratch
+        self.pc_r_rel = self.r_rel
+        bad = int(status.sum().item())
+        if bad:
+            raise RuntimeError(f"{bad} body mobility blocks are not positive definite")
+
     def preconditioner_state(self):
-        """The built preconditioner (M_b^-1, M_b^-1 K_b, N_b and the K_b
-        offsets it used), for reuse on a later configuration."""
+        """The built preconditioner (M_b^-1, M_b^-1 K_b, N_b, the K_b offsets
+        it used and, if built, the Cholesky factors), for reuse on a later
+        configuration."""
         if self.minv is None:
             self.build_preconditioner()
-        return (self.minv, self.minvk, self.nbody, self.pc_r_rel.clone())
+        return (self.minv, self.minvk, self.nbody, self.pc_r_rel.clone(), self.chol, self.chol_inv)
 
     def adopt_preconditioner(self, state):
         """Apply a preconditioner built on an earlier configuration of the
-        same bodies (multi_bodies.py:766-819 with update_PC > 1: the stale
-        M_b^-1, N_b and K_b are used until the next rebuild)."""
-        self.minv, self.minvk, self.nbody, self.pc_r_rel = state
+        same bodies (multi_bodies.py:766-819 and :525-555 with update_PC > 1:
+        the stale M_b^-1, N_b, K_b and factors are used until the next
+        rebuild)."""
+        self.minv, self.minvk, self.nbody, self.pc_r_rel, self.chol, self.chol_inv = state
+
+    def _block_apply(self, blocks: torch.Tensor, x: torch.Tensor, y: torch.Tensor, trans: bool):
+        """y_b = B_b x_b (or B_b^T x_b) for every body and each of the k rows
+        of x, y (k, 3N); B_b lower triangular."""
+        k = 1 if x.dim() == 1 else x.shape[0]
+        _lib.check(self.lib.rmw_rigid_block_apply(self._offp(), self.nbodies, blocks.data_ptr(), int(trans), 1,
+                                                  x.data_ptr(), y.data_ptr(), k, 3 * self.Nblobs, self.ws.data_ptr(),
+                                                  self.ws.numel(), _stream()))
+
+    def mobility_pc(self, X: torch.Tensor, Y: torch.Tensor, precision: Optional[str] = None):
+        """Y[r] = P^T M P X[r] with P = L^-T per body (the `mobility_pc` closure
+        of multi_bodies.py:591-606) for X, Y (k, 3N) contiguous; its block
+        diagonal is the identity."""
+        if self.chol_inv is None:
+            raise RuntimeError("build_stochastic_preconditioner() first")
+        k = X.shape[0]
+        t1 = torch.empty_like(X)
+        self._block_apply(self.chol_inv, X, t1, trans=True)
+        t2 = torch.empty((k, self.Nblobs, 3), dtype=torch.float64, device=self.device)
+        wall_rpy.apply(RMW_UF, self.r_blobs, force=t1.view(k, self.Nblobs, 3), eta=self.eta, a=self.a,
+                       periodic_length=self.L, precision=precision or self.precision, k=k, out_velocity=t2)
+        self._block_apply(self.chol_inv, t2.view(k, -1), Y, trans=False)
+
+    def P_inv_mult(self, x: torch.Tensor) -> torch.Tensor:
+        """L x per body (the `P_inv_mult` closure of multi_bodies.py:608-615,
+        out of place)."""
+        y = torch.empty_like(x)
+        self._block_apply(self.chol, x.contiguous(), y, trans=False)
+        return y
+
+    def brownian_slip(self, Z: torch.Tensor, factor: float, tolerance: float, max_iter: int = 1000,
+                      precision: Optional[str] = None, strict: bool = False):
+        """factor * L (P^T M P)^1/2 z for the k rows of Z (k, 3N): the blob
+        noise of stochastic_EM (quaternion_integrator_multi_bodies.py:288-298),
+        Lanczos on the preconditioned mobility with L_mult = L.  Its
+        covariance is factor^2 L (P^T M P) L^T = factor^2 M.  Returns
+        (noise (k, 3N), iterations)."""
+        from . import lanczos
+        if self.chol is None:
+            self.build_stochastic_preconditioner()
+        return lanczos.lanczos_block(lambda X, Y: self.mobility_pc(X, Y, precision), Z, factor=factor,
+                                     tolerance=tolerance, max_iter=max_iter, L_mult=self.P_inv_mult, strict=strict)
 
     def pc_apply(self, x: torch.Tensor, out: torch.Tensor):
         if self.minv is None:
@@ -250,7 +332,9 @@ class RigidSuspension:
         the one that ends the solve, is FP64.  A start whose inner-operator
         residual is within 10x of `inner_floor` is judged by its FP64
         residual instead.  Returns a GmresResult whose iterations and
-        matvecs sum over the inner solves; final_residual is the FP64 one."""
+        matvecs sum over the inner solves (at most `maxiter` iterations in
+        all); final_residual is the FP64 one, and info is 0 only when it meets
+        tol (else a positive count, as scipy's gmres)."""
         if rhs is None:
             rhs = self.rhs(np.zeros((self.Nblobs, 3)) if slip is None else slip,
                            np.zeros((self.nbodies, 6)) if force_torque is None else force_torque)
@@ -294,8 +378,9 @@ class RigidSuspension:
         for outer in range(max_outer + 1):
             if outer > 0:
                 rel = fp64_residual()
-                if rel <= tol or outer == max_outer:
-                    out.info = 0 if rel <= tol else out.iterations
+                if rel <= tol or outer == max_outer or out.iterations >= maxiter:
+                    # info: 0 when converged, else a positive iteration count
+                    out.info = 0 if rel <= tol else max(out.iterations, 1)
                     break
             if rel == 0.0:
                 continue             # the start solves the inner operator exactly: judge it in FP64
@@ -309,8 +394,9 @@ class RigidSuspension:
             rn = rel * bnorm
             rs = torch.empty_like(r)
             ops.axpby(1.0 / rn, r, 0.0, rs)
+            # the inner solves share the caller's iteration budget
             res = krylov.gmres(lambda v, o: self.operator(v, o, inner_precision), rs, Pinv=self.pc_apply, tol=inner_tol,
-                               restart=restart, maxiter=maxiter, verify=False)
+                               restart=restart, maxiter=max(1, maxiter - out.iterations), verify=False)
             out.iterations += res.iterations
             out.matvecs += res.matvecs
             ops.axpby(rn, res.x, 1.0, x)

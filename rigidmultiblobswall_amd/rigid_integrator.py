@@ -1,10 +1,15 @@
-"""Deterministic time stepping of rigid multiblob bodies above a wall.
+"""Time stepping of rigid multiblob bodies above a wall.
 
 Mirror of QuaternionIntegrator (quaternion_integrator/
 quaternion_integrator_multi_bodies.py) for its GMRES schemes
 
   deterministic_forward_euler     :75-107
   deterministic_adams_bashforth   :142-186
+  stochastic_EM                   :262-300  (Euler-Maruyama: blob noise
+      factor L (P^T M P)^1/2 z with factor sqrt(2 kT / dt) from Lanczos on
+      the stochastic block-diagonal preconditioner, multi_bodies.py:480-623,
+      subtracted from the slip of the GMRES right-hand side; a step that
+      moves a blob below the wall is redrawn)
 
 with solve_mobility_problem (:1441-1548), the body forces of
 force_torque_calculator_sort_by_bodies (multi_bodies/multi_bodies_functions.py
@@ -24,11 +29,17 @@ every step like active_body_slip (multi_bodies_functions.py:123-140).
 ``solver="mixed"`` (no counterpart in the reference) solves each step with
 RigidSuspension.solve_mixed: FP32-pair inner GMRES refined against FP64
 residuals to the same FP64 tolerance.
+
+Random numbers come from a torch generator on the device (the reference
+draws z from numpy's global generator inside stochastic_forcing_lanczos);
+``noise_source(kind, shape)`` with kind "z" replaces them, one call per
+attempt, in the reference's order.
 """
 from __future__ import annotations
 
 import ctypes
-from typing import Optional, Sequence
+import math
+from typing import Callable, Optional, Sequence
 
 import numpy as np
 import torch
@@ -36,7 +47,7 @@ import torch
 from . import _lib, forces
 from .rigid import RigidSuspension
 
-SCHEMES = ("deterministic_forward_euler", "deterministic_adams_bashforth")
+SCHEMES = ("deterministic_forward_euler", "deterministic_adams_bashforth", "stochastic_EM")
 
 
 def _stream():
@@ -50,7 +61,8 @@ class RigidIntegrator:
                  blob_blob_force_implementation: str = "None", repulsion_strength: float = 1.0,
                  debye_length: float = 1.0, periodic_length=None, tolerance: float = 1e-8,
                  precision: str = "fp64", slip_reference=None, domain: str = "single_wall", restart: int = 60,
-                 update_PC: int = 1, solver: str = "fp64"):
+                 update_PC: int = 1, solver: str = "fp64", kT: float = 0.0, seed: Optional[int] = None,
+                 noise_source: Optional[Callable] = None, lanczos_precision: Optional[str] = None):
         if scheme not in SCHEMES:
             raise ValueError(f"scheme {scheme!r} is not provided; use one of {list(SCHEMES)}")
         if solver not in ("fp64", "mixed"):
@@ -90,6 +102,16 @@ class RigidIntegrator:
         self.velocities_previous_step: Optional[torch.Tensor] = None
         self.invalid_configuration_count = 0
         self.det_iterations_count = 0
+        self.stoch_iterations_count = 0
+        self.kT = float(kT)
+        # pair arithmetic of the Lanczos products (default: the solve's)
+        self.lanczos_precision = lanczos_precision
+        self.gen = torch.Generator(device=self.device)
+        if seed is not None:
+            self.gen.manual_seed(int(seed))
+        else:
+            self.gen.seed()
+        self.noise_source = noise_source
         self._S: Optional[RigidSuspension] = None       # the current configuration
         self._S_next: Optional[RigidSuspension] = None  # buffers of the trial configuration
         self._slip_ref_dev: Optional[torch.Tensor] = None
@@ -168,11 +190,22 @@ class RigidIntegrator:
                                            S.ws.numel(), _stream()))
         return rot
 
-    def _preconditioner(self, S: RigidSuspension, step: int) -> None:
+    def _draw(self, kind: str, shape) -> torch.Tensor:
+        if self.noise_source is not None:
+            t = torch.as_tensor(self.noise_source(kind, shape))
+            return t.to(self.device, torch.float64).reshape(shape).contiguous()
+        return torch.randn(shape, dtype=torch.float64, device=self.device, generator=self.gen)
+
+    def _preconditioner(self, S: RigidSuspension, step: int, stochastic: bool = False) -> None:
         """Rebuilt when step % update_PC == 0 or none exists, else the last
-        one is reused (multi_bodies.py:766-819)."""
-        if self._pc_state is None or step % self.update_PC == 0:
-            S.build_preconditioner()
+        one is reused (multi_bodies.py:766-819; the stochastic one with its
+        Cholesky factors, :522-555)."""
+        stale_kind = stochastic and self._pc_state is not None and self._pc_state[4] is None
+        if self._pc_state is None or step % self.update_PC == 0 or stale_kind:
+            if stochastic:
+                S.build_stochastic_preconditioner()
+            else:
+                S.build_preconditioner()
             self.pc_builds += 1
             self._pc_state = S.preconditioner_state() if self.update_PC > 1 else None
             if self.update_PC > 1:
@@ -180,10 +213,15 @@ class RigidIntegrator:
         else:
             S.adopt_preconditioner(self._pc_state)
 
-    def solve_mobility_problem(self, S: RigidSuspension, step: Optional[int] = None):
-        """Body velocities (B, 6) of the current configuration (:1441-1548)."""
-        self._preconditioner(S, self.step_count if step is None else step)
+    def solve_mobility_problem(self, S: RigidSuspension, step: Optional[int] = None,
+                               noise: Optional[torch.Tensor] = None, stochastic: bool = False):
+        """Body velocities (B, 6) of the current configuration (:1441-1548);
+        `noise` (3N) is subtracted from the slip of the right-hand side
+        (:1490-1492)."""
+        self._preconditioner(S, self.step_count if step is None else step, stochastic=stochastic)
         rhs = S.rhs(self.slip(S), self.force_torque(S))
+        if noise is not None:
+            rhs[: 3 * S.Nblobs] -= noise.reshape(-1)
         rnorm = float(torch.linalg.norm(rhs))
         x0 = None if (self.first_guess is None or rnorm == 0.0) else self.first_guess * rnorm
         if self.solver == "mixed":
@@ -196,9 +234,10 @@ class RigidIntegrator:
         self.first_guess = res.x / rnorm if rnorm > 0 else torch.zeros_like(res.x)
         return res.x[3 * S.Nblobs:].reshape(S.nbodies, 6).contiguous()
 
-    def _advance(self, dt: float, adams_bashforth: bool, step: Optional[int] = None) -> None:
-        S = self._current()
-        U = self.solve_mobility_problem(S, step)
+    def _trial(self, S: RigidSuspension, U: torch.Tensor, dt: float, ab: bool) -> bool:
+        """Move the bodies with velocities U (and the previous step's, for
+        Adams-Bashforth) into the trial configuration; True when every blob
+        stays above the wall (check_positions, :1779-1829)."""
         if self._S_next is None:
             self._S_next = self.suspension()
             nb = ctypes.c_size_t()
@@ -206,7 +245,6 @@ class RigidIntegrator:
             self._count_ws = torch.empty(max(nb.value, 256), dtype=torch.uint8, device=self.device)
             self._count = torch.empty(1, dtype=torch.int64, device=self.device)
         Sn = self._S_next
-        ab = adams_bashforth and not self.first_step
         # the new centres and quaternions go straight into the trial suspension
         _lib.check(self.lib.rmw_bodies_advance(S.centres.data_ptr(), S.quats.data_ptr(), U.data_ptr(),
                                                self.velocities_previous_step.data_ptr() if ab else None,
@@ -216,7 +254,19 @@ class RigidIntegrator:
         self._count.zero_()
         _lib.check(self.lib.rmw_count_below(Sn.r_blobs.data_ptr(), Sn.Nblobs, 0.0, self._count.data_ptr(),
                                             self._count_ws.data_ptr(), self._count_ws.numel(), _stream()))
-        if int(self._count.item()) > 0:
+        return int(self._count.item()) == 0
+
+    def _accept(self, S: RigidSuspension) -> None:
+        self._S, self._S_next = self._S_next, S
+        self._centres = None
+        self._quats = None
+        self.step_count += 1
+
+    def _advance(self, dt: float, adams_bashforth: bool, step: Optional[int] = None) -> None:
+        S = self._current()
+        U = self.solve_mobility_problem(S, step)
+        ab = adams_bashforth and not self.first_step
+        if not self._trial(S, U, dt, ab):
             # check_positions fails; a deterministic step would be recomputed
             # unchanged forever by the reference
             self.invalid_configuration_count += 1
@@ -224,16 +274,40 @@ class RigidIntegrator:
         if adams_bashforth:
             self.first_step = False
             self.velocities_previous_step = U
-        self._S, self._S_next = Sn, S
-        self._centres = None
-        self._quats = None
-        self.step_count += 1
+        self._accept(S)
 
     def deterministic_forward_euler(self, dt: float, *args, **kwargs):
         self._advance(dt, adams_bashforth=False, step=kwargs.get("step"))
 
     def deterministic_adams_bashforth(self, dt: float, *args, **kwargs):
         self._advance(dt, adams_bashforth=True, step=kwargs.get("step"))
+
+    def brownian_slip(self, S: RigidSuspension, dt: float) -> torch.Tensor:
+        """sqrt(2 kT / dt) L (P^T M P)^1/2 z on the blobs (3N), Lanczos to the
+        solver tolerance (:287-296)."""
+        if self.kT == 0.0:
+            return torch.zeros(3 * S.Nblobs, dtype=torch.float64, device=self.device)
+        z = self._draw("z", (1, 3 * S.Nblobs))
+        noise, its = S.brownian_slip(z, factor=math.sqrt(2.0 * self.kT / dt), tolerance=self.tolerance,
+                                     precision=self.lanczos_precision)
+        self.stoch_iterations_count += its
+        return noise[0]
+
+    def stochastic_EM(self, dt: float, *args, **kwargs):
+        """Euler-Maruyama step (:262-300): build the stochastic preconditioner
+        (every update_PC steps), draw the blob noise, solve the mobility
+        problem with it, move the bodies with the solved velocities; redraw
+        while a blob ends below the wall."""
+        step = kwargs.get("step")
+        while True:
+            S = self._current()
+            self._preconditioner(S, self.step_count if step is None else step, stochastic=True)
+            noise = self.brownian_slip(S, dt)
+            U = self.solve_mobility_problem(S, step, noise=noise, stochastic=True)
+            if self._trial(S, U, dt, ab=False):
+                self._accept(S)
+                return
+            self.invalid_configuration_count += 1
 
     def advance_time_step(self, dt: float, *args, **kwargs):
         """One step of the selected scheme (:68-72)."""

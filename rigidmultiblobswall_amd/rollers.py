@@ -32,12 +32,7 @@ class Rollers:
 
     def grand_matvec(self, X: torch.Tensor, Y: torch.Tensor, precision=None):
         """Y[r] = [UF UT; WF WT] X[r] for X, Y (k, 6N) with X[r] = [f; t]."""
-        k = X.shape[0]
-        n3 = 3 * self.n
-        ld = X.stride(0)
-        assert Y.stride(0) == ld and X.is_contiguous() and Y.is_contiguous()
-        wall_rpy.apply_strided(GRAND, self.r, self.n, k, ld, X.data_ptr(), X.data_ptr() + 8 * n3, Y.data_ptr(),
-                               Y.data_ptr() + 8 * n3, self.eta, self.a, self.L, precision or self.precision)
+        wall_rpy.apply_strided(GRAND, self.r, self.n, X, Y, self.eta, self.a, self.L, precision or self.precision)
 
     def brownian_noise(self, Z: torch.Tensor, factor: float = 1.0, tolerance: float = 1e-3, max_iter: int = 1000,
                        precision=None):
@@ -74,12 +69,7 @@ class RaggedRollers:
 
     def grand_matvec(self, X: torch.Tensor, Y: torch.Tensor, precision=None):
         """Y[r] = blockdiag_s([UF UT; WF WT]_s) X[r] for X, Y (k, 6N)."""
-        k = X.shape[0]
-        n3 = 3 * self.n
-        ld = X.stride(0)
-        assert Y.stride(0) == ld and X.is_contiguous() and Y.is_contiguous()
-        wall_rpy.apply_ragged_strided(GRAND, self.r, self.offsets, k, ld, X.data_ptr(), X.data_ptr() + 8 * n3,
-                                      Y.data_ptr(), Y.data_ptr() + 8 * n3, self.eta, self.a, self.L,
+        wall_rpy.apply_ragged_strided(GRAND, self.r, self.offsets, X, Y, self.eta, self.a, self.L,
                                       precision or self.precision, tuning=self.tuning)
 
     def brownian_noise(self, Z: torch.Tensor, factor: float = 1.0, tolerance: float = 1e-3, max_iter: int = 1000,

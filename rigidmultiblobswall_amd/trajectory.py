@@ -11,13 +11,18 @@ Forces follow the reference's blob-force model
 and body force/torque are sums over blobs (force_torque_calculator_sort_by_
 bodies, :411-445).
 
-Time step (Euler-Maruyama): the deterministic velocity U is the rigid
-mobility solve (GPU, FP64, warm-started from the previous step); the
-Brownian displacement sqrt(2 kT dt) N_b^1/2 W uses each body's 6x6 mobility
-N_b = pinv(K^T M_b^-1 K) from the block-diagonal preconditioner, i.e. it
-omits the hydrodynamic correlation between bodies in the noise (the drift
-is exact).  Quaternions are advanced by the rotation vector
-w dt + dtheta (quaternion_integrator/quaternion.py:15-38).
+Time step (Euler-Maruyama, stochastic_EM of quaternion_integrator/
+quaternion_integrator_multi_bodies.py:262-300): the Brownian blob noise
+sqrt(2 kT / dt) L (P^T M P)^1/2 z comes from Lanczos on the stochastic
+block-diagonal preconditioner (multi_bodies.py:480-623), so it carries the
+hydrodynamic correlation between all blobs; the rigid solve of the
+right-hand side [-noise; -F] gives the body velocities.  By linearity that
+solve is split into the deterministic part [0; -F] (its solution is what a
+snapshot stores: the (geometry, lambda) pair the warm start learns, solved
+warm-started from the previous step) and the noise part [-noise; 0]; both
+are solved to the same FP64 residual.  Quaternions are advanced by the
+rotation vector w dt (quaternion.py:15-38); a step that moves a blob below
+the wall is redrawn.
 """
 from __future__ import annotations
 
@@ -104,17 +109,39 @@ class Trajectory:
 
 def run_c4_trajectory(centres, quats, ref, a, eta, nsteps=200, dt=1e-3, kT=0.1, seed=4, tol=1e-8,
                       fp: Optional[ForceParams] = None, precision="fp64", log=None,
-                      field_radius: float = 0.9) -> Trajectory:
+                      field_radius: float = 0.9, solver: str = "mixed", lanczos_tol: float = 1e-3,
+                      lanczos_precision: str = "fp32_blocked") -> Trajectory:
+    """Brownian trajectory of rigid bodies; solver 'mixed' (FP32-pair inner
+    GMRES refined to the FP64 tolerance) or 'fp64' for both solves of a
+    step."""
     fp = fp or ForceParams()
-    rng = np.random.default_rng(seed + 1000)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed) + 1000)
     centres = np.array(centres, dtype=np.float64)
     quats = np.array(quats, dtype=np.float64)
     traj = Trajectory(params=dict(nsteps=nsteps, dt=dt, kT=kT, eta=eta, a=a, tol=tol, forces=fp.__dict__,
-                                  precision=precision))
+                                  precision=precision, solver=solver, lanczos_tol=lanczos_tol,
+                                  lanczos_precision=lanczos_precision,
+                                  noise="sqrt(2 kT/dt) L (P^T M P)^1/2 z (Lanczos, stochastic block-diagonal "
+                                        "preconditioner)"))
+    S = RigidSuspension(centres, quats, ref, eta, a, precision=precision)
+    factor = float(np.sqrt(2.0 * kT / dt))
+
+    def solve(rhs, x0):
+        if solver == "mixed":
+            res = S.solve_mixed(rhs=rhs, tol=tol, x0=x0)
+        else:
+            res = S.solve_mobility_problem(rhs=rhs, tol=tol, x0=x0)
+        if res.final_residual is not None and res.final_residual > 1.5 * tol:
+            raise RuntimeError(f"GMRES did not reach {tol} (residual {res.final_residual:.3e})")
+        return res
+
     prev = None
     for step in range(nsteps):
-        S = RigidSuspension(centres, quats, ref, eta, a, precision=precision)
-        S.build_preconditioner()
+        if step:
+            S.set_configuration(centres, quats)
+        S.build_stochastic_preconditioner()
         ft = body_forces(centres, quats, ref, a, fp)
         rhs = S.rhs(np.zeros((S.Nblobs, 3)), ft)
         pc = torch.empty_like(rhs)
@@ -122,27 +149,31 @@ def run_c4_trajectory(centres, quats, ref, a, eta, nsteps=200, dt=1e-3, kT=0.1, 
         base = S.point_field_guess(rhs, ft, radius=field_radius)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = S.solve_mobility_problem(rhs=rhs, tol=tol, x0=prev)
+        res = solve(rhs, prev)
         torch.cuda.synchronize()
         ts = time.perf_counter() - t0
-        if res.info != 0:
-            raise RuntimeError(f"GMRES did not converge at step {step}")
         traj.snapshots.append(Snapshot(centres.copy(), quats.copy(), ft, res.x.clone(), pc, res.iterations, ts, base))
-        if log:
-            log(f"step {step}: GMRES iterations {res.iterations}, {ts * 1e3:.1f} ms")
         prev = res.x
-        U = S.velocities(res.x)                       # (B, 6) [v, w]
-        N = S.nbody.cpu().numpy()
-        N = 0.5 * (N + np.transpose(N, (0, 2, 1)))
-        L = np.linalg.cholesky(N + 1e-14 * np.eye(6))
-        for _ in range(20):
-            W = rng.standard_normal((len(centres), 6))
-            dX = np.sqrt(2.0 * kT * dt) * np.einsum("bij,bj->bi", L, W)
-            c_new = centres + U[:, :3] * dt + dX[:, :3]
-            q_new = quat_mul(quat_from_rotation(U[:, 3:] * dt + dX[:, 3:]), quats)
+        U_det = S.velocities(res.x)                   # (B, 6) [v, w]
+        attempts = 0
+        while True:
+            attempts += 1
+            z = torch.randn((1, 3 * S.Nblobs), dtype=torch.float64, device=dev, generator=gen)
+            noise, its = S.brownian_slip(z, factor=factor, tolerance=lanczos_tol, precision=lanczos_precision)
+            rhs_n = torch.zeros_like(rhs)
+            rhs_n[: 3 * S.Nblobs] = -noise[0]
+            x0_n = torch.empty_like(rhs)
+            S.pc_apply(rhs_n, x0_n)
+            res_n = solve(rhs_n, x0_n)
+            U = U_det + S.velocities(res_n.x)
+            c_new = centres + U[:, :3] * dt
+            q_new = quat_mul(quat_from_rotation(U[:, 3:] * dt), quats)
             q_new /= np.linalg.norm(q_new, axis=1, keepdims=True)
             blobs_z = (np.einsum("bij,kj->bki", rotation_matrices(q_new), ref)[..., 2] + c_new[:, None, 2])
-            if blobs_z.min() > 0.0:       # check_positions, qi_multi_bodies.py:1779
+            if blobs_z.min() >= 0.0 or attempts >= 20:   # check_positions, qi_multi_bodies.py:1779
                 break
+        if log:
+            log(f"step {step}: GMRES iterations {res.iterations} (noise part {res_n.iterations}), Lanczos {its}, "
+                f"{ts * 1e3:.1f} ms, {attempts} attempt(s)")
         centres, quats = c_new, q_new
     return traj

@@ -53,6 +53,23 @@ def _as_device_f64(x, n: int, k: Optional[int], device, name: str) -> torch.Tens
     return t
 
 
+def _check_out(t: torch.Tensor, count: int, device, name: str) -> torch.Tensor:
+    """A caller-supplied output buffer must be a contiguous float64 tensor on
+    the launch device with exactly `count` entries: the kernels write
+    through its raw pointer."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name} must be float64, got {t.dtype}")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, the product runs on {device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() != count:
+        raise ValueError(f"{name} must have {count} entries, got {t.numel()}")
+    return t
+
+
 def _L_array(periodic_length) -> ctypes.Array:
     L = np.zeros(3) if periodic_length is None else np.asarray(periodic_length, dtype=np.float64).reshape(-1)
     if L.size != 3:
@@ -85,7 +102,7 @@ DOMAINS = {"single_wall": 0, "no_wall": 1, "in_plane": 2, "free_surface": 3}
 def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: float = 1.0,
           periodic_length=None, precision="fp32", k: int = 1, tuning=None,
           out_velocity: Optional[torch.Tensor] = None, out_omega: Optional[torch.Tensor] = None,
-          domain: str = "single_wall"):
+          domain: str = "single_wall", check_finite: bool = False):
     """Evaluate the blocks of ``mask`` on one suspension.
 
     r_vectors: (N, 3); force / torque: (k, N, 3) (or (N, 3) for k = 1).
@@ -93,6 +110,9 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     (squeezed to (N, 3) when k == 1), None for outputs the mask does not write.
     domain: 'single_wall' (default), 'no_wall' (all blocks), 'in_plane'
     (UF, UT) or 'free_surface' (UF).
+    check_finite: scan the outputs after the product (the call then waits for
+    the stream) and raise RmwError with code RMW_E_NOTFINITE if an entry is
+    inf or NaN.
     """
     if domain not in DOMAINS:
         raise ValueError(f"unknown domain {domain!r}; use one of {sorted(DOMAINS)}")
@@ -105,17 +125,24 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     has_w = bool(mask & (RMW_WF | RMW_WT))
     f = _as_device_f64(force, n, k, device, "force") if mask & (RMW_UF | RMW_WF) else None
     t = _as_device_f64(torque, n, k, device, "torque") if mask & (RMW_UT | RMW_WT) else None
-    u = out_velocity if out_velocity is not None else (
-        torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_u else None)
-    w = out_omega if out_omega is not None else (
-        torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None)
+    if out_velocity is not None and has_u:
+        u = _check_out(out_velocity, 3 * n * k, device, "out_velocity")
+    else:
+        u = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_u else None
+    if out_omega is not None and has_w:
+        w = _check_out(out_omega, 3 * n * k, device, "out_omega")
+    else:
+        w = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None
     prec = _lib.precision_code(precision)
     dom = DOMAINS[domain]
     if dom == 0:
-        tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
+        tun = _tuning_for(tuning, mask, precision, k, float(n) * n, False)
     else:
-        # the other domains run the ordered kernel; the cache holds wall shapes
-        tp = _lib.tuning_ptr(None if tuning is None or isinstance(tuning, str) else tuning)
+        # the cache holds wall shapes
+        tun = None if tuning is None or isinstance(tuning, str) else tuning
+    if check_finite:
+        tun = _lib.with_flags(tun, _lib.RMW_FLAG_CHECK_FINITE)
+    tp = _lib.tuning_ptr(tun)
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_mobility_workspace_size(dom, mask, prec, n, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device)
@@ -130,18 +157,40 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     return u, w
 
 
-def apply_strided(mask: int, r: torch.Tensor, n: int, k: int, ld: int, force_ptr, torque_ptr, vel_ptr, omg_ptr,
-                  eta: float, a: float, periodic_length=None, precision="fp32", tuning=None):
-    """Low-level product on raw device pointers with k right-hand sides `ld`
-    doubles apart (e.g. force and torque halves of stacked 6N vectors)."""
+def _grand_rows(X: torch.Tensor, Y: torch.Tensor, n: int, device):
+    """Pointers of the [force; torque] halves of the k rows of stacked grand
+    vectors X, Y (k, >= 6n), after checking that every row the kernels read
+    or write lies inside the tensors."""
+    for name, t in (("X", X), ("Y", Y)):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or t.device != device:
+            raise ValueError(f"{name} must be a float64 tensor on {device}")
+        if t.dim() != 2 or t.stride(1) != 1:
+            raise ValueError(f"{name} must be (k, ld) with unit column stride")
+    if X.shape[0] != Y.shape[0] or X.stride(0) != Y.stride(0):
+        raise ValueError("X and Y must have the same rows and row stride")
+    k, ld = X.shape[0], X.stride(0)
+    if min(X.shape[1], Y.shape[1]) < 6 * n or (k > 1 and ld < 6 * n):
+        raise ValueError(f"rows of X and Y must hold 6N = {6 * n} values")
+    return k, ld, X.data_ptr(), X.data_ptr() + 24 * n, Y.data_ptr(), Y.data_ptr() + 24 * n
+
+
+def apply_strided(mask: int, r: torch.Tensor, n: int, X: torch.Tensor, Y: torch.Tensor, eta: float, a: float,
+                  periodic_length=None, precision="fp32", tuning=None):
+    """Y[r] = [[UF, UT], [WF, WT]] X[r] (blocks of ``mask``) for the k rows of
+    stacked grand vectors X, Y (k, 6N): forces then torques in each row, rows
+    X.stride(0) doubles apart.  One launch per pass serves every row."""
     lib = _lib.load()
     device = _device()
+    if not isinstance(r, torch.Tensor) or r.dtype != torch.float64 or r.device != device or \
+            not r.is_contiguous() or r.numel() != 3 * n:
+        raise ValueError(f"r must be a contiguous float64 (N, 3) tensor on {device} with N = {n}")
+    k, ld, fp, tp_, vp, wp = _grand_rows(X, Y, n, device)
     prec = _lib.precision_code(precision)
     tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device)
-    _lib.check(lib.rmw_wall_rpy_apply(mask, prec, r.data_ptr(), force_ptr, torque_ptr, vel_ptr, omg_ptr, n, k, ld,
+    _lib.check(lib.rmw_wall_rpy_apply(mask, prec, r.data_ptr(), fp, tp_, vp, wp, n, k, ld,
                                       float(eta), float(a), _L_array(periodic_length), tp, ws.data_ptr(),
                                       ws.numel(), _stream_ptr(device)))
 
@@ -180,14 +229,19 @@ def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torqu
     return u, w
 
 
-def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int, ld: int, force_ptr, torque_ptr,
-                         vel_ptr, omg_ptr, eta: float, a: float, periodic_length=None, precision="fp32_blocked",
-                         tuning=None):
-    """Ragged product on raw device pointers, k right-hand sides `ld` doubles
-    apart (force and torque parts of stacked grand vectors of a batch)."""
+def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, X: torch.Tensor, Y: torch.Tensor,
+                         eta: float, a: float, periodic_length=None, precision="fp32_blocked", tuning=None):
+    """Ragged product on stacked grand vectors of a batch: X, Y (k, 6N) hold
+    the forces of all blobs, then their torques; system s owns blobs
+    [offsets[s], offsets[s+1])."""
     lib = _lib.load()
     device = _device()
     off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+    n = int(off[-1])
+    if not isinstance(r, torch.Tensor) or r.dtype != torch.float64 or r.device != device or \
+            not r.is_contiguous() or r.numel() != 3 * n:
+        raise ValueError(f"r must be a contiguous float64 (N, 3) tensor on {device} with N = offsets[-1] = {n}")
+    k, ld, fp, tp_, vp, wp = _grand_rows(X, Y, n, device)
     prec = _lib.precision_code(precision)
     pairs = float(np.sum(np.diff(off).astype(np.float64) ** 2))
     tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True))
@@ -195,7 +249,7 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, k: int
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, off.size - 1, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device, slot="ragged")
-    _lib.check(lib.rmw_wall_rpy_ragged_apply(mask, prec, r.data_ptr(), force_ptr, torque_ptr, vel_ptr, omg_ptr, offp,
+    _lib.check(lib.rmw_wall_rpy_ragged_apply(mask, prec, r.data_ptr(), fp, tp_, vp, wp, offp,
                                              off.size - 1, k, ld, float(eta), float(a), _L_array(periodic_length),
                                              tp, ws.data_ptr(), ws.numel(), _stream_ptr(device)))
 
@@ -234,7 +288,8 @@ def apply_radii(source, target, force, radius_source, radius_target, eta: float,
     rt = torch.as_tensor(radius_target).to(device=device, dtype=torch.float64).contiguous().reshape(-1)
     if rs.numel() != ns or rt.numel() != nt:
         raise ValueError("one radius per source and per target blob is required")
-    u = out if out is not None else torch.empty((nt, 3), dtype=torch.float64, device=device)
+    u = _check_out(out, 3 * nt, device, "out") if out is not None else \
+        torch.empty((nt, 3), dtype=torch.float64, device=device)
     prec = _lib.precision_code(precision)
     tp = _lib.tuning_ptr(None if isinstance(tuning, str) else tuning)
     nbytes = ctypes.c_size_t()
@@ -263,7 +318,8 @@ def apply_radii_self(r_vectors, force, radius, eta: float, periodic_length=None,
     rad = torch.as_tensor(radius).to(device=device, dtype=torch.float64).contiguous().reshape(-1)
     if rad.numel() != n:
         raise ValueError("one radius per blob is required")
-    u = out if out is not None else torch.empty((n, 3), dtype=torch.float64, device=device)
+    u = _check_out(out, 3 * n, device, "out") if out is not None else \
+        torch.empty((n, 3), dtype=torch.float64, device=device)
     prec = _lib.precision_code(precision)
     tp = _lib.tuning_ptr(None if isinstance(tuning, str) else tuning)
     L = _L_array(periodic_length)

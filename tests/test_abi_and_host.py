@@ -177,3 +177,37 @@ def test_workload_shapes_and_ranges():
     c5 = W.c5_ragged_rollers()
     sizes = np.diff(c5["offsets"])
     assert sizes.size == 64 and sizes.min() >= 128 and sizes.max() <= 4096
+
+
+def test_output_buffer_check_host():
+    """The host-side check of caller-supplied output buffers (size, dtype,
+    device, layout) refuses a wrong buffer before any launch."""
+    import torch
+    from rigidmultiblobswall_amd.wall_rpy import _check_out
+    cpu = torch.device("cpu")
+    ok = torch.empty((1, 10, 3), dtype=torch.float64)
+    assert _check_out(ok, 30, cpu, "out") is ok
+    for bad, count in ((torch.empty((1, 9, 3), dtype=torch.float64), 30),
+                       (torch.empty((1, 10, 3), dtype=torch.float32), 30),
+                       (torch.empty((1, 3, 10), dtype=torch.float64).transpose(1, 2), 30),
+                       (np.empty((10, 3)), 30)):
+        with pytest.raises(ValueError):
+            _check_out(bad, count, cpu, "out")
+
+
+def test_tuning_flags_layout_and_merge():
+    """rmw_tuning's flags field sits where include/rmw_hip.h puts it, and
+    the finite-check flag is merged into any launch shape a call resolves
+    (none, a dict, a cached Tuning) without changing the shape."""
+    import ctypes
+    from rigidmultiblobswall_amd import _lib
+    assert ctypes.sizeof(_lib.Tuning) == 24 and _lib.Tuning.flags.offset == 20
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "rmw_hip.h")).read()
+    assert re.search(r"#define RMW_FLAG_CHECK_FINITE 1\b", hdr) and re.search(r"#define RMW_E_NOTFINITE \(-4\)", hdr)
+    t = _lib.with_flags(None, _lib.RMW_FLAG_CHECK_FINITE)
+    assert (t.chunk_len, t.max_wgs, t.rhs_group, t.symmetric, t.flags) == (0, 0, 0, 0, 1)
+    t = _lib.with_flags({"chunk_len": 128, "symmetric": 1}, 1)
+    assert (t.chunk_len, t.symmetric, t.flags) == (128, 1, 1)
+    base = _lib.Tuning(704, 0, 4, 1, 0)
+    t = _lib.with_flags(base, 1)
+    assert (t.chunk_len, t.rhs_group, t.symmetric, t.flags) == (704, 4, 1, 1) and base.flags == 0

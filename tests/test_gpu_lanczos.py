@@ -77,3 +77,25 @@ def test_c3_rollers_block_lanczos_invariants(gpu):
     # a single right-hand side reproduces its row of the block
     one, _ = R.brownian_noise(Z[:1].contiguous(), factor=1.0, tolerance=1e-3)
     assert rel(one[0].cpu().numpy(), noise[0].cpu().numpy()) < 2e-3
+
+
+def test_unconverged_lanczos_is_reported(gpu):
+    """max_iter steps that do not reach the tolerance: the last noise is
+    returned with iterations = max_iter, as the reference does, plus a
+    LanczosNotConverged warning; strict=True raises instead."""
+    rng = np.random.default_rng(12)
+    n, a = 20, 0.3
+    r = np.column_stack([rng.uniform(0, 2, n), rng.uniform(0, 2, n), rng.uniform(0.4, 1.5, n)])
+    M = torch.from_numpy(O.single_wall_fluid_mobility(r, 1.0, a)).to(gpu)
+    Z = torch.from_numpy(rng.standard_normal((2, 3 * n))).to(gpu)
+    mv = lambda X, Y: Y.copy_(X @ M)   # noqa: E731  (M symmetric)
+    print("case unconverged Lanczos, max_iter 2, tol 1e-14", flush=True)
+    with pytest.warns(lanczos.LanczosNotConverged):
+        noise, it = lanczos.lanczos_block(mv, Z, tolerance=1e-14, max_iter=2)
+    assert it == 2 and bool(torch.isfinite(noise).all())
+    with pytest.raises(RuntimeError):
+        lanczos.lanczos_block(mv, Z, tolerance=1e-14, max_iter=2, strict=True)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("error", lanczos.LanczosNotConverged)
+        lanczos.lanczos_block(mv, Z, tolerance=1e-6, max_iter=200)     # converges: no warning
