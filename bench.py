@@ -255,29 +255,48 @@ def gmres_leg(steps: int, epochs: int, evals: int, log):
     from rigidmultiblobswall_amd.trajectory import run_c4_trajectory
     w = workloads.c4_monolayer()
     t0 = time.perf_counter()
-    traj = run_c4_trajectory(w["centres"], w["quats"], w["ref"], w["a"], w["eta"], nsteps=steps)
+    traj = run_c4_trajectory(w["centres"], w["quats"], w["ref"], w["a"], w["eta"], nsteps=steps,
+                             log=lambda m: log(m) if m.startswith("step ") and int(m.split()[1][:-1]) % 25 == 0
+                             else None)
     t_traj = time.perf_counter() - t0
     log(f"gmres leg: trajectory {steps} steps in {t_traj:.1f} s")
     ntr = int(0.75 * steps)
     ws = warmstart.WarmStart(w["ref"], w["a"], w["eta"])
     fit = ws.fit(traj.snapshots[:ntr], epochs=epochs)
     log(f"gmres leg: trained {fit['steps']} Adam steps in {fit['train_seconds']:.1f} s, loss {fit['loss_history'][-1]:.3e}")
+    ws_prev = warmstart.WarmStart(w["ref"], w["a"], w["eta"], mode="previous")
+    fit_prev = ws_prev.fit(traj.snapshots[:ntr], epochs=epochs)
     held = traj.snapshots[ntr:]
+    heldout_prev = ws_prev.heldout_loss(held)
+    log(f"gmres leg: previous-step network trained {fit_prev['steps']} Adam steps in {fit_prev['train_seconds']:.1f} s, "
+        f"loss {fit_prev['loss_history'][-1]:.3e}, held-out {heldout_prev:.3e}")
     picks = np.linspace(0, len(held) - 1, min(evals, len(held))).astype(int)
-    rows = warmstart.evaluate(ws, [held[i] for i in picks], [traj.snapshots[ntr + i - 1] for i in picks])
+    rows = warmstart.evaluate(ws, [held[i] for i in picks], [traj.snapshots[ntr + i - 1] for i in picks],
+                              ws_prev=ws_prev)
     sm = warmstart.summarise(rows)
     pick = lambda k: {"ms": sm[k]["total_ms"], "iterations": sm[k]["iterations"],  # noqa: E731
                       "x0_rel_err": sm[k]["x0_rel_err"], "converged": sm[k]["all_converged"]}
-    out = {"workload": f"C4 monolayer: 1024 spheres x 42 blobs = 43008 blobs, tol 1e-8, {steps}-step trajectory, "
-                       f"trained on {ntr} steps, {len(picks)} held-out solves",
+    out = {"workload": f"C4 monolayer: 1024 spheres x 42 blobs = 43008 blobs, tol 1e-8, {steps}-step Brownian "
+                       f"trajectory (stochastic Euler-Maruyama, dt 1e-3), trained on {ntr} steps, "
+                       f"{len(picks)} held-out solves",
+           "trajectory_seconds": t_traj,
            "plain_fp64": pick("zero/fp64"), "learned_fp64": pick("gnn/fp64"), "learned_mixed": pick("gnn/mixed"),
            "zero_mixed": pick("zero/mixed"), "point_field_mixed": pick("point/mixed"),
-           "previous_step_mixed": pick("previous/mixed"),
+           "previous_step_fp64": pick("previous/fp64"), "previous_step_mixed": pick("previous/mixed"),
+           "learned_previous_fp64": pick("gnn_prev/fp64"), "learned_previous_mixed": pick("gnn_prev/mixed"),
            "point_field_ms": float(np.mean([r["point_field_ms"] for r in rows])),
            "gnn_inference_ms": float(np.mean([r["gnn_inference_ms"] for r in rows])),
-           "train_seconds": fit["train_seconds"], "parameters": fit["parameters"]}
+           "gnn_prev_inference_ms": float(np.mean([r["gnn_prev_inference_ms"] for r in rows])),
+           "train_seconds": fit["train_seconds"], "train_seconds_previous": fit_prev["train_seconds"],
+           "parameters": fit["parameters"], "parameters_previous": fit_prev["parameters"]}
+    # the learned guesses against the guesses without a network, in FP64
+    # GMRES alone (the network's own gain) and with the mixed solve
     out["speedup_learned_vs_plain"] = out["plain_fp64"]["ms"] / out["learned_mixed"]["ms"]
     out["speedup_learned_fp64_vs_plain"] = out["plain_fp64"]["ms"] / out["learned_fp64"]["ms"]
+    out["speedup_mixed_alone_vs_plain"] = out["plain_fp64"]["ms"] / out["zero_mixed"]["ms"]
+    out["speedup_learned_previous_fp64_vs_previous_fp64"] = (out["previous_step_fp64"]["ms"]
+                                                             / out["learned_previous_fp64"]["ms"])
+    out["speedup_learned_previous_mixed_vs_plain"] = out["plain_fp64"]["ms"] / out["learned_previous_mixed"]["ms"]
     return out
 
 

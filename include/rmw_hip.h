@@ -49,11 +49,14 @@ typedef struct rmw_tuning {
   int64_t chunk_len;  /* sources per work item (rounded up to a multiple of 64; 0 = auto) */
   int32_t max_wgs;    /* cap on resident workgroups of persistent launches (0 = auto) */
   int32_t rhs_group;  /* right-hand sides per pair-kernel pass: 1, 2 or 4 (0 = auto) */
-  int32_t symmetric;  /* UF alone, UF + UT (wall) or all four blocks, without periodic
-                         images: each unordered pair evaluated once for both blobs
-                         (UF_ji = UF_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, WT_ji = WT_ij^T).
-                         0 = when the tile pairs fill the device, 1 = required (error
-                         otherwise), -1 = off */
+  int32_t symmetric;  /* each unordered pair evaluated once for both blobs
+                         (UF_ji = UF_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, WT_ji = WT_ij^T):
+                         the UF block alone, UF + UT (wall) or all four blocks (wall,
+                         no wall); above the wall also with pseudo-periodic images, one
+                         right-hand side per pass; the no-wall, in-plane and
+                         free-surface domains without images.  0 = when the tile
+                         pairs fill the device, 1 = required (error otherwise),
+                         -1 = off */
   int32_t flags;      /* RMW_FLAG_* bits (0 = none) */
 } rmw_tuning;
 
@@ -121,7 +124,10 @@ int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double
  *                            r_j* = (x_j, y_j, -z_j), D = diag(1,1,-1), UF
  *                            only: free_surface_mobility_trans_times_force_
  *                            numba, :1770-1938 (wrapper mobility.py:1390-1406)
- * One right-hand side per pass (k > 1 runs k passes); ordered kernels only.
+ * One right-hand side per pass (k > 1 runs k passes).  The symmetric
+ * kernels serve the UF block of every domain and the grand mobility of the
+ * unbounded one (without periodic images); the other masks and periodic
+ * calls outside the wall domain run the ordered kernels.
  */
 #define RMW_DOMAIN_WALL 0
 #define RMW_DOMAIN_NO_WALL 1
@@ -343,6 +349,13 @@ int rmw_count_below(const double* r, int64_t n, double level, int64_t* count, vo
 int rmw_dense_wall_mobility(const double* r_blobs, const int64_t* body_offsets, int nbodies,
                             double eta, double a, double* out, void* workspace,
                             size_t workspace_bytes, void* stream);
+/* The same for domain RMW_DOMAIN_WALL (= rmw_dense_wall_mobility) or
+ * RMW_DOMAIN_NO_WALL: the unbounded RPY tensor of each body, no height
+ * shift or damping, replacing rotne_prager_tensor, mobility/mobility.py
+ * :967-1012 ('python_no_wall' / 'C++_no_wall' of set_mobility_blobs,
+ * multi_bodies/multi_bodies.py:218-221). */
+int rmw_dense_mobility(int domain, const double* r_blobs, const int64_t* body_offsets, int nbodies, double eta,
+                       double a, double* out, void* workspace, size_t workspace_bytes, void* stream);
 
 /* Block-diagonal body preconditioner build: per body M_b^-1 (via Cholesky,
  * minv: packed blocks as above, minv_doubles long), M_b^-1 K (minvk:
@@ -356,17 +369,21 @@ int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, const int64_t
                        double* nbody, int* status, double* scratch, size_t scratch_doubles,
                        void* workspace, size_t workspace_bytes, void* stream);
 
-/* The stochastic variant of the build: besides minv, minvk and nbody as
- * rmw_rigid_pc_build, per body the lower Cholesky factor L_b of M_b
+/* The general build: rmw_rigid_pc_build for domain RMW_DOMAIN_WALL or
+ * RMW_DOMAIN_NO_WALL (M_b the unbounded RPY tensor: rotne_prager_tensor,
+ * mobility/mobility.py:967-1012, as mobility_blobs_implementation
+ * 'python_no_wall' / 'C++_no_wall' selects it, multi_bodies.py:218-221),
+ * and with chol, chol_inv non-NULL the stochastic variant: besides minv,
+ * minvk and nbody, per body the lower Cholesky factor L_b of M_b
  * (M_b = L_b L_b^T; chol) and L_b^-1 (chol_inv), row-major blocks packed as
- * minv with zero upper triangles.  Replaces the factorisation loop of
- * build_block_diagonal_preconditioners_det_stoch, multi_bodies/
+ * minv with zero upper triangles.  That variant replaces the factorisation
+ * loop of build_block_diagonal_preconditioners_det_stoch, multi_bodies/
  * multi_bodies.py:480-560 (its M_factorization_blobs = L_b; its
  * M_factorization_blobs_inv = U_b^-1 = L_b^-T for scipy's M_b = U_b^T U_b). */
-int rmw_rigid_pc_build_det_stoch(const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
-                                 int nbodies, double eta, double a, double* minv, double* minvk, double* nbody,
-                                 double* chol, double* chol_inv, int* status, double* scratch,
-                                 size_t scratch_doubles, void* workspace, size_t workspace_bytes, void* stream);
+int rmw_rigid_pc_build_ex(int domain, const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
+                          int nbodies, double eta, double a, double* minv, double* minvk, double* nbody, double* chol,
+                          double* chol_inv, int* status, double* scratch, size_t scratch_doubles, void* workspace,
+                          size_t workspace_bytes, void* stream);
 /* y_b = B_b x_b (trans = 0) or B_b^T x_b (trans = 1) for every body b and
  * each of k vectors ld doubles apart (0 = 3N), B packed as minv; lower != 0
  * reads only the lower triangle of each block.  With chol / chol_inv these
@@ -514,6 +531,15 @@ int rmw_gnn_knn(const double* centres, int nb, int deg, int* nbr, void* stream);
 /* node (10) and edge (16 per neighbour) features in each body's frame */
 int rmw_gnn_features(const double* centres, const double* quats, const double* force_torque,
                      const int* nbr, int nb, int deg, double fscale, float* Xn, float* Xe, void* stream);
+/* node (22) and edge (16) features of the network that corrects the previous
+ * step's solution: those of rmw_gnn_features plus the body-frame motion
+ * since (centres0, quats0, force_torque0) -- displacement / dscale, rotation
+ * vector / rscale, force and torque changes / fscale -- with the edge's
+ * relative orientation replaced by the change of the neighbour offset */
+int rmw_gnn_motion_features(const double* centres, const double* quats, const double* force_torque,
+                            const double* centres0, const double* quats0, const double* force_torque0, const int* nbr,
+                            int nb, int deg, double fscale, double dscale, double rscale, float* Xn, float* Xe,
+                            void* stream);
 /* mode 0: T = standardised body-frame (sol - pc) / fscale; mode 1: sol = pc + R (T sd + mean) fscale */
 int rmw_gnn_body_frame(const double* quats, const int64_t* body_offsets_dev, int nb, int64_t nblobs,
                        int blobs_per_body, double* sol, const double* pc, float* T, const float* mean,

@@ -2,7 +2,7 @@
 
 Forward and analytic backward of the same architecture as the GPU
 implementation (edge MLP 16 -> 64 -> 64, sum over 8 edges, node MLP
-80 -> 128 -> 128 -> out, SiLU), in numpy float64.  The tests pin the
+(nin + 64, padded to 16) -> 128 -> 128 -> out, SiLU), in numpy float64.  The tests pin the
 analytic backward with central finite differences and compare the GPU's
 BF16 MFMA forward and gradients with it.  The GPU rounds operands to BF16;
 `bf16_round` lets a test apply the same rounding to inputs and weights.
@@ -31,19 +31,19 @@ def dsilu(z):
 
 def forward(P, Xn, Xe):
     """P: dict of float64 parameters (W [out][in]).  Returns (Y, cache)."""
-    B = Xn.shape[0]
+    B, nin = Xn.shape
     Ze1 = Xe @ P["We1"].T + P["be1"]
     He1 = silu(Ze1)
     Msg = He1 @ P["We2"].T + P["be2"]
-    Xc = np.zeros((B, NC))
-    Xc[:, :NIN] = Xn
-    Xc[:, NIN:NIN + NMSG] = Msg.reshape(B, DEG, NMSG).sum(axis=1)
+    Xc = np.zeros((B, P["Wn1"].shape[1]))     # node inputs, message, zero padding
+    Xc[:, :nin] = Xn
+    Xc[:, nin:nin + NMSG] = Msg.reshape(B, DEG, NMSG).sum(axis=1)
     Zn1 = Xc @ P["Wn1"].T + P["bn1"]
     Hn1 = silu(Zn1)
     Zn2 = Hn1 @ P["Wn2"].T + P["bn2"]
     Hn2 = silu(Zn2)
     Y = Hn2 @ P["Wn3"].T + P["bn3"]
-    return Y, dict(Xe=Xe, Ze1=Ze1, He1=He1, Xc=Xc, Zn1=Zn1, Hn1=Hn1, Zn2=Zn2, Hn2=Hn2)
+    return Y, dict(Xe=Xe, Ze1=Ze1, He1=He1, Xc=Xc, Zn1=Zn1, Hn1=Hn1, Zn2=Zn2, Hn2=Hn2, nin=nin)
 
 
 def loss(P, Xn, Xe, T):
@@ -62,11 +62,12 @@ def backward(P, cache, dY):
     g["bn2"] = dZ2.sum(0)
     dH1 = dZ2 @ P["Wn2"]
     dZ1 = dH1 * dsilu(cache["Zn1"])
+    nin = cache["nin"]
     g["Wn1"] = dZ1.T @ cache["Xc"]
-    g["Wn1"][:, NIN + NMSG:] = 0.0
+    g["Wn1"][:, nin + NMSG:] = 0.0
     g["bn1"] = dZ1.sum(0)
     dXc = dZ1 @ P["Wn1"]
-    dM = np.repeat(dXc[:, NIN:NIN + NMSG], DEG, axis=0)
+    dM = np.repeat(dXc[:, nin:nin + NMSG], DEG, axis=0)
     g["We2"] = dM.T @ cache["He1"]
     g["be2"] = dM.sum(0)
     dHe = dM @ P["We2"]

@@ -16,8 +16,7 @@ LIB_PATH = os.path.join(HERE, "librmw_hip.so")
 RMW_OK = 0
 RMW_UF, RMW_UT, RMW_WF, RMW_WT = 1, 2, 4, 8
 RMW_PREC_F32_F64ACC, RMW_PREC_F32_KAHAN, RMW_PREC_F64, RMW_PREC_F32_BLOCKED = 0, 1, 2, 3
-RMW_E_NOTFINITE = -4
-RMW_E_LIMIT = -5
+RMW_E_ARG, RMW_E_HIP, RMW_E_WORKSPACE, RMW_E_NOTFINITE, RMW_E_LIMIT = -1, -2, -3, -4, -5
 RMW_FLAG_CHECK_FINITE = 1
 
 PRECISIONS = {
@@ -122,9 +121,11 @@ PROTOTYPES = {
                                                _vp, _sz, _vp]),
     "rmw_rigid_pc_build": (ctypes.c_int, [_dp, _dp, _i64p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp,
                                           _dp, _dp, _vp, _dp, _sz, _vp, _sz, _vp]),
-    "rmw_rigid_pc_build_det_stoch": (ctypes.c_int, [_dp, _dp, _i64p, ctypes.c_int, ctypes.c_double,
-                                                    ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _vp, _dp, _sz, _vp,
-                                                    _sz, _vp]),
+    "rmw_rigid_pc_build_ex": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _i64p, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _vp, _dp, _sz, _vp, _sz,
+                                             _vp]),
+    "rmw_dense_mobility": (ctypes.c_int, [ctypes.c_int, _dp, _i64p, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, _dp, _vp, _sz, _vp]),
     "rmw_rigid_block_apply": (ctypes.c_int, [_i64p, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                              ctypes.c_int, _i64, _vp, _sz, _vp]),
     "rmw_rigid_point_field": (ctypes.c_int, [_dp, _dp, _dp, _i64, _i64p, ctypes.c_int, ctypes.c_double,
@@ -162,6 +163,8 @@ PROTOTYPES = {
     "rmw_gnn_knn": (ctypes.c_int, [_dp, ctypes.c_int, ctypes.c_int, _dp, _vp]),
     "rmw_gnn_features": (ctypes.c_int, [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                         _vp]),
+    "rmw_gnn_motion_features": (ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _vp]),
     "rmw_gnn_body_frame": (ctypes.c_int, [_dp, _dp, ctypes.c_int, _i64, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                           ctypes.c_double, ctypes.c_int, _vp]),
     # Lanczos

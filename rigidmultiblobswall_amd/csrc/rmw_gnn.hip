@@ -407,6 +407,88 @@ __global__ void features_kernel(const double* __restrict__ c, const double* __re
   }
 }
 
+// Features of the motion since a previous configuration (c0, q0, ft0), for
+// the network that corrects the previous step's solution.
+// Node features (22): the 10 of features_kernel, then in the body frame
+//   the displacement R_b^T (c_b - c0_b) / dscale (3), the rotation vector of
+//   q_b q0_b^-1 / rscale (3), and the changes of force and torque / fscale (6).
+// Edge features (16): those of features_kernel with entries 5..8 (the
+//   relative orientation) replaced by the change of the offset to the
+//   neighbour since the previous configuration, R_b^T (d - d0) / dscale (3),
+//   and the change of its length / dscale (1).
+__global__ void motion_features_kernel(const double* __restrict__ c, const double* __restrict__ q,
+                                       const double* __restrict__ ft, const double* __restrict__ c0,
+                                       const double* __restrict__ q0, const double* __restrict__ ft0,
+                                       const int* __restrict__ nbr, int nb, int deg, double fscale, double dscale,
+                                       double rscale, float* __restrict__ Xn, float* __restrict__ Xe)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double R[9];
+  quat_to_rot(q + 4 * b, R);
+  auto tobody = [&](const double* v, double* o) {
+    o[0] = R[0] * v[0] + R[3] * v[1] + R[6] * v[2];
+    o[1] = R[1] * v[0] + R[4] * v[1] + R[7] * v[2];
+    o[2] = R[2] * v[0] + R[5] * v[1] + R[8] * v[2];
+  };
+  const double zhat[3] = {0.0, 0.0, 1.0};
+  double t3[3];
+  float* xn = Xn + (int64_t)b * 22;
+  tobody(zhat, t3);
+  xn[0] = (float)t3[0]; xn[1] = (float)t3[1]; xn[2] = (float)t3[2];
+  xn[3] = (float)c[3 * b + 2];
+  tobody(ft + 6 * b, t3);
+  xn[4] = (float)(t3[0] / fscale); xn[5] = (float)(t3[1] / fscale); xn[6] = (float)(t3[2] / fscale);
+  tobody(ft + 6 * b + 3, t3);
+  xn[7] = (float)(t3[0] / fscale); xn[8] = (float)(t3[1] / fscale); xn[9] = (float)(t3[2] / fscale);
+  const double dc[3] = {c[3 * b] - c0[3 * b], c[3 * b + 1] - c0[3 * b + 1], c[3 * b + 2] - c0[3 * b + 2]};
+  tobody(dc, t3);
+  xn[10] = (float)(t3[0] / dscale); xn[11] = (float)(t3[1] / dscale); xn[12] = (float)(t3[2] / dscale);
+  // q q0^-1 = (s, p) * (s0, -p0): the rotation that carried q0 to q (lab frame)
+  {
+    const double s1 = q[4 * b], x1 = q[4 * b + 1], y1 = q[4 * b + 2], z1 = q[4 * b + 3];
+    const double s2 = q0[4 * b], x2 = -q0[4 * b + 1], y2 = -q0[4 * b + 2], z2 = -q0[4 * b + 3];
+    double rs = s1 * s2 - x1 * x2 - y1 * y2 - z1 * z2;
+    double rx = s1 * x2 + s2 * x1 + y1 * z2 - z1 * y2;
+    double ry = s1 * y2 + s2 * y1 + z1 * x2 - x1 * z2;
+    double rz = s1 * z2 + s2 * z1 + x1 * y2 - y1 * x2;
+    if (rs < 0) { rs = -rs; rx = -rx; ry = -ry; rz = -rz; }
+    const double pn = sqrt(rx * rx + ry * ry + rz * rz);
+    const double k = pn > 0.0 ? 2.0 * atan2(pn, rs) / pn : 2.0;
+    const double th[3] = {k * rx, k * ry, k * rz};
+    tobody(th, t3);
+    xn[13] = (float)(t3[0] / rscale); xn[14] = (float)(t3[1] / rscale); xn[15] = (float)(t3[2] / rscale);
+  }
+  const double dF[3] = {ft[6 * b] - ft0[6 * b], ft[6 * b + 1] - ft0[6 * b + 1], ft[6 * b + 2] - ft0[6 * b + 2]};
+  const double dT[3] = {ft[6 * b + 3] - ft0[6 * b + 3], ft[6 * b + 4] - ft0[6 * b + 4],
+                        ft[6 * b + 5] - ft0[6 * b + 5]};
+  tobody(dF, t3);
+  xn[16] = (float)(t3[0] / fscale); xn[17] = (float)(t3[1] / fscale); xn[18] = (float)(t3[2] / fscale);
+  tobody(dT, t3);
+  xn[19] = (float)(t3[0] / fscale); xn[20] = (float)(t3[1] / fscale); xn[21] = (float)(t3[2] / fscale);
+  for (int e = 0; e < deg; ++e) {
+    const int n = nbr[b * deg + e];
+    float* xe = Xe + ((int64_t)b * deg + e) * 16;
+    const double dl[3] = {c[3 * n] - c[3 * b], c[3 * n + 1] - c[3 * b + 1], c[3 * n + 2] - c[3 * b + 2]};
+    const double d0[3] = {c0[3 * n] - c0[3 * b], c0[3 * n + 1] - c0[3 * b + 1], c0[3 * n + 2] - c0[3 * b + 2]};
+    tobody(dl, t3);
+    const double dn = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+    const double dn0 = sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+    xe[0] = (float)t3[0]; xe[1] = (float)t3[1]; xe[2] = (float)t3[2];
+    xe[3] = (float)dn;
+    xe[4] = (float)(dn > 0 ? 1.0 / dn : 0.0);
+    const double dd[3] = {dl[0] - d0[0], dl[1] - d0[1], dl[2] - d0[2]};
+    tobody(dd, t3);
+    xe[5] = (float)(t3[0] / dscale); xe[6] = (float)(t3[1] / dscale); xe[7] = (float)(t3[2] / dscale);
+    xe[8] = (float)((dn - dn0) / dscale);
+    xe[9] = (float)c[3 * n + 2];
+    tobody(ft + 6 * n, t3);
+    xe[10] = (float)(t3[0] / fscale); xe[11] = (float)(t3[1] / fscale); xe[12] = (float)(t3[2] / fscale);
+    tobody(ft + 6 * n + 3, t3);
+    xe[13] = (float)(t3[0] / fscale); xe[14] = (float)(t3[1] / fscale); xe[15] = (float)(t3[2] / fscale);
+  }
+}
+
 // Body-frame targets from a solution: T[b] = (R_b^T (lambda_b - lpc_b) per blob,
 // R_b^T (U_b - Upc_b)) / fscale, standardised by (mean, inv_std) per column.
 // mode 0: solution -> target; mode 1: prediction -> x0 = pc + R_b (pred * std + mean) * fscale
@@ -589,6 +671,22 @@ extern "C" int rmw_gnn_features(const double* centres, const double* quats, cons
     RMW_REQUIRE(fscale > 0, "fscale must be positive");
     hipLaunchKernelGGL(features_kernel, dim3((nb + 127) / 128), dim3(128), 0, (hipStream_t)stream, centres, quats,
                        force_torque, nbr, nb, deg, fscale, Xn, Xe);
+    RMW_CHECK_HIP(hipGetLastError());
+  });
+}
+
+extern "C" int rmw_gnn_motion_features(const double* centres, const double* quats, const double* force_torque,
+                                       const double* centres0, const double* quats0, const double* force_torque0,
+                                       const int* nbr, int nb, int deg, double fscale, double dscale, double rscale,
+                                       float* Xn, float* Xe, void* stream)
+{
+  return guarded([&] {
+    RMW_REQUIRE(fscale > 0 && dscale > 0 && rscale > 0, "fscale, dscale and rscale must be positive");
+    RMW_REQUIRE(centres && quats && force_torque && centres0 && quats0 && force_torque0 && nbr && Xn && Xe,
+                "null pointer");
+    hipLaunchKernelGGL(motion_features_kernel, dim3((nb + 127) / 128), dim3(128), 0, (hipStream_t)stream, centres,
+                       quats, force_torque, centres0, quats0, force_torque0, nbr, nb, deg, fscale, dscale, rscale, Xn,
+                       Xe);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

@@ -109,6 +109,30 @@ RMW_HD Geom<T> make_geom(T dx, T dy, T dz, T zi, T zj)
   return g;
 }
 
+// make_geom of a pair that may not be taken (on = false): then 1/|d| =
+// 1/|R| = 0 and |d| = 3 (the far-field branches), which makes every
+// coefficient of every block and boundary exactly zero, so the pair adds +-0
+// to both of its blobs without a select per output.  d and the heights must
+// be finite; a taken pair gets the same values as make_geom.
+template <typename T>
+RMW_HD Geom<T> make_geom_on(T dx, T dy, T dz, T zi, T zj, bool on)
+{
+  Geom<T> g;
+  g.dx = dx; g.dy = dy; g.dz = dz;
+  g.Rz = zi + zj;
+  T p2 = dx * dx + dy * dy;
+  g.r2 = p2 + dz * dz;
+  const T ir = rsq(g.r2);
+  g.invr = on ? ir : T(0);
+  g.r = on ? g.r2 * ir : T(3);
+  T R2 = p2 + g.Rz * g.Rz;
+  const T iR = rsq(R2);
+  g.invR = on ? iR : T(0);
+  g.ez = g.Rz * g.invR;
+  g.ez2 = g.ez * g.ez;
+  return g;
+}
+
 // ---------------------------------------------------------------------------
 // UF: translation from force.   mobility_numba.py:209-276
 // ---------------------------------------------------------------------------

@@ -206,24 +206,28 @@ __global__ __launch_bounds__(256) void count_sum_kernel(const int64_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Dense wall mobility entries (FP64), B M(z_eff) B, one 3x3 block.
+// Dense blob mobility entries (FP64), one 3x3 block: above the wall
+// B M(z_eff) B (single_wall_fluid_mobility, mobility.py:1018-1116), or the
+// unbounded RPY tensor (rotne_prager_tensor, mobility.py:967-1012) for
+// dom = kDomNoWall.
 // ---------------------------------------------------------------------------
-__device__ inline void wall_uf_block(const double* ri, const double* rj, bool self, double a, double inva,
-                                     double norm, double m[9])
+__device__ inline void uf_block(int dom, const double* ri, const double* rj, bool self, double a, double inva,
+                                double norm, double m[9])
 {
+  const bool wall = dom == kDomWall;
   const double zi_raw = ri[2], zj_raw = rj[2];
-  const double zie = zi_raw <= a ? a : zi_raw, zje = zj_raw <= a ? a : zj_raw;
-  const double Bi = zi_raw < a ? zi_raw * inva : 1.0, Bj = zj_raw < a ? zj_raw * inva : 1.0;
+  const double zie = wall && zi_raw <= a ? a : zi_raw, zje = wall && zj_raw <= a ? a : zj_raw;
+  const double Bi = wall && zi_raw < a ? zi_raw * inva : 1.0, Bj = wall && zj_raw < a ? zj_raw * inva : 1.0;
   const double zi = zie * inva, zj = zje * inva;
   for (int c = 0; c < 3; ++c) {
     double v[3] = {0.0, 0.0, 0.0};
     v[c] = 1.0;
     Vec3<double> u;
     if (self) {
-      u = self_uf(zi, v[0], v[1], v[2]);
+      u = self_uf_dom(dom, zi, v[0], v[1], v[2]);
     } else {
       Geom<double> g = make_geom((ri[0] - rj[0]) * inva, (ri[1] - rj[1]) * inva, (zie - zje) * inva, zi, zj);
-      u = apply_uf(g, coef_uf(g, zi, zj), v[0], v[1], v[2]);
+      u = apply_uf(g, coef_uf(g, zi, zj, dom), v[0], v[1], v[2]);
     }
     const double s = norm * Bi * Bj;
     m[0 * 3 + c] = s * u.x;
@@ -234,7 +238,7 @@ __device__ inline void wall_uf_block(const double* ri, const double* rj, bool se
 
 // Dense 3N x 3N wall mobility of each body (or of one system = one body),
 // row-major at out + moff[b].
-__global__ void dense_mobility_kernel(const double* __restrict__ r, const int64_t* __restrict__ offs,
+__global__ void dense_mobility_kernel(int dom, const double* __restrict__ r, const int64_t* __restrict__ offs,
                                       const int64_t* __restrict__ moff, double a, double eta,
                                       double* __restrict__ out)
 {
@@ -247,7 +251,7 @@ __global__ void dense_mobility_kernel(const double* __restrict__ r, const int64_
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = idx / nb, j = idx - (idx / nb) * nb;
     double m[9];
-    wall_uf_block(r + 3 * (i0 + i), r + 3 * (i0 + j), i == j, a, inva, norm, m);
+    uf_block(dom, r + 3 * (i0 + i), r + 3 * (i0 + j), i == j, a, inva, norm, m);
     for (int p = 0; p < 3; ++p)
       for (int q = 0; q < 3; ++q) M[(3 * i + p) * n + 3 * j + q] = m[p * 3 + q];
   }
@@ -324,6 +328,7 @@ struct PcBuildParams {
   int use_lds;
   double* chol;           // out (optional): lower Cholesky factor L of M_b (M_b = L L^T), packed as minv
   double* chol_inv;       // out (optional): L^-1 (lower), packed as minv
+  int dom;                // kDomWall or kDomNoWall: the blob mobility M_b is built for
 };
 
 template <bool LDS>
@@ -348,7 +353,7 @@ __global__ __launch_bounds__(kRigidBlock) void pc_build_kernel(PcBuildParams p)
   for (int64_t idx = tid; idx < nb * nb; idx += nt) {
     const int64_t i = idx / nb, j = idx - (idx / nb) * nb;
     double m[9];
-    wall_uf_block(p.r + 3 * (i0 + i), p.r + 3 * (i0 + j), i == j, p.a, inva, norm, m);
+    uf_block(p.dom, p.r + 3 * (i0 + i), p.r + 3 * (i0 + j), i == j, p.a, inva, norm, m);
     for (int q = 0; q < 3; ++q)
       for (int s = 0; s < 3; ++s) A[(3 * i + q) * n + 3 * j + s] = m[q * 3 + s];
   }
@@ -490,7 +495,7 @@ __global__ __launch_bounds__(kRigidBlock) void pc_build_gj_kernel(PcBuildParams 
   for (int64_t idx = tid; idx < nb * nb; idx += nt) {
     const int64_t i = idx / nb, j = idx - (idx / nb) * nb;
     double m[9];
-    wall_uf_block(p.r + 3 * (i0 + i), p.r + 3 * (i0 + j), i == j, p.a, inva, norm, m);
+    uf_block(p.dom, p.r + 3 * (i0 + i), p.r + 3 * (i0 + j), i == j, p.a, inva, norm, m);
     for (int q = 0; q < 3; ++q)
       for (int s = 0; s < 3; ++s) A[(3 * i + q) * n + 3 * j + s] = m[q * 3 + s];
   }
@@ -993,11 +998,26 @@ extern "C" int rmw_count_below(const double* r, int64_t n, double level, int64_t
   });
 }
 
+static void check_rigid_domain(int domain)
+{
+  RMW_REQUIRE(domain == RMW_DOMAIN_WALL || domain == RMW_DOMAIN_NO_WALL,
+              "rigid bodies: domain must be RMW_DOMAIN_WALL or RMW_DOMAIN_NO_WALL");
+}
+
 extern "C" int rmw_dense_wall_mobility(const double* r_blobs, const int64_t* body_offsets, int nbodies,
                                        double eta, double a, double* out, void* workspace,
                                        size_t workspace_bytes, void* stream)
 {
+  return rmw_dense_mobility(RMW_DOMAIN_WALL, r_blobs, body_offsets, nbodies, eta, a, out, workspace,
+                            workspace_bytes, stream);
+}
+
+extern "C" int rmw_dense_mobility(int domain, const double* r_blobs, const int64_t* body_offsets, int nbodies,
+                                  double eta, double a, double* out, void* workspace, size_t workspace_bytes,
+                                  void* stream)
+{
   return guarded([&] {
+    check_rigid_domain(domain);
     RigidLayout L = rigid_layout(body_offsets, nbodies);
     RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
     RMW_REQUIRE(eta > 0 && a > 0, "eta and a must be positive");
@@ -1009,7 +1029,8 @@ extern "C" int rmw_dense_wall_mobility(const double* r_blobs, const int64_t* bod
       maxpairs = std::max(maxpairs, nb * nb);
     }
     const int gx = (int)std::min<int64_t>((maxpairs + 255) / 256, 4096);
-    hipLaunchKernelGGL(dense_mobility_kernel, dim3(gx, nbodies), dim3(256), 0, st, r_blobs,
+    hipLaunchKernelGGL(dense_mobility_kernel, dim3(gx, nbodies), dim3(256), 0, st,
+                       domain == RMW_DOMAIN_WALL ? (int)kDomWall : (int)kDomNoWall, r_blobs,
                        (const int64_t*)(ws + L.off_offs), (const int64_t*)(ws + L.off_moff), a, eta, out);
     RMW_CHECK_HIP(hipGetLastError());
   });
@@ -1020,56 +1041,23 @@ extern "C" int rmw_rigid_pc_build(const double* r_blobs, const double* r_rel, co
                                   double* nbody, int* status, double* scratch, size_t scratch_doubles,
                                   void* workspace, size_t workspace_bytes, void* stream)
 {
+  return rmw_rigid_pc_build_ex(RMW_DOMAIN_WALL, r_blobs, r_rel, body_offsets, nbodies, eta, a, minv, minvk, nbody,
+                               nullptr, nullptr, status, scratch, scratch_doubles, workspace, workspace_bytes, stream);
+}
+
+extern "C" int rmw_rigid_pc_build_ex(int domain, const double* r_blobs, const double* r_rel,
+                                     const int64_t* body_offsets, int nbodies, double eta, double a, double* minv,
+                                     double* minvk, double* nbody, double* chol, double* chol_inv, int* status,
+                                     double* scratch, size_t scratch_doubles, void* workspace,
+                                     size_t workspace_bytes, void* stream)
+{
   return guarded([&] {
+    check_rigid_domain(domain);
     RigidLayout L = rigid_layout(body_offsets, nbodies);
     RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
     RMW_REQUIRE(eta > 0 && a > 0, "eta and a must be positive");
     RMW_REQUIRE(r_blobs && r_rel && minv && minvk && nbody, "null pointer");
-    hipStream_t st = (hipStream_t)stream;
-    char* ws = (char*)workspace;
-    PcBuildParams p{};
-    p.r = r_blobs; p.rrel = r_rel;
-    p.offs = (const int64_t*)(ws + L.off_offs);
-    p.moff = (const int64_t*)(ws + L.off_moff);
-    p.minv = minv; p.minvk = minvk; p.nbody = nbody;
-    RMW_REQUIRE(scratch_doubles >= L.scratch_doubles && (L.scratch_doubles == 0 || scratch != nullptr),
-                "factorisation scratch too small (bodies above 45 blobs need it)");
-    p.scratch = L.scratch_doubles ? scratch : nullptr;
-    p.scratch_vec = L.scratch_doubles ? scratch + L.minv_doubles : nullptr;
-    p.status = status ? status : (int*)(ws + L.off_status);
-    p.a = a; p.eta = eta;
-    int small = 0, big = 0;
-    int max_small = 0;
-    for (int b = 0; b < nbodies; ++b) {
-      int n = (int)(3 * (L.offs[b + 1] - L.offs[b]));
-      if (n <= kLdsMaxN) { ++small; max_small = std::max(max_small, n); } else { ++big; }
-    }
-    if (small) {
-      // Gauss-Jordan in registers, R x R elements per thread for the largest body
-      const int R = std::max(1, (max_small + 15) / 16);
-      const size_t lds = ((size_t)max_small * max_small + std::max<size_t>(6 * (size_t)max_small, 4 * 16 * (size_t)R)) *
-                         sizeof(double);
-      RMW_REQUIRE(lds <= 160 * 1024, "preconditioner build exceeds LDS");
-      launch_pc_gj(R, p, nbodies, lds, st);
-    }
-    if (big) {
-      hipLaunchKernelGGL(pc_build_kernel<false>, dim3(nbodies), dim3(kRigidBlock), 0, st, p);
-      RMW_CHECK_HIP(hipGetLastError());
-    }
-  });
-}
-
-extern "C" int rmw_rigid_pc_build_det_stoch(const double* r_blobs, const double* r_rel, const int64_t* body_offsets,
-                                            int nbodies, double eta, double a, double* minv, double* minvk,
-                                            double* nbody, double* chol, double* chol_inv, int* status,
-                                            double* scratch, size_t scratch_doubles, void* workspace,
-                                            size_t workspace_bytes, void* stream)
-{
-  return guarded([&] {
-    RigidLayout L = rigid_layout(body_offsets, nbodies);
-    RMW_REQUIRE(workspace_bytes >= L.total, "workspace too small");
-    RMW_REQUIRE(eta > 0 && a > 0, "eta and a must be positive");
-    RMW_REQUIRE(r_blobs && r_rel && minv && minvk && nbody && chol && chol_inv, "null pointer");
+    RMW_REQUIRE((chol == nullptr) == (chol_inv == nullptr), "chol and chol_inv go together");
     hipStream_t st = (hipStream_t)stream;
     char* ws = (char*)workspace;
     PcBuildParams p{};
@@ -1078,6 +1066,7 @@ extern "C" int rmw_rigid_pc_build_det_stoch(const double* r_blobs, const double*
     p.moff = (const int64_t*)(ws + L.off_moff);
     p.minv = minv; p.minvk = minvk; p.nbody = nbody;
     p.chol = chol; p.chol_inv = chol_inv;
+    p.dom = domain == RMW_DOMAIN_WALL ? (int)kDomWall : (int)kDomNoWall;
     RMW_REQUIRE(scratch_doubles >= L.scratch_doubles && (L.scratch_doubles == 0 || scratch != nullptr),
                 "factorisation scratch too small (bodies above 45 blobs need it)");
     p.scratch = L.scratch_doubles ? scratch : nullptr;
@@ -1090,8 +1079,16 @@ extern "C" int rmw_rigid_pc_build_det_stoch(const double* r_blobs, const double*
       const int n = (int)(3 * (L.offs[b + 1] - L.offs[b]));
       if (n <= kLdsMaxN) max_small = std::max(max_small, n); else big = true;
     }
-    if (max_small) {
-      // the factorisation in LDS: M_b (n^2), then n + 6 n of vectors
+    if (max_small && chol == nullptr) {
+      // Gauss-Jordan in registers, R x R elements per thread for the largest body
+      const int R = std::max(1, (max_small + 15) / 16);
+      const size_t lds = ((size_t)max_small * max_small + std::max<size_t>(6 * (size_t)max_small, 4 * 16 * (size_t)R)) *
+                         sizeof(double);
+      RMW_REQUIRE(lds <= 160 * 1024, "preconditioner build exceeds LDS");
+      launch_pc_gj(R, p, nbodies, lds, st);
+    } else if (max_small) {
+      // the Cholesky factorisation in LDS (it also writes L and L^-1): M_b
+      // (n^2), then n + 6 n of vectors
       const size_t lds = ((size_t)max_small * max_small + 8 * (size_t)max_small) * sizeof(double);
       RMW_REQUIRE(lds <= 160 * 1024, "preconditioner build exceeds LDS");
       RMW_CHECK_HIP(hipFuncSetAttribute((const void*)pc_build_kernel<true>,

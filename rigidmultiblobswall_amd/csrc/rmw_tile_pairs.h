@@ -27,9 +27,12 @@ __host__ __device__ __forceinline__ int64_t items_before(int64_t T, int64_t n_t,
 }
 
 // Lane l receives lane (l + 1) mod 64 (DPP wave_rol:1).
+// (bound_ctrl set: every lane of the wave is active where this is used, so it
+// changes nothing, and it lets the rotate fuse into the add that consumes it,
+// v_add_f32_dpp, instead of a separate v_mov_b32_dpp)
 __device__ __forceinline__ float rol1(float v)
 {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double rol1(double v)
 {

@@ -572,9 +572,11 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
             s_t[k][0][il] = (R)(c * v[0]); s_t[k][1][il] = (R)(c * v[1]); s_t[k][2][il] = (R)(pz * c * v[2]);
           }
         }
-      } else if constexpr (kMat) {
-        // zero source vectors beyond the system: the matrix form multiplies
-        // them by the masked (zero) blocks, so they must be finite
+      } else {
+        // slots beyond the system: finite zeros, since pairs not taken are
+        // masked through their geometry (make_geom_on) and still multiply
+        // these values by zero coefficients
+        s_x[il] = 0.0; s_y[il] = 0.0; s_z[il] = 0.0; s_zr[il] = R(0);
 #pragma unroll
         for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -613,7 +615,8 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
           };
           // one image d = (dx, dy, dz) of the pair, applied on both sides
           auto pair_apply = [&](R dx, R dy, R dz) {
-          const Geom<R> g = make_geom(dx, dy, dz, zi, zj);
+          // a pair not taken contributes exact zeros (make_geom_on)
+          const Geom<R> g = MASKED ? make_geom_on(dx, dy, dz, zi, zj, on) : make_geom(dx, dy, dz, zi, zj);
           CoefUF<R> cuf;
           CoefRot<R> cut, cwf;
           CoefWT<R> cwt;
@@ -621,23 +624,12 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
           if constexpr (UT || WF) { cut = coef_rot(g, zi, DOM); cwf = coef_rot(g, zj, DOM); }
           if constexpr (WT) cwt = coef_wt(g, DOM);
           if constexpr (kMat) {
-            // the pair's blocks as 3x3 matrices, zeroed for pairs not taken
+            // the pair's blocks as 3x3 matrices (zero for pairs not taken)
             Mat3<R> Mu, Mc, Md, Mw;   // UF, UT, WF, WT of (i, j)
             if constexpr (UF) Mu = mat_uf(g, cuf);
             if constexpr (UT) Mc = mat_ut(g, cut);
             if constexpr (WF) Md = mat_wf(g, cwf);
             if constexpr (WT) Mw = mat_wt(g, cwt);
-            if constexpr (MASKED) {
-#pragma unroll
-              for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  if constexpr (UF) Mu.m[r][c] = on ? Mu.m[r][c] : R(0);
-                  if constexpr (UT) Mc.m[r][c] = on ? Mc.m[r][c] : R(0);
-                  if constexpr (WF) Md.m[r][c] = on ? Md.m[r][c] : R(0);
-                  if constexpr (WT) Mw.m[r][c] = on ? Mw.m[r][c] : R(0);
-                }
-            }
 #pragma unroll
             for (int k = 0; k < K; ++k) {
               R fj[3], tj[3];
@@ -682,50 +674,45 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
           } else {
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            // target side: u_i += UF f_j + UT t_j, w_i += WF f_j + WT t_j
-            Vec3<R> ui{R(0), R(0), R(0)}, wi{R(0), R(0), R(0)};
-            // source side: u_j += UF^T f_i + WF^T t_i, w_j += UT^T f_i + WT^T t_i
-            Vec3<R> uj{R(0), R(0), R(0)}, wj{R(0), R(0), R(0)};
+            // target side: u_i = UF f_j + UT t_j, w_i = WF f_j + WT t_j;
+            // source side: u_j = UF^T f_i + WF^T t_i, w_j = UT^T f_i + WT^T t_i
+            // (the first block present initialises each sum: no 0 + x)
+            Vec3<R> ui, wi, uj, wj;
+            auto put = [](Vec3<R>& s, const Vec3<R>& v, bool first) {
+              if (first) { s = v; } else { s.x += v.x; s.y += v.y; s.z += v.z; }
+            };
             if constexpr (UF) {
-              const Vec3<R> a = apply_uf(g, cuf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
-              const Vec3<R> bt = apply_uf_t(g, cuf, fi[k][0], fi[k][1], fi[k][2]);
-              ui.x += a.x; ui.y += a.y; ui.z += a.z;
-              uj.x += bt.x; uj.y += bt.y; uj.z += bt.z;
+              put(ui, apply_uf(g, cuf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]), true);
+              put(uj, apply_uf_t(g, cuf, fi[k][0], fi[k][1], fi[k][2]), true);
             }
             if constexpr (UT) {
-              const Vec3<R> a = apply_ut(g, cut, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]);
-              const Vec3<R> bt = apply_wf_t(g, cwf, ti[k][0], ti[k][1], ti[k][2]);
-              ui.x += a.x; ui.y += a.y; ui.z += a.z;
-              uj.x += bt.x; uj.y += bt.y; uj.z += bt.z;
+              put(ui, apply_ut(g, cut, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]), !UF);
+              put(uj, apply_wf_t(g, cwf, ti[k][0], ti[k][1], ti[k][2]), !UF);
             }
             if constexpr (WF) {
-              const Vec3<R> a = apply_wf(g, cwf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]);
-              const Vec3<R> bt = apply_ut_t(g, cut, fi[k][0], fi[k][1], fi[k][2]);
-              wi.x += a.x; wi.y += a.y; wi.z += a.z;
-              wj.x += bt.x; wj.y += bt.y; wj.z += bt.z;
+              put(wi, apply_wf(g, cwf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]), true);
+              put(wj, apply_ut_t(g, cut, fi[k][0], fi[k][1], fi[k][2]), true);
             }
             if constexpr (WT) {
-              const Vec3<R> a = apply_wt(g, cwt, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]);
-              const Vec3<R> bt = apply_wt_t(g, cwt, ti[k][0], ti[k][1], ti[k][2]);
-              wi.x += a.x; wi.y += a.y; wi.z += a.z;
-              wj.x += bt.x; wj.y += bt.y; wj.z += bt.z;
+              put(wi, apply_wt(g, cwt, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]), !WF);
+              put(wj, apply_wt_t(g, cwt, ti[k][0], ti[k][1], ti[k][2]), !WF);
             }
             if constexpr (BT::HU) {
-              u[k][0].add(on ? ui.x : R(0));
-              u[k][1].add(on ? ui.y : R(0));
-              u[k][2].add(on ? ui.z : R(0));
-              acc_j(ja[k][0], (on ? (JT)uj.x : JT(0)));
-              acc_j(ja[k][1], (on ? (JT)uj.y : JT(0)));
-              acc_j(ja[k][2], (on ? (JT)uj.z : JT(0)));
+              u[k][0].add(ui.x);
+              u[k][1].add(ui.y);
+              u[k][2].add(ui.z);
+              acc_j(ja[k][0], (JT)uj.x);
+              acc_j(ja[k][1], (JT)uj.y);
+              acc_j(ja[k][2], (JT)uj.z);
             }
             if constexpr (BT::HW) {
               constexpr int o = BT::HU ? 3 : 0;
-              w[k][0].add(on ? wi.x : R(0));
-              w[k][1].add(on ? wi.y : R(0));
-              w[k][2].add(on ? wi.z : R(0));
-              acc_j(ja[k][o], (on ? (JT)wj.x : JT(0)));
-              acc_j(ja[k][o + 1], (on ? (JT)wj.y : JT(0)));
-              acc_j(ja[k][o + 2], (on ? (JT)wj.z : JT(0)));
+              w[k][0].add(wi.x);
+              w[k][1].add(wi.y);
+              w[k][2].add(wi.z);
+              acc_j(ja[k][o], (JT)wj.x);
+              acc_j(ja[k][o + 1], (JT)wj.y);
+              acc_j(ja[k][o + 2], (JT)wj.z);
             }
           }
           }
@@ -1182,9 +1169,11 @@ static int64_t default_chunk(double pairs, int cu_count)
   return std::max<int64_t>(kBlock, (int64_t)std::ceil(c / kChunkQuantum) * kChunkQuantum);
 }
 
-// offsets: host array of nsys + 1 blob offsets (offsets[0] = 0).
+// offsets: host array of nsys + 1 blob offsets (offsets[0] = 0).  periodic:
+// the call has pseudo-periodic images (the symmetric wall kernels then take
+// one right-hand side per pass, so their partial sums are sized for one).
 static Plan make_plan(int mask, int prec, const int64_t* offsets, int nsys, int k,
-                      const rmw_tuning* tuning, bool ragged, int domain = kDomWall)
+                      const rmw_tuning* tuning, bool ragged, int domain = kDomWall, bool periodic = false)
 {
   RMW_REQUIRE(valid_mask(mask), "mask must be a non-empty subset of UF|UT|WF|WT");
   RMW_REQUIRE(domain >= kDomWall && domain <= kDomFreeSurface, "unknown domain");
@@ -1261,7 +1250,8 @@ static Plan make_plan(int mask, int prec, const int64_t* offsets, int nsys, int 
     // up to 4 right-hand sides per pass (each pair's blocks formed once as
     // 3x3 matrices and applied to all of them), fewer when the tuning caps it
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = domain == kDomWall && mask != (kUF | kUT) ? group_k(k, rc > 0 ? std::min(rc, 4) : 4) : 1;
+    pl.kmax_mut = domain == kDomWall && mask != (kUF | kUT) && !periodic ? group_k(k, rc > 0 ? std::min(rc, 4) : 4)
+                                                                         : 1;
     if (tuning && tuning->chunk_len > 0) {
       // 64 or 128 sources: a quarter or half of a tile pair per work item
       // (finer work for suspensions of few tiles); else whole source tiles
@@ -1642,6 +1632,21 @@ static void check_outputs(const Plan& pl, const double* vel, const double* omg, 
                                                        "blobs or non-finite inputs)");
 }
 
+static bool has_images(const double* L)
+{
+  return L && (L[0] > 0.0 || L[1] > 0.0 || L[2] > 0.0);
+}
+
+// Workspace bytes for a call whose lengths L the query does not see: the
+// larger of the plans without and with pseudo-periodic images.
+static size_t plan_bytes_any_images(int mask, int prec, const int64_t* offsets, int nsys, int k,
+                                    const rmw_tuning* tuning, bool ragged, int domain = kDomWall)
+{
+  const Plan a = make_plan(mask, prec, offsets, nsys, k, tuning, ragged, domain, false);
+  const Plan b = make_plan(mask, prec, offsets, nsys, k, tuning, ragged, domain, true);
+  return std::max(a.total, b.total);
+}
+
 static void run_plan(const Plan& pl, const double* r, const double* f, const double* t, double* vel,
                      double* omg, int k, int64_t ld, double eta, double a, const double* L,
                      const rmw_tuning* tuning, void* ws, size_t ws_bytes, hipStream_t st, bool ragged)
@@ -1763,8 +1768,7 @@ extern "C" int rmw_wall_rpy_workspace_size(int mask, int prec, int64_t n, int k,
     RMW_REQUIRE(bytes != nullptr, "bytes is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     const int64_t off[2] = {0, n};
-    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false);
-    *bytes = pl.total;
+    *bytes = plan_bytes_any_images(mask, prec, off, 1, k, tuning, false);
   });
 }
 
@@ -1775,7 +1779,7 @@ extern "C" int rmw_wall_rpy_describe(int mask, int prec, int64_t n, int k, const
     RMW_REQUIRE(info != nullptr, "info is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     const int64_t off[2] = {0, n};
-    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false);
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, kDomWall, periodic != 0);
     const bool mut = pl.mut_ok && (!periodic || pl.domain == kDomWall);
     info->symmetric = mut ? 1 : 0;
     info->rhs_group = mut ? (periodic ? 1 : pl.kmax_mut) : pl.kmax;
@@ -1793,7 +1797,7 @@ extern "C" int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, c
   return guarded([&] {
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     const int64_t off[2] = {0, n};
-    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false);
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, kDomWall, has_images(L));
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
   });
@@ -1806,8 +1810,7 @@ extern "C" int rmw_mobility_workspace_size(int domain, int mask, int prec, int64
     RMW_REQUIRE(bytes != nullptr, "bytes is null");
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     const int64_t off[2] = {0, n};
-    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, domain);
-    *bytes = pl.total;
+    *bytes = plan_bytes_any_images(mask, prec, off, 1, k, tuning, false, domain);
   });
 }
 
@@ -1819,7 +1822,7 @@ extern "C" int rmw_mobility_apply(int domain, int mask, int prec, const double* 
   return guarded([&] {
     RMW_REQUIRE(n >= 0, "n must be >= 0");
     const int64_t off[2] = {0, n};
-    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, domain);
+    Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, domain, has_images(L));
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
   });
@@ -1830,8 +1833,7 @@ extern "C" int rmw_wall_rpy_ragged_workspace_size(int mask, int prec, const int6
 {
   return guarded([&] {
     RMW_REQUIRE(bytes != nullptr && offsets != nullptr && nsys >= 1, "bad arguments");
-    Plan pl = make_plan(mask, prec, offsets, nsys, k, tuning, true);
-    *bytes = pl.total;
+    *bytes = plan_bytes_any_images(mask, prec, offsets, nsys, k, tuning, true);
   });
 }
 
@@ -1843,7 +1845,7 @@ extern "C" int rmw_wall_rpy_ragged_apply(int mask, int prec, const double* r_vec
 {
   return guarded([&] {
     RMW_REQUIRE(offsets != nullptr && nsys >= 1, "bad offsets");
-    Plan pl = make_plan(mask, prec, offsets, nsys, k, tuning, true);
+    Plan pl = make_plan(mask, prec, offsets, nsys, k, tuning, true, kDomWall, has_images(L));
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, true);
   });

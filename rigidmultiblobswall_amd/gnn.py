@@ -34,28 +34,34 @@ from . import _lib
 
 DEG, NIN, EIN, HE, NMSG, HN = 8, 10, 16, 64, 64, 128
 NC = 80  # node input width: 10 + 64 padded to a multiple of 16
+NIN_MOTION = 22   # node inputs of the network that corrects the previous step's solution
+
+
+def node_width(nin: int) -> int:
+    """Node-layer input width: nin features + the message, padded to 16."""
+    return (nin + NMSG + 15) // 16 * 16
 
 
 def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
-def param_shapes(nout: int) -> List[Tuple[str, Tuple[int, ...]]]:
+def param_shapes(nout: int, nin: int = NIN) -> List[Tuple[str, Tuple[int, ...]]]:
     return [("We1", (HE, EIN)), ("be1", (HE,)), ("We2", (NMSG, HE)), ("be2", (NMSG,)),
-            ("Wn1", (HN, NC)), ("bn1", (HN,)), ("Wn2", (HN, HN)), ("bn2", (HN,)),
+            ("Wn1", (HN, node_width(nin))), ("bn1", (HN,)), ("Wn2", (HN, HN)), ("bn2", (HN,)),
             ("Wn3", (nout, HN)), ("bn3", (nout,))]
 
 
-def init_params(nout: int, seed: int = 0) -> Dict[str, np.ndarray]:
+def init_params(nout: int, seed: int = 0, nin: int = NIN) -> Dict[str, np.ndarray]:
     """Glorot-uniform weights, zero biases (FP32)."""
     rng = np.random.default_rng(seed)
     out = {}
-    for name, shp in param_shapes(nout):
+    for name, shp in param_shapes(nout, nin):
         if len(shp) == 2:
             lim = np.sqrt(6.0 / (shp[0] + shp[1]))
             w = rng.uniform(-lim, lim, shp).astype(np.float32)
             if name == "Wn1":
-                w[:, NIN + NMSG:] = 0.0   # padding columns multiply zero inputs
+                w[:, nin + NMSG:] = 0.0   # padding columns multiply zero inputs
             if name == "Wn3":
                 w *= 0.1                  # small initial correction
             out[name] = w
@@ -66,7 +72,7 @@ def init_params(nout: int, seed: int = 0) -> Dict[str, np.ndarray]:
 
 @dataclass
 class Batch:
-    Xn: torch.Tensor   # (B, 10) f32
+    Xn: torch.Tensor   # (B, nin) f32
     Xe: torch.Tensor   # (8B, 16) f32
     T: Optional[torch.Tensor] = None  # (B, nout) f32 standardised targets
 
@@ -75,16 +81,17 @@ class GraphNet:
     """Device-resident network: FP32 master parameters in one flat buffer,
     BF16 copies, gradients and Adam moments alongside."""
 
-    def __init__(self, blobs_per_body: int, device=None, seed: int = 0):
+    def __init__(self, blobs_per_body: int, device=None, seed: int = 0, nin: int = NIN):
         self.lib = _lib.load()
         self.dev = device or torch.device("cuda", torch.cuda.current_device())
         self.nbb = blobs_per_body
         self.nout = 3 * blobs_per_body + 6
-        shapes = param_shapes(self.nout)
+        self.nin, self.nc = int(nin), node_width(nin)
+        shapes = param_shapes(self.nout, self.nin)
         self.sizes = [int(np.prod(s)) for _, s in shapes]
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.nparam = int(self.offsets[-1])
-        init = init_params(self.nout, seed)
+        init = init_params(self.nout, seed, self.nin)
         flat = np.concatenate([init[n].ravel() for n, _ in shapes]).astype(np.float32)
         self.w = torch.from_numpy(flat).to(self.dev)
         self.wb = torch.empty(self.nparam, dtype=torch.int16, device=self.dev)
@@ -98,6 +105,8 @@ class GraphNet:
         self.mean = torch.zeros(self.nout, dtype=torch.float32, device=self.dev)
         self.sd = torch.ones(self.nout, dtype=torch.float32, device=self.dev)
         self.fscale = 1.0
+        self.dscale = 1.0     # motion features: displacement and rotation scales
+        self.rscale = 1.0
         self._act = {}
 
     # parameter views
@@ -162,12 +171,15 @@ class GraphNet:
                    bias=self.W("be1"), act=1)
         Msg = self._empty((E, NMSG))
         self._gemm(0, 1, E, NMSG, HE, He1, HE, self.Wb("We2"), HE, C=Msg, ldc=NMSG, bias=self.W("be2"))
-        Xc = self._empty((B, NC), torch.int16)
-        _lib.check(self.lib.rmw_gnn_gather(batch.Xn.data_ptr(), NIN, Msg.data_ptr(), NMSG, DEG, B, Xc.data_ptr(), NC,
-                                           _stream()))
+        nc = self.nc
+        if batch.Xn.shape[1] != self.nin:
+            raise ValueError(f"the network takes {self.nin} node features, the batch has {batch.Xn.shape[1]}")
+        Xc = self._empty((B, nc), torch.int16)
+        _lib.check(self.lib.rmw_gnn_gather(batch.Xn.data_ptr(), self.nin, Msg.data_ptr(), NMSG, DEG, B, Xc.data_ptr(),
+                                           nc, _stream()))
         Zn1 = self._empty((B, HN))
         Hn1 = self._empty((B, HN), torch.int16)
-        self._gemm(0, 1, B, HN, NC, Xc, NC, self.Wb("Wn1"), NC, C=Zn1, ldc=HN, H=Hn1, ldh=HN, bias=self.W("bn1"),
+        self._gemm(0, 1, B, HN, nc, Xc, nc, self.Wb("Wn1"), nc, C=Zn1, ldc=HN, H=Hn1, ldh=HN, bias=self.W("bn1"),
                    act=1)
         Zn2 = self._empty((B, HN))
         Hn2 = self._empty((B, HN), torch.int16)
@@ -201,13 +213,14 @@ class GraphNet:
         dZ1 = self._empty((B, HN), torch.int16)
         _lib.check(self.lib.rmw_silu_backward(dH1.data_ptr(), a["Zn1"].data_ptr(), dZ1.data_ptr(), B * HN,
                                               _stream()))
-        self._gemm_wgrad(HN, NC, B, dZ1, HN, a["Xc"], NC, self.G("Wn1"), NC)
+        nc = self.nc
+        self._gemm_wgrad(HN, nc, B, dZ1, HN, a["Xc"], nc, self.G("Wn1"), nc)
         _lib.check(self.lib.rmw_colsum_bf16(dZ1.data_ptr(), HN, B, HN, self.G("bn1").data_ptr(), 0.0, _stream()))
-        dXc = self._empty((B, NC))
-        self._gemm(0, 0, B, NC, HN, dZ1, HN, self.Wb("Wn1"), NC, C=dXc, ldc=NC)
+        dXc = self._empty((B, nc))
+        self._gemm(0, 0, B, nc, HN, dZ1, HN, self.Wb("Wn1"), nc, C=dXc, ldc=nc)
         # messages -> edge layer 2
         dM = self._empty((E, NMSG), torch.int16)
-        _lib.check(self.lib.rmw_gnn_scatter(dXc.data_ptr(), NC, NIN, NMSG, DEG, B, dM.data_ptr(), _stream()))
+        _lib.check(self.lib.rmw_gnn_scatter(dXc.data_ptr(), nc, self.nin, NMSG, DEG, B, dM.data_ptr(), _stream()))
         self._gemm_wgrad(NMSG, HE, E, dM, NMSG, a["He1"], HE, self.G("We2"), HE)
         _lib.check(self.lib.rmw_colsum_bf16(dM.data_ptr(), NMSG, E, NMSG, self.G("be2").data_ptr(), 0.0, _stream()))
         dHe = self._empty((E, HE))
@@ -219,7 +232,7 @@ class GraphNet:
         self._gemm_wgrad(HE, EIN, E, dZe, HE, a["Xe_b"], EIN, self.G("We1"), EIN)
         _lib.check(self.lib.rmw_colsum_bf16(dZe.data_ptr(), HE, E, HE, self.G("be1").data_ptr(), 0.0, _stream()))
         # the padding columns of Wn1 stay zero
-        self.G("Wn1")[:, NIN + NMSG:].zero_()
+        self.G("Wn1")[:, self.nin + NMSG:].zero_()
 
     def loss_and_grad(self, batch: Batch, weights: Optional[torch.Tensor] = None) -> float:
         Y = self.forward(batch, keep=True)
@@ -246,15 +259,30 @@ class GraphFeatures:
         self.lib = net.lib
         self.dev = net.dev
 
-    def batch(self, centres: torch.Tensor, quats: torch.Tensor, force_torque: torch.Tensor) -> Batch:
+    def batch(self, centres: torch.Tensor, quats: torch.Tensor, force_torque: torch.Tensor, previous=None) -> Batch:
+        """Inputs of the network; previous = (centres0, quats0, force_torque0)
+        of the earlier configuration for the motion features (nin 22)."""
         nb = centres.shape[0]
         nbr = torch.empty(nb * DEG, dtype=torch.int32, device=self.dev)
         _lib.check(self.lib.rmw_gnn_knn(centres.data_ptr(), nb, DEG, nbr.data_ptr(), _stream()))
-        Xn = torch.empty((nb, NIN), dtype=torch.float32, device=self.dev)
         Xe = torch.empty((nb * DEG, EIN), dtype=torch.float32, device=self.dev)
-        _lib.check(self.lib.rmw_gnn_features(centres.data_ptr(), quats.data_ptr(), force_torque.data_ptr(),
-                                             nbr.data_ptr(), nb, DEG, float(self.net.fscale), Xn.data_ptr(),
-                                             Xe.data_ptr(), _stream()))
+        if previous is None:
+            if self.net.nin != NIN:
+                raise ValueError("this network needs the previous configuration")
+            Xn = torch.empty((nb, NIN), dtype=torch.float32, device=self.dev)
+            _lib.check(self.lib.rmw_gnn_features(centres.data_ptr(), quats.data_ptr(), force_torque.data_ptr(),
+                                                 nbr.data_ptr(), nb, DEG, float(self.net.fscale), Xn.data_ptr(),
+                                                 Xe.data_ptr(), _stream()))
+        else:
+            if self.net.nin != NIN_MOTION:
+                raise ValueError(f"motion features need a network of {NIN_MOTION} node inputs")
+            c0, q0, f0 = previous
+            Xn = torch.empty((nb, NIN_MOTION), dtype=torch.float32, device=self.dev)
+            _lib.check(self.lib.rmw_gnn_motion_features(centres.data_ptr(), quats.data_ptr(), force_torque.data_ptr(),
+                                                        c0.data_ptr(), q0.data_ptr(), f0.data_ptr(), nbr.data_ptr(),
+                                                        nb, DEG, float(self.net.fscale), float(self.net.dscale),
+                                                        float(self.net.rscale), Xn.data_ptr(), Xe.data_ptr(),
+                                                        _stream()))
         return Batch(Xn=Xn, Xe=Xe)
 
     def targets(self, quats, offs_dev, nblobs, sol, pc, raw: bool = False) -> torch.Tensor:

@@ -73,11 +73,14 @@ def lanczos_block(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch
     if mmax > 1024:
         raise ValueError("max_iter above 1022 is not supported")
     ops = krylov.KrylovOps(dim, mmax + 1, dev)
-    V = torch.empty((mmax, k, dim), dtype=torch.float64, device=dev)   # iteration-major basis
+    # iteration-major basis and the tridiagonals, grown geometrically (the
+    # solves of a well-conditioned operator stop after a few tens of steps)
+    mcap = min(mmax, 32)
+    V = torch.empty((mcap, k, dim), dtype=torch.float64, device=dev)
     W = torch.empty((k, dim), dtype=torch.float64, device=dev)
-    H = torch.zeros((k, mmax, mmax + 1), dtype=torch.float64, device=dev)
+    H = torch.zeros((k, mcap, mcap + 1), dtype=torch.float64, device=dev)
     hs = torch.empty(mmax + 1, dtype=torch.float64, device=dev)
-    C = torch.empty((k, mmax), dtype=torch.float64, device=dev)
+    C = torch.empty((k, mmax + 1), dtype=torch.float64, device=dev)   # last column: QL status
     scale = torch.empty(k, dtype=torch.float64, device=dev)
     zsrc = Z.to(torch.float64).contiguous()
     znorm = np.zeros(k)
@@ -90,24 +93,42 @@ def lanczos_block(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torch
     ldv = k * dim
     status = torch.zeros(k, dtype=torch.int32, device=dev)
     nscr = ctypes.c_size_t()
+    # scratch of the tridiagonal solve (k m^2 doubles), grown geometrically,
+    # and a pinned host buffer for the coefficients the stop test reads
+    zscr = None
+    c_host = torch.empty((k, mmax + 1), dtype=torch.float64, pin_memory=True)
     c_prev = [None] * k
     converged = np.zeros(k, dtype=bool)
     it_conv = np.full(k, max_iter)
     i_last = 0
     for i in range(max_iter + 1):
+        if i + 2 > mcap:
+            m2 = min(mmax, 2 * mcap)
+            V2 = torch.empty((m2, k, dim), dtype=torch.float64, device=dev)
+            V2[:mcap] = V
+            H2 = torch.zeros((k, m2, m2 + 1), dtype=torch.float64, device=dev)
+            H2[:, :mcap, :mcap + 1] = H
+            V, H, mcap = V2, H2, m2
         matvec(V[i], W)
         for r in range(k):
             _lib.check(lib.rmw_arnoldi_cgs2(V[0, r].data_ptr(), ldv, i, W[r].data_ptr(), dim, H[r, i].data_ptr(),
                                             hs.data_ptr(), ops.ws.data_ptr(), ops.ws.numel(), _stream()))
         m = i + 1
         _lib.check(lib.rmw_lanczos_scratch_doubles(m, k, ctypes.byref(nscr)))
-        zscr = torch.empty(max(nscr.value, 1), dtype=torch.float64, device=dev)
+        if zscr is None or zscr.numel() < nscr.value:
+            _lib.check(lib.rmw_lanczos_scratch_doubles(min(mmax, max(2 * m, 32)), k, ctypes.byref(nscr)))
+            zscr = torch.empty(nscr.value, dtype=torch.float64, device=dev)
         _lib.check(lib.rmw_lanczos_sqrt_coeffs(H.data_ptr(), H.stride(0), H.stride(1), m, k, scale.data_ptr(),
-                                               C.data_ptr(), C.stride(0), zscr.data_ptr(), nscr.value,
+                                               C.data_ptr(), C.stride(0), zscr.data_ptr(), zscr.numel(),
                                                status.data_ptr(), _stream()))
-        c_now = C[:, :m].cpu().numpy()
-        if int(status.sum().item()):
+        # one read-back per step: the coefficients and the QL status together
+        C[:, mmax] = status.to(torch.float64)
+        c_host[:, :m].copy_(C[:, :m], non_blocking=True)
+        c_host[:, mmax].copy_(C[:, mmax], non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        if float(c_host[:, mmax].sum()) != 0.0:
             raise RuntimeError("tridiagonal eigen-solve did not converge")
+        c_now = c_host[:, :m].numpy().copy()
         i_last = i
         if i > 0:
             for r in range(k):
@@ -231,6 +252,9 @@ def lanczos_ragged(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torc
     m_used = np.ones((k, nsys), dtype=int)
     c_prev = None
     i_last = 0
+    zscr = None
+    C = torch.empty((R, mmax + 1), dtype=torch.float64, device=dev)       # last column: QL status
+    c_host = torch.empty((R, mmax + 1), dtype=torch.float64, pin_memory=True)
     for i in range(max_iter + 1):
         if i + 2 > mcap:                      # grow the basis and H geometrically
             m2 = min(mmax, 2 * mcap)
@@ -246,14 +270,20 @@ def lanczos_ragged(matvec: Callable[[torch.Tensor, torch.Tensor], None], Z: torc
                                                 ws.numel(), _stream()))
         m = i + 1
         _lib.check(lib.rmw_lanczos_scratch_doubles(m, R, ctypes.byref(nscr)))
-        zscr = torch.empty(max(nscr.value, 1), dtype=torch.float64, device=dev)
-        C = torch.empty((R, m), dtype=torch.float64, device=dev)
+        if zscr is None or zscr.numel() < nscr.value:
+            _lib.check(lib.rmw_lanczos_scratch_doubles(min(mmax, max(2 * m, 32)), R, ctypes.byref(nscr)))
+            zscr = torch.empty(nscr.value, dtype=torch.float64, device=dev)
         _lib.check(lib.rmw_lanczos_sqrt_coeffs(H.data_ptr(), H.stride(1), H.stride(2), m, R, scale.data_ptr(),
-                                               C.data_ptr(), C.stride(0), zscr.data_ptr(), nscr.value,
+                                               C.data_ptr(), C.stride(0), zscr.data_ptr(), zscr.numel(),
                                                status.data_ptr(), _stream()))
-        c_now = C.cpu().numpy().reshape(k, nsys, m)
-        if int(status.sum().item()):
+        # one read-back per step: the coefficients and the QL status together
+        C[:, mmax] = status.to(torch.float64)
+        c_host[:, :m].copy_(C[:, :m], non_blocking=True)
+        c_host[:, mmax].copy_(C[:, mmax], non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        if float(c_host[:, mmax].sum()) != 0.0:
             raise RuntimeError("tridiagonal eigen-solve did not converge")
+        c_now = c_host[:, :m].numpy().reshape(k, nsys, m).copy()
         i_last = i
         if i > 0:
             prev = np.concatenate([c_prev, np.zeros((k, nsys, 1))], axis=2)

@@ -168,10 +168,7 @@ __all__ = [
 ]
 
 
-def single_wall_fluid_mobility_hip(r_vectors, eta, a, *args, **kwargs):
-    """Dense 3N x 3N wall mobility B M(z_eff) B (FP64, built on the GPU);
-    drop-in for single_wall_fluid_mobility, mobility/mobility.py:1018-1116,
-    as selected by mobility_blobs_implementation."""
+def _dense_mobility(domain: int, r_vectors, eta, a):
     import ctypes
     from . import _lib
     lib = _lib.load()
@@ -188,12 +185,26 @@ def single_wall_fluid_mobility_hip(r_vectors, eta, a, *args, **kwargs):
     out = torch.empty((3 * n, 3 * n), dtype=torch.float64, device=device)
     stream = torch.cuda.current_stream(device).cuda_stream
     _lib.check(lib.rmw_rigid_setup(offp, 1, ws.data_ptr(), ws.numel(), stream))
-    _lib.check(lib.rmw_dense_wall_mobility(r.data_ptr(), offp, 1, float(eta), float(a), out.data_ptr(),
-                                           ws.data_ptr(), ws.numel(), stream))
+    _lib.check(lib.rmw_dense_mobility(domain, r.data_ptr(), offp, 1, float(eta), float(a), out.data_ptr(),
+                                      ws.data_ptr(), ws.numel(), stream))
     return out.cpu().numpy()
 
 
-__all__.append("single_wall_fluid_mobility_hip")
+def single_wall_fluid_mobility_hip(r_vectors, eta, a, *args, **kwargs):
+    """Dense 3N x 3N wall mobility B M(z_eff) B (FP64, built on the GPU);
+    drop-in for single_wall_fluid_mobility, mobility/mobility.py:1018-1116,
+    as selected by mobility_blobs_implementation."""
+    return _dense_mobility(wall_rpy.DOMAINS["single_wall"], r_vectors, eta, a)
+
+
+def rotne_prager_tensor_hip(r_vectors, eta, a, *args, **kwargs):
+    """Dense 3N x 3N unbounded RPY mobility (FP64, built on the GPU); drop-in
+    for rotne_prager_tensor, mobility/mobility.py:967-1012, as selected by
+    mobility_blobs_implementation 'python_no_wall' / 'C++_no_wall'."""
+    return _dense_mobility(wall_rpy.DOMAINS["no_wall"], r_vectors, eta, a)
+
+
+__all__ += ["single_wall_fluid_mobility_hip", "rotne_prager_tensor_hip"]
 
 
 # ---------------------------------------------------------------------------

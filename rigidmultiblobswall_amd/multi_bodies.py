@@ -69,8 +69,14 @@ def set_mobility_vector_prod(implementation, *args, **kwargs):
 
 
 def set_mobility_blobs(implementation):
-    """Dense (3N, 3N) blob mobility, signature g(r_vectors, eta, a).  The dense
+    """Dense (3N, 3N) blob mobility, signature g(r_vectors, eta, a), as
+    multi_bodies.py:207-230 selects it: 'hip[_<precision>]' above the wall
+    (the reference's 'python' / 'C++'), 'hip_no_wall[_<precision>]' the
+    unbounded RPY tensor ('python_no_wall' / 'C++_no_wall').  The dense
     builder is always FP64."""
+    if implementation.startswith("hip_no_wall"):
+        _precision("hip" + implementation[len("hip_no_wall"):])
+        return mb.rotne_prager_tensor_hip
     _precision(implementation)
     return mb.single_wall_fluid_mobility_hip
 

@@ -37,7 +37,10 @@ def _stream():
 
 
 class RigidSuspension:
-    """Rigid bodies sharing blob radius `a` above a no-slip wall.
+    """Rigid bodies sharing blob radius `a` above a no-slip wall
+    (domain 'single_wall') or in unbounded fluid ('no_wall': the unbounded
+    RPY product and the dense RPY tensor in the preconditioner, as the
+    reference's 'numba_no_wall' / 'python_no_wall' implementations).
 
     locations (B, 3); orientations (B, 4) unit quaternions (s, p);
     reference: one (nb, 3) array shared by all bodies, or a list of per-body
@@ -45,9 +48,13 @@ class RigidSuspension:
     """
 
     def __init__(self, locations, orientations, reference, eta: float, a: float, periodic_length=None,
-                 precision: str = "fp64", device=None):
+                 precision: str = "fp64", device=None, domain: str = "single_wall"):
         if not torch.cuda.is_available():
             raise RuntimeError("RigidSuspension needs a ROCm GPU (there is no CPU fallback)")
+        if domain not in ("single_wall", "no_wall"):
+            raise ValueError("rigid bodies: domain 'single_wall' or 'no_wall'")
+        self.domain = domain
+        self._dom = wall_rpy.DOMAINS[domain]
         self.lib = _lib.load()
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         loc = np.ascontiguousarray(np.asarray(locations, dtype=np.float64).reshape(-1, 3))
@@ -116,7 +123,8 @@ class RigidSuspension:
     def mobility_times(self, lam: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
         """out = B M(z_eff) B lam on all blobs (wall-RPY UF block)."""
         wall_rpy.apply(RMW_UF, self.r_blobs, force=lam, eta=self.eta, a=self.a, periodic_length=self.L,
-                       precision=precision or self.precision, out_velocity=out.view(1, self.Nblobs, 3))
+                       precision=precision or self.precision, out_velocity=out.view(1, self.Nblobs, 3),
+                       domain=self.domain)
 
     def operator(self, x: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
         """out = [M lambda - K U; -K^T lambda]."""
@@ -134,12 +142,12 @@ class RigidSuspension:
         status = torch.zeros(self.nbodies, dtype=torch.int32, device=dev)
         scratch = (torch.empty(self.pc_scratch_doubles, dtype=torch.float64, device=dev)
                    if self.pc_scratch_doubles else None)
-        _lib.check(self.lib.rmw_rigid_pc_build(self.r_blobs.data_ptr(), self.r_rel.data_ptr(), self._offp(),
-                                               self.nbodies, self.eta, self.a, self.minv.data_ptr(),
-                                               self.minvk.data_ptr(), self.nbody.data_ptr(), status.data_ptr(),
-                                               scratch.data_ptr() if scratch is not None else None,
-                                               self.pc_scratch_doubles, self.ws.data_ptr(), self.ws.numel(),
-                                               _stream()))
+        _lib.check(self.lib.rmw_rigid_pc_build_ex(self._dom, self.r_blobs.data_ptr(), self.r_rel.data_ptr(),
+                                                  self._offp(), self.nbodies, self.eta, self.a, self.minv.data_ptr(),
+                                                  self.minvk.data_ptr(), self.nbody.data_ptr(), None, None,
+                                                  status.data_ptr(), scratch.data_ptr() if scratch is not None else None,
+                                                  self.pc_scratch_doubles, self.ws.data_ptr(), self.ws.numel(),
+                                                  _stream()))
         del scratch
         self.pc_r_rel = self.r_rel
         bad = int(status.sum().item())
@@ -164,8 +172,8 @@ class RigidSuspension:
         status = torch.zeros(self.nbodies, dtype=torch.int32, device=dev)
         scratch = (torch.empty(self.pc_scratch_doubles, dtype=torch.float64, device=dev)
                    if self.pc_scratch_doubles else None)
-        _lib.check(self.lib.rmw_rigid_pc_build_det_stoch(
-            self.r_blobs.data_ptr(), self.r_rel.data_ptr(), self._offp(), self.nbodies, self.eta, self.a,
+        _lib.check(self.lib.rmw_rigid_pc_build_ex(
+            self._dom, self.r_blobs.data_ptr(), self.r_rel.data_ptr(), self._offp(), self.nbodies, self.eta, self.a,
             self.minv.data_ptr(), self.minvk.data_ptr(), self.nbody.data_ptr(), self.chol.data_ptr(),
             self.chol_inv.data_ptr(), status.data_ptr(), scratch.data_ptr() if scratch is not None else None,
             self.pc_scratch_doubles, self.ws.data_ptr(), self.ws.numel(), _stream()))
@@ -209,7 +217,8 @@ class RigidSuspension:
         self._block_apply(self.chol_inv, X, t1, trans=True)
         t2 = torch.empty((k, self.Nblobs, 3), dtype=torch.float64, device=self.device)
         wall_rpy.apply(RMW_UF, self.r_blobs, force=t1.view(k, self.Nblobs, 3), eta=self.eta, a=self.a,
-                       periodic_length=self.L, precision=precision or self.precision, k=k, out_velocity=t2)
+                       periodic_length=self.L, precision=precision or self.precision, k=k, out_velocity=t2,
+                       domain=self.domain)
         self._block_apply(self.chol_inv, t2.view(k, -1), Y, trans=False)
 
     def P_inv_mult(self, x: torch.Tensor) -> torch.Tensor:
@@ -241,10 +250,12 @@ class RigidSuspension:
                                                self.ws.data_ptr(), self.ws.numel(), _stream()))
 
     # -- initial guesses -----------------------------------------------------------------
-    def point_field(self, force_torque, radius: float = 0.9, wall: bool = True) -> torch.Tensor:
+    def point_field(self, force_torque, radius: float = 0.9, wall: Optional[bool] = None) -> torch.Tensor:
         """(Nblobs, 3) velocity at every blob from the net forces of the other
         bodies, each a sphere of `radius` at its centre (different-radii
         wall RPY)."""
+        if wall is None:
+            wall = self.domain == "single_wall"
         ft = torch.as_tensor(force_torque, dtype=torch.float64).to(self.device).reshape(self.nbodies, -1).contiguous()
         if ft.shape[1] < 3:
             raise ValueError("force_torque must hold at least the three force components per body")

@@ -68,8 +68,9 @@ class RigidIntegrator:
         if solver not in ("fp64", "mixed"):
             raise ValueError(f"solver must be 'fp64' or 'mixed', not {solver!r}")
         self.solver = solver
-        if domain != "single_wall":
-            raise ValueError("rigid bodies: domain single_wall only")
+        if domain not in ("single_wall", "no_wall"):
+            raise ValueError("rigid bodies: domain 'single_wall' or 'no_wall'")
+        self.domain = domain
         self.scheme = scheme
         self.lib = _lib.load()
         self.device = torch.device("cuda", torch.cuda.current_device())
@@ -145,7 +146,7 @@ class RigidIntegrator:
     def suspension(self, centres=None, quats=None) -> RigidSuspension:
         return RigidSuspension(self.centres if centres is None else centres, self.quats if quats is None else quats,
                                self.reference, self.eta, self.a, periodic_length=self.periodic_length,
-                               precision=self.precision)
+                               precision=self.precision, domain=self.domain)
 
     def _current(self) -> RigidSuspension:
         if self._S is None:
@@ -251,6 +252,8 @@ class RigidIntegrator:
                                                S.nbodies, 1.5 if ab else 1.0, 0.5 if ab else 0.0, float(dt),
                                                Sn.centres.data_ptr(), Sn.quats.data_ptr(), _stream()))
         Sn.set_configuration(Sn.centres, Sn.quats)
+        if self.domain != "single_wall":
+            return True          # check_positions tests the wall only above one (:1786)
         self._count.zero_()
         _lib.check(self.lib.rmw_count_below(Sn.r_blobs.data_ptr(), Sn.Nblobs, 0.0, self._count.data_ptr(),
                                             self._count_ws.data_ptr(), self._count_ws.numel(), _stream()))

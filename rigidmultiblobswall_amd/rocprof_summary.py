@@ -11,6 +11,9 @@
       scaled by what the same counter reads for the calibration copy kernel
       (rmw_axpby with beta = 0: B bytes read, B bytes written; bench.py
       --calibrate-copy)
+  python -m rigidmultiblobswall_amd.rocprof_summary counters DIR --kernel SUBSTR
+      mean per-dispatch value of every counter of a `rocprofv3 --pmc` run for
+      the kernels whose name contains SUBSTR
 """
 from __future__ import annotations
 
@@ -84,6 +87,20 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: float, copy
             "fetch_scale": f_scale, "write_scale": w_scale, "traffic_bytes": tb}
 
 
+def counter_summary(d: str, kernel: str) -> Dict[str, float]:
+    """Mean per-dispatch value of every counter collected for the kernels
+    whose name contains `kernel`."""
+    cnt = counters(d)
+    out: Dict[str, List[float]] = defaultdict(list)
+    for name, cs in cnt.items():
+        if kernel in name:
+            for c, vals in cs.items():
+                out[c].extend(vals)
+    if not out:
+        raise KeyError(f"no counters for a kernel matching {kernel!r}")
+    return {c: sum(v) / len(v) for c, v in sorted(out.items())}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -97,8 +114,17 @@ def main(argv=None):
     s2.add_argument("--kernel", required=True)
     s2.add_argument("--copy-bytes", type=float, required=True)
     s2.add_argument("--copy-kernel", default="axpby_kernel")
+    s2.add_argument("--label", default=None,
+                    help="kernel family name recorded in the output (default: the --kernel match)")
     s2.add_argument("--out", default=None)
+    s3 = sub.add_parser("counters")
+    s3.add_argument("dir")
+    s3.add_argument("--kernel", required=True)
     args = ap.parse_args(argv)
+    if args.cmd == "counters":
+        for c, v in counter_summary(args.dir, args.kernel).items():
+            print(f"{c:28s} {v:16.1f}")
+        return 0
     if args.cmd == "stats":
         rows = kernel_stats(args.dir)
         print(f"{'kernel':88s} {'calls':>6s} {'total_us':>11s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} "
@@ -109,6 +135,9 @@ def main(argv=None):
     else:
         t = traffic(args.fetch, args.write, args.kernel, args.copy_bytes, args.copy_fetch, args.copy_write,
                     args.copy_kernel)
+        if args.label:
+            t["kernel_match"] = t["kernel"]
+            t["kernel"] = args.label
         js = json.dumps(t, indent=1)
         print(js)
         if args.out:

@@ -1,8 +1,8 @@
 """Device-level wall-RPY products (torch ROCm tensors in, torch tensors out).
 
 Thin host layer over ``rmw_wall_rpy_apply`` / ``rmw_wall_rpy_ragged_apply``:
-it validates shapes, owns a per-device workspace that only grows, and
-passes raw device pointers and the current HIP stream.  PyTorch is used
+it validates shapes, owns a workspace per (device, stream) that only grows,
+and passes raw device pointers and the current HIP stream.  PyTorch is used
 only to own device memory and streams.
 """
 from __future__ import annotations
@@ -28,9 +28,12 @@ def _device() -> torch.device:
 
 
 def workspace(nbytes: int, device: Optional[torch.device] = None, slot: str = "default") -> torch.Tensor:
-    """A persistent uint8 device buffer of at least nbytes (grown, never shrunk)."""
+    """A persistent uint8 device buffer of at least nbytes (grown, never
+    shrunk), one per (device, current stream, slot): calls issued on
+    different streams never share scratch, so they may overlap.  A buffer is
+    allocated on, used on and (when grown) released on its own stream."""
     device = device or _device()
-    key = (device.index, slot)
+    key = (device.index, torch.cuda.current_stream(device).cuda_stream, slot)
     with _ws_lock:
         buf = _workspaces.get(key)
         if buf is None or buf.numel() < nbytes:
@@ -81,15 +84,24 @@ def _stream_ptr(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
-def _tuning_for(tuning, mask: int, precision: str, k: int, pairs: float, ragged: bool):
+def _periodic(periodic_length) -> bool:
+    return periodic_length is not None and bool(np.any(np.asarray(periodic_length, dtype=np.float64) > 0))
+
+
+def _tuning_for(tuning, mask: int, precision: str, k: int, pairs: float, ragged: bool, periodic: bool = False):
     """Launch shape for a call: the caller's, the autotuner's cached one
-    (tuning=None), or the library's built-in heuristic (tuning="heuristic")."""
+    (tuning=None), or the library's built-in heuristic (tuning="heuristic").
+    The cache holds shapes timed without periodic images (each pair then
+    costs 9-27 image evaluations), so calls with images use the built-in
+    heuristic unless the caller passes a shape."""
     if isinstance(tuning, str):
         if tuning != "heuristic":
             raise ValueError("tuning must be None, 'heuristic', a dict or an rmw_tuning")
         return None
     if tuning is not None:
         return tuning
+    if periodic:
+        return None
     from . import autotune
     return autotune.lookup(mask, precision if isinstance(precision, str) else str(precision), k, pairs, ragged)
 
@@ -136,7 +148,7 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
     prec = _lib.precision_code(precision)
     dom = DOMAINS[domain]
     if dom == 0:
-        tun = _tuning_for(tuning, mask, precision, k, float(n) * n, False)
+        tun = _tuning_for(tuning, mask, precision, k, float(n) * n, False, _periodic(periodic_length))
     else:
         # the cache holds wall shapes
         tun = None if tuning is None or isinstance(tuning, str) else tuning
@@ -186,7 +198,7 @@ def apply_strided(mask: int, r: torch.Tensor, n: int, X: torch.Tensor, Y: torch.
         raise ValueError(f"r must be a contiguous float64 (N, 3) tensor on {device} with N = {n}")
     k, ld, fp, tp_, vp, wp = _grand_rows(X, Y, n, device)
     prec = _lib.precision_code(precision)
-    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False, _periodic(periodic_length)))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_workspace_size(mask, prec, n, k, tp, ctypes.byref(nbytes)))
     ws = workspace(nbytes.value, device)
@@ -213,7 +225,7 @@ def apply_ragged(mask: int, r_vectors, offsets: Sequence[int], force=None, torqu
     w = torch.empty((k, n, 3), dtype=torch.float64, device=device) if has_w else None
     prec = _lib.precision_code(precision)
     pairs = float(np.sum(np.diff(off).astype(np.float64) ** 2))
-    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True))
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True, _periodic(periodic_length)))
     offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, nsys, k, tp, ctypes.byref(nbytes)))
@@ -244,7 +256,7 @@ def apply_ragged_strided(mask: int, r: torch.Tensor, offsets: np.ndarray, X: tor
     k, ld, fp, tp_, vp, wp = _grand_rows(X, Y, n, device)
     prec = _lib.precision_code(precision)
     pairs = float(np.sum(np.diff(off).astype(np.float64) ** 2))
-    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True))
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, pairs, True, _periodic(periodic_length)))
     offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     nbytes = ctypes.c_size_t()
     _lib.check(lib.rmw_wall_rpy_ragged_workspace_size(mask, prec, offp, off.size - 1, k, tp, ctypes.byref(nbytes)))
@@ -349,7 +361,7 @@ def describe(mask: int, n: int, k: int = 1, precision="fp32", periodic_length=No
     work items), with the same tuning resolution as ``apply``."""
     lib = _lib.load()
     _device()
-    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False))
+    tp = _lib.tuning_ptr(_tuning_for(tuning, mask, precision, k, float(n) * n, False, _periodic(periodic_length)))
     L = np.zeros(3) if periodic_length is None else np.asarray(periodic_length, dtype=np.float64)
     info = _lib.PlanInfo()
     _lib.check(lib.rmw_wall_rpy_describe(mask, _lib.precision_code(precision), n, k, tp, int(bool(np.any(L > 0))),

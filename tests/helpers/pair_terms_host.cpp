@@ -117,3 +117,51 @@ extern "C" int pt_radii_matrix_f32(const double* ri, const double* rj, double a,
 {
   return radii_matrix<float>(ri, rj, a, b, eta, wall, m9);
 }
+
+// make_geom_on: every block of every boundary, dyadic and explicit forms,
+// applied to a unit vector, for a pair that is (on = 1) or is not (on = 0)
+// taken.  m_out: the four blocks and their transposes in every domain
+// (8 block forms x 4 domains x 3 unit vectors x 3 components = 288 values),
+// then 4 explicit blocks x 9 (36 values).
+template <typename T>
+static int masked_blocks(const double* ri, const double* rj, int on, double* m_out)
+{
+  T dx = T(ri[0] - rj[0]), dy = T(ri[1] - rj[1]), dz = T(ri[2] - rj[2]);
+  T zi = T(ri[2]), zj = T(rj[2]);
+  Geom<T> g = make_geom_on(dx, dy, dz, zi, zj, on != 0);
+  int o = 0;
+  for (int dom = 0; dom < 4; ++dom) {
+    const CoefUF<T> cu = coef_uf(g, zi, zj, dom);
+    const CoefRot<T> ci = coef_rot(g, zi, dom), cj = coef_rot(g, zj, dom);
+    const CoefWT<T> cw = coef_wt(g, dom);
+    for (int c = 0; c < 3; ++c) {
+      T v[3] = {T(0), T(0), T(0)};
+      v[c] = T(1);
+      const Vec3<T> r[8] = {apply_uf(g, cu, v[0], v[1], v[2]),   apply_uf_t(g, cu, v[0], v[1], v[2]),
+                            apply_ut(g, ci, v[0], v[1], v[2]),   apply_ut_t(g, ci, v[0], v[1], v[2]),
+                            apply_wf(g, cj, v[0], v[1], v[2]),   apply_wf_t(g, cj, v[0], v[1], v[2]),
+                            apply_wt(g, cw, v[0], v[1], v[2]),   apply_wt_t(g, cw, v[0], v[1], v[2])};
+      for (int b = 0; b < 8; ++b) {
+        m_out[o++] = double(r[b].x);
+        m_out[o++] = double(r[b].y);
+        m_out[o++] = double(r[b].z);
+      }
+    }
+  }
+  const CoefUF<T> cu = coef_uf(g, zi, zj);
+  const Mat3<T> M[4] = {mat_uf(g, cu), mat_ut(g, coef_rot(g, zi)), mat_wf(g, coef_rot(g, zj)), mat_wt(g, coef_wt(g))};
+  for (int b = 0; b < 4; ++b)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) m_out[o++] = double(M[b].m[r][c]);
+  return o;
+}
+
+extern "C" int pt_masked_blocks_f32(const double* ri, const double* rj, int on, double* m_out)
+{
+  return masked_blocks<float>(ri, rj, on, m_out);
+}
+
+extern "C" int pt_masked_blocks_f64(const double* ri, const double* rj, int on, double* m_out)
+{
+  return masked_blocks<double>(ri, rj, on, m_out);
+}

@@ -211,3 +211,29 @@ def test_tuning_flags_layout_and_merge():
     base = _lib.Tuning(704, 0, 4, 1, 0)
     t = _lib.with_flags(base, 1)
     assert (t.chunk_len, t.rhs_group, t.symmetric, t.flags) == (704, 4, 1, 1) and base.flags == 0
+
+
+def test_masked_geometry_gives_exact_zeros():
+    """A pair the symmetric kernels do not take (the self pair, the lower
+    half of a diagonal tile, padding beyond the suspension) is masked
+    through its geometry (make_geom_on: 1/|d| = 1/|R| = 0, far-field
+    branches): every block of every boundary, dyadic and explicit, both
+    orientations, is then exactly zero -- also at d = 0 and for near pairs --
+    and a taken pair is unchanged."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(REPO, "tests", "helpers", "libpair_terms_host.so"))
+    dp = ctypes.POINTER(ctypes.c_double)
+    P = lambda x: x.ctypes.data_as(dp)   # noqa: E731
+    for fn in (lib.pt_masked_blocks_f32, lib.pt_masked_blocks_f64):
+        fn.argtypes = [dp, dp, ctypes.c_int, dp]
+    for fn32, fn in ((True, lib.pt_masked_blocks_f32), (False, lib.pt_masked_blocks_f64)):
+        for ri, rj in ((np.array([1.0, 2.0, 1.5]), np.array([1.0, 2.0, 1.5])),      # d = 0
+                       (np.array([0.3, 0.1, 0.9]), np.array([0.0, -0.4, 1.2])),     # overlapping
+                       (np.array([5.0, 2.0, 3.5]), np.array([-1.0, 0.4, 0.8]))):    # far
+            off = np.full(324, np.nan)
+            on = np.full(324, np.nan)
+            assert fn(P(ri), P(rj), 0, P(off)) == 324
+            assert np.all(off == 0.0), (fn32, ri, rj)
+            if np.any(ri != rj):
+                assert fn(P(ri), P(rj), 1, P(on)) == 324
+                assert np.all(np.isfinite(on)) and np.abs(on).max() > 0.0

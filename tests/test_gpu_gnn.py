@@ -128,7 +128,8 @@ def test_forward_and_gradients_match_oracle(gpu):
         cos = float((g * r).sum() / (np.linalg.norm(g) * np.linalg.norm(r) + 1e-300))
         err = np.linalg.norm(g - r) / (np.linalg.norm(r) + 1e-300)
         print(f"  {name}: rel {err:.2e} cos {cos:.6f}")
-        assert cos > 0.998 and err < 6e-2, name
+        # measured on MI355X: rel <= 4e-3, cos >= 0.99999 (BF16 operands)
+        assert cos > 0.9999 and err < 1e-2, name
 
 
 def test_training_reduces_loss(gpu):
@@ -231,3 +232,85 @@ def test_small_warm_start_end_to_end(gpu):
         # the point-field base beats P^-1 b, and the network improves on its base
         assert row["point/fp64"]["x0_rel_err"] < row["pc/fp64"]["x0_rel_err"]
         assert row["gnn/fp64"]["x0_rel_err"] < row["point/fp64"]["x0_rel_err"]
+
+
+def test_motion_features(gpu):
+    """Motion features of the previous-step network: the first 10 node
+    features equal the static ones; a configuration compared with itself has
+    zero motion; a known displacement and rotation of every body appear in
+    the body frame (rotation vector, displacement, neighbour-offset change)."""
+    from rigidmultiblobswall_amd.trajectory import quat_from_rotation, quat_mul
+    w = workloads.c4_monolayer(nbodies=64, seed=8)
+    net = gnn.GraphNet(42, gpu, nin=gnn.NIN_MOTION)
+    net.fscale, net.dscale, net.rscale = 2.0, 0.01, 0.02
+    feats = gnn.GraphFeatures(net)
+    rng = np.random.default_rng(3)
+    c0 = w["centres"]
+    q0 = w["quats"]
+    f0 = rng.standard_normal((64, 6))
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(gpu)   # noqa: E731
+    print("case motion features, 64 bodies", flush=True)
+    same = feats.batch(dev(c0), dev(q0), dev(f0), previous=(dev(c0), dev(q0), dev(f0)))
+    Xn = same.Xn.cpu().numpy()
+    # q q^-1 is the identity up to rounding: rotation vectors of ~1e-16 / rscale
+    assert Xn.shape == (64, 22) and np.abs(Xn[:, 10:]).max() < 1e-12
+    assert np.abs(same.Xe.cpu().numpy().reshape(64, 8, 16)[:, :, 5:9]).max() == 0.0
+    net10 = gnn.GraphNet(42, gpu)
+    net10.fscale = 2.0
+    static = gnn.GraphFeatures(net10).batch(dev(c0), dev(q0), dev(f0))
+    assert np.array_equal(Xn[:, :10], static.Xn.cpu().numpy())
+    # move every body
+    dc = 0.01 * rng.standard_normal((64, 3))
+    th = 0.02 * rng.standard_normal((64, 3))
+    c1 = c0 + dc
+    q1 = quat_mul(quat_from_rotation(th), q0)
+    f1 = f0 + 0.1 * rng.standard_normal((64, 6))
+    b = feats.batch(dev(c1), dev(q1), dev(f1), previous=(dev(c0), dev(q0), dev(f0)))
+    Xn = b.Xn.cpu().numpy().astype(np.float64)
+    R1 = workloads.rotation_matrices(q1)
+    body = lambda v: np.einsum("bji,bj->bi", R1, v)   # noqa: E731  R^T v per body
+    assert np.allclose(Xn[:, 10:13], body(dc) / 0.01, atol=1e-4)
+    assert np.allclose(Xn[:, 13:16], body(th) / 0.02, atol=1e-4)
+    assert np.allclose(Xn[:, 16:19], body(f1[:, :3] - f0[:, :3]) / 2.0, atol=1e-5)
+    assert np.allclose(Xn[:, 19:22], body(f1[:, 3:] - f0[:, 3:]) / 2.0, atol=1e-5)
+    d = np.linalg.norm(c1[:, None] - c1[None], axis=-1)
+    np.fill_diagonal(d, np.inf)
+    nn = np.argsort(d, axis=1)[:, :8]
+    Xe = b.Xe.cpu().numpy().reshape(64, 8, 16).astype(np.float64)
+    for i in (0, 31, 63):
+        dd = (c1[nn[i]] - c1[i]) - (c0[nn[i]] - c0[i])
+        assert np.allclose(Xe[i, :, 5:8], (dd @ R1[i]) / 0.01, atol=1e-4)
+        dl = np.linalg.norm(c1[nn[i]] - c1[i], axis=1) - np.linalg.norm(c0[nn[i]] - c0[i], axis=1)
+        assert np.allclose(Xe[i, :, 8], dl / 0.01, atol=1e-4)
+
+
+def test_motion_network_gradients_match_oracle(gpu):
+    """The 22-input network (node width 96): GPU forward and gradients =
+    the FP64 oracle on BF16-rounded operands."""
+    rng = np.random.default_rng(13)
+    B = 41
+    net = gnn.GraphNet(12, gpu, seed=4, nin=gnn.NIN_MOTION)
+    Xn = rng.standard_normal((B, gnn.NIN_MOTION)).astype(np.float32)
+    Xe = rng.standard_normal((B * gnn.DEG, gnn.EIN)).astype(np.float32)
+    T = rng.standard_normal((B, net.nout)).astype(np.float32)
+    batch = gnn.Batch(Xn=torch.from_numpy(Xn).to(gpu), Xe=torch.from_numpy(Xe).to(gpu), T=torch.from_numpy(T).to(gpu))
+    P = {k: GO.bf16_round(v).astype(np.float64) for k, v in net.params_numpy().items()}
+    for bname in ("be1", "be2", "bn1", "bn2", "bn3"):
+        v = (0.1 * rng.standard_normal(net.W(bname).shape)).astype(np.float32)
+        net.W(bname).copy_(torch.from_numpy(v))
+        P[bname] = v.astype(np.float64)
+    print("case motion network forward/backward B=41, nin 22", flush=True)
+    loss = net.loss_and_grad(batch)
+    Y = net.forward(batch).cpu().numpy()
+    Xn64, Xe64 = GO.bf16_round(Xn), GO.bf16_round(Xe)
+    Yr, _ = GO.forward(P, Xn64, Xe64)
+    assert np.linalg.norm(Y - Yr) <= 3e-2 * np.linalg.norm(Yr)
+    Lr, gr = GO.loss_and_grad(P, Xn64, Xe64, T.astype(np.float64))
+    assert abs(loss * net.nout - Lr) <= 5e-2 * abs(Lr)
+    for name in net.names:
+        g = net.G(name).cpu().numpy().astype(np.float64)
+        r = gr[name]
+        cos = float((g * r).sum() / (np.linalg.norm(g) * np.linalg.norm(r) + 1e-300))
+        err = np.linalg.norm(g - r) / (np.linalg.norm(r) + 1e-300)
+        print(f"  {name}: rel {err:.2e} cos {cos:.6f}")
+        assert cos > 0.9999 and err < 1e-2, name
