@@ -107,6 +107,29 @@ int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, const double
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * rmw_wall_rpy_apply without a workspace argument and without any device,
+ * pinned or event allocation: the partial sums go to a fixed array of
+ * RMW_NOALLOC_SCRATCH_BYTES in device memory that the library holds on
+ * every device it runs on until the process ends.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as rmw_wall_rpy_apply; when the launch shape
+ * rmw_wall_rpy_apply would choose needs more partial-sum space than the
+ * array holds, the ordered kernel runs with longer source chunks (and
+ * fewer right-hand sides per pass) so that it fits; sizes for which even
+ * that does not fit (about 2.7 million blobs per right-hand side and
+ * output block) return RMW_E_LIMIT before any launch.  Every call on one
+ * device shares that device's array: calls that overlap in time on one
+ * device (two streams, two threads) race and must be serialised by the
+ * caller.  Pair-kernel timing records (rmw_profile_enable) are not taken.
+ */
+#define RMW_NOALLOC_SCRATCH_BYTES (64u << 20)
+int rmw_wall_rpy_apply_noalloc(int mask, int prec, const double* r_vectors, const double* force,
+                               const double* torque, double* velocity, double* omega, int64_t n, int k,
+                               int64_t ld, double eta, double a, const double* L, const rmw_tuning* tuning,
+                               void* stream);
+
+/*
  * The same products for the other boundaries of the fluid.  domain:
  *   RMW_DOMAIN_WALL          = rmw_wall_rpy_apply
  *   RMW_DOMAIN_NO_WALL       unbounded RPY, no height shift or damping, any

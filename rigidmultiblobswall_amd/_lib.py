@@ -18,6 +18,7 @@ RMW_UF, RMW_UT, RMW_WF, RMW_WT = 1, 2, 4, 8
 RMW_PREC_F32_F64ACC, RMW_PREC_F32_KAHAN, RMW_PREC_F64, RMW_PREC_F32_BLOCKED = 0, 1, 2, 3
 RMW_E_ARG, RMW_E_HIP, RMW_E_WORKSPACE, RMW_E_NOTFINITE, RMW_E_LIMIT = -1, -2, -3, -4, -5
 RMW_FLAG_CHECK_FINITE = 1
+RMW_NOALLOC_SCRATCH_BYTES = 64 << 20     # include/rmw_hip.h
 
 PRECISIONS = {
     "fp32": RMW_PREC_F32_F64ACC,
@@ -103,6 +104,9 @@ PROTOTYPES = {
                                              ctypes.POINTER(PlanInfo)]),
     "rmw_wall_rpy_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64, ctypes.c_int,
                                           _i64, ctypes.c_double, ctypes.c_double, _hdp, _tp, _vp, _sz, _vp]),
+    "rmw_wall_rpy_apply_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64,
+                                                  ctypes.c_int, _i64, ctypes.c_double, ctypes.c_double, _hdp, _tp,
+                                                  _vp]),
     "rmw_wall_rpy_ragged_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int,
                                                           ctypes.c_int, _tp, _szp]),
     "rmw_wall_rpy_ragged_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64p,

@@ -1140,14 +1140,17 @@ static const void* pair_kernel_ptr(int mask, int prec, int K)
 // device at once (register and LDS limited), cached per device.
 static int resident_slots(int mask, int prec, int K)
 {
+  // fixed table (no heap use on the launch path): one entry per
+  // (device, mask, prec, K) seen, first come first kept
   static std::mutex mu;
-  static std::vector<std::pair<int64_t, int>> cache;
+  static std::pair<int64_t, int> cache[256];
+  static int used = 0;
   const DeviceProps& dp = device_props();
   const int64_t key = ((((int64_t)dp.device * 16 + mask) * 8 + prec) * 8) + K;
   {
     std::lock_guard<std::mutex> lk(mu);
-    for (auto& e : cache)
-      if (e.first == key) return e.second;
+    for (int i = 0; i < used; ++i)
+      if (cache[i].first == key) return cache[i].second;
   }
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pair_kernel_ptr(mask, prec, K), kBlock, 0) !=
@@ -1156,7 +1159,7 @@ static int resident_slots(int mask, int prec, int K)
     per_cu = 1;
   const int slots = per_cu * std::max(dp.cu_count, 1);
   std::lock_guard<std::mutex> lk(mu);
-  cache.emplace_back(key, slots);
+  if (used < 256) cache[used++] = {key, slots};
   return slots;
 }
 
@@ -1517,13 +1520,17 @@ static void dispatch_mutual(int domain, int mask, int prec, int K, const MutPara
   }
 }
 
+// Set while the calling thread runs rmw_wall_rpy_apply_noalloc: its
+// launches take no timing records (each record creates two HIP events).
+static thread_local bool tl_no_records = false;
+
 // Pair-kernel timing records (rmw_profile_*).
 void prof_begin(ProfRec& rec, bool& on, hipStream_t st, int kind)
 {
   rec.kind = kind;
   {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    on = g_prof_on;
+    on = g_prof_on && !tl_no_records;
   }
   if (on) {
     RMW_CHECK_HIP(hipEventCreate(&rec.a));
@@ -1750,6 +1757,66 @@ static void run_plan(const Plan& pl, const double* r, const double* f, const dou
   check_outputs(pl, HU ? vel : nullptr, HW ? omg : nullptr, k, stride, base, st);
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation entry point: the partial sums live in a fixed device array
+// instead of a caller workspace.  Each device holds its own copy of the
+// array (for the life of the process); its address on the current device
+// is looked up once and kept.  Nothing in it must start at zero: every
+// partial-sum slot a reduce reads is written by the pair kernel of the same
+// call first.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[RMW_NOALLOC_SCRATCH_BYTES];
+
+static void* scratch_address()
+{
+  static std::mutex mu;
+  static void* ptrs[64] = {};
+  int dev = 0;
+  RMW_CHECK_HIP(hipGetDevice(&dev));
+  RMW_REQUIRE(dev >= 0 && dev < 64, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (ptrs[dev] == nullptr) {
+    void* p = nullptr;
+    RMW_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    RMW_REQUIRE(p != nullptr, "no device address for the scratch array");
+    ptrs[dev] = p;
+  }
+  return ptrs[dev];
+}
+
+// The plan of rmw_wall_rpy_apply for these sizes if its partial sums fit
+// in `cap` bytes; otherwise (unless the tuning requires the symmetric
+// kernel) the ordered kernel with source chunks as long as the fit needs,
+// and fewer right-hand sides per pass if one chunk of k_group is still too
+// large.  Throws RMW_E_LIMIT when nothing fits.
+static Plan plan_within(int mask, int prec, int64_t n, int k, const rmw_tuning* tuning, size_t cap, bool periodic)
+{
+  const int64_t off[2] = {0, n};
+  Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, kDomWall, periodic);
+  if (pl.total <= cap) return pl;
+  const bool sym_required = tuning && tuning->symmetric > 0;
+  if (!sym_required) {
+    const int OC = 3 * (((mask & (kUF | kUT)) ? 1 : 0) + ((mask & (kWF | kWT)) ? 1 : 0));
+    rmw_tuning t = tuning ? *tuning : rmw_tuning{};
+    t.symmetric = -1;
+    for (int g : {4, 2, 1}) {
+      if (t.rhs_group != 0 && g > t.rhs_group) continue;
+      const int kg = group_k(k, g);
+      const double per_chunk = (double)kg * (double)n * OC * sizeof(double);
+      const int64_t chunks = (int64_t)std::floor((double)(cap - 256) / per_chunk);
+      if (chunks < 1) continue;
+      const int64_t len = (n + chunks - 1) / chunks;
+      t.chunk_len = std::max<int64_t>(t.chunk_len, (len + kChunkQuantum - 1) / kChunkQuantum * kChunkQuantum);
+      t.rhs_group = kg;
+      Plan p2 = make_plan(mask, prec, off, 1, k, &t, false, kDomWall, periodic);
+      if (p2.total <= cap) return p2;
+    }
+  }
+  throw Error(RMW_E_LIMIT, "rmw_wall_rpy_apply_noalloc: n = " + std::to_string(n) + ", k = " + std::to_string(k) +
+                               " needs " + std::to_string(pl.total) + " bytes of partial sums, more than the " +
+                               std::to_string((size_t)RMW_NOALLOC_SCRATCH_BYTES) + "-byte scratch array");
+}
+
 }  // namespace rmw
 
 // ---------------------------------------------------------------------------
@@ -1800,6 +1867,30 @@ extern "C" int rmw_wall_rpy_apply(int mask, int prec, const double* r_vectors, c
     Plan pl = make_plan(mask, prec, off, 1, k, tuning, false, kDomWall, has_images(L));
     run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, tuning, workspace,
              workspace_bytes, (hipStream_t)stream, false);
+  });
+}
+
+extern "C" int rmw_wall_rpy_apply_noalloc(int mask, int prec, const double* r_vectors, const double* force,
+                                          const double* torque, double* velocity, double* omega, int64_t n, int k,
+                                          int64_t ld, double eta, double a, const double* L,
+                                          const rmw_tuning* tuning, void* stream)
+{
+  struct NoRecords {
+    NoRecords() { tl_no_records = true; }
+    ~NoRecords() { tl_no_records = false; }
+  } guard;
+  return guarded([&] {
+    RMW_REQUIRE(n >= 0, "n must be >= 0");
+    RMW_REQUIRE(!(tuning && (tuning->flags & RMW_FLAG_CHECK_FINITE)),
+                "the finite check waits for the stream; use rmw_wall_rpy_apply for it");
+    Plan pl = plan_within(mask, prec, n, k, tuning, RMW_NOALLOC_SCRATCH_BYTES, has_images(L));
+    if (pl.ntot == 0) return;
+    // the plan may differ from the caller's tuning (longer ordered chunks);
+    // run_plan reads only max_wgs and rhs_group from it
+    rmw_tuning t = tuning ? *tuning : rmw_tuning{};
+    if (!pl.mut_ok) t.rhs_group = pl.kmax;
+    run_plan(pl, r_vectors, force, torque, velocity, omega, k, ld, eta, a, L, &t, scratch_address(),
+             RMW_NOALLOC_SCRATCH_BYTES, (hipStream_t)stream, false);
   });
 }
 

@@ -66,3 +66,16 @@ def test_finite_check_passes_valid_products(gpu):
                                 check_finite=True)
         assert torch.equal(u0, u1) and torch.equal(w0, w1)
 
+
+def test_noalloc_refuses_the_finite_check(gpu):
+    import ctypes
+    lib = _lib.load()
+    n = 64
+    r = torch.from_numpy(_suspension(n, 1, coincident=False)).to(gpu)
+    f = torch.zeros((n, 3), dtype=torch.float64, device=gpu)
+    u = torch.zeros((n, 3), dtype=torch.float64, device=gpu)
+    t = _lib.Tuning(0, 0, 0, 0, _lib.RMW_FLAG_CHECK_FINITE)
+    rc = lib.rmw_wall_rpy_apply_noalloc(RMW_UF, 3, r.data_ptr(), f.data_ptr(), None, u.data_ptr(), None, n, 1, 0,
+                                        1.0, 0.5, (ctypes.c_double * 3)(0, 0, 0), ctypes.byref(t),
+                                        torch.cuda.current_stream().cuda_stream)
+    assert rc == _lib.RMW_E_ARG
