@@ -588,11 +588,13 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       __syncthreads();
       const bool diag = S == T;
       // full off-diagonal tile pair: every lane's pair is taken, no masks.
-      // Only the unbounded and free-surface kernels take this path (8 % and
-      // 1.5 % faster at 32768 blobs); with the wall terms (wall, in plane)
-      // the unmasked loop, 8 fewer VALU per pair, measured 3 % slower than
-      // the masked one on the same device.
-      constexpr bool kUnmasked = DOM == kDomNoWall || DOM == kDomFreeSurface;
+      // Measured on MI355X with every kernel in one process, alternating:
+      // the unmasked loop (8 fewer VALU per pair) makes the wall UF product
+      // 17 % faster at 32768 blobs (fp32_blocked; 4 % in FP64) and UF + UT
+      // 3 % faster; the two-output (grand) kernels keep the masked loop
+      // only, as the second copy of their loop costs registers (K = 1:
+      // 1 % slower; matrix form: 40 % slower).
+      constexpr bool kUnmasked = DOM == kDomNoWall || DOM == kDomFreeSurface || (!kMat && OC == 3);
       const bool full = kUnmasked && !diag && cnt == kBlock && (T + 1) * kBlock <= Sy.n;
       const int bb0 = S == S0 ? (int)(blk0 - 4 * S) : 0;
       const int bb1 = blk1 - 4 * S < kBlock / 64 ? (int)(blk1 - 4 * S) : kBlock / 64;
@@ -642,31 +644,36 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
               for (int r = 0; r < 3; ++r) {
                 // target side: u_i += UF f_j + UT t_j, w_i += WF f_j + WT t_j
                 // source side: u_j += UF^T f_i + WF^T t_i, w_j += UT^T f_i + WT^T t_i
+                // (where the accumulator allows, its running sum seeds the
+                // target-side chain: no separate add)
+                constexpr bool kChain = Acc<ACC>::kChain;
                 if constexpr (BT::HU) {
                   R x = R(0), y = R(0);
+                  if constexpr (kChain) x = u[k][r].run();
                   if constexpr (UF) {
-                    x = fma_(Mu.m[r][0], fj[0], fma_(Mu.m[r][1], fj[1], Mu.m[r][2] * fj[2]));
+                    x = fma_(Mu.m[r][0], fj[0], fma_(Mu.m[r][1], fj[1], kChain ? fma_(Mu.m[r][2], fj[2], x) : Mu.m[r][2] * fj[2]));
                     y = fma_(Mu.m[0][r], fi[k][0], fma_(Mu.m[1][r], fi[k][1], Mu.m[2][r] * fi[k][2]));
                   }
                   if constexpr (UT)
                     x = fma_(Mc.m[r][0], tj[0], fma_(Mc.m[r][1], tj[1], r == 2 ? x : fma_(Mc.m[r][2], tj[2], x)));
                   if constexpr (WF)
                     y = fma_(Md.m[0][r], ti[k][0], fma_(Md.m[1][r], ti[k][1], r == 2 ? y : fma_(Md.m[2][r], ti[k][2], y)));
-                  u[k][r].add(x);
+                  if constexpr (kChain) u[k][r].run() = x; else u[k][r].add(x);
                   acc_j(ja[k][r], (JT)y);
                 }
                 if constexpr (BT::HW) {
                   constexpr int o = BT::HU ? 3 : 0;
                   R x = R(0), y = R(0);
+                  if constexpr (kChain) x = w[k][r].run();
                   if constexpr (WT) {
-                    x = fma_(Mw.m[r][0], tj[0], fma_(Mw.m[r][1], tj[1], Mw.m[r][2] * tj[2]));
+                    x = fma_(Mw.m[r][0], tj[0], fma_(Mw.m[r][1], tj[1], kChain ? fma_(Mw.m[r][2], tj[2], x) : Mw.m[r][2] * tj[2]));
                     y = fma_(Mw.m[0][r], ti[k][0], fma_(Mw.m[1][r], ti[k][1], Mw.m[2][r] * ti[k][2]));
                   }
                   if constexpr (WF)
                     x = fma_(Md.m[r][0], fj[0], fma_(Md.m[r][1], fj[1], r == 2 ? x : fma_(Md.m[r][2], fj[2], x)));
                   if constexpr (UT)
                     y = fma_(Mc.m[0][r], fi[k][0], fma_(Mc.m[1][r], fi[k][1], r == 2 ? y : fma_(Mc.m[2][r], fi[k][2], y)));
-                  w[k][r].add(x);
+                  if constexpr (kChain) w[k][r].run() = x; else w[k][r].add(x);
                   acc_j(ja[k][o + r], (JT)y);
                 }
               }
