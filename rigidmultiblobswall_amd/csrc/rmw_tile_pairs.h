@@ -4,28 +4,37 @@
 // lane rotation that carries source-side sums across a wave.
 #pragma once
 
-#include <hip/hip_runtime.h>
-
 #include <cstdint>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define RMW_TP __host__ __device__ __forceinline__
+#else
+#define RMW_TP inline   // host build of the schedule for the CPU tests
+#endif
 
 namespace rmw {
 
-__host__ __device__ __forceinline__ int64_t tri_index(int64_t S, int64_t T) { return S * (S + 1) / 2 + T; }
+RMW_TP int64_t tri_index(int64_t S, int64_t T) { return S * (S + 1) / 2 + T; }
 
-// sum_{m=1}^{M} ceil(m / C)
-__host__ __device__ __forceinline__ int64_t ceil_sum(int64_t M, int64_t C)
+// sum_{m=1}^{M} ceil(m / C).  M and C are tile counts (< 2^31): the
+// quotient is a 32-bit division (a 64-bit one expands to ~40 instructions,
+// and the work-item decode below evaluates this in a binary search).
+RMW_TP int64_t ceil_sum(int64_t M, int64_t C)
 {
-  const int64_t q = M / C, r = M - q * C;
+  const uint32_t m = (uint32_t)M, c = (uint32_t)C;
+  const int64_t q = m / c, r = m - (uint32_t)q * c;
   return C * q * (q + 1) / 2 + r * (q + 1);
 }
 
 // Work items of target tiles 0..T-1 of a system (target tile t has
 // ceil((n_t - t) / C) runs of C source tiles): closed form, no table.
-__host__ __device__ __forceinline__ int64_t items_before(int64_t T, int64_t n_t, int64_t C)
+RMW_TP int64_t items_before(int64_t T, int64_t n_t, int64_t C)
 {
   return ceil_sum(n_t, C) - ceil_sum(n_t - T, C);
 }
 
+#ifdef __HIPCC__
 // Lane l receives lane (l + 1) mod 64 (DPP wave_rol:1).
 // (bound_ctrl set: every lane of the wave is active where this is used, so it
 // changes nothing, and it lets the rotate fuse into the add that consumes it,
@@ -40,21 +49,22 @@ __device__ __forceinline__ double rol1(double v)
   const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
+#endif
 
 // Work items counted in runs of C 64-source blocks (C = 1, 2 or a multiple
 // of 4; four blocks to a tile): target tile t has ceil(4 (n_t - t) / C)
 // runs of blocks from its own tile on.  sum_{m=1}^{M} ceil(4 m / C):
-__host__ __device__ __forceinline__ int64_t block_ceil_sum(int64_t M, int64_t C)
+RMW_TP int64_t block_ceil_sum(int64_t M, int64_t C)
 {
-  return C % 4 == 0 ? ceil_sum(M, C / 4) : (4 / C) * M * (M + 1) / 2;
+  return (C & 3) == 0 ? ceil_sum(M, C >> 2) : (C == 1 ? 4 : 2) * (M * (M + 1) >> 1);
 }
 
-__host__ __device__ __forceinline__ int64_t items_before_blocks(int64_t T, int64_t n_t, int64_t C)
+RMW_TP int64_t items_before_blocks(int64_t T, int64_t n_t, int64_t C)
 {
   return block_ceil_sum(n_t, C) - block_ceil_sum(n_t - T, C);
 }
 
-__device__ __forceinline__ int64_t pair_target_tile_blocks(int64_t local, int64_t n_t, int64_t C)
+RMW_TP int64_t pair_target_tile_blocks(int64_t local, int64_t n_t, int64_t C)
 {
   int64_t lo = 0, hi = n_t - 1;
   while (lo < hi) {
@@ -64,7 +74,7 @@ __device__ __forceinline__ int64_t pair_target_tile_blocks(int64_t local, int64_
   return lo;
 }
 
-__device__ __forceinline__ int64_t pair_target_tile(int64_t local, int64_t n_t, int64_t C)
+RMW_TP int64_t pair_target_tile(int64_t local, int64_t n_t, int64_t C)
 {
   int64_t lo = 0, hi = n_t - 1;
   while (lo < hi) {

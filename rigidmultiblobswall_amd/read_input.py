@@ -29,8 +29,79 @@ def _strip(line: str) -> str:
     return line.strip()
 
 
-def _floats(text, default: str) -> np.ndarray:
-    return np.array([float(v) for v in (text or default).split()], dtype=np.float64)
+def _num(kind):
+    return lambda text, default: kind(text or default)
+
+
+def _array(kind):
+    return lambda text, default: np.array([kind(v) for v in (text or default).split()],
+                                          dtype=np.float64 if kind is float else None)
+
+
+def _raw(text, default):
+    return text if text is not None else default
+
+
+def _flag(text, default):
+    return str(text or default) == "True"
+
+
+# The option table of the reference's input files (name, default and how the
+# value is read: read_input/read_input.py:48-102): attribute, key in the
+# file, converter, default.  Empty values take the default, as there.
+_OPTIONS = [
+    ("n_steps", "n_steps", _num(int), 0),
+    ("initial_step", "initial_step", _num(int), 0),
+    ("n_save", "n_save", _num(int), 1),
+    ("n_relaxation", "n_relaxation", _num(int), 0),
+    ("dt", "dt", _num(float), 0.0),
+    ("eta", "eta", _num(float), 1.0),
+    ("g", "g", _num(float), 1.0),
+    ("theta", "tilt_angle", _num(float), 0.0),
+    ("blob_radius", "blob_radius", _num(float), 1.0),
+    ("tracer_radius", "tracer_radius", _num(float), 0.0),
+    ("kT", "kT", _num(float), 1.0),
+    ("scheme", "scheme", _num(str), "deterministic_forward_euler"),
+    ("output_name", "output_name", _num(str), "run"),
+    ("random_state", "random_state", _raw, None),
+    ("seed", "seed", _raw, None),
+    ("repulsion_strength_wall", "repulsion_strength_wall", _num(float), 1.0),
+    ("debye_length_wall", "debye_length_wall", _num(float), 1.0),
+    ("mobility_blobs_implementation", "mobility_blobs_implementation", _num(str), "python"),
+    ("mobility_vector_prod_implementation", "mobility_vector_prod_implementation", _num(str), "python"),
+    ("repulsion_strength", "repulsion_strength", _num(float), 1.0),
+    ("debye_length", "debye_length", _num(float), 1.0),
+    ("blob_blob_force_implementation", "blob_blob_force_implementation", _num(str), "None"),
+    ("body_body_force_torque_implementation", "body_body_force_torque_implementation", _num(str), "None"),
+    ("save_body_mobility", "save_body_mobility", _num(str), "False"),
+    ("save_blobs_mobility", "save_blobs_mobility", _num(str), "False"),
+    ("save_velocities", "save_velocities", _num(str), "False"),
+    ("slip_file", "slip_file", _raw, None),
+    ("force_file", "force_file", _raw, None),
+    ("velocity_file", "velocity_file", _raw, None),
+    ("solver_tolerance", "solver_tolerance", _num(float), 1e-08),
+    ("nonlinear_solver_tolerance", "nonlinear_solver_tolerance", _num(float), 1e-08),
+    ("rf_delta", "rf_delta", _num(float), 1e-03),
+    ("save_clones", "save_clones", _num(str), "one_file_per_step"),
+    ("periodic_length", "periodic_length", _array(float), "0 0 0"),
+    ("omega_one_roller", "omega_one_roller", _array(float), "0 0 0"),
+    ("free_kinematics", "free_kinematics", _num(str), "True"),
+    ("hydro_interactions", "hydro_interactions", _num(int), 1),
+    ("update_PC", "update_PC", _num(int), 1),
+    ("domain", "domain", _num(str), "single_wall"),
+    ("green_particles", "green_particles", _array(int), "0 0"),
+    ("cells", "cells", _array(int), "1 1"),
+    ("sample_HydroGrid", "sample_HydroGrid", _num(int), 1),
+    ("save_HydroGrid", "save_HydroGrid", _num(int), 0),
+    ("call_HydroGrid", "call_HydroGrid", _flag, "False"),
+    ("repulsion_strength_firm", "repulsion_strength_firm", _num(float), 0.0),
+    ("firm_delta", "firm_delta", _num(float), 1e-02),
+    ("Lub_Cut", "Lub_Cut", _num(float), 4.5),
+    ("zmin", "zmin", _num(float), 0.0),
+    ("zmax", "zmax", _num(float), 1e7),
+    ("domType", "domType", _num(str), "RPB"),
+    ("diffusion_coefficient", "diffusion_coefficient", _num(float), 1.0),
+]
 
 
 class ReadInput:
@@ -54,58 +125,9 @@ class ReadInput:
                     counts[option] += 1
                     option += str(idx)
                 self.options[option] = value
+        for attr, key, conv, default in _OPTIONS:
+            setattr(self, attr, conv(self.options.get(key), default))
         o = self.options.get
-        self.n_steps = int(o("n_steps") or 0)
-        self.initial_step = int(o("initial_step") or 0)
-        self.n_save = int(o("n_save") or 1)
-        self.n_relaxation = int(o("n_relaxation") or 0)
-        self.dt = float(o("dt") or 0.0)
-        self.eta = float(o("eta") or 1.0)
-        self.g = float(o("g") or 1.0)
-        self.theta = float(o("tilt_angle") or 0.0)
-        self.blob_radius = float(o("blob_radius") or 1.0)
-        self.tracer_radius = float(o("tracer_radius") or 0.0)
-        self.kT = float(o("kT") or 1.0)
-        self.scheme = str(o("scheme") or "deterministic_forward_euler")
-        self.output_name = str(o("output_name") or "run")
-        self.random_state = o("random_state")
-        self.seed = o("seed")
-        self.repulsion_strength_wall = float(o("repulsion_strength_wall") or 1.0)
-        self.debye_length_wall = float(o("debye_length_wall") or 1.0)
-        self.mobility_blobs_implementation = str(o("mobility_blobs_implementation") or "python")
-        self.mobility_vector_prod_implementation = str(o("mobility_vector_prod_implementation") or "python")
-        self.repulsion_strength = float(o("repulsion_strength") or 1.0)
-        self.debye_length = float(o("debye_length") or 1.0)
-        self.blob_blob_force_implementation = str(o("blob_blob_force_implementation") or "None")
-        self.body_body_force_torque_implementation = str(o("body_body_force_torque_implementation") or "None")
-        self.save_body_mobility = str(o("save_body_mobility") or "False")
-        self.save_blobs_mobility = str(o("save_blobs_mobility") or "False")
-        self.save_velocities = str(o("save_velocities") or "False")
-        self.slip_file = o("slip_file")
-        self.force_file = o("force_file")
-        self.velocity_file = o("velocity_file")
-        self.solver_tolerance = float(o("solver_tolerance") or 1e-08)
-        self.nonlinear_solver_tolerance = float(o("nonlinear_solver_tolerance") or 1e-08)
-        self.rf_delta = float(o("rf_delta") or 1e-03)
-        self.save_clones = str(o("save_clones") or "one_file_per_step")
-        self.periodic_length = _floats(o("periodic_length"), "0 0 0")
-        self.omega_one_roller = _floats(o("omega_one_roller"), "0 0 0")
-        self.free_kinematics = str(o("free_kinematics") or "True")
-        self.hydro_interactions = int(o("hydro_interactions") or 1)
-        self.update_PC = int(o("update_PC") or 1)
-        self.domain = str(o("domain") or "single_wall")
-        self.green_particles = np.array([int(v) for v in (o("green_particles") or "0 0").split()])
-        self.cells = np.array([int(v) for v in (o("cells") or "1 1").split()])
-        self.sample_HydroGrid = int(o("sample_HydroGrid") or 1)
-        self.save_HydroGrid = int(o("save_HydroGrid") or 0)
-        self.call_HydroGrid = str(o("call_HydroGrid") or "False") == "True"
-        self.repulsion_strength_firm = float(o("repulsion_strength_firm") or 0.0)
-        self.firm_delta = float(o("firm_delta") or 1e-02)
-        self.Lub_Cut = float(o("Lub_Cut") or 4.5)
-        self.zmin = float(o("zmin") or 0)
-        self.zmax = float(o("zmax") or 1e7)
-        self.domType = str(o("domType") or "RPB")
-        self.diffusion_coefficient = float(o("diffusion_coefficient") or 1.0)
 
         self.num_free_bodies = counts["structure"]
         self.structures: List[List[str]] = []

@@ -165,3 +165,47 @@ extern "C" int pt_masked_blocks_f64(const double* ri, const double* rj, int on, 
 {
   return masked_blocks<double>(ri, rj, on, m_out);
 }
+
+// ---------------------------------------------------------------------------
+// Work-item schedule of the symmetric kernels (rmw_tile_pairs.h): count the
+// (target tile, source block or tile) pairs that a decode of every item
+// visits other than exactly once; 0 when each unordered pair with source
+// >= target is covered once and nothing else.
+#include <vector>
+#include "../../rigidmultiblobswall_amd/csrc/rmw_tile_pairs.h"
+
+extern "C" int64_t tp_block_cover_errors(int64_t n_t, int64_t C)
+{
+  const int64_t nb = 4 * n_t, items = items_before_blocks(n_t, n_t, C);
+  std::vector<int> cov((size_t)(n_t * nb), 0);
+  int64_t bad = 0;
+  for (int64_t local = 0; local < items; ++local) {
+    const int64_t T = pair_target_tile_blocks(local, n_t, C);
+    const int64_t b0 = 4 * T + (local - items_before_blocks(T, n_t, C)) * C;
+    const int64_t b1 = b0 + C < nb ? b0 + C : nb;
+    if (T < 0 || T >= n_t || b0 < 4 * T || b0 >= nb) { ++bad; continue; }
+    for (int64_t b = b0; b < b1; ++b) ++cov[(size_t)(T * nb + b)];
+  }
+  for (int64_t T = 0; T < n_t; ++T)
+    for (int64_t b = 0; b < nb; ++b) bad += cov[(size_t)(T * nb + b)] != (b >= 4 * T ? 1 : 0);
+  return bad;
+}
+
+extern "C" int64_t tp_tile_cover_errors(int64_t n_t, int64_t C)
+{
+  const int64_t items = items_before(n_t, n_t, C);
+  std::vector<int> cov((size_t)(n_t * n_t), 0);
+  int64_t bad = 0;
+  for (int64_t it = 0; it < items; ++it) {
+    const int64_t T = pair_target_tile(it, n_t, C);
+    const int64_t S0 = T + (it - items_before(T, n_t, C)) * C;
+    const int64_t S1 = S0 + C < n_t ? S0 + C : n_t;
+    if (T < 0 || T >= n_t || S0 < T || S0 >= n_t) { ++bad; continue; }
+    for (int64_t S = S0; S < S1; ++S) ++cov[(size_t)(T * n_t + S)];
+  }
+  for (int64_t T = 0; T < n_t; ++T)
+    for (int64_t S = 0; S < n_t; ++S) bad += cov[(size_t)(T * n_t + S)] != (S >= T ? 1 : 0);
+  return bad;
+}
+
+extern "C" int64_t tp_ceil_sum(int64_t M, int64_t C) { return ceil_sum(M, C); }

@@ -237,3 +237,25 @@ def test_masked_geometry_gives_exact_zeros():
             if np.any(ri != rj):
                 assert fn(P(ri), P(rj), 1, P(on)) == 324
                 assert np.all(np.isfinite(on)) and np.abs(on).max() > 0.0
+
+
+def test_work_item_schedule_covers_every_tile_pair_once():
+    """The work-item decode of the symmetric kernels (rmw_tile_pairs.h,
+    host build): items of C 64-source blocks (C = 1, 2 or a multiple of 4)
+    and of C whole tiles cover every unordered (target tile, source block /
+    tile >= target) pair exactly once and nothing else, for every size up to
+    130 tiles; the closed-form run count equals the direct sum."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(REPO, "tests", "helpers", "libpair_terms_host.so"))
+    for fn in (lib.tp_block_cover_errors, lib.tp_tile_cover_errors, lib.tp_ceil_sum):
+        fn.argtypes = [ctypes.c_int64, ctypes.c_int64]
+        fn.restype = ctypes.c_int64
+    for C in (1, 2, 4, 8, 12, 44):
+        bad = [n_t for n_t in range(1, 131) if lib.tp_block_cover_errors(n_t, C)]
+        assert not bad, (C, bad[:5])
+    for C in range(1, 21):
+        bad = [n_t for n_t in range(1, 101) if lib.tp_tile_cover_errors(n_t, C)]
+        assert not bad, (C, bad[:5])
+    for C in range(1, 41):
+        for M in range(0, 301, 7):
+            assert lib.tp_ceil_sum(M, C) == sum(-(-m // C) for m in range(1, M + 1))
