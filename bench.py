@@ -452,7 +452,8 @@ def lanczos_bench(args, log, ragged: bool):
                                  "pair_kernel_ms": tw / args.steps,
                                  "lanczos_iterations_max": int(its_np.max()) if its_np.size else 0}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": None if ragged else load_traffic("c3lz", prec, kernel="grand_k4_kernel"),
                      "kernel": "wall-RPY grand pair kernel" + (" (ragged)" if ragged else f" ({k} RHS)"),
                      "kernel_mean_ms": mw, "kernel_launches": nw, "flop_per_pair": 314 + 72 * k},
     }
