@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "rmw_accum.h"
 #include "rmw_internal.h"
@@ -295,13 +296,18 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
       }
       __syncthreads();
       const bool diag = S == Tt;
+      // full off-diagonal tile pair: every lane's pair is taken on both
+      // sides, so the step loop runs without its selects
+      const bool full = !diag && cnt == kRBlock && (Tt + 1) * kRBlock <= p.n;
       for (int b = 0; b < kRBlock / 64 && 64 * b < cnt; ++b) {
         JT ja[3] = {JT(0), JT(0), JT(0)};
+        auto steps = [&](auto masked_c) {
+        constexpr bool MASKED = decltype(masked_c)::value;
         for (int st = 0; st < 64; ++st) {
           const int jl = 64 * b + ((lane + st) & 63);
           // the self pair (the RPY self mobility and its image) once, on the target side
-          const bool on = valid && jl < cnt && (!diag || jl >= il);
-          const bool on_j = on && (!diag || jl != il);
+          const bool on = !MASKED || (valid && jl < cnt && (!diag || jl >= il));
+          const bool on_j = !MASKED || (on && (!diag || jl != il));
           T dx, dy, dz;
           if constexpr (sizeof(T) == 4) {
             dx = (T)(xi - s_x[jl]); dy = (T)(yi - s_y[jl]); dz = (T)(zi - s_z[jl]);
@@ -313,12 +319,21 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
           const CoefRadii<T> c = coef_radii<T>(dx, dy, dz, Rz, zir, zj, ar, s_b[jl], inva, s_ib[jl], BND);
           const Vec3<T> vi = apply_radii<T>(dx, dy, dz, Rz, c, s_f[0][jl], s_f[1][jl], s_f[2][jl]);
           const Vec3<T> vj = apply_radii_t<T>(dx, dy, dz, Rz, c, fi[0], fi[1], fi[2]);
-          u[0].add(on ? vi.x : T(0));
-          u[1].add(on ? vi.y : T(0));
-          u[2].add(on ? vi.z : T(0));
-          ja[0] = rol1(ja[0]) + (on_j ? (JT)vj.x : JT(0));
-          ja[1] = rol1(ja[1]) + (on_j ? (JT)vj.y : JT(0));
-          ja[2] = rol1(ja[2]) + (on_j ? (JT)vj.z : JT(0));
+          if constexpr (MASKED) {
+            u[0].add(on ? vi.x : T(0));
+            u[1].add(on ? vi.y : T(0));
+            u[2].add(on ? vi.z : T(0));
+            ja[0] = rol1(ja[0]) + (on_j ? (JT)vj.x : JT(0));
+            ja[1] = rol1(ja[1]) + (on_j ? (JT)vj.y : JT(0));
+            ja[2] = rol1(ja[2]) + (on_j ? (JT)vj.z : JT(0));
+          } else {
+            u[0].add(vi.x);
+            u[1].add(vi.y);
+            u[2].add(vi.z);
+            ja[0] = rol1(ja[0]) + (JT)vj.x;
+            ja[1] = rol1(ja[1]) + (JT)vj.y;
+            ja[2] = rol1(ja[2]) + (JT)vj.z;
+          }
           if ((st & (kSub - 1)) == kSub - 1) {
             u[0].flush(); u[1].flush(); u[2].flush();
             if constexpr (kFoldJ) {
@@ -333,6 +348,8 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
             }
           }
         }
+        };
+        if (full) steps(std::false_type{}); else steps(std::true_type{});
         if constexpr (!kFoldJ) {
           const int jw = (lane + 63) & 63;
 #pragma unroll
