@@ -51,8 +51,9 @@ typedef struct rmw_tuning {
   int32_t rhs_group;  /* right-hand sides per pair-kernel pass: 1, 2 or 4 (0 = auto) */
   int32_t symmetric;  /* each unordered pair evaluated once for both blobs
                          (UF_ji = UF_ij^T, UT_ji = WF_ij^T, WF_ji = UT_ij^T, WT_ji = WT_ij^T):
-                         the UF block alone, UF + UT (wall) or all four blocks (wall,
-                         no wall); above the wall also with pseudo-periodic images, one
+                         the UF block alone, all four blocks (wall, no wall), UF + UT
+                         or one of UT, WF, WT alone (wall, one right-hand side per
+                         pass); above the wall also with pseudo-periodic images, one
                          right-hand side per pass; the no-wall, in-plane and
                          free-surface domains without images.  0 = when the tile
                          pairs fill the device, 1 = required (error otherwise),

@@ -10,8 +10,9 @@ sources.  Three launch parameters are tuned (``rmw_tuning``):
              workgroup per work item, ragged batches default to 8 per CU);
   rhs_group  right-hand sides per pass (1, 2 or 4): more share the
              geometry of a pair, but need more registers;
-  symmetric  for the UF block alone or all four blocks: each unordered
-             pair once for both blobs (on) or the ordered kernel (off).
+  symmetric  for the blocks that have it (UF, UF + UT, UT, WF or WT alone,
+             all four): each unordered pair once for both blobs (on) or the
+             ordered kernel (off).
 
 `tune` times candidates with HIP events on the launch stream (whole
 product: pair kernel + reduce) in three stages -- every chunk length (with

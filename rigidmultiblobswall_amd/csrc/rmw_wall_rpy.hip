@@ -1239,15 +1239,20 @@ static Plan make_plan(int mask, int prec, const int64_t* offsets, int nsys, int 
   pl.total_items = item;
   pl.partial_doubles = (size_t)part;
 
-  // Symmetric layout: the UF block alone or all four blocks, unless switched off.
+  // Symmetric layout, unless switched off: the UF block alone or all four
+  // blocks; with the wall also UF + UT and the single UT, WF and WT blocks
+  // (UT_ji = WF_ij^T, WT_ji = WT_ij^T: a pair's source side of one of them is
+  // the transposed other), one right-hand side per pass; the other
+  // boundaries: the UF block (and the unbounded grand mobility), one
+  // right-hand side per pass
   const int sym = tuning ? tuning->symmetric : 0;
-  // the other boundaries: the UF block (and the unbounded grand mobility),
-  // one right-hand side per pass
-  pl.mut_ok = (mask == kUF || (mask == (kUF | kUT) && domain == kDomWall) ||
+  const bool single_rot = mask == kUT || mask == kWF || mask == kWT;
+  pl.mut_ok = (mask == kUF || ((mask == (kUF | kUT) || single_rot) && domain == kDomWall) ||
                (mask == (kUF | kUT | kWF | kWT) && (domain == kDomWall || domain == kDomNoWall))) &&
               pl.ntot > 0 && sym >= 0;
   RMW_REQUIRE(!(sym > 0 && !pl.mut_ok),
-              "symmetric evaluation needs the UF block alone, UF + UT (wall) or all four blocks");
+              "symmetric evaluation needs the UF block alone, all four blocks, or (wall) UF + UT or one of "
+              "UT, WF, WT");
   int64_t tile_pairs = 0;
   for (int s = 0; s < nsys; ++s) {
     const int64_t nt = (offsets[s + 1] - offsets[s] + kBlock - 1) / kBlock;
@@ -1260,8 +1265,9 @@ static Plan make_plan(int mask, int prec, const int64_t* offsets, int nsys, int 
     // up to 4 right-hand sides per pass (each pair's blocks formed once as
     // 3x3 matrices and applied to all of them), fewer when the tuning caps it
     const int rc = rhs_cap(tuning);
-    pl.kmax_mut = domain == kDomWall && mask != (kUF | kUT) && !periodic ? group_k(k, rc > 0 ? std::min(rc, 4) : 4)
-                                                                         : 1;
+    pl.kmax_mut = domain == kDomWall && mask != (kUF | kUT) && !single_rot && !periodic
+                      ? group_k(k, rc > 0 ? std::min(rc, 4) : 4)
+                      : 1;
     if (tuning && tuning->chunk_len > 0) {
       // 64 or 128 sources: a quarter or half of a tile pair per work item
       // (finer work for suspensions of few tiles); else whole source tiles
@@ -1505,6 +1511,9 @@ static void dispatch_mutual_periodic(int mask, int prec, const MutParams& q, int
 {
   if (mask == kUF) dispatch_mutual_per_m<kUF>(prec, q, grid, st);
   else if (mask == (kUF | kUT)) dispatch_mutual_per_m<kUF | kUT>(prec, q, grid, st);
+  else if (mask == kUT) dispatch_mutual_per_m<kUT>(prec, q, grid, st);
+  else if (mask == kWF) dispatch_mutual_per_m<kWF>(prec, q, grid, st);
+  else if (mask == kWT) dispatch_mutual_per_m<kWT>(prec, q, grid, st);
   else dispatch_mutual_per_m<kUF | kUT | kWF | kWT>(prec, q, grid, st);
 }
 
@@ -1522,6 +1531,14 @@ static void dispatch_mutual(int domain, int mask, int prec, int K, const MutPara
   } else if (mask == (kUF | kUT)) {
     // u = UF f + UT t: target side UF, UT; source side UF^T f_i + WF^T t_i
     dispatch_mutual_dom<kUF | kUT, kDomWall>(prec, q, grid, st);
+  } else if (mask == kUT) {
+    // u = UT t: target side UT t_j, source side WF^T t_i
+    dispatch_mutual_dom<kUT, kDomWall>(prec, q, grid, st);
+  } else if (mask == kWF) {
+    // w = WF f: target side WF f_j, source side UT^T f_i
+    dispatch_mutual_dom<kWF, kDomWall>(prec, q, grid, st);
+  } else if (mask == kWT) {
+    dispatch_mutual_dom<kWT, kDomWall>(prec, q, grid, st);
   } else {
     dispatch_mutual_m<kUF | kUT | kWF | kWT>(prec, K, q, grid, st);
   }

@@ -9,7 +9,7 @@ import torch
 
 from oracle import oracle as O
 from rigidmultiblobswall_amd import _lib, wall_rpy, workloads
-from rigidmultiblobswall_amd._lib import RMW_UF, RMW_UT
+from rigidmultiblobswall_amd._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
 
 pytestmark = pytest.mark.gpu
 
@@ -100,8 +100,8 @@ def test_symmetric_c3_size_matches_ordered(gpu):
 def test_symmetric_rejected_where_it_does_not_apply(gpu):
     r = _geom(50, 1)
     f = np.ones((50, 3))
-    with pytest.raises(_lib.RmwError):      # only UF alone or all four blocks
-        wall_rpy.apply(RMW_UT, r, torque=f, eta=1.0, a=0.2, tuning=_lib.Tuning(0, 0, 0, 1, 0))
+    with pytest.raises(_lib.RmwError):      # not UT + WT (two outputs from one source vector)
+        wall_rpy.apply(RMW_UT | RMW_WT, r, torque=f, eta=1.0, a=0.2, tuning=_lib.Tuning(0, 0, 0, 1, 0))
     # periodic images: the ordered kernel is used (auto), and agrees with the oracle
     L = np.array([6.0, 5.0, 0.0])
     u, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=1.0, a=0.2, periodic_length=L, precision="fp64")
@@ -329,3 +329,59 @@ def test_symmetric_periodic_ragged_and_items(gpu):
     ordered = run(-1)
     for chunk, wgs in ((0, 0), (64, 0), (512, 5)):
         assert rel(run(1, chunk, wgs), ordered) <= 2e-6, (chunk, wgs)
+
+
+SINGLE_ROT = {RMW_UT: ("UT", "torque"), RMW_WF: ("WF", "force"), RMW_WT: ("WT", "torque")}
+
+
+def _single(mask, r, v, prec, k, sym, L=None, offsets=None, chunk=0):
+    kw = dict(eta=0.9, a=0.3, precision=prec, tuning=_lib.Tuning(chunk, 0, 0, sym, 0))
+    kw[SINGLE_ROT[mask][1]] = v
+    if L is not None:
+        kw["periodic_length"] = np.array(L)
+    if offsets is None:
+        u, w = wall_rpy.apply(mask, r, k=k, **kw)
+    else:
+        u, w = wall_rpy.apply_ragged(mask, r, offsets, **kw)
+    out = u if mask == RMW_UT else w
+    return out.cpu().numpy().reshape(k, -1)
+
+
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp32", "fp32_kahan", "fp64"])
+@pytest.mark.parametrize("mask", [RMW_UT, RMW_WF, RMW_WT])
+def test_symmetric_single_rotational_block(gpu, prec, mask):
+    """One rotational block alone through the symmetric kernel: u = UT t
+    (source side WF_ij^T t_i = UT_ji t_i), w = WF f (source side UT_ij^T f_i)
+    or w = WT t (source side WT_ij^T t_i); against the oracle's block and
+    the ordered kernel, 700 blobs (a ragged last tile), two right-hand sides,
+    without and with pseudo-periodic images."""
+    n, k = 700, 2
+    rng = np.random.default_rng(90 + mask)
+    r = np.column_stack([rng.uniform(-4, 9, n), rng.uniform(0, 5, n), rng.uniform(0.25, 2.0, n)])
+    v = rng.standard_normal((k, n, 3))
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    block = getattr(O, SINGLE_ROT[mask][0])
+    for L in [None] + PERIODIC_L:
+        print(f"case symmetric single block mask={mask} L={L} {prec}", flush=True)
+        sym = _lib.Tuning(0, 0, 0, 1, 0)
+        assert wall_rpy.describe(mask, n, k, prec, periodic_length=L, tuning=sym)["kernel"] == "symmetric"
+        s_ = _single(mask, r, v, prec, k, 1, L)
+        o_ = _single(mask, r, v, prec, k, -1, L)
+        for j in range(k):
+            ref = O.block_apply(block, r, v[j], 0.9, 0.3, periodic_length=L if L is not None else [0, 0, 0])
+            assert rel(s_[j], ref) <= tol, (L, j)
+            assert rel(o_[j], ref) <= tol, (L, j)
+
+
+@pytest.mark.parametrize("mask", [RMW_UT, RMW_WF, RMW_WT])
+def test_symmetric_single_rotational_ragged_and_items(gpu, mask):
+    """Ragged batches (empty and one-blob systems included) and work items
+    of one, two and eight 64-source blocks, against the ordered kernel."""
+    off = np.concatenate([[0], np.cumsum(SIZES_R)]).astype(np.int64)
+    n = int(off[-1])
+    r = np.vstack([_geom(m, 40 + m) for m in SIZES_R if m > 0])
+    v = np.random.default_rng(91).standard_normal((1, n, 3))
+    print(f"case symmetric single block ragged mask={mask}", flush=True)
+    ordered = _single(mask, r, v[0], "fp32_blocked", 1, -1, offsets=off)
+    for chunk in (0, 64, 128, 512):
+        assert rel(_single(mask, r, v[0], "fp32_blocked", 1, 1, offsets=off, chunk=chunk), ordered) <= 2e-6, chunk
