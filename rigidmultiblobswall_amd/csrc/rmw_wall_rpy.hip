@@ -1239,20 +1239,19 @@ static Plan make_plan(int mask, int prec, const int64_t* offsets, int nsys, int 
   pl.total_items = item;
   pl.partial_doubles = (size_t)part;
 
-  // Symmetric layout, unless switched off: the UF block alone or all four
-  // blocks; with the wall also UF + UT and the single UT, WF and WT blocks
-  // (UT_ji = WF_ij^T, WT_ji = WT_ij^T: a pair's source side of one of them is
-  // the transposed other), one right-hand side per pass; the other
-  // boundaries: the UF block (and the unbounded grand mobility), one
-  // right-hand side per pass
+  // Symmetric layout, unless switched off: the UF block alone, UF + UT, the
+  // single UT, WF and WT blocks (UT_ji = WF_ij^T, WT_ji = WT_ij^T: a pair's
+  // source side of one of them is the transposed other) and all four
+  // blocks, as far as the boundary serves them (the free surface: UF only;
+  // in plane: UF, UT, UF + UT); several right-hand sides per pass for UF
+  // and all four blocks above the wall, one otherwise
   const int sym = tuning ? tuning->symmetric : 0;
   const bool single_rot = mask == kUT || mask == kWF || mask == kWT;
-  pl.mut_ok = (mask == kUF || ((mask == (kUF | kUT) || single_rot) && domain == kDomWall) ||
+  pl.mut_ok = (mask == kUF || ((mask == (kUF | kUT) || single_rot) && domain != kDomFreeSurface) ||
                (mask == (kUF | kUT | kWF | kWT) && (domain == kDomWall || domain == kDomNoWall))) &&
               pl.ntot > 0 && sym >= 0;
   RMW_REQUIRE(!(sym > 0 && !pl.mut_ok),
-              "symmetric evaluation needs the UF block alone, all four blocks, or (wall) UF + UT or one of "
-              "UT, WF, WT");
+              "symmetric evaluation needs the UF block alone, UF + UT, one of UT, WF, WT, or all four blocks");
   int64_t tile_pairs = 0;
   for (int s = 0; s < nsys; ++s) {
     const int64_t nt = (offsets[s + 1] - offsets[s] + kBlock - 1) / kBlock;
@@ -1521,9 +1520,15 @@ static void dispatch_mutual(int domain, int mask, int prec, int K, const MutPara
 {
   if (domain == kDomNoWall) {
     if (mask == kUF) dispatch_mutual_dom<kUF, kDomNoWall>(prec, q, grid, st);
+    else if (mask == (kUF | kUT)) dispatch_mutual_dom<kUF | kUT, kDomNoWall>(prec, q, grid, st);
+    else if (mask == kUT) dispatch_mutual_dom<kUT, kDomNoWall>(prec, q, grid, st);
+    else if (mask == kWF) dispatch_mutual_dom<kWF, kDomNoWall>(prec, q, grid, st);
+    else if (mask == kWT) dispatch_mutual_dom<kWT, kDomNoWall>(prec, q, grid, st);
     else dispatch_mutual_dom<kUF | kUT | kWF | kWT, kDomNoWall>(prec, q, grid, st);
   } else if (domain == kDomInPlane) {
-    dispatch_mutual_dom<kUF, kDomInPlane>(prec, q, grid, st);
+    if (mask == kUF) dispatch_mutual_dom<kUF, kDomInPlane>(prec, q, grid, st);
+    else if (mask == (kUF | kUT)) dispatch_mutual_dom<kUF | kUT, kDomInPlane>(prec, q, grid, st);
+    else dispatch_mutual_dom<kUT, kDomInPlane>(prec, q, grid, st);
   } else if (domain == kDomFreeSurface) {
     dispatch_mutual_dom<kUF, kDomFreeSurface>(prec, q, grid, st);
   } else if (mask == kUF) {

@@ -146,11 +146,15 @@ def test_rollers_in_other_domains(gpu, domain, free):
     assert np.linalg.norm(g.locations() - o.r) <= 1e-8 * np.linalg.norm(o.r - r0)
 
 
-@pytest.mark.parametrize("domain,mask", [("no_wall", 1), ("no_wall", 15), ("in_plane", 1), ("free_surface", 1)])
+@pytest.mark.parametrize("domain,mask", [("no_wall", 1), ("no_wall", 15), ("no_wall", 3), ("no_wall", 2),
+                                         ("no_wall", 4), ("no_wall", 8), ("in_plane", 1), ("in_plane", 3),
+                                         ("in_plane", 2), ("free_surface", 1)])
 @pytest.mark.parametrize("prec", ["fp32_blocked", "fp64"])
 def test_symmetric_kernel_in_other_domains(gpu, domain, mask, prec):
     """The symmetric kernel (each unordered pair once) serves the UF block of
-    every boundary and the unbounded grand mobility."""
+    every boundary, UF + UT and the single UT, WF and WT blocks where the
+    boundary has them (unbounded; in plane: UT) and the unbounded grand
+    mobility."""
     from rigidmultiblobswall_amd import _lib
     rng = np.random.default_rng(7)
     n, a, eta = 777, 0.3, 0.8
@@ -161,6 +165,7 @@ def test_symmetric_kernel_in_other_domains(gpu, domain, mask, prec):
         u, w = wall_rpy.apply(mask, r, force=f, torque=t, eta=eta, a=a, precision=prec, domain=domain,
                               tuning=_lib.Tuning(chunk, 0, 0, 1, 0))
         ur, wr = _ref(domain, mask, r, f, t, eta, a, None)
-        assert rel(u.cpu().numpy().ravel(), ur) <= TOL[prec]
-        if mask & RMW_WF:
-            assert rel(w.cpu().numpy().ravel(), wr) <= TOL[prec]
+        if mask & (RMW_UF | RMW_UT):
+            assert rel(u.cpu().numpy().ravel(), ur) <= TOL[prec], chunk
+        if mask & (RMW_WF | RMW_WT):
+            assert rel(w.cpu().numpy().ravel(), wr) <= TOL[prec], chunk
