@@ -508,10 +508,16 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
   // several right-hand sides: form each pair's blocks once as 3x3 matrices
   constexpr bool kMat = K >= 2;
   using JT = typename std::conditional<kFoldJ, float, double>::type;
-  __shared__ double s_x[kBlock], s_y[kBlock], s_z[kBlock];
-  __shared__ R s_zr[kBlock];
-  __shared__ R s_f[BT::NF ? K : 1][3][kBlock];
-  __shared__ R s_t[BT::NT ? K : 1][3][kBlock];
+  // single-output kernels of one right-hand side store each 64-source
+  // block twice (entry 128 b + m, m < 128, holds source 64 b + m mod 64), so
+  // the rotated read of step s, entry 128 b + lane + s, needs no wrap
+  // (1.5 % faster UF product at 32768 blobs; not for the two-output kernel)
+  constexpr bool kDbl = !kMat && !PER && OC == 3;
+  constexpr int kLds = kDbl ? 2 * kBlock : kBlock;
+  __shared__ double s_x[kLds], s_y[kLds], s_z[kLds];
+  __shared__ R s_zr[kLds];
+  __shared__ R s_f[BT::NF ? K : 1][3][kLds];
+  __shared__ R s_t[BT::NT ? K : 1][3][kLds];
   __shared__ double s_jd[kBlock / 64][K][OC][64];  // per-wave source-side sums of one block
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int il = threadIdx.x;
@@ -554,36 +560,49 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       const int64_t s0 = S * kBlock;
       const int cnt = (int)(Sy.n - s0 < kBlock ? Sy.n - s0 : kBlock);
       __syncthreads();  // the previous tile has been consumed
-      if (il < cnt) {
-        const int64_t gj = Sy.b0 + s0 + il;
-        double X, Y, Z, B;
-        load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, DOM);
-        s_x[il] = X; s_y[il] = Y; s_z[il] = Z; s_zr[il] = (R)Z;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if constexpr (BT::NF) {
-            const double* v = p.f + k * p.rhs_stride + 3 * gj;
-            const double c = p.c1 * B;
-            s_f[k][0][il] = (R)(c * v[0]); s_f[k][1][il] = (R)(c * v[1]); s_f[k][2][il] = (R)(pz * c * v[2]);
-          }
-          if constexpr (BT::NT) {
-            const double* v = p.t + k * p.rhs_stride + 3 * gj;
-            const double c = p.c2 * B;
-            s_t[k][0][il] = (R)(c * v[0]); s_t[k][1][il] = (R)(c * v[1]); s_t[k][2][il] = (R)(pz * c * v[2]);
-          }
-        }
-      } else {
-        // slots beyond the system: finite zeros, since pairs not taken are
-        // masked through their geometry (make_geom_on) and still multiply
-        // these values by zero coefficients
-        s_x[il] = 0.0; s_y[il] = 0.0; s_z[il] = 0.0; s_zr[il] = R(0);
+      {
+        // the source record of this thread (finite zeros beyond the system:
+        // pairs not taken are masked through their geometry, make_geom_on,
+        // and still multiply these values by zero coefficients)
+        double X = 0.0, Y = 0.0, Z = 0.0;
+        R fv[BT::NF ? K : 1][3], tv[BT::NT ? K : 1][3];
 #pragma unroll
         for (int k = 0; k < K; ++k)
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            if constexpr (BT::NF) s_f[k][c][il] = R(0);
-            if constexpr (BT::NT) s_t[k][c][il] = R(0);
+            fv[k][c] = R(0);
+            tv[k][c] = R(0);
           }
+        if (il < cnt) {
+          const int64_t gj = Sy.b0 + s0 + il;
+          double B;
+          load_blob(p.r, gj, p.a, p.inva, X, Y, Z, B, DOM);
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if constexpr (BT::NF) {
+              const double* v = p.f + k * p.rhs_stride + 3 * gj;
+              const double c = p.c1 * B;
+              fv[k][0] = (R)(c * v[0]); fv[k][1] = (R)(c * v[1]); fv[k][2] = (R)(pz * c * v[2]);
+            }
+            if constexpr (BT::NT) {
+              const double* v = p.t + k * p.rhs_stride + 3 * gj;
+              const double c = p.c2 * B;
+              tv[k][0] = (R)(c * v[0]); tv[k][1] = (R)(c * v[1]); tv[k][2] = (R)(pz * c * v[2]);
+            }
+          }
+        }
+        const int e0 = kDbl ? 128 * (il >> 6) + (il & 63) : il;
+#pragma unroll
+        for (int e = e0; e <= e0 + (kDbl ? 64 : 0); e += 64) {
+          s_x[e] = X; s_y[e] = Y; s_z[e] = Z; s_zr[e] = (R)Z;
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              if constexpr (BT::NF) s_f[k][c][e] = fv[k][c];
+              if constexpr (BT::NT) s_t[k][c][e] = tv[k][c];
+            }
+        }
       }
       __syncthreads();
       const bool diag = S == T;
@@ -607,9 +626,10 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
         auto steps = [&](auto masked_c) {
         constexpr bool MASKED = decltype(masked_c)::value;
         for (int st = 0; st < 64; ++st) {
-          const int jl = 64 * b + ((lane + st) & 63);
+          const int jl = 64 * b + ((lane + st) & 63);   // the source, within the tile
+          const int jr = kDbl ? 128 * b + lane + st : jl;   // its LDS entry
           const bool on = !MASKED || (valid && jl < cnt && (!diag || jl > il));
-          const R zj = s_zr[jl];
+          const R zj = s_zr[jr];
           // source-side sums move one lane per step (fused into the add; with
           // images, once per step before them)
           auto acc_j = [&](JT& a, JT x) {
@@ -637,8 +657,8 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
               R fj[3], tj[3];
 #pragma unroll
               for (int c = 0; c < 3; ++c) {
-                if constexpr (BT::NF) fj[c] = s_f[k][c][jl];
-                if constexpr (BT::NT) tj[c] = s_t[k][c][jl];
+                if constexpr (BT::NF) fj[c] = s_f[k][c][jr];
+                if constexpr (BT::NT) tj[c] = s_t[k][c][jr];
               }
 #pragma unroll
               for (int r = 0; r < 3; ++r) {
@@ -689,19 +709,19 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
               if (first) { s = v; } else { s.x += v.x; s.y += v.y; s.z += v.z; }
             };
             if constexpr (UF) {
-              put(ui, apply_uf(g, cuf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]), true);
+              put(ui, apply_uf(g, cuf, s_f[k][0][jr], s_f[k][1][jr], s_f[k][2][jr]), true);
               put(uj, apply_uf_t(g, cuf, fi[k][0], fi[k][1], fi[k][2]), true);
             }
             if constexpr (UT) {
-              put(ui, apply_ut(g, cut, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]), !UF);
+              put(ui, apply_ut(g, cut, s_t[k][0][jr], s_t[k][1][jr], s_t[k][2][jr]), !UF);
               put(uj, apply_wf_t(g, cwf, ti[k][0], ti[k][1], ti[k][2]), !UF);
             }
             if constexpr (WF) {
-              put(wi, apply_wf(g, cwf, s_f[k][0][jl], s_f[k][1][jl], s_f[k][2][jl]), true);
+              put(wi, apply_wf(g, cwf, s_f[k][0][jr], s_f[k][1][jr], s_f[k][2][jr]), true);
               put(wj, apply_ut_t(g, cut, fi[k][0], fi[k][1], fi[k][2]), true);
             }
             if constexpr (WT) {
-              put(wi, apply_wt(g, cwt, s_t[k][0][jl], s_t[k][1][jl], s_t[k][2][jl]), !WF);
+              put(wi, apply_wt(g, cwt, s_t[k][0][jr], s_t[k][1][jr], s_t[k][2][jr]), !WF);
               put(wj, apply_wt_t(g, cwt, ti[k][0], ti[k][1], ti[k][2]), !WF);
             }
             if constexpr (BT::HU) {
@@ -726,16 +746,16 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
           };
           if constexpr (!PER) {
             if constexpr (sizeof(R) == 4)
-              pair_apply((R)(Xi - s_x[jl]), (R)(Yi - s_y[jl]), (R)(Zi - s_z[jl]));
+              pair_apply((R)(Xi - s_x[jr]), (R)(Yi - s_y[jr]), (R)(Zi - s_z[jr]));
             else
-              pair_apply(Xi - s_x[jl], Yi - s_y[jl], Zi - s_z[jl]);
+              pair_apply(Xi - s_x[jr], Yi - s_y[jr], Zi - s_z[jr]);
           } else {
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
               for (int c = 0; c < OC; ++c) ja[k][c] = rol1(ja[k][c]);
             // minimum image of the FP64 difference, then the neighbouring boxes
-            double ddx = Xi - s_x[jl], ddy = Yi - s_y[jl], ddz = Zi - s_z[jl];
+            double ddx = Xi - s_x[jr], ddy = Yi - s_y[jr], ddz = Zi - s_z[jr];
             if (pbox[0]) ddx -= image_count(ddx, p.L[0]) * p.L[0];
             if (pbox[1]) ddy -= image_count(ddy, p.L[1]) * p.L[1];
             if (pbox[2]) ddz -= image_count(ddz, p.L[2]) * p.L[2];
