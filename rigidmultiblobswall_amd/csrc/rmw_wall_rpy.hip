@@ -769,12 +769,21 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
             if constexpr (kFoldJ) {
               // lane l holds the sum of source (l + st) mod 64 of the block
               const int jw = (lane + st) & 63;
+              // every read first, then the adds and writes: one LDS round
+              // trip per fold (at the first fold of a block the reads return
+              // stale values, a previous block's sums or uninitialised
+              // memory, which the select drops)
+              double prev[K][OC];
+#pragma unroll
+              for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < OC; ++c) prev[k][c] = s_jd[wv][k][c][jw];
+              const bool first = st <= kFold;
 #pragma unroll
               for (int k = 0; k < K; ++k)
 #pragma unroll
                 for (int c = 0; c < OC; ++c) {
-                  const double prev = st <= kFold ? 0.0 : s_jd[wv][k][c][jw];
-                  s_jd[wv][k][c][jw] = prev + (double)ja[k][c];
+                  s_jd[wv][k][c][jw] = (first ? 0.0 : prev[k][c]) + (double)ja[k][c];
                   ja[k][c] = JT(0);
                 }
             }
@@ -1026,10 +1035,21 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
     const int64_t step = (int64_t)p.K * kBlock * OC;
     const double* Pi = p.part_i + (((Sy.pi0 + it0) * p.K + k) * kBlock + l) * OC + c;
     const double* Pj = p.part_j + (((Sy.pj0 + tri_index(X, 0)) * p.K + k) * kBlock + l) * OC + c;
-    double s0 = 0.0, s1 = 0.0;
-    for (int64_t m = g; m < it1 - it0; m += kRedGroups) s0 += Pi[m * step];
-    for (int64_t m = g; m <= X; m += kRedGroups) s1 += Pj[m * step];
-    sum = s0 + s1;
+    // four loads in flight per thread (the loop is latency-bound: one
+    // dependent load per slot took ~2x longer at 32768 blobs)
+    auto strided_sum = [&](const double* P, int64_t cnt) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int64_t m = g;
+      for (; m + 3 * kRedGroups < cnt; m += 4 * kRedGroups) {
+        a0 += P[m * step];
+        a1 += P[(m + kRedGroups) * step];
+        a2 += P[(m + 2 * kRedGroups) * step];
+        a3 += P[(m + 3 * kRedGroups) * step];
+      }
+      for (; m < cnt; m += kRedGroups) a0 += P[m * step];
+      return (a0 + a1) + (a2 + a3);
+    };
+    sum = strided_sum(Pi, it1 - it0) + strided_sum(Pj, X + 1);
   }
   part[g][ol] = sum;
   __syncthreads();
