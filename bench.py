@@ -300,6 +300,20 @@ def gmres_leg(steps: int, epochs: int, evals: int, log):
     return out
 
 
+def timed_steps(advance, steps: int) -> float:
+    """Mean milliseconds per call of ``advance`` over ``steps`` calls, from
+    HIP events around the whole loop (device synchronised on both sides)."""
+    import torch
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(steps):
+        advance()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
 def rollers_step_bench(args, log):
     """One step = one stochastic Adams-Bashforth step of the rollers
     integrator (rollers example parameters) on the 32768-roller strip:
@@ -322,21 +336,23 @@ def rollers_step_bench(args, log):
         integ.advance_time_step(p["dt"])
     torch.cuda.synchronize()
     prod0, it0, inv0 = dict(integ.products), integ.stoch_iterations_count, integ.invalid_configuration_count
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with wall_rpy.PairKernelTimer() as pk:
-        torch.cuda.synchronize()
-        start.record()
-        for _ in range(args.steps):
-            integ.advance_time_step(p["dt"])
-        stop.record()
-        torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
+    advance = lambda: integ.advance_time_step(p["dt"])
+    ms = timed_steps(advance, args.steps)
     per = {k: (integ.products[k] - prod0[k]) / args.steps for k in integ.products}
     lanczos_its = (integ.stoch_iterations_count - it0) / args.steps
+    inv = integ.invalid_configuration_count - inv0
+    # kernel times from per-launch event records, in steps of their own (the
+    # records cost host time, so the step time above is taken without them)
+    ksteps = max(1, min(args.steps, 10))
+    prod1 = dict(integ.products)
+    with wall_rpy.PairKernelTimer() as pk:
+        timed_steps(advance, ksteps)
+    kper = {k: (integ.products[k] - prod1[k]) / ksteps for k in integ.products}
     nw, tw, mw = pk.by_kind["wall_rpy"]
     nf, tf, mf = pk.by_kind["forces"]
+    tw, tf = tw / ksteps * args.steps, tf / ksteps * args.steps
     pairs = float(n) * n
-    flop = (per["UF"] * FLOP_PER_PAIR["UF"] + per["UF+UT"] * (FLOP_PER_PAIR["UF"] + FLOP_PER_PAIR["UT"])) \
+    flop = (kper["UF"] * FLOP_PER_PAIR["UF"] + kper["UF+UT"] * (FLOP_PER_PAIR["UF"] + FLOP_PER_PAIR["UT"])) \
         * args.steps * pairs
     achieved = flop / (tw * 1e-3) / 1e12
     peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
@@ -352,9 +368,10 @@ def rollers_step_bench(args, log):
                                "free kinematics, blob-blob and wall repulsion, Lanczos tolerance 1e-3",
                    "blobs": n, "precision": args.precision + PRECISION_NOTE.get(args.precision, ""),
                    "per_step": {"products": per, "lanczos_iterations": lanczos_its,
-                                "invalid_configurations": integ.invalid_configuration_count - inv0,
+                                "invalid_configurations": inv,
                                 "wall_rpy_kernel_ms": tw / args.steps, "force_kernel_ms": tf / args.steps,
-                                "pair_kernel_share": (tw + tf) / (ms * args.steps)}},
+                                "pair_kernel_share": (tw + tf) / (ms * args.steps),
+                                "kernel_timing_steps": ksteps}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": None, "kernel": "wall-RPY pair kernels of the step (UF, UF+UT)",
                      "kernel_mean_ms": mw, "kernel_launches": nw,
@@ -510,23 +527,22 @@ def rigid_step_bench(args, log):
         integ.advance_time_step(dt)
     torch.cuda.synchronize()
     it0 = integ.det_iterations_count
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with wall_rpy.PairKernelTimer() as pk:
-        torch.cuda.synchronize()
-        start.record()
-        for _ in range(args.steps):
-            integ.advance_time_step(dt)
-        stop.record()
-        torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
+    advance = lambda: integ.advance_time_step(dt)
+    ms = timed_steps(advance, args.steps)
     its = (integ.det_iterations_count - it0) / args.steps
+    # kernel times from per-launch event records, in steps of their own (the
+    # records cost host time, so the step time above is taken without them)
+    ksteps = max(1, min(args.steps, 10))
+    with wall_rpy.PairKernelTimer() as pk:
+        timed_steps(advance, ksteps)
     nw, tw, mw = pk.by_kind["wall_rpy"]
+    nw_step, tw_step = nw / ksteps, tw / ksteps
     n = w["r"].shape[0]
     pairs = float(n) * n
     peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
     achieved = FLOP_PER_PAIR["UF"] * pairs * nw / (tw * 1e-3) / 1e12 if tw else 0.0
     log(f"rigid step ({args.rigid_solver} solver): {ms:.3f} ms/step, {its:.1f} GMRES iterations/step, wall-RPY "
-        f"kernels {tw / args.steps:.3f} ms/step ({nw / args.steps:.1f} launches)")
+        f"kernels {tw_step:.3f} ms/step ({nw_step:.1f} launches)")
     mixed = args.rigid_solver == "mixed"
     if mixed:
         # FP32-pair inner products and FP64 residuals: the FLOP rate of all
@@ -541,9 +557,9 @@ def rigid_step_bench(args, log):
         "config": {"workload": "C2 suspension: 256 spheres x 42 blobs = 10752 blobs, a = 0.243, eta = 1, dt 1e-3, "
                                "gravity 0.02 per blob, wall and blob-blob repulsion (0.02, 0.1)",
                    "blobs": n, "precision": prec + PRECISION_NOTE.get(prec, ""),
-                   "per_step": {"gmres_iterations": its, "wall_rpy_launches": nw / args.steps,
-                                "wall_rpy_kernel_ms": tw / args.steps,
-                                "pair_kernel_share": tw / (ms * args.steps)}},
+                   "per_step": {"gmres_iterations": its, "wall_rpy_launches": nw_step,
+                                "wall_rpy_kernel_ms": tw_step, "pair_kernel_share": tw_step / ms,
+                                "kernel_timing_steps": ksteps}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "kernel": "wall-RPY UF pair kernel of the GMRES operator",
                      "kernel_mean_ms": mw, "kernel_launches": nw, "flop_per_pair": FLOP_PER_PAIR["UF"]},
@@ -801,16 +817,12 @@ def main(argv=None):
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms = timed_steps(step, args.steps)
+    # the pair kernel's own duration: events recorded around each launch on
+    # its stream, over as many products again (taken apart from the product
+    # time above, which the records would otherwise include)
     with wall_rpy.PairKernelTimer() as pk:
-        torch.cuda.synchronize()
-        start.record()
-        for _ in range(args.steps):
-            step()
-        stop.record()
-        torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
+        timed_steps(step, args.steps)
     pairs = float(n) * n
     value = pairs / (ms * 1e-3)
     op = ("forces" if fparams else ("radii" if args.domain == "single_wall" else f"radii_{args.domain}") if radii

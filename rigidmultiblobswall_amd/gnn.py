@@ -31,6 +31,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import _stream as _stream_mod
 
 DEG, NIN, EIN, HE, NMSG, HN = 8, 10, 16, 64, 64, 128
 NC = 80  # node input width: 10 + 64 padded to a multiple of 16
@@ -43,7 +44,7 @@ def node_width(nin: int) -> int:
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    return _stream_mod.current()
 
 
 def param_shapes(nout: int, nin: int = NIN) -> List[Tuple[str, Tuple[int, ...]]]:

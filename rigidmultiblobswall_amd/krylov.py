@@ -24,6 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import _stream as _stream_mod
 
 
 @dataclass
@@ -37,7 +38,21 @@ class GmresResult:
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    return _stream_mod.current()
+
+
+# pinned host buffers for the Arnoldi column, one per (length, stream): a
+# solve reads through it only between its own stream synchronisations
+_pinned: dict = {}
+
+
+def _pinned_column(length: int, stream_handle: int) -> torch.Tensor:
+    key = (length, stream_handle)
+    buf = _pinned.get(key)
+    if buf is None:
+        buf = torch.empty(length, dtype=torch.float64, pin_memory=True)
+        _pinned[key] = buf
+    return buf
 
 
 class KrylovOps:
@@ -103,8 +118,8 @@ def gmres(A: Callable[[torch.Tensor, torch.Tensor], None], b: torch.Tensor,
     ycoef = torch.empty(m + 1, dtype=torch.float64, device=device)
     # pinned host copy of the Arnoldi column: one short copy and a stream
     # synchronisation per step
-    h_host = torch.empty(m + 2, dtype=torch.float64, pin_memory=True)
     stream = torch.cuda.current_stream(device)
+    h_host = _pinned_column(m + 2, stream.cuda_stream)
     x = torch.zeros(n, dtype=torch.float64, device=device) if x0 is None else x0.clone()
     ident = Pinv is None
     bnorm = ops.norm(b)

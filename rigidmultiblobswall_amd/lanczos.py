@@ -29,10 +29,11 @@ import numpy as np
 import torch
 
 from . import _lib, krylov
+from . import _stream as _stream_mod
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    return _stream_mod.current()
 
 
 class LanczosNotConverged(RuntimeWarning):

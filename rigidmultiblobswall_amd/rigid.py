@@ -29,11 +29,12 @@ import numpy as np
 import torch
 
 from . import _lib, krylov, wall_rpy
+from . import _stream as _stream_mod
 from ._lib import RMW_UF
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    return _stream_mod.current()
 
 
 class RigidSuspension:
@@ -94,6 +95,8 @@ class RigidSuspension:
                                            self._offp(), self.nbodies, self.r_blobs.data_ptr(),
                                            self.r_rel.data_ptr(), self.ws.data_ptr(), self.ws.numel(), _stream()))
         self._mlam = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
+        self._products = {}      # precision -> wall_rpy.Product on r_blobs
+        self._ops = None         # KrylovOps of the solves (vectors of self.size)
         self.minv = None
         self.pc_r_rel = None     # body-frame blob offsets the preconditioner was built with
         self.chol = None         # per body L_b (M_b = L_b L_b^T), packed like minv
@@ -122,9 +125,15 @@ class RigidSuspension:
     # -- blob mobility and operator -------------------------------------------------
     def mobility_times(self, lam: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
         """out = B M(z_eff) B lam on all blobs (wall-RPY UF block)."""
-        wall_rpy.apply(RMW_UF, self.r_blobs, force=lam, eta=self.eta, a=self.a, periodic_length=self.L,
-                       precision=precision or self.precision, out_velocity=out.view(1, self.Nblobs, 3),
-                       domain=self.domain)
+        prec = precision or self.precision
+        prod = self._products.get(prec)
+        if prod is None:
+            # r_blobs is rewritten in place when the bodies move: one prepared
+            # product per precision serves every configuration
+            prod = wall_rpy.Product(RMW_UF, self.r_blobs, eta=self.eta, a=self.a, periodic_length=self.L,
+                                    precision=prec, domain=self.domain)
+            self._products[prec] = prod
+        prod(force=lam.reshape(-1).contiguous(), out_velocity=out.view(-1))
 
     def operator(self, x: torch.Tensor, out: torch.Tensor, precision: Optional[str] = None):
         """out = [M lambda - K U; -K^T lambda]."""
@@ -278,6 +287,16 @@ class RigidSuspension:
         return x
 
     # -- solve ------------------------------------------------------------------------
+    def _krylov_ops(self) -> krylov.KrylovOps:
+        if self._ops is None:
+            self._ops = krylov.KrylovOps(self.size, 2, self.device)
+        return self._ops
+
+    def norm(self, v: torch.Tensor) -> float:
+        """||v|| of a device vector of self.size entries (the fixed-order
+        reduction the solves normalise with)."""
+        return self._krylov_ops().norm(v)
+
     def rhs(self, slip, force_torque) -> torch.Tensor:
         """[slip; -force_torque] as in solve_mobility_problem (no constraints)."""
         s = torch.as_tensor(slip, dtype=torch.float64).reshape(-1)
@@ -289,12 +308,13 @@ class RigidSuspension:
     def solve_mobility_problem(self, slip=None, force_torque=None, rhs: Optional[torch.Tensor] = None,
                                tol: float = 1e-8, restart: int = 60, maxiter: int = 1000,
                                x0: Optional[torch.Tensor] = None, precision: Optional[str] = None,
-                               callback=None) -> krylov.GmresResult:
+                               callback=None, rhs_norm: Optional[float] = None) -> krylov.GmresResult:
         """Solve [M -K; -K^T 0][lambda; U] = [slip; -F] to relative residual tol.
 
         The RHS is normalised to unit norm and the solution rescaled, as in
         quaternion_integrator_multi_bodies.py:1515-1536; x0 (unnormalised)
-        is used as a true warm start.  Returns a GmresResult whose x is
+        is used as a true warm start.  rhs_norm: ||rhs|| when the caller has
+        it from norm() (saves a read-back).  Returns a GmresResult whose x is
         [lambda; U] (U_b = (v_b, w_b))."""
         if rhs is None:
             if slip is None:
@@ -302,8 +322,8 @@ class RigidSuspension:
             if force_torque is None:
                 force_torque = np.zeros((self.nbodies, 6))
             rhs = self.rhs(slip, force_torque)
-        ops = krylov.KrylovOps(self.size, 2, self.device)
-        rnorm = ops.norm(rhs)
+        ops = self._krylov_ops()
+        rnorm = ops.norm(rhs) if rhs_norm is None else float(rhs_norm)
         if rnorm == 0.0:
             return krylov.GmresResult(x=torch.zeros_like(rhs), info=0, iterations=0, matvecs=0, final_residual=0.0)
         b = torch.empty_like(rhs)
@@ -325,7 +345,7 @@ class RigidSuspension:
     def solve_mixed(self, slip=None, force_torque=None, rhs: Optional[torch.Tensor] = None, tol: float = 1e-8,
                     restart: int = 60, maxiter: int = 1000, x0: Optional[torch.Tensor] = None,
                     inner_precision: str = "fp32_blocked", inner_floor: float = 1e-8, richardson: float = 2.5,
-                    max_outer: int = 8):
+                    max_outer: int = 8, rhs_norm: Optional[float] = None):
         """Mixed-precision solve to the FP64 tolerance: FP64 residuals
         r = b - A_64 x (outer), GMRES on A_32 d = r with FP32-pair products
         and FP64 Krylov algebra (inner), x += d.
@@ -342,7 +362,8 @@ class RigidSuspension:
         reaches; it never decides convergence -- every later residual, and
         the one that ends the solve, is FP64.  A start whose inner-operator
         residual is within 10x of `inner_floor` is judged by its FP64
-        residual instead.  Returns a GmresResult whose iterations and
+        residual instead.  rhs_norm: ||rhs|| when the caller has it from
+        norm().  Returns a GmresResult whose iterations and
         matvecs sum over the inner solves (at most `maxiter` iterations in
         all); final_residual is the FP64 one, and info is 0 only when it meets
         tol (else a positive count, as scipy's gmres)."""
@@ -351,8 +372,8 @@ class RigidSuspension:
                            np.zeros((self.nbodies, 6)) if force_torque is None else force_torque)
         if self.minv is None:
             self.build_preconditioner()
-        ops = krylov.KrylovOps(self.size, 2, self.device)
-        bnorm = ops.norm(rhs)
+        ops = self._krylov_ops()
+        bnorm = ops.norm(rhs) if rhs_norm is None else float(rhs_norm)
         x = torch.zeros_like(rhs) if x0 is None else x0.to(self.device, torch.float64).reshape(-1).clone()
         r = torch.empty_like(rhs)
         out = krylov.GmresResult(x=x, info=0, iterations=0, matvecs=0)

@@ -45,13 +45,14 @@ import numpy as np
 import torch
 
 from . import _lib, forces
+from . import _stream as _stream_mod
 from .rigid import RigidSuspension
 
 SCHEMES = ("deterministic_forward_euler", "deterministic_adams_bashforth", "stochastic_EM")
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    return _stream_mod.current()
 
 
 class RigidIntegrator:
@@ -223,12 +224,14 @@ class RigidIntegrator:
         rhs = S.rhs(self.slip(S), self.force_torque(S))
         if noise is not None:
             rhs[: 3 * S.Nblobs] -= noise.reshape(-1)
-        rnorm = float(torch.linalg.norm(rhs))
+        rnorm = S.norm(rhs)
         x0 = None if (self.first_guess is None or rnorm == 0.0) else self.first_guess * rnorm
         if self.solver == "mixed":
-            res = S.solve_mixed(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0)
+            res = S.solve_mixed(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0,
+                                rhs_norm=rnorm)
         else:
-            res = S.solve_mobility_problem(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0)
+            res = S.solve_mobility_problem(rhs=rhs, tol=self.tolerance, restart=self.restart, maxiter=1000, x0=x0,
+                                           rhs_norm=rnorm)
         if res.info != 0:
             raise RuntimeError(f"GMRES did not reach {self.tolerance} (residual {res.final_residual:.3e})")
         self.det_iterations_count += res.iterations

@@ -34,6 +34,7 @@ import numpy as np
 import torch
 
 from . import _lib, forces, krylov, lanczos, wall_rpy
+from . import _stream as _stream_mod
 from ._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
 
 SCHEMES = ("deterministic_forward_euler", "stochastic_first_order", "deterministic_adams_bashforth",
@@ -116,7 +117,7 @@ class RollersIntegrator:
 
     # ------------------------------------------------------------------
     def _stream(self):
-        return torch.cuda.current_stream(self.device).cuda_stream
+        return _stream_mod.current(self.device)
 
     def _draw(self, kind: str, shape) -> torch.Tensor:
         if self.noise_source is not None:

@@ -14,7 +14,7 @@ from typing import Optional, Sequence
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, _stream
 from ._lib import RMW_UF, RMW_UT, RMW_WF, RMW_WT
 
 _ws_lock = threading.Lock()
@@ -33,7 +33,10 @@ def workspace(nbytes: int, device: Optional[torch.device] = None, slot: str = "d
     different streams never share scratch, so they may overlap.  A buffer is
     allocated on, used on and (when grown) released on its own stream."""
     device = device or _device()
-    key = (device.index, torch.cuda.current_stream(device).cuda_stream, slot)
+    return _workspace_of((device.index, _stream.current(device), slot), nbytes, device)
+
+
+def _workspace_of(key, nbytes: int, device) -> torch.Tensor:
     with _ws_lock:
         buf = _workspaces.get(key)
         if buf is None or buf.numel() < nbytes:
@@ -81,7 +84,7 @@ def _L_array(periodic_length) -> ctypes.Array:
 
 
 def _stream_ptr(device) -> int:
-    return torch.cuda.current_stream(device).cuda_stream
+    return _stream.current(device)
 
 
 def _periodic(periodic_length) -> bool:
@@ -167,6 +170,65 @@ def apply(mask: int, r_vectors, force=None, torque=None, eta: float = 1.0, a: fl
         u = u.view(n, 3) if u is not None else None
         w = w.view(n, 3) if w is not None else None
     return u, w
+
+
+class Product:
+    """A prepared product of the blocks of ``mask`` on the positions held by
+    ``r`` (a contiguous float64 (N, 3) device tensor, read at every call, so
+    it may be rewritten in place between calls): the shape checks, the launch
+    shape lookup and the workspace size are settled once, and each call
+    passes the right-hand side and output pointers and the current stream.
+    Products of a Krylov operator run one after another from Python; this
+    keeps the host time between them short.  Same arguments and results as
+    apply() with k = 1 and caller-supplied outputs."""
+
+    def __init__(self, mask: int, r: torch.Tensor, eta: float = 1.0, a: float = 1.0, periodic_length=None,
+                 precision="fp32", tuning=None, domain: str = "single_wall"):
+        if domain not in DOMAINS:
+            raise ValueError(f"unknown domain {domain!r}; use one of {sorted(DOMAINS)}")
+        self.lib = _lib.load()
+        self.device = _device()
+        if not isinstance(r, torch.Tensor) or r.dtype != torch.float64 or r.device != self.device or \
+                not r.is_contiguous() or r.numel() % 3:
+            raise ValueError(f"r must be a contiguous float64 (N, 3) tensor on {self.device}")
+        self.r = r
+        self.n = n = r.numel() // 3
+        self.mask = int(mask)
+        self.has_u = bool(mask & (RMW_UF | RMW_UT))
+        self.has_w = bool(mask & (RMW_WF | RMW_WT))
+        self.dom = DOMAINS[domain]
+        self.prec = _lib.precision_code(precision)
+        if self.dom == 0:
+            self.tuning = _tuning_for(tuning, mask, precision, 1, float(n) * n, False, _periodic(periodic_length))
+        else:
+            self.tuning = None if tuning is None or isinstance(tuning, str) else tuning
+        if isinstance(self.tuning, dict):
+            self.tuning = _lib.with_flags(self.tuning, 0)
+        self.tp = _lib.tuning_ptr(self.tuning)
+        self.L = _L_array(periodic_length)
+        self.eta, self.a = float(eta), float(a)
+        nbytes = ctypes.c_size_t()
+        _lib.check(self.lib.rmw_mobility_workspace_size(self.dom, self.mask, self.prec, n, 1, self.tp,
+                                                        ctypes.byref(nbytes)))
+        self.nbytes = nbytes.value
+
+    def _ptr(self, t, name: str, needed: bool):
+        if not needed:
+            return None
+        if t is None:
+            raise ValueError(f"{name} is required by this block mask")
+        return _check_out(t, 3 * self.n, self.device, name).data_ptr()
+
+    def __call__(self, force=None, torque=None, out_velocity=None, out_omega=None) -> None:
+        m = self.mask
+        fp = self._ptr(force, "force", bool(m & (RMW_UF | RMW_WF)))
+        tp_ = self._ptr(torque, "torque", bool(m & (RMW_UT | RMW_WT)))
+        up = self._ptr(out_velocity, "out_velocity", self.has_u)
+        wp = self._ptr(out_omega, "out_omega", self.has_w)
+        st = _stream.current(self.device)
+        ws = _workspace_of((self.device.index, st, "default"), self.nbytes, self.device)
+        _lib.check(self.lib.rmw_mobility_apply(self.dom, m, self.prec, self.r.data_ptr(), fp, tp_, up, wp, self.n, 1,
+                                               0, self.eta, self.a, self.L, self.tp, ws.data_ptr(), ws.numel(), st))
 
 
 def _grand_rows(X: torch.Tensor, Y: torch.Tensor, n: int, device):

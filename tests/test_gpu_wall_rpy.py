@@ -118,6 +118,48 @@ def test_two_block_masks(gpu, mask):
         assert rel(w.cpu().numpy().ravel(), wr) <= 1e-12
 
 
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp64"])
+@pytest.mark.parametrize("mask", [RMW_UF, RMW_UF | RMW_UT, 15])
+def test_prepared_product(gpu, mask, prec):
+    """wall_rpy.Product (shape, launch shape and workspace settled once)
+    gives apply()'s result bit for bit, follows positions rewritten in place,
+    matches the oracle, and refuses wrong buffers before any launch."""
+    import torch
+    rng = np.random.default_rng(16)
+    n, a, eta = 1500, 0.4, 1.5
+    r0 = np.column_stack([rng.uniform(0, 30, n), rng.uniform(0, 30, n), rng.uniform(0.2, 2.5, n)])
+    r1 = r0 + rng.normal(0, 0.05, r0.shape)
+    r1[:, 2] = np.abs(r1[:, 2])
+    f, t = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    print(f"case Product mask={mask} prec={prec} N={n}", flush=True)
+    dev = torch.device("cuda", 0)
+    r = torch.from_numpy(r0).to(dev)
+    fd, td = torch.from_numpy(f).to(dev), torch.from_numpy(t).to(dev)
+    p = wall_rpy.Product(mask, r, eta=eta, a=a, precision=prec)
+    u = torch.empty(3 * n, dtype=torch.float64, device=dev)
+    w = torch.empty(3 * n, dtype=torch.float64, device=dev)
+    has_w = bool(mask & (RMW_WF | RMW_WT))
+    for rr in (r0, r1):
+        r.copy_(torch.from_numpy(rr))
+        p(force=fd, torque=td if mask & (RMW_UT | RMW_WT) else None, out_velocity=u,
+          out_omega=w if has_w else None)
+        ue, we = wall_rpy.apply(mask, r, force=fd, torque=td, eta=eta, a=a, precision=prec)
+        torch.cuda.synchronize()
+        assert torch.equal(u, ue.reshape(-1))
+        if has_w:
+            assert torch.equal(w, we.reshape(-1))
+        ur = O.uf(rr, f, eta, a) + (O.ut(rr, t, eta, a) if mask & RMW_UT else 0.0)
+        assert rel(u.cpu().numpy(), ur) <= TOL[prec]
+        if has_w:
+            assert rel(w.cpu().numpy(), O.wf(rr, f, eta, a) + O.wt(rr, t, eta, a)) <= TOL[prec]
+    with pytest.raises(ValueError):
+        p(force=fd[:-3], torque=td, out_velocity=u, out_omega=w)
+    with pytest.raises(ValueError):
+        p(force=fd, torque=td, out_velocity=u.float(), out_omega=w)
+    with pytest.raises(ValueError):
+        p(force=None, torque=td, out_velocity=u, out_omega=w)
+
+
 def test_trans_times_force_torque_dropin(gpu):
     rng = np.random.default_rng(15)
     n, a, eta = 400, 0.5, 1.0
