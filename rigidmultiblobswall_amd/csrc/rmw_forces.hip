@@ -27,8 +27,9 @@
 // max(r, 1e-25)) and dr = 0.
 //
 // Precision: the geometry (differences, r, the exponent) and every sum are
-// FP64 in both modes.  RMW_PREC_F64 evaluates sqrt, the quotient and exp in
-// FP64; the other modes take 1/r from the hardware FP64 reciprocal square
+// FP64 in both modes.  RMW_PREC_F64 takes 1/r from the FP64 reciprocal
+// square root of the pair terms (rsq: <= 1.4e-16 relative) and exp in FP64;
+// the other modes take 1/r from the hardware FP64 reciprocal square
 // root refined by one Newton step (relative error ~1e-13) and the
 // exponential from the hardware FP32 exp2 of the FP64 exponent rounded
 // once, which leaves a relative error of ~1e-7 t e^-t per pair with
@@ -41,6 +42,7 @@
 
 #include "rmw_accum.h"
 #include "rmw_internal.h"
+#include "rmw_pair_terms.h"
 #include "rmw_tile_pairs.h"
 
 namespace rmw {
@@ -83,8 +85,8 @@ __device__ __forceinline__ double pair_g(const ForceParams& p, double r2, double
   r2 = fmax(r2, 1e-50);
   double r, inv, e;
   if constexpr (F64) {
-    r = sqrt(r2);
-    inv = 1.0 / r;
+    inv = rsq(r2);
+    r = r2 * inv;
     e = r > A ? exp(-(r - A) * p.invb) : 1.0;
   } else {
     const double y0 = __builtin_amdgcn_rsq(r2);
