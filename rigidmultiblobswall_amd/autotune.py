@@ -151,10 +151,10 @@ def chunk_candidates(nmax: int, max_chunks: int = 48) -> List[int]:
 
 
 def candidates(nmax: int, k: int, cu: int, ragged: bool, symmetric_possible: bool = False) -> List[dict]:
-    """Second-stage shapes for one chunk length: grid caps (none, 4 and 8
-    per CU, plus 16 per CU for ragged batches) x RHS groups x (where the
+    """Second-stage shapes for one chunk length: grid caps (none, 4, 8 and
+    10 per CU, plus 16 per CU for ragged batches) x RHS groups x (where the
     symmetric UF kernel applies) symmetric on / off."""
-    wgs = [0, 4 * cu, 8 * cu] + ([16 * cu] if ragged else [])
+    wgs = [0, 4 * cu, 8 * cu, 10 * cu] + ([16 * cu] if ragged else [])
     groups = ([0] + [g for g in (1, 2) if g < min(k, 4)] + ([4] if k >= 4 else [])) if k > 1 else [0]
     syms = [0, -1] if symmetric_possible else [0]
     return [{"max_wgs": w, "rhs_group": g, "symmetric": sy} for w in wgs for g in groups for sy in syms]

@@ -1026,6 +1026,11 @@ struct MutReduceParams {
 // outputs of one slot), then group 0 adds the eight partial sums in order.
 constexpr int kRedGroups = 8, kRedOut = 256 / kRedGroups;
 
+// PER: pseudo-periodic calls, which add the self pair's outer images; the
+// image loop takes 168 VGPRs, so calls without images run an instantiation
+// without it (70 VGPRs: seven waves per SIMD instead of three for this
+// latency-bound loop).
+template <bool PER>
 __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
 {
   __shared__ double part[kRedGroups][kRedOut];
@@ -1084,7 +1089,7 @@ __global__ __launch_bounds__(256) void mutual_reduce_kernel(MutReduceParams p)
 #pragma unroll
     for (int q = 1; q < kRedGroups; ++q) s += part[q][ol];
     finish_output(p.r, p.f, p.t, p.vel, p.omg, p.mask, k, gi, c, s, p.rhs_stride, p.a, p.inva, p.c1, p.c2,
-                  p.domain, p.periodic, p.L[0], p.L[1], p.L[2]);
+                  p.domain, PER ? p.periodic : 0, p.L[0], p.L[1], p.L[2]);
   }
 }
 
@@ -1704,7 +1709,11 @@ static void run_mutual(const Plan& pl, const double* r, const double* f, const d
     for (int d = 0; d < 3; ++d) rp.L[d] = Lp[d];
     const int OC = 3 * ((HU ? 1 : 0) + (HW ? 1 : 0));
     const int64_t nout = pl.ntot * K * OC;
-    hipLaunchKernelGGL(mutual_reduce_kernel, dim3((unsigned)((nout + kRedOut - 1) / kRedOut)), dim3(256), 0, st, rp);
+    const dim3 rgrid((unsigned)((nout + kRedOut - 1) / kRedOut));
+    if (periodic)
+      hipLaunchKernelGGL(mutual_reduce_kernel<true>, rgrid, dim3(256), 0, st, rp);
+    else
+      hipLaunchKernelGGL(mutual_reduce_kernel<false>, rgrid, dim3(256), 0, st, rp);
     RMW_CHECK_HIP(hipGetLastError());
   }
 }

@@ -43,9 +43,9 @@ def test_candidates():
     assert autotune.chunk_candidates(100) == [64, 128]
     cs = autotune.candidates(nmax=10752, k=1, cu=256, ragged=False)
     assert {c["rhs_group"] for c in cs} == {0}
-    assert {c["max_wgs"] for c in cs} == {0, 1024, 2048}
+    assert {c["max_wgs"] for c in cs} == {0, 1024, 2048, 2560}
     cs4 = autotune.candidates(nmax=32768, k=4, cu=256, ragged=False)
     assert {c["rhs_group"] for c in cs4} == {0, 1, 2, 4}
     cs2 = autotune.candidates(nmax=100, k=2, cu=256, ragged=True)
     assert {c["rhs_group"] for c in cs2} == {0, 1}
-    assert {c["max_wgs"] for c in cs2} == {0, 1024, 2048, 4096}
+    assert {c["max_wgs"] for c in cs2} == {0, 1024, 2048, 2560, 4096}
