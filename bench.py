@@ -11,7 +11,11 @@ Prints ONE JSON line (rank 0):
   oracle_error relative L2 error of the timed product against the FP64 C
                oracle (strict IEEE build) on a sample of 512 target rows
   fp64         the same product with FP64 pair terms: time, pairs/s, fraction
-               of the FP64 vector peak and its oracle error
+               of the FP64 vector peak and its oracle error (run before the
+               FP32 warm-up: its sustained load brings the GPU to its
+               sustained clock, which the first ~30 ms of load from idle
+               do not reach; lines without this leg repeat their product,
+               untimed, for 40 ms before the warm-up instead)
   roofline     pair kernel: algorithmic FLOP per launch (164 FLOP/pair x N^2)
                / mean pair-kernel duration (HIP events on the launch stream),
                against the FP32 vector peak; traffic from committed rocprof
@@ -193,8 +197,10 @@ def oracle_error(u, r_h, f_h, eta, a, domain="single_wall"):
                       "(strict IEEE build)"}
 
 
-def fp64_leg(r, f, u, r_h, f_h, eta, a, steps, warmup):
-    """The same product with FP64 pair terms and sums (the 1e-12 mode)."""
+def fp64_leg(r, f, u, r_h, f_h, eta, a, steps, warmup, check: bool = True):
+    """The same product with FP64 pair terms and sums (the 1e-12 mode);
+    check=False leaves out the oracle comparison (the caller makes it later
+    from u)."""
     import torch
     from rigidmultiblobswall_amd import wall_rpy
     from rigidmultiblobswall_amd._lib import RMW_UF
@@ -218,7 +224,7 @@ def fp64_leg(r, f, u, r_h, f_h, eta, a, steps, warmup):
     return {"ms_per_step": ms, "pairs_per_s": pairs / (ms * 1e-3), "kernel_mean_ms": pk.mean_ms,
             "kernel": f"{'mutual_kernel<UF,K=1,fp64>' if launch['kernel'] == 'symmetric' else 'pair_kernel_f64<UF,K=1>'}",
             "achieved_tflops": achieved, "peak_tflops": PEAK_FP64_TFLOPS, "frac": achieved / PEAK_FP64_TFLOPS,
-            "oracle_error": oracle_error(u, r_h, f_h, eta, a)}
+            **({"oracle_error": oracle_error(u, r_h, f_h, eta, a)} if check else {})}
 
 
 def end_to_end(r_h, f_h, eta, a, precision, reps=20):
@@ -298,6 +304,23 @@ def gmres_leg(steps: int, epochs: int, evals: int, log):
                                                              / out["learned_previous_fp64"]["ms"])
     out["speedup_learned_previous_mixed_vs_plain"] = out["plain_fp64"]["ms"] / out["learned_previous_mixed"]["ms"]
     return out
+
+
+def settle(step, ms: float = 40.0) -> None:
+    """Repeat `step` (untimed) until `ms` milliseconds of GPU work have run:
+    from idle the GPU needs ~30 ms of sustained load to reach its sustained
+    clock (the pair kernels' first launches run up to 30 % longer)."""
+    import torch
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    done = 0.0
+    while done < ms:
+        for _ in range(4):
+            step()
+        stop.record()
+        stop.synchronize()
+        done = start.elapsed_time(stop)
 
 
 def timed_steps(advance, steps: int) -> float:
@@ -815,6 +838,18 @@ def main(argv=None):
             wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u,
                            domain=args.domain)
 
+    # the FP64-mode product first: its ~30 ms of sustained load bring the GPU
+    # to its sustained clock before the timed FP32 products (from idle, the
+    # pair kernel's first launches at 32768 blobs take 1.2-1.4 ms and settle
+    # at ~1.08 ms after ~30 ms of load); its oracle comparison comes later,
+    # so that no host work separates it from the timed region
+    u64 = None
+    if not radii and not fparams and args.domain == "single_wall" and args.precision != "fp64" \
+            and not args.no_fp64:
+        u64 = torch.empty_like(u)
+        fp64_res = fp64_leg(r, f, u64, r_h, f_h, eta, a, max(5, args.steps // 2), 8, check=False)
+    else:
+        settle(step)
     for _ in range(args.warmup):
         step()
     ms = timed_steps(step, args.steps)
@@ -898,8 +933,9 @@ def main(argv=None):
     if not radii and not fparams and args.domain == "single_wall":
         out["oracle_error"] = oracle_error(u, r_h, f_h, eta, a)
         log(f"oracle error (rel L2, {out['oracle_error']['rows']} rows): {out['oracle_error']['rel_l2']:.3e}")
-        if args.precision != "fp64" and not args.no_fp64:
-            out["fp64"] = fp64_leg(r, f, u, r_h, f_h, eta, a, max(5, args.steps // 2), 2)
+        if u64 is not None:
+            fp64_res["oracle_error"] = oracle_error(u64, r_h, f_h, eta, a)
+            out["fp64"] = fp64_res
             log(f"fp64 mode: {out['fp64']['ms_per_step']:.4f} ms/step, pair kernel {out['fp64']['kernel_mean_ms']:.4f} ms, "
                 f"oracle error {out['fp64']['oracle_error']['rel_l2']:.3e}")
         out["end_to_end"] = end_to_end(r_h, f_h.reshape(-1, 3), eta, a, args.precision)
