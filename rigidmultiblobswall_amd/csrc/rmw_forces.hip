@@ -282,23 +282,36 @@ __global__ __launch_bounds__(256) void force_reduce_kernel(const double* __restr
 }
 
 // Symmetric reduce for blob i of tile X: the target-side slots of the items
-// of tile X, then the source-side slots (X, T) for T = 0..X, in that order.
+// of tile X and the source-side slots (X, T) for T = 0..X, split over
+// kFRedGroups threads per output, whose sums are added in a fixed order.
+constexpr int kFRedGroups = 8, kFRedOut = 256 / kFRedGroups;
 __global__ __launch_bounds__(256) void force_mutual_reduce_kernel(const double* __restrict__ part_i,
                                                                   const double* __restrict__ part_j, int64_t n,
                                                                   int64_t n_t, int64_t chunk_tiles, double scale,
                                                                   double* __restrict__ out)
 {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= n * 3) return;
-  const int64_t i = tid / 3;
-  const int c = (int)(tid - 3 * i);
-  const int64_t X = i / kFBlock, l = i - X * kFBlock;
-  const int64_t it0 = items_before(X, n_t, chunk_tiles), it1 = items_before(X + 1, n_t, chunk_tiles);
+  __shared__ double part[kFRedGroups][kFRedOut];
+  const int g = threadIdx.x / kFRedOut, ol = threadIdx.x - g * kFRedOut;
+  const int64_t tid = (int64_t)blockIdx.x * kFRedOut + ol;
+  const bool valid = tid < n * 3;
   double s = 0.0;
-  for (int64_t it = it0; it < it1; ++it) s += part_i[(it * kFBlock + l) * 3 + c];
-  const double* Pj = part_j + (tri_index(X, 0) * kFBlock + l) * 3 + c;
-  for (int64_t T = 0; T <= X; ++T) s += Pj[T * kFBlock * 3];
-  out[tid] = scale * s;
+  if (valid) {
+    const int64_t i = tid / 3;
+    const int c = (int)(tid - 3 * i);
+    const int64_t X = i / kFBlock, l = i - X * kFBlock;
+    const int64_t it0 = items_before(X, n_t, chunk_tiles), it1 = items_before(X + 1, n_t, chunk_tiles);
+    const int64_t step = (int64_t)kFBlock * 3;
+    s = group_strided_sum<kFRedGroups>(part_i + (it0 * kFBlock + l) * 3 + c, it1 - it0, step, g) +
+        group_strided_sum<kFRedGroups>(part_j + (tri_index(X, 0) * kFBlock + l) * 3 + c, X + 1, step, g);
+  }
+  part[g][ol] = s;
+  __syncthreads();
+  if (g == 0 && valid) {
+    double t = part[0][ol];
+#pragma unroll
+    for (int q = 1; q < kFRedGroups; ++q) t += part[q][ol];
+    out[tid] = scale * t;
+  }
 }
 
 struct ForcePlan {
@@ -434,8 +447,8 @@ extern "C" int rmw_blob_forces(int prec, const double* r_vectors, const double* 
     const double scale = -repulsion_strength / debye_length;
     const dim3 rg((unsigned)((n * 3 + 255) / 256)), rb(256);
     if (pl.mutual)
-      hipLaunchKernelGGL(force_mutual_reduce_kernel, rg, rb, 0, st, q.partial, q.part_j, n, pl.n_t, pl.chunk_tiles,
-                         scale, force);
+      hipLaunchKernelGGL(force_mutual_reduce_kernel, dim3((unsigned)((n * 3 + kFRedOut - 1) / kFRedOut)), rb, 0, st,
+                         q.partial, q.part_j, n, pl.n_t, pl.chunk_tiles, scale, force);
     else
       hipLaunchKernelGGL(force_reduce_kernel, rg, rb, 0, st, q.partial, pl.n_chunks, n, scale, force);
     RMW_CHECK_HIP(hipGetLastError());

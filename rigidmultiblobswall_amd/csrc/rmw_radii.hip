@@ -372,8 +372,10 @@ __global__ __launch_bounds__(kRBlock) void radii_mutual_kernel(RadiiMutParams p)
   }
 }
 
-// Blob i of tile X: the target-side sums of the items of tile X, then the
-// source-side sums (X, T) for T = 0..X, in that order; then the damping.
+// Blob i of tile X: the target-side sums of the items of tile X and the
+// source-side sums (X, T) for T = 0..X, split over kRRedGroups threads per
+// output whose sums are added in a fixed order; then the damping.
+constexpr int kRRedGroups = 8, kRRedOut = 256 / kRRedGroups;
 __global__ __launch_bounds__(256) void radii_mutual_reduce_kernel(const double* __restrict__ part_i,
                                                                   const double* __restrict__ part_j,
                                                                   const double* __restrict__ r,
@@ -381,19 +383,31 @@ __global__ __launch_bounds__(256) void radii_mutual_reduce_kernel(const double* 
                                                                   int64_t n_t, int64_t chunk_tiles,
                                                                   double* __restrict__ out)
 {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= n * 3) return;
-  const int64_t i = tid / 3;
-  const int c = (int)(tid - 3 * i);
-  const int64_t X = i / kRBlock, l = i - X * kRBlock;
-  const int64_t it0 = items_before(X, n_t, chunk_tiles), it1 = items_before(X + 1, n_t, chunk_tiles);
+  __shared__ double part[kRRedGroups][kRRedOut];
+  const int g = threadIdx.x / kRRedOut, ol = threadIdx.x - g * kRRedOut;
+  const int64_t tid = (int64_t)blockIdx.x * kRRedOut + ol;
+  const bool valid = tid < n * 3;
   double s = 0.0;
-  for (int64_t it = it0; it < it1; ++it) s += part_i[(it * kRBlock + l) * 3 + c];
-  const double* Pj = part_j + (tri_index(X, 0) * kRBlock + l) * 3 + c;
-  for (int64_t T = 0; T <= X; ++T) s += Pj[T * kRBlock * 3];
-  double ze, B;
-  wall_shift(r[3 * i + 2], rad[i], bnd, ze, B);
-  out[tid] = B * s;
+  if (valid) {
+    const int64_t i = tid / 3;
+    const int c = (int)(tid - 3 * i);
+    const int64_t X = i / kRBlock, l = i - X * kRBlock;
+    const int64_t it0 = items_before(X, n_t, chunk_tiles), it1 = items_before(X + 1, n_t, chunk_tiles);
+    const int64_t step = (int64_t)kRBlock * 3;
+    s = group_strided_sum<kRRedGroups>(part_i + (it0 * kRBlock + l) * 3 + c, it1 - it0, step, g) +
+        group_strided_sum<kRRedGroups>(part_j + (tri_index(X, 0) * kRBlock + l) * 3 + c, X + 1, step, g);
+  }
+  part[g][ol] = s;
+  __syncthreads();
+  if (g == 0 && valid) {
+    double t = part[0][ol];
+#pragma unroll
+    for (int q = 1; q < kRRedGroups; ++q) t += part[q][ol];
+    const int64_t i = tid / 3;
+    double ze, B;
+    wall_shift(r[3 * i + 2], rad[i], bnd, ze, B);
+    out[tid] = B * t;
+  }
 }
 
 constexpr size_t kRMutAutoBytes = size_t(2) << 30;
@@ -592,8 +606,8 @@ extern "C" int rmw_radii_self_apply(int prec, int boundary, const double* r, con
     else launch_radii_mutual<kRadiiUnbounded>(prec, g, b, st, q);
     RMW_CHECK_HIP(hipGetLastError());
     prof_end(rec, prof, st);
-    hipLaunchKernelGGL(radii_mutual_reduce_kernel, dim3((unsigned)((n * 3 + 255) / 256)), dim3(256), 0, st, q.part_i,
-                       q.part_j, r, radius, q.bnd, n, pl.n_t, pl.chunk_tiles, velocity);
+    hipLaunchKernelGGL(radii_mutual_reduce_kernel, dim3((unsigned)((n * 3 + kRRedOut - 1) / kRRedOut)), dim3(256), 0,
+                       st, q.part_i, q.part_j, r, radius, q.bnd, n, pl.n_t, pl.chunk_tiles, velocity);
     RMW_CHECK_HIP(hipGetLastError());
   });
 }

@@ -49,6 +49,25 @@ __device__ __forceinline__ double rol1(double v)
   const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
+
+// Partial-sum reduces: G threads share one output; thread g adds the slots
+// P[m * step] for m = g, g + G, g + 2G, ... below cnt with four loads in
+// flight (the loop is latency-bound), in a fixed order.  The caller adds
+// the G results in order g = 0..G-1.
+template <int G>
+__device__ __forceinline__ double group_strided_sum(const double* __restrict__ P, int64_t cnt, int64_t step, int g)
+{
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int64_t m = g;
+  for (; m + 3 * G < cnt; m += 4 * G) {
+    a0 += P[m * step];
+    a1 += P[(m + G) * step];
+    a2 += P[(m + 2 * G) * step];
+    a3 += P[(m + 3 * G) * step];
+  }
+  for (; m < cnt; m += G) a0 += P[m * step];
+  return (a0 + a1) + (a2 + a3);
+}
 #endif
 
 // Work items counted in runs of C 64-source blocks (C = 1, 2 or a multiple
