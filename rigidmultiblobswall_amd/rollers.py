@@ -29,10 +29,16 @@ class Rollers:
         self.eta, self.a = float(eta), float(a)
         self.L = periodic_length
         self.precision = precision
+        self._products = {}      # precision -> wall_rpy.StridedProduct on self.r
 
     def grand_matvec(self, X: torch.Tensor, Y: torch.Tensor, precision=None):
         """Y[r] = [UF UT; WF WT] X[r] for X, Y (k, 6N) with X[r] = [f; t]."""
-        wall_rpy.apply_strided(GRAND, self.r, self.n, X, Y, self.eta, self.a, self.L, precision or self.precision)
+        prec = precision or self.precision
+        prod = self._products.get(prec)
+        if prod is None:
+            prod = wall_rpy.StridedProduct(GRAND, self.r, self.eta, self.a, self.L, prec)
+            self._products[prec] = prod
+        prod(X, Y)
 
     def brownian_noise(self, Z: torch.Tensor, factor: float = 1.0, tolerance: float = 1e-3, max_iter: int = 1000,
                        precision=None):
@@ -66,11 +72,17 @@ class RaggedRollers:
         self.precision = precision
         self.tuning = tuning
         self.seg = lanczos.Segments(self.offsets, comp=3, nparts=2, device=self.device)
+        self._products = {}      # precision -> wall_rpy.StridedProduct on self.r
 
     def grand_matvec(self, X: torch.Tensor, Y: torch.Tensor, precision=None):
         """Y[r] = blockdiag_s([UF UT; WF WT]_s) X[r] for X, Y (k, 6N)."""
-        wall_rpy.apply_ragged_strided(GRAND, self.r, self.offsets, X, Y, self.eta, self.a, self.L,
-                                      precision or self.precision, tuning=self.tuning)
+        prec = precision or self.precision
+        prod = self._products.get(prec)
+        if prod is None:
+            prod = wall_rpy.StridedProduct(GRAND, self.r, self.eta, self.a, self.L, prec, tuning=self.tuning,
+                                           offsets=self.offsets)
+            self._products[prec] = prod
+        prod(X, Y)
 
     def brownian_noise(self, Z: torch.Tensor, factor: float = 1.0, tolerance: float = 1e-3, max_iter: int = 1000,
                        precision=None):

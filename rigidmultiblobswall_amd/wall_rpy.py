@@ -231,6 +231,78 @@ class Product:
                                                0, self.eta, self.a, self.L, self.tp, ws.data_ptr(), ws.numel(), st))
 
 
+class StridedProduct:
+    """A prepared apply_strided (offsets None: one suspension) or
+    apply_ragged_strided (offsets: the batch's system boundaries) on the
+    positions held by ``r`` (a contiguous float64 (N, 3) device tensor, read
+    at every call): the launch shape and workspace size are settled once per
+    number of rows k.  Same arguments and results as those functions; the
+    Lanczos loops call it once per step, between read-backs of their
+    coefficients, where host time leaves the GPU idle."""
+
+    def __init__(self, mask: int, r: torch.Tensor, eta: float, a: float, periodic_length=None,
+                 precision="fp32_blocked", tuning=None, offsets=None):
+        self.lib = _lib.load()
+        self.device = _device()
+        if not isinstance(r, torch.Tensor) or r.dtype != torch.float64 or r.device != self.device or \
+                not r.is_contiguous() or r.numel() % 3:
+            raise ValueError(f"r must be a contiguous float64 (N, 3) tensor on {self.device}")
+        self.r = r
+        self.n = r.numel() // 3
+        self.mask = int(mask)
+        self.precision = precision if isinstance(precision, str) else str(precision)
+        self.prec = _lib.precision_code(precision)
+        self.tuning = tuning
+        self.periodic = _periodic(periodic_length)
+        self.L = _L_array(periodic_length)
+        self.eta, self.a = float(eta), float(a)
+        if offsets is None:
+            self.off = None
+            self.pairs = float(self.n) * self.n
+        else:
+            self.off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+            if self.off.size < 2 or int(self.off[-1]) != self.n:
+                raise ValueError(f"offsets must end at N = {self.n}")
+            self.offp = self.off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+            self.pairs = float(np.sum(np.diff(self.off).astype(np.float64) ** 2))
+        self._shapes: dict = {}
+
+    def _shape(self, k: int):
+        sh = self._shapes.get(k)
+        if sh is None:
+            tun = _tuning_for(self.tuning, self.mask, self.precision, k, self.pairs, self.off is not None,
+                              self.periodic)
+            if isinstance(tun, dict):
+                tun = _lib.with_flags(tun, 0)
+            tp = _lib.tuning_ptr(tun)
+            nbytes = ctypes.c_size_t()
+            if self.off is None:
+                _lib.check(self.lib.rmw_wall_rpy_workspace_size(self.mask, self.prec, self.n, k, tp,
+                                                                ctypes.byref(nbytes)))
+            else:
+                _lib.check(self.lib.rmw_wall_rpy_ragged_workspace_size(self.mask, self.prec, self.offp,
+                                                                       self.off.size - 1, k, tp,
+                                                                       ctypes.byref(nbytes)))
+            sh = (tun, tp, nbytes.value)
+            self._shapes[k] = sh
+        return sh
+
+    def __call__(self, X: torch.Tensor, Y: torch.Tensor) -> None:
+        k, ld, fp, tp_, vp, wp = _grand_rows(X, Y, self.n, self.device)
+        _, tp, nbytes = self._shape(k)
+        st = _stream.current(self.device)
+        if self.off is None:
+            ws = _workspace_of((self.device.index, st, "default"), nbytes, self.device)
+            _lib.check(self.lib.rmw_wall_rpy_apply(self.mask, self.prec, self.r.data_ptr(), fp, tp_, vp, wp, self.n,
+                                                   k, ld, self.eta, self.a, self.L, tp, ws.data_ptr(), ws.numel(),
+                                                   st))
+        else:
+            ws = _workspace_of((self.device.index, st, "ragged"), nbytes, self.device)
+            _lib.check(self.lib.rmw_wall_rpy_ragged_apply(self.mask, self.prec, self.r.data_ptr(), fp, tp_, vp, wp,
+                                                          self.offp, self.off.size - 1, k, ld, self.eta, self.a,
+                                                          self.L, tp, ws.data_ptr(), ws.numel(), st))
+
+
 def _grand_rows(X: torch.Tensor, Y: torch.Tensor, n: int, device):
     """Pointers of the [force; torque] halves of the k rows of stacked grand
     vectors X, Y (k, >= 6n), after checking that every row the kernels read

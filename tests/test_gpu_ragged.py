@@ -245,3 +245,44 @@ def test_c5_ragged_lanczos_per_replica(gpu, c5):
             Z[:, torch.from_numpy(idx).to(gpu)].contiguous(), tolerance=1e-3)
         assert it1 == its64[0, s]
         assert rel(n64[idx], one[0].cpu().numpy()) < 1e-9, s
+
+
+@pytest.mark.parametrize("prec", ["fp32_blocked", "fp64"])
+def test_prepared_strided_products(gpu, prec):
+    """wall_rpy.StridedProduct (launch shape and workspace settled once per
+    row count) equals apply_strided / apply_ragged_strided bit for bit, for
+    one suspension and for a ragged batch, at two row counts, and refuses
+    rows that do not hold the grand vectors."""
+    rng = np.random.default_rng(21)
+    sizes = [130, 7, 300, 1, 129]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(off[-1])
+    r = np.column_stack([rng.uniform(0, 25, n), rng.uniform(0, 25, n), rng.uniform(0.7, 3.0, n)])
+    rd = torch.from_numpy(r).to(gpu)
+    eta, a = 1.0, 0.45
+    print(f"case StridedProduct prec={prec} N={n} systems={len(sizes)}", flush=True)
+    single = wall_rpy.StridedProduct(GRAND, rd, eta, a, precision=prec)
+    ragged = wall_rpy.StridedProduct(GRAND, rd, eta, a, precision=prec, offsets=off)
+    for k in (3, 1, 3):
+        X = torch.from_numpy(rng.standard_normal((k, 6 * n))).to(gpu)
+        Y1, Y2 = torch.empty_like(X), torch.empty_like(X)
+        single(X, Y1)
+        wall_rpy.apply_strided(GRAND, rd, n, X, Y2, eta, a, precision=prec)
+        torch.cuda.synchronize()
+        assert torch.equal(Y1, Y2)
+        ragged(X, Y1)
+        wall_rpy.apply_ragged_strided(GRAND, rd, off, X, Y2, eta, a, precision=prec)
+        torch.cuda.synchronize()
+        assert torch.equal(Y1, Y2)
+    # every system's block of the ragged product is that system alone
+    b0, b1 = int(off[2]), int(off[3])
+    rs = torch.from_numpy(np.ascontiguousarray(r[b0:b1])).to(gpu)
+    Xs = torch.cat([X[:, 3 * b0:3 * b1], X[:, 3 * n + 3 * b0:3 * n + 3 * b1]], dim=1).contiguous()
+    Ys = torch.empty_like(Xs)
+    wall_rpy.StridedProduct(GRAND, rs, eta, a, precision=prec)(Xs, Ys)
+    got = torch.cat([Y1[:, 3 * b0:3 * b1], Y1[:, 3 * n + 3 * b0:3 * n + 3 * b1]], dim=1)
+    assert rel(got.cpu().numpy(), Ys.cpu().numpy()) <= (1e-13 if prec == "fp64" else 2e-6)
+    with pytest.raises(ValueError):
+        single(X[:, : 6 * n - 3], Y1)
+    with pytest.raises(ValueError):
+        wall_rpy.StridedProduct(GRAND, rd, eta, a, precision=prec, offsets=off[:-1])
