@@ -209,8 +209,15 @@ class RollersIntegrator:
         r = self.r if r is None else r
         n = self.n
         z = self._draw("z", (3 * n,))
-        noise, it = lanczos.lanczos_block(lambda X, Y: self._uf(r, X[0].reshape(n, 3), Y[0].reshape(n, 3)),
-                                          z.reshape(1, -1), factor=math.sqrt(2 * self.kT / dt),
+        # one prepared product serves every Lanczos step on these positions
+        prod = wall_rpy.Product(RMW_UF, r.contiguous(), eta=self.eta, a=self.a, periodic_length=self.periodic_length,
+                                precision=self.precision, domain=self.domain)
+
+        def mv(X, Y):
+            self.products["UF"] += 1
+            prod(force=X[0], out_velocity=Y[0])
+
+        noise, it = lanczos.lanczos_block(mv, z.reshape(1, -1), factor=math.sqrt(2 * self.kT / dt),
                                           tolerance=self.tolerance, max_iter=1000)
         self.stoch_iterations_count += it
         v = noise.reshape(n, 3)
