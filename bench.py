@@ -774,8 +774,8 @@ def main(argv=None):
     ap.add_argument("--gmres-evals", type=int, default=5)
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md but not built)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (the product runs on one "
+              "device; sharding the targets over several GPUs is not built)", file=sys.stderr)
         return 2
 
     import torch
