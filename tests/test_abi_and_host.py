@@ -9,6 +9,7 @@
 import ctypes
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -259,3 +260,13 @@ def test_work_item_schedule_covers_every_tile_pair_once():
     for C in range(1, 41):
         for M in range(0, 301, 7):
             assert lib.tp_ceil_sum(M, C) == sum(-(-m // C) for m in range(1, M + 1))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with status 2 and a message,
+    before touching a GPU."""
+    import subprocess
+    out = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120, cwd=REPO)
+    assert out.returncode == 2
+    assert "--gpus 2 is not supported" in out.stderr
