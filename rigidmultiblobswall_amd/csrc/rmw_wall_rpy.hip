@@ -559,8 +559,12 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
     for (int64_t S = S0; S < S1; ++S) {
       const int64_t s0 = S * kBlock;
       const int cnt = (int)(Sy.n - s0 < kBlock ? Sy.n - s0 : kBlock);
+      // the 64-source blocks of this tile the item covers: only their
+      // sources are staged (an item of one block reads a quarter of the tile)
+      const int bb0 = S == S0 ? (int)(blk0 - 4 * S) : 0;
+      const int bb1 = blk1 - 4 * S < kBlock / 64 ? (int)(blk1 - 4 * S) : kBlock / 64;
       __syncthreads();  // the previous tile has been consumed
-      {
+      if ((il >> 6) >= bb0 && (il >> 6) < bb1) {
         // the source record of this thread (finite zeros beyond the system:
         // pairs not taken are masked through their geometry, make_geom_on,
         // and still multiply these values by zero coefficients)
@@ -615,8 +619,6 @@ __global__ __launch_bounds__(kBlock) void mutual_kernel(MutParams p)
       // 1 % slower; matrix form: 40 % slower).
       constexpr bool kUnmasked = DOM == kDomNoWall || DOM == kDomFreeSurface || (!kMat && OC == 3);
       const bool full = kUnmasked && !diag && cnt == kBlock && (T + 1) * kBlock <= Sy.n;
-      const int bb0 = S == S0 ? (int)(blk0 - 4 * S) : 0;
-      const int bb1 = blk1 - 4 * S < kBlock / 64 ? (int)(blk1 - 4 * S) : kBlock / 64;
       for (int b = bb0; b < bb1 && 64 * b < cnt; ++b) {
         JT ja[K][OC];
 #pragma unroll
