@@ -235,3 +235,30 @@ def test_c3_size_properties(gpu):
     My = mob.single_wall_mobility_trans_times_force_hip(r, y, eta, a, precision="fp64")
     s1, s2 = float(x @ My), float(y @ u64)
     assert abs(s1 - s2) <= 1e-11 * (abs(s1) + abs(s2))
+
+
+@pytest.mark.parametrize("domain", ["no_wall", "in_plane", "free_surface"])
+def test_prepared_product_domains(gpu, domain):
+    """wall_rpy.Product in the other boundaries gives apply()'s result bit
+    for bit, on the current stream and on a side stream."""
+    import torch
+    rng = np.random.default_rng(17)
+    n, a, eta = 1200, 0.5, 1.0
+    r = torch.from_numpy(np.column_stack([rng.uniform(0, 30, n), rng.uniform(0, 30, n),
+                                          rng.uniform(0.6, 3.0, n)])).to(gpu)
+    f = torch.from_numpy(rng.standard_normal(3 * n)).to(gpu)
+    print(f"case Product domain={domain} N={n}", flush=True)
+    p = wall_rpy.Product(RMW_UF, r, eta=eta, a=a, precision="fp32_blocked", domain=domain)
+    u = torch.empty(3 * n, dtype=torch.float64, device=gpu)
+    p(force=f, out_velocity=u)
+    ue, _ = wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision="fp32_blocked", domain=domain)
+    torch.cuda.synchronize()
+    assert torch.equal(u, ue.reshape(-1))
+    side = torch.cuda.Stream()
+    u2 = torch.empty_like(u)
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        p(force=f, out_velocity=u2)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    assert torch.equal(u2, u)
