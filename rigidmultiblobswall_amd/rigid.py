@@ -96,7 +96,7 @@ class RigidSuspension:
                                            self.r_rel.data_ptr(), self.ws.data_ptr(), self.ws.numel(), _stream()))
         self._mlam = torch.empty((self.Nblobs, 3), dtype=torch.float64, device=dev)
         self._products = {}      # precision -> wall_rpy.Product on r_blobs
-        self._ops = None         # KrylovOps of the solves (vectors of self.size)
+        self._ops = {}           # stream handle -> KrylovOps of the solves (vectors of self.size)
         self.minv = None
         self.pc_r_rel = None     # body-frame blob offsets the preconditioner was built with
         self.chol = None         # per body L_b (M_b = L_b L_b^T), packed like minv
@@ -288,9 +288,13 @@ class RigidSuspension:
 
     # -- solve ------------------------------------------------------------------------
     def _krylov_ops(self) -> krylov.KrylovOps:
-        if self._ops is None:
-            self._ops = krylov.KrylovOps(self.size, 2, self.device)
-        return self._ops
+        # one per stream: the reductions' scratch is not shared across streams
+        key = _stream_mod.current(self.device)
+        ops = self._ops.get(key)
+        if ops is None:
+            ops = krylov.KrylovOps(self.size, 2, self.device)
+            self._ops[key] = ops
+        return ops
 
     def norm(self, v: torch.Tensor) -> float:
         """||v|| of a device vector of self.size entries (the fixed-order
