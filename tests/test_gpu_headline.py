@@ -77,6 +77,41 @@ def test_c2_operator_shape_vs_oracle(gpu, tuning):
     assert e <= 1e-6
 
 
+def test_c3_lanczos_block_product_vs_oracle(gpu):
+    """The C3 Lanczos block's product at full size: the grand mobility on 4
+    right-hand sides in one symmetric pass (explicit 3x3 blocks, the cache's
+    launch shape), through the prepared strided product the Lanczos loop
+    calls, against the oracle's four blocks on sampled rows of every
+    right-hand side."""
+    from rigidmultiblobswall_amd.rollers import GRAND, Rollers
+    w = workloads.c3_rollers()
+    r, eta, a = w["r"], w["eta"], w["a"]
+    n = r.shape[0]
+    Z = np.random.default_rng(9).standard_normal((4, 6 * n))
+    plan = wall_rpy.describe(GRAND, n, 4, "fp32_blocked")
+    print(f"case C3 grand k=4 N={n} fp32_blocked: {plan}", flush=True)
+    assert plan["kernel"] == "symmetric" and plan["rhs_group"] == 4
+    R = Rollers(r, eta, a, precision="fp32_blocked")
+    X = torch.from_numpy(Z).to(gpu)
+    Y = torch.empty_like(X)
+    R.grand_matvec(X, Y)
+    got = Y.cpu().numpy()
+    blocks = [(0, 48), (n // 2, n // 2 + 48), (n - 32, n)]
+    for q in range(4):
+        f, t = Z[q, : 3 * n], Z[q, 3 * n:]
+        num = den = 0.0
+        for t0, t1 in blocks:
+            ur, wr = O.grand(r, f, t, eta, a, rows=(t0, t1))
+            for part, ref in ((got[q, : 3 * n], ur), (got[q, 3 * n:], wr)):
+                g = part.reshape(n, 3)[t0:t1]
+                rr = ref.reshape(n, 3)[t0:t1]
+                num += float(np.sum((g - rr) ** 2))
+                den += float(np.sum(rr ** 2))
+        e = float(np.sqrt(num / den))
+        print(f"  rhs {q}: rel L2 vs oracle (128 rows, u and w) = {e:.3e}")
+        assert e <= 1e-6
+
+
 def test_output_buffers_are_checked(gpu):
     """A caller's output buffer of the wrong size, type, device or layout is
     refused before any launch."""
