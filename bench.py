@@ -928,7 +928,7 @@ def main(argv=None):
     if fparams:
         out["metric"] = "blob-blob soft-repulsion forces, blob-pairs/s"
         out["dtype"] = "f64"
-        out["config"]["precision"] = {"fp64": "FP64 sqrt, quotient and exp; FP64 sums",
+        out["config"]["precision"] = {"fp64": "FP64 1/r and exp; FP64 sums",
                                       "fp32": "FP64 geometry and sums, FP32 exponential"}[fprec]
     if not radii and not fparams and args.domain == "single_wall":
         out["oracle_error"] = oracle_error(u, r_h, f_h, eta, a)

@@ -255,9 +255,10 @@ int rmw_radii_self_apply(int prec, int boundary, const double* r_vectors, const 
  *   f0 = -(eps/b) exp(-(r - 2a)/b) / r  for r > 2a,  -(eps/b) / r  otherwise,
  * a = blob_radius, or (a_i + a_j)/2 when radius_blobs is given.  cutoff > 0
  * drops the pairs with r > cutoff (the neighbour-list form uses
- * 2a + 30b); 0 keeps every pair.  prec RMW_PREC_F64: FP64 sqrt, quotient
- * and exp; any other mode: FP64 geometry and sums with a hardware
- * reciprocal square root (one Newton step) and an FP32 exponential.
+ * 2a + 30b); 0 keeps every pair.  prec RMW_PREC_F64: FP64 1/r (hardware
+ * estimate and one third-order Newton step, within 1.4e-16) and FP64 exp;
+ * any other mode: FP64 geometry and sums with a hardware reciprocal square
+ * root (one Newton step) and an FP32 exponential.
  * Replaces
  *   blob_blob_force_numba / calc_blob_blob_forces_numba
  *                                  multi_bodies/forces_numba.py:12-72

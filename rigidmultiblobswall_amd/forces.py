@@ -36,8 +36,8 @@ def blob_forces(r_vectors, repulsion_strength: float, debye_length: float, blob_
 
     radius_blobs (N,) switches to the different-radii form (a = (a_i+a_j)/2);
     cutoff > 0 keeps only pairs with r <= cutoff.  precision "fp64": FP64
-    sqrt/quotient/exp; "fp32" (any FP32 mode): FP64 geometry and sums, FP32
-    exponential."""
+    1/r (to 1.4e-16) and exp; "fp32" (any FP32 mode): FP64 geometry and
+    sums, FP32 exponential."""
     lib = _lib.load()
     device = _device()
     r = torch.as_tensor(r_vectors)
