@@ -204,7 +204,9 @@ def fp64_leg(r, f, u, r_h, f_h, eta, a, steps, warmup, check: bool = True):
     import torch
     from rigidmultiblobswall_amd import wall_rpy
     from rigidmultiblobswall_amd._lib import RMW_UF
-    step = lambda: wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision="fp64", out_velocity=u)  # noqa: E731
+    prod = wall_rpy.Product(RMW_UF, r, eta=eta, a=a, precision="fp64")
+    step = lambda: prod(force=f, out_velocity=u)  # noqa: E731
+    settle(step)
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
@@ -222,6 +224,7 @@ def fp64_leg(r, f, u, r_h, f_h, eta, a, steps, warmup, check: bool = True):
     achieved = FLOP_PER_PAIR["UF"] * pairs / (pk.mean_ms * 1e-3) / 1e12
     launch = wall_rpy.describe(RMW_UF, n, 1, "fp64")
     return {"ms_per_step": ms, "pairs_per_s": pairs / (ms * 1e-3), "kernel_mean_ms": pk.mean_ms,
+            "kernel_median_ms": pk.median_ms,
             "kernel": f"{'mutual_kernel<UF,K=1,fp64>' if launch['kernel'] == 'symmetric' else 'pair_kernel_f64<UF,K=1>'}",
             "achieved_tflops": achieved, "peak_tflops": PEAK_FP64_TFLOPS, "frac": achieved / PEAK_FP64_TFLOPS,
             **({"oracle_error": oracle_error(u, r_h, f_h, eta, a)} if check else {})}
@@ -829,27 +832,31 @@ def main(argv=None):
         from rigidmultiblobswall_amd import forces
         fprec = "fp64" if args.precision == "fp64" else "fp32"
 
+    # the UF lines run the prepared product (launch shape, workspace and
+    # checks settled once; the same launches as wall_rpy.apply, bit for bit),
+    # so that host time per call stays far below the kernel's
+    prod = None if radii or fparams else \
+        wall_rpy.Product(RMW_UF, r, eta=eta, a=a, precision=args.precision, domain=args.domain)
+
     def step():
         if fparams:
             forces.blob_forces(r, precision=fprec, out=u2, **fparams)
         elif radii:
             wall_rpy.apply_radii_self(r, f, rad, eta, precision=args.precision, out=u2, boundary=rbnd)
         else:
-            wall_rpy.apply(RMW_UF, r, force=f, eta=eta, a=a, precision=args.precision, out_velocity=u,
-                           domain=args.domain)
+            prod(force=f, out_velocity=u)
 
-    # the FP64-mode product first: its ~30 ms of sustained load bring the GPU
-    # to its sustained clock before the timed FP32 products (from idle, the
-    # pair kernel's first launches at 32768 blobs take 1.2-1.4 ms and settle
-    # at ~1.08 ms after ~30 ms of load); its oracle comparison comes later,
-    # so that no host work separates it from the timed region
+    # the FP64-mode product first (its oracle comparison comes later, so that
+    # no host work separates it from the timed region)
     u64 = None
     if not radii and not fparams and args.domain == "single_wall" and args.precision != "fp64" \
             and not args.no_fp64:
         u64 = torch.empty_like(u)
         fp64_res = fp64_leg(r, f, u64, r_h, f_h, eta, a, max(5, args.steps // 2), 8, check=False)
-    else:
-        settle(step)
+    # then the timed product itself for 40 ms, untimed: from idle, and again
+    # after the FP64 leg, the FP32 pair kernel's first launches at 32768 blobs
+    # take 1.2-1.4 ms and settle at ~1.06 ms after ~35 ms of its own load
+    settle(step)
     for _ in range(args.warmup):
         step()
     ms = timed_steps(step, args.steps)
@@ -857,7 +864,7 @@ def main(argv=None):
     # its stream, over as many products again (taken apart from the product
     # time above, which the records would otherwise include)
     with wall_rpy.PairKernelTimer() as pk:
-        timed_steps(step, args.steps)
+        ms_rec = timed_steps(step, args.steps)
     pairs = float(n) * n
     value = pairs / (ms * 1e-3)
     op = ("forces" if fparams else ("radii" if args.domain == "single_wall" else f"radii_{args.domain}") if radii
@@ -920,11 +927,14 @@ def main(argv=None):
                                              else "pair_kernel_uf", kname)
                                 if args.domain == "single_wall" else None,
                      "kernel": kname,
-                     "kernel_mean_ms": kernel_ms, "kernel_launches": pk.launches,
+                     "kernel_mean_ms": kernel_ms, "kernel_median_ms": pk.median_ms, "kernel_launches": pk.launches,
+                     # the product time of the kernel-timed loop (its event records included)
+                     "kernel_loop_ms_per_step": ms_rec,
                      "flop_per_pair": FLOP_PER_PAIR[op]},
     }
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    log(f"matvec: {ms:.4f} ms/step, {value:.4g} pairs/s, pair kernel {kernel_ms:.4f} ms")
+    log(f"matvec: {ms:.4f} ms/step, {value:.4g} pairs/s, pair kernel {kernel_ms:.4f} ms (median {pk.median_ms:.4f}; "
+        f"{ms_rec:.4f} ms/step in its timed loop)")
     if fparams:
         out["metric"] = "blob-blob soft-repulsion forces, blob-pairs/s"
         out["dtype"] = "f64"

@@ -321,6 +321,14 @@ int rmw_profile_collect(int64_t* launches, double* total_ms, double* mean_ms);
 /* The same record split by kernel family (arrays of 3: wall RPY, different
  * radii, blob-blob forces); also clears the record. */
 int rmw_profile_collect_kinds(int64_t* launches, double* total_ms);
+/* The record launch by launch, in launch order: the number of launches
+ * recorded so far (rmw_profile_pending), then each one's kernel family
+ * (0 wall RPY, 1 different radii, 2 blob-blob forces) and duration in ms,
+ * the first cap of them written; *launches = all of them; clears the
+ * record.  Medians and spreads of the launch times come from this (no
+ * counterpart in the reference: timing of this library's own launches). */
+int rmw_profile_pending(int64_t* launches);
+int rmw_profile_collect_launches(int32_t* kinds, double* ms, int64_t cap, int64_t* launches);
 
 /* ------------------------------------------------------------------------
  * Rigid multiblob bodies (FP64).  body_offsets is a HOST array of nbodies+1

@@ -90,6 +90,8 @@ PROTOTYPES = {
     "rmw_rollers_advance": (ctypes.c_int, [_dp, _dp, _dp, _dp, _i64, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, _dp, _dp, _vp, _sz, _vp]),
     "rmw_profile_collect_kinds": (ctypes.c_int, [_i64p, _hdp]),
+    "rmw_profile_pending": (ctypes.c_int, [_i64p]),
+    "rmw_profile_collect_launches": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), _hdp, _i64, _i64p]),
     "rmw_mobility_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int,
                                                    _tp, _szp]),
     "rmw_mobility_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _i64,

@@ -2104,6 +2104,39 @@ extern "C" int rmw_profile_collect_kinds(int64_t* launches, double* total_ms)
   });
 }
 
+extern "C" int rmw_profile_pending(int64_t* launches)
+{
+  return guarded([&] {
+    RMW_REQUIRE(launches != nullptr, "launches is null");
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    *launches = (int64_t)g_prof.size();
+  });
+}
+
+extern "C" int rmw_profile_collect_launches(int32_t* kinds, double* ms, int64_t cap, int64_t* launches)
+{
+  return guarded([&] {
+    RMW_REQUIRE(kinds && ms && launches && cap >= 0, "kinds, ms and launches are required");
+    std::vector<ProfRec> recs;
+    {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      recs.swap(g_prof);
+    }
+    *launches = (int64_t)recs.size();
+    for (size_t i = 0; i < recs.size(); ++i) {
+      RMW_CHECK_HIP(hipEventSynchronize(recs[i].b));
+      float t = 0.f;
+      RMW_CHECK_HIP(hipEventElapsedTime(&t, recs[i].a, recs[i].b));
+      if ((int64_t)i < cap) {
+        kinds[i] = recs[i].kind;
+        ms[i] = t;
+      }
+      RMW_CHECK_HIP(hipEventDestroy(recs[i].a));
+      RMW_CHECK_HIP(hipEventDestroy(recs[i].b));
+    }
+  });
+}
+
 extern "C" int rmw_device_info(int* cu_count, int* clock_khz, int* wavefront, char* arch_name, int arch_len)
 {
   return guarded([&] {

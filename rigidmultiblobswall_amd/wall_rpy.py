@@ -526,9 +526,10 @@ PROFILE_KINDS = ("wall_rpy", "radii", "forces")
 class PairKernelTimer:
     """Context manager timing every pair-kernel launch with HIP events.
 
-    After the block: launches / total_ms / mean_ms over all pair kernels and
-    ``by_kind[name] = (launches, total_ms, mean_ms)`` per kernel family
-    (wall RPY, different radii, blob-blob forces)."""
+    After the block: launches / total_ms / mean_ms / median_ms over all pair
+    kernels, ``by_kind[name] = (launches, total_ms, mean_ms)`` per kernel
+    family (wall RPY, different radii, blob-blob forces) and
+    ``durations[name]``, each launch's time in ms in launch order."""
 
     def __enter__(self):
         lib = _lib.load()
@@ -539,11 +540,20 @@ class PairKernelTimer:
     def __exit__(self, *exc):
         lib = _lib.load()
         _lib.check(lib.rmw_profile_enable(0))
-        n = (ctypes.c_int64 * 3)()
-        tot = (ctypes.c_double * 3)()
-        _lib.check(lib.rmw_profile_collect_kinds(n, tot))
-        self.by_kind = {k: (n[i], tot[i], tot[i] / n[i] if n[i] else 0.0) for i, k in enumerate(PROFILE_KINDS)}
-        self.launches = sum(n)
-        self.total_ms = sum(tot)
-        self.mean_ms = self.total_ms / self.launches if self.launches else 0.0
+        pending = ctypes.c_int64()
+        _lib.check(lib.rmw_profile_pending(ctypes.byref(pending)))
+        cap = pending.value
+        kinds = (ctypes.c_int32 * max(cap, 1))()
+        ms = (ctypes.c_double * max(cap, 1))()
+        got = ctypes.c_int64()
+        _lib.check(lib.rmw_profile_collect_launches(kinds, ms, cap, ctypes.byref(got)))
+        if got.value != cap:  # recording is off, so nothing can be added in between
+            raise RuntimeError(f"pair-kernel record changed while collected ({cap} -> {got.value})")
+        self.durations = {k: [ms[j] for j in range(cap) if kinds[j] == i] for i, k in enumerate(PROFILE_KINDS)}
+        self.by_kind = {k: (len(d), sum(d), sum(d) / len(d) if d else 0.0) for k, d in self.durations.items()}
+        every = [ms[j] for j in range(cap)]
+        self.launches = cap
+        self.total_ms = sum(every)
+        self.mean_ms = self.total_ms / cap if cap else 0.0
+        self.median_ms = float(np.median(every)) if cap else 0.0
         return False

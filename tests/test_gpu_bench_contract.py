@@ -34,4 +34,5 @@ def test_bench_json_line(gpu):
     assert rf["peak"] == 157.3 and abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-12
     # the pair kernel runs inside the product
     assert 0.0 < rf["kernel_mean_ms"] <= d["ms_per_step"] * 1.15
+    assert 0.0 < rf["kernel_median_ms"] <= d["ms_per_step"] * 1.15 and rf["kernel_launches"] == 3
     assert d["oracle_error"]["rel_l2"] <= 1e-6 and d["fp64"]["oracle_error"]["rel_l2"] <= 1e-12

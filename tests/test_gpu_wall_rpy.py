@@ -262,3 +262,28 @@ def test_prepared_product_domains(gpu, domain):
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     assert torch.equal(u2, u)
+
+
+def test_pair_kernel_timer_launch_record(gpu):
+    """PairKernelTimer reads the launch record launch by launch: one entry
+    per pair-kernel launch in the wall-RPY family, positive times whose sum
+    and median agree with its totals, nothing recorded outside the block."""
+    import torch
+    rng = np.random.default_rng(21)
+    n = 4096
+    r = torch.from_numpy(np.column_stack([rng.uniform(0, 60, n), rng.uniform(0, 60, n),
+                                          rng.uniform(0.8, 2.0, n)])).to(gpu)
+    f = torch.from_numpy(rng.standard_normal(3 * n)).to(gpu)
+    print(f"case PairKernelTimer UF N={n} x5", flush=True)
+    wall_rpy.apply(RMW_UF, r, force=f, eta=1.0, a=0.5, precision="fp32_blocked")   # not recorded
+    with wall_rpy.PairKernelTimer() as pk:
+        for _ in range(5):
+            wall_rpy.apply(RMW_UF, r, force=f, eta=1.0, a=0.5, precision="fp32_blocked")
+    d = pk.durations
+    assert pk.launches == 5 and len(d["wall_rpy"]) == 5 and not d["radii"] and not d["forces"]
+    assert all(t > 0.0 for t in d["wall_rpy"])
+    assert abs(pk.total_ms - sum(d["wall_rpy"])) <= 1e-9 * pk.total_ms
+    assert pk.by_kind["wall_rpy"][0] == 5 and min(d["wall_rpy"]) <= pk.median_ms <= max(d["wall_rpy"])
+    with wall_rpy.PairKernelTimer() as empty:
+        pass
+    assert empty.launches == 0 and empty.median_ms == 0.0
