@@ -310,8 +310,9 @@ def gmres_leg(steps: int, epochs: int, evals: int, log):
 
 
 def settle(step, ms: float = 40.0) -> None:
-    """Repeat `step` (untimed) until `ms` milliseconds of GPU work have run:
-    from idle the GPU needs ~30 ms of sustained load to reach its sustained
+    """Repeat `step` (untimed) until `ms` milliseconds have passed on the
+    device (whole steps, four at a time): from idle, and after a change of
+    kernel, the GPU needs ~30 ms of sustained load to reach its sustained
     clock (the pair kernels' first launches run up to 30 % longer)."""
     import torch
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -358,6 +359,7 @@ def rollers_step_bench(args, log):
               omega_one_roller=p["omega_one_roller"], free_kinematics=p["free_kinematics"],
               tolerance=p["solver_tolerance"])
     integ = RollersIntegrator(r0, blob_blob_force_implementation="hip", precision=args.precision, seed=11, **kw)
+    settle(lambda: integ.advance_time_step(p["dt"]))
     for _ in range(args.warmup):
         integ.advance_time_step(p["dt"])
     torch.cuda.synchronize()
@@ -458,6 +460,7 @@ def lanczos_bench(args, log, ragged: bool):
         desc = (f"C3 rollers: N={n}, a=0.656, eta=1e-3, grand mobility (trans + rot), block of {k} RHS sharing "
                 f"each product, Lanczos tol 1e-3")
     solve = lambda: R.brownian_noise(Z, factor=1.0, tolerance=1e-3)   # noqa: E731
+    settle(solve)
     for _ in range(args.warmup):
         solve()
     torch.cuda.synchronize()
@@ -549,6 +552,7 @@ def rigid_step_bench(args, log):
     prec = "fp64" if args.precision == "fp32_blocked" else args.precision
     integ = RigidIntegrator(w["centres"], w["quats"], w["ref"], precision=prec, solver=args.rigid_solver, **kw)
     dt = 1e-3
+    settle(lambda: integ.advance_time_step(dt))
     for _ in range(args.warmup):
         integ.advance_time_step(dt)
     torch.cuda.synchronize()
@@ -681,6 +685,7 @@ def prod_timing_bench(args, log):
             torch.cuda.synchronize()
         return st.elapsed_time(sp) / steps, pk
 
+    settle(lambda: [run_block(*blk) for blk in blocks])
     for _ in range(args.warmup):
         for blk in blocks:
             run_block(*blk)
